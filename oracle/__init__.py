@@ -1,0 +1,245 @@
+"""CPU oracle for the ufunc / reduction hot path (test infrastructure only).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg may import this package; the product package ``anionpy_amd`` never does.
+
+The arithmetic lives in ``ionp_oracle.c`` (a scalar C restatement of the
+reference's algorithm, see the citations there); this module only marshals
+NumPy arrays (any strides, including negative and zero) into its descriptors
+and allocates outputs with NumPy's K-order layout.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+MAXDIM = 16
+DTYPES = [np.bool_, np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32,
+          np.int64, np.uint64, np.float32, np.float64]
+_CODE = {np.dtype(d): i for i, d in enumerate(DTYPES)}
+
+OPS = ["add", "subtract", "multiply", "true_divide", "maximum", "minimum",
+       "less", "less_equal", "greater", "greater_equal", "equal", "not_equal"]
+OP = {n: i for i, n in enumerate(OPS)}
+
+BUFSIZE = 8192  # np.getbufsize() of NumPy 2.2.6
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int32), ("ndim", ctypes.c_int32),
+                ("shape", ctypes.c_int64 * MAXDIM), ("strides", ctypes.c_int64 * MAXDIM),
+                ("data", ctypes.c_void_p)]
+
+
+class OracleError(Exception):
+    def __init__(self, code):
+        super().__init__(f"oracle status {code}")
+        self.code = code
+
+
+def build():
+    """Compile liboracle.so from ionp_oracle.c (gcc, no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        src = os.path.join(_HERE, "ionp_oracle.c")
+        if not os.path.exists(_LIB_PATH) or os.path.getmtime(_LIB_PATH) < os.path.getmtime(src):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.POINTER(_Desc)
+        for name, args in [("ora_binary", [ctypes.c_int, P, P, P]),
+                           ("ora_where", [P, P, P, P]),
+                           ("ora_cast", [P, P]),
+                           ("ora_reduce", [ctypes.c_int, P, ctypes.c_char_p, P, ctypes.c_int64]),
+                           ("ora_argext", [ctypes.c_int, P, ctypes.c_int, P])]:
+            getattr(L, name).argtypes = args
+            getattr(L, name).restype = ctypes.c_int
+        L.ora_result_type.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.ora_result_type.restype = ctypes.c_int
+        L.ora_binary_types.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.ora_binary_types.restype = ctypes.c_int
+        L.ora_reduce_dtype.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.ora_reduce_dtype.restype = ctypes.c_int
+        L.ora_korder_strides.argtypes = [ctypes.c_int, ctypes.POINTER(P), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        L.ora_korder_strides.restype = ctypes.c_int
+        L.ora_pairwise_f64.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        L.ora_pairwise_f64.restype = ctypes.c_double
+        L.ora_pairwise_f32.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        L.ora_pairwise_f32.restype = ctypes.c_float
+        L.ora_axis_order.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                     ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]
+        L.ora_axis_order.restype = None
+        _lib = L
+    return _lib
+
+
+def _desc(a):
+    a = np.asarray(a)
+    d = _Desc()
+    d.dtype = _CODE[a.dtype]
+    d.ndim = a.ndim
+    for i in range(a.ndim):
+        d.shape[i] = a.shape[i]
+        assert a.strides[i] % a.itemsize == 0
+        d.strides[i] = a.strides[i] // a.itemsize
+    d.data = a.__array_interface__["data"][0]
+    return d
+
+
+def _check(rc):
+    if rc:
+        raise OracleError(rc)
+
+
+def _empty_strided(shape, strides_elems, dtype):
+    dtype = np.dtype(dtype)
+    span = 1 + sum((s - 1) * abs(st) for s, st in zip(shape, strides_elems)) if all(shape) else 0
+    base = np.empty(max(span, 1), dtype=dtype)
+    return np.lib.stride_tricks.as_strided(base, shape=shape,
+                                           strides=[st * dtype.itemsize for st in strides_elems])
+
+
+def korder_strides(arrays, shape):
+    descs = [_desc(a) for a in arrays]
+    arr = (ctypes.POINTER(_Desc) * len(descs))(*[ctypes.pointer(d) for d in descs])
+    shp = (ctypes.c_int64 * MAXDIM)(*shape)
+    out = (ctypes.c_int64 * MAXDIM)()
+    _check(lib().ora_korder_strides(len(descs), arr, len(shape), shp, out))
+    return [out[i] for i in range(len(shape))]
+
+
+def result_type(a, b):
+    return np.dtype(DTYPES[lib().ora_result_type(_CODE[np.dtype(a)], _CODE[np.dtype(b)])])
+
+
+def binary_types(op, a, b):
+    lo, ou = ctypes.c_int(), ctypes.c_int()
+    _check(lib().ora_binary_types(OP[op], _CODE[np.dtype(a)], _CODE[np.dtype(b)],
+                                  ctypes.byref(lo), ctypes.byref(ou)))
+    return np.dtype(DTYPES[lo.value]), np.dtype(DTYPES[ou.value])
+
+
+def broadcast_shape(*shapes):
+    nd = max(len(s) for s in shapes)
+    out = []
+    for i in range(nd):
+        dims = [s[i - (nd - len(s))] if i >= nd - len(s) else 1 for s in shapes]
+        big = [d for d in dims if d != 1]
+        if big and any(d != big[0] for d in big):
+            raise OracleError(1)
+        out.append(big[0] if big else 1)
+    return tuple(out)
+
+
+def binary(op, a, b):
+    """np.<op>(a, b) with NumPy's K-order output layout."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    _, odt = binary_types(op, a.dtype, b.dtype)
+    shape = broadcast_shape(a.shape, b.shape)
+    out = _empty_strided(shape, korder_strides([a, b], shape), odt)
+    da, db, do = _desc(a), _desc(b), _desc(out)
+    _check(lib().ora_binary(OP[op], ctypes.byref(da), ctypes.byref(db), ctypes.byref(do)))
+    return out
+
+
+def where(c, x, y):
+    c, x, y = np.asarray(c), np.asarray(x), np.asarray(y)
+    odt = result_type(x.dtype, y.dtype)
+    shape = broadcast_shape(c.shape, x.shape, y.shape)
+    out = _empty_strided(shape, korder_strides([c, x, y], shape), odt)
+    dc, dx, dy, do = _desc(c), _desc(x), _desc(y), _desc(out)
+    _check(lib().ora_where(ctypes.byref(dc), ctypes.byref(dx), ctypes.byref(dy), ctypes.byref(do)))
+    return out
+
+
+def cast(x, dtype):
+    """x.astype(dtype) (order='K')."""
+    x = np.asarray(x)
+    out = _empty_strided(x.shape, korder_strides([x], x.shape), dtype)
+    dx, do = _desc(x), _desc(out)
+    _check(lib().ora_cast(ctypes.byref(dx), ctypes.byref(do)))
+    return out
+
+
+def _norm_axes(axis, ndim):
+    if axis is None:
+        return list(range(ndim))
+    if isinstance(axis, int):
+        axis = (axis,)
+    out = []
+    for a in axis:
+        n = a + max(ndim, 1) if a < 0 else a
+        if n < 0 or n >= max(ndim, 1):
+            raise OracleError(6)
+        out.append(n)
+    if len(set(out)) != len(out):
+        raise OracleError(4)
+    return [] if ndim == 0 else out
+
+
+def reduce(op, x, axis=0, bufsize=BUFSIZE):
+    """np.<op>.reduce(x, axis) for op in add/maximum/minimum."""
+    x = np.asarray(x)
+    axes = _norm_axes(axis, x.ndim)
+    mask = bytes(1 if k in axes else 0 for k in range(x.ndim)) or b"\0"
+    odt = DTYPES[lib().ora_reduce_dtype(OP[op], _CODE[x.dtype])]
+    kept = [k for k in range(x.ndim) if k not in axes]
+    shape = tuple(x.shape[k] for k in kept)
+    # K-order of the kept axes, following the input's iterator order
+    perm = (ctypes.c_int * MAXDIM)()
+    if x.ndim:
+        shp = (ctypes.c_int64 * MAXDIM)(*x.shape)
+        st = (ctypes.c_int64 * MAXDIM)(*[s // x.itemsize for s in x.strides])
+        lib().ora_axis_order(x.ndim, shp, st, perm)
+    order = [perm[i] for i in range(x.ndim) if perm[i] in kept]
+    ostr = [0] * len(kept)
+    acc = 1
+    for a in order:
+        ostr[kept.index(a)] = acc
+        acc *= max(x.shape[a], 1)
+    out = _empty_strided(shape, ostr, odt)
+    dx, do = _desc(x), _desc(out)
+    _check(lib().ora_reduce(OP[op], ctypes.byref(dx), mask, ctypes.byref(do), bufsize))
+    return out[()] if out.ndim == 0 else out
+
+
+def argext(ismax, x, axis=None):
+    x = np.asarray(x)
+    if axis is None:
+        out = np.empty((), dtype=np.int64)
+        ax = -1
+    else:
+        ax = _norm_axes(axis, x.ndim)[0]
+        out = np.empty(tuple(s for k, s in enumerate(x.shape) if k != ax), dtype=np.int64)
+    dx, do = _desc(x), _desc(out)
+    _check(lib().ora_argext(1 if ismax else 0, ctypes.byref(dx), ax, ctypes.byref(do)))
+    return out[()] if out.ndim == 0 else out
+
+
+def argmax(x, axis=None):
+    return argext(True, x, axis)
+
+
+def argmin(x, axis=None):
+    return argext(False, x, axis)
+
+
+def pairwise(x):
+    """NumPy's pairwise tree over one contiguous run (no buffer windows)."""
+    x = np.ascontiguousarray(x)
+    if x.dtype == np.float64:
+        return np.float64(lib().ora_pairwise_f64(x.ctypes.data, x.size))
+    return np.float32(lib().ora_pairwise_f32(x.ctypes.data, x.size))
