@@ -1,0 +1,557 @@
+"""anionpy_amd -- MI355X (gfx950) backend for anionpy's ufunc and reduction engine.
+
+NumPy-named API over device arrays, mirroring the reference's Python surface
+(anionpy/__init__.py re-exports of the ionp-py Ufunc objects,
+ionp-py/src/lib.rs:8697 `Ufunc`, ionp-py/src/reductions.rs sum/max/argmax,
+where).  Every call runs on the GPU through the C ABI in include/ionp_hip.h;
+there is no CPU fallback: without the built library, import fails.
+
+    import anionpy_amd as ap
+    A = ap.asarray(np_array)            # host -> HBM
+    C = ap.add(A, b)                     # broadcast, K-order output
+    s = ap.add.reduce(C, axis=0)         # NumPy's pairwise tree per buffer window
+    C.get()                              # HBM -> host numpy (same strides)
+"""
+import builtins
+import ctypes
+import numbers
+import operator
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import IonpError
+
+__all__ = ["ndarray", "asarray", "empty", "zeros", "Ufunc", "add", "subtract", "multiply", "divide",
+           "true_divide", "maximum", "minimum", "less", "less_equal", "greater", "greater_equal", "equal",
+           "not_equal", "where", "sum", "max", "amax", "min", "amin", "argmax", "argmin", "result_type",
+           "AxisError", "UFuncTypeError", "IonpError", "setbufsize", "getbufsize"]
+
+_NP_DTYPES = [np.bool_, np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64,
+              np.uint64, np.float32, np.float64]
+_CODE = {np.dtype(d): i for i, d in enumerate(_NP_DTYPES)}
+
+
+class AxisError(ValueError, IndexError):
+    """numpy.exceptions.AxisError counterpart (error.rs AxisError)."""
+
+
+class UFuncTypeError(TypeError):
+    """numpy's _UFuncNoLoopError counterpart (error.rs NoUfuncLoop)."""
+
+
+def _raise(e):
+    # IonpError -> Python exception type (lib.rs:65-75 to_py_err)
+    code = e.code
+    msg = str(e)
+    if code in (1, 2, 5):
+        raise ValueError(msg) from None
+    if code == 3:
+        raise IndexError(msg) from None
+    if code == 4:
+        raise TypeError(msg) from None
+    if code == 6:
+        raise UFuncTypeError(msg) from None
+    if code == 7:
+        raise AxisError(msg) from None
+    if code == 21:
+        raise NotImplementedError(msg) from None
+    raise RuntimeError(msg) from None
+
+
+def _call(fn, *args):
+    code = fn(*args)
+    if code != 0:
+        try:
+            _lib.check(code)
+        except IonpError as e:
+            _raise(e)
+
+
+def _dtype(dt):
+    dt = np.dtype(dt)
+    if dt not in _CODE:
+        raise TypeError(f"dtype {dt} is not supported by the MI355X backend")
+    return dt
+
+
+def _device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("anionpy_amd needs a HIP device (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+class ndarray:
+    """Strided view of device (HBM) memory: dtype, shape, element strides and
+    an element offset into a byte allocation (ionp-core/src/array.rs:24-29)."""
+
+    __slots__ = ("_buf", "dtype", "shape", "strides_el", "offset", "__weakref__")
+    __array_priority__ = 100
+
+    def __init__(self, buf, dtype, shape, strides_el, offset=0):
+        self._buf = buf  # torch.uint8 CUDA tensor owning the bytes
+        self.dtype = np.dtype(dtype)
+        self.shape = tuple(int(s) for s in shape)
+        self.strides_el = tuple(int(s) for s in strides_el)
+        self.offset = int(offset)
+
+    # ---- descriptors ---------------------------------------------------------
+    def _desc(self):
+        d = _lib.ArrayDesc()
+        d.dtype = _CODE[self.dtype]
+        d.ndim = len(self.shape)
+        for i, (n, s) in enumerate(zip(self.shape, self.strides_el)):
+            d.shape[i] = n
+            d.strides[i] = s
+        d.offset = self.offset
+        d.data = self._buf.data_ptr() if self._buf is not None and self._buf.numel() else 0
+        return d
+
+    # ---- numpy-like attributes ------------------------------------------------
+    @property
+    def ndim(self):
+        return len(self.shape)
+
+    @property
+    def size(self):
+        n = 1
+        for s in self.shape:
+            n *= s
+        return n
+
+    @property
+    def itemsize(self):
+        return self.dtype.itemsize
+
+    @property
+    def strides(self):
+        return tuple(s * self.itemsize for s in self.strides_el)
+
+    @property
+    def nbytes(self):
+        return self.size * self.itemsize
+
+    @property
+    def T(self):
+        return self.transpose()
+
+    def __len__(self):
+        if not self.shape:
+            raise TypeError("len() of unsized object")
+        return self.shape[0]
+
+    def __repr__(self):
+        return f"anionpy_amd.ndarray(shape={self.shape}, dtype={self.dtype}, strides={self.strides})"
+
+    # ---- views ---------------------------------------------------------------------
+    def transpose(self, *axes):
+        if not axes or axes == (None,):
+            axes = tuple(range(self.ndim))[::-1]
+        elif len(axes) == 1 and isinstance(axes[0], (tuple, list)):
+            axes = tuple(axes[0])
+        axes = tuple(a % self.ndim for a in axes)
+        if sorted(axes) != list(range(self.ndim)):
+            raise ValueError("axes don't match array")
+        return ndarray(self._buf, self.dtype, [self.shape[a] for a in axes],
+                       [self.strides_el[a] for a in axes], self.offset)
+
+    def __getitem__(self, key):
+        """Basic indexing: ints, slices (any step), Ellipsis and None."""
+        if not isinstance(key, tuple):
+            key = (key,)
+        nexp = builtins.sum(1 for k in key if k is not None and k is not Ellipsis)
+        if any(k is Ellipsis for k in key):
+            i = next(i for i, k in enumerate(key) if k is Ellipsis)
+            key = key[:i] + (slice(None),) * (self.ndim - nexp) + key[i + 1:]
+        key = key + (slice(None),) * (self.ndim - builtins.sum(1 for k in key if k is not None))
+        shape, strides, off, ax = [], [], self.offset, 0
+        for k in key:
+            if k is None:
+                shape.append(1)
+                strides.append(0)
+                continue
+            n, st = self.shape[ax], self.strides_el[ax]
+            if isinstance(k, slice):
+                start, stop, step = k.indices(n)
+                cnt = len(range(start, stop, step))
+                off += start * st if cnt else 0
+                shape.append(cnt)
+                strides.append(st * step)
+            elif isinstance(k, numbers.Integral):
+                i = operator.index(k)
+                if i < -n or i >= n:
+                    raise IndexError(f"index {i} is out of bounds for axis {ax} with size {n}")
+                off += (i % n) * st
+            else:
+                raise IndexError("only basic indexing (ints, slices, None, ...) is supported")
+            ax += 1
+        return ndarray(self._buf, self.dtype, shape, strides, off)
+
+    def reshape(self, *shape):
+        if len(shape) == 1 and isinstance(shape[0], (tuple, list)):
+            shape = tuple(shape[0])
+        shape = list(shape)
+        if -1 in shape:
+            known = 1
+            for s in shape:
+                if s != -1:
+                    known *= s
+            shape[shape.index(-1)] = self.size // known if known else 0
+        n = 1
+        for s in shape:
+            n *= s
+        if n != self.size:
+            raise ValueError(f"cannot reshape array of size {self.size} into shape "
+                             f"({','.join(str(s) for s in shape)}{',' if len(shape) == 1 else ''})")
+        src = self if self.flags_c_contiguous() else self.copy()
+        st, acc = [0] * len(shape), 1
+        for i in range(len(shape) - 1, -1, -1):
+            st[i] = acc
+            acc *= shape[i]
+        return ndarray(src._buf, src.dtype, shape, st, src.offset)
+
+    def flags_c_contiguous(self):
+        acc = 1
+        for n, s in zip(self.shape[::-1], self.strides_el[::-1]):
+            if n != 1 and s != acc:
+                return False
+            acc *= n
+        return True
+
+    def copy(self):
+        return self.astype(self.dtype, order="C")
+
+    def astype(self, dtype, order="K"):
+        dtype = _dtype(dtype)
+        t = _lib.ArrayDesc()
+        d = self._desc()
+        _call(_lib.load().ionp_plan_cast, ctypes.byref(d), _CODE[dtype], ctypes.byref(t))
+        strides = _c_strides(self.shape) if order == "C" else list(t.strides)[: self.ndim]
+        out = empty_strided(self.shape, strides, dtype)
+        o = out._desc()
+        _call(_lib.load().ionp_cast, ctypes.byref(d), ctypes.byref(o), None, _lib.stream_handle())
+        return out
+
+    # ---- transfer ---------------------------------------------------------------------
+    def get(self):
+        """Copy to a host numpy array with the same shape and strides."""
+        if self.size == 0:
+            return np.empty(self.shape, dtype=self.dtype)
+        lo, hi = _span(self.shape, self.strides_el)
+        isz = self.itemsize
+        nbytes = (hi - lo + 1) * isz
+        start = (self.offset + lo) * isz
+        torch.cuda.current_stream().synchronize()
+        host = self._buf[start:start + nbytes].cpu().numpy()
+        base = host.view(self.dtype)
+        return np.lib.stride_tricks.as_strided(base[-lo:] if lo else base, shape=self.shape,
+                                               strides=[s * isz for s in self.strides_el])
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.get()
+        return a.astype(dtype) if dtype is not None else a
+
+    def item(self):
+        return self.get().item()
+
+    def __float__(self):
+        return float(self.item())
+
+    def __int__(self):
+        return int(self.item())
+
+    def __bool__(self):
+        return bool(self.item())
+
+    # ---- reductions as methods (ndarray_attrs.rs mirrors do_reduce_axis) --------------
+    def sum(self, axis=None, dtype=None, keepdims=False):
+        return sum(self, axis=axis, dtype=dtype, keepdims=keepdims)
+
+    def max(self, axis=None, keepdims=False):
+        return amax(self, axis=axis, keepdims=keepdims)
+
+    def min(self, axis=None, keepdims=False):
+        return amin(self, axis=axis, keepdims=keepdims)
+
+    def argmax(self, axis=None, keepdims=False):
+        return argmax(self, axis=axis, keepdims=keepdims)
+
+    def argmin(self, axis=None, keepdims=False):
+        return argmin(self, axis=axis, keepdims=keepdims)
+
+    # ---- operators -----------------------------------------------------------------------
+    def __add__(self, o): return add(self, o)
+    def __radd__(self, o): return add(o, self)
+    def __sub__(self, o): return subtract(self, o)
+    def __rsub__(self, o): return subtract(o, self)
+    def __mul__(self, o): return multiply(self, o)
+    def __rmul__(self, o): return multiply(o, self)
+    def __truediv__(self, o): return true_divide(self, o)
+    def __rtruediv__(self, o): return true_divide(o, self)
+    def __lt__(self, o): return less(self, o)
+    def __le__(self, o): return less_equal(self, o)
+    def __gt__(self, o): return greater(self, o)
+    def __ge__(self, o): return greater_equal(self, o)
+    def __eq__(self, o): return equal(self, o)
+    def __ne__(self, o): return not_equal(self, o)
+    __hash__ = None
+
+
+def _c_strides(shape):
+    st, acc = [0] * len(shape), 1
+    for i in range(len(shape) - 1, -1, -1):
+        st[i] = acc
+        acc *= builtins.max(shape[i], 1)
+    return st
+
+
+def _span(shape, strides):
+    lo = builtins.sum(builtins.min(0, (n - 1) * s) for n, s in zip(shape, strides))
+    hi = builtins.sum(builtins.max(0, (n - 1) * s) for n, s in zip(shape, strides))
+    return lo, hi
+
+
+def empty_strided(shape, strides_el, dtype):
+    dtype = _dtype(dtype)
+    shape = tuple(int(s) for s in shape)
+    if any(s == 0 for s in shape):
+        nel = 0
+        lo = 0
+    else:
+        lo, hi = _span(shape, strides_el)
+        nel = hi - lo + 1
+    buf = torch.empty(builtins.max(nel, 1) * dtype.itemsize, dtype=torch.uint8, device=_device())
+    return ndarray(buf, dtype, shape, strides_el, -lo)
+
+
+def empty(shape, dtype=np.float64):
+    if isinstance(shape, numbers.Integral):
+        shape = (shape,)
+    return empty_strided(shape, _c_strides(shape), dtype)
+
+
+def zeros(shape, dtype=np.float64):
+    a = empty(shape, dtype)
+    a._buf.zero_()
+    return a
+
+
+def asarray(x, dtype=None):
+    """Host (numpy / scalar / list) -> device array.  Views keep their strides
+    (including negative ones): the memory span they cover is uploaded."""
+    if isinstance(x, ndarray):
+        return x if dtype is None or np.dtype(dtype) == x.dtype else x.astype(dtype)
+    a = np.asarray(x, dtype=dtype)
+    _dtype(a.dtype)
+    isz = a.itemsize
+    if any(s % isz for s in a.strides) or not a.flags.aligned:
+        a = np.ascontiguousarray(a)
+    st = [s // isz for s in a.strides]
+    if a.size == 0:
+        return empty(a.shape, a.dtype)
+    lo, hi = _span(a.shape, st)
+    nbytes = (hi - lo + 1) * isz
+    ptr = a.__array_interface__["data"][0] + lo * isz
+    raw = np.ctypeslib.as_array((ctypes.c_uint8 * nbytes).from_address(ptr)).copy()
+    buf = torch.from_numpy(raw).to(_device())
+    return ndarray(buf, a.dtype, a.shape, st, -lo)
+
+
+def setbufsize(n):
+    """NumPy's iterator buffer size, which sets add.reduce's window (default 8192)."""
+    old = getbufsize()
+    _call(_lib.load().ionp_set_bufsize, int(n))
+    return old
+
+
+def getbufsize():
+    return int(_lib.load().ionp_get_bufsize())
+
+
+def result_type(a, b):
+    out = ctypes.c_int32()
+    _call(_lib.load().ionp_result_type, _CODE[_dtype(a)], _CODE[_dtype(b)], ctypes.byref(out))
+    return np.dtype(_NP_DTYPES[out.value])
+
+
+# ---- operands --------------------------------------------------------------------------
+def _is_weak(x):
+    return isinstance(x, (bool, int, float)) and not isinstance(x, np.generic)
+
+
+def _weak_dtype(x, other):
+    """NEP 50: a Python scalar takes the other operand's dtype within its kind."""
+    od = other.dtype
+    if isinstance(x, bool):
+        return od
+    if isinstance(x, int):
+        if od.kind in "iu":
+            info = np.iinfo(od)
+            if not info.min <= x <= info.max:
+                raise OverflowError(f"Python integer {x} out of bounds for {od}")
+            return od
+        if od.kind == "b":
+            return np.dtype(np.int64)
+        return od
+    # float
+    return od if od.kind == "f" else np.dtype(np.float64)
+
+
+def _operands(a, b):
+    if _is_weak(a) and _is_weak(b):
+        a, b = asarray(a), asarray(b)
+    elif _is_weak(a):
+        b = asarray(b)
+        a = asarray(np.array(a, dtype=_weak_dtype(a, b)))
+    elif _is_weak(b):
+        a = asarray(a)
+        b = asarray(np.array(b, dtype=_weak_dtype(b, a)))
+    else:
+        a, b = asarray(a), asarray(b)
+    return a, b
+
+
+def _finish(out):
+    return out
+
+
+class Ufunc:
+    """Binary ufunc object: __call__ and .reduce (ionp-py/src/lib.rs:8697-10227)."""
+
+    nin = 2
+    nout = 1
+
+    def __init__(self, name, code, reducible):
+        self.__name__ = name
+        self._code = code
+        self._reducible = reducible
+
+    def __repr__(self):
+        return f"<ufunc '{self.__name__}'>"
+
+    def __call__(self, a, b, out=None, dtype=None):
+        """ufunc.__call__ (lib.rs:9204) -> binary_op (ionp-core/src/ufunc.rs:3628)."""
+        a, b = _operands(a, b)
+        if dtype is not None:
+            dt = _dtype(dtype)
+            a, b = a.astype(dt), b.astype(dt)
+        L = _lib.load()
+        da, db = a._desc(), b._desc()
+        if out is None:
+            t = _lib.ArrayDesc()
+            _call(L.ionp_plan_binary, self._code, ctypes.byref(da), ctypes.byref(db), ctypes.byref(t))
+            out = empty_strided(tuple(t.shape[: t.ndim]), tuple(t.strides[: t.ndim]), _NP_DTYPES[t.dtype])
+        do = out._desc()
+        _call(L.ionp_binary, self._code, ctypes.byref(da), ctypes.byref(db), ctypes.byref(do), None,
+              _lib.stream_handle())
+        return out
+
+    def reduce(self, a, axis=0, dtype=None, out=None, keepdims=False):
+        """ufunc.reduce (lib.rs:9958) -> reduce_axis (ionp-core/src/ufunc.rs:7634)."""
+        if not self._reducible:
+            raise NotImplementedError(f"{self.__name__}.reduce is not provided by the MI355X backend")
+        return _reduce(self._code, asarray(a), axis, dtype, keepdims, out)
+
+
+def _axes_arg(axis):
+    if axis is None:
+        return None, -1
+    if isinstance(axis, numbers.Integral):
+        axis = (int(axis),)
+    axis = tuple(int(a) for a in axis)
+    arr = (ctypes.c_int32 * builtins.max(len(axis), 1))(*axis)
+    return arr, len(axis)
+
+
+def _reduce(code, a, axis, dtype, keepdims, out=None):
+    L = _lib.load()
+    axes, naxes = _axes_arg(axis)
+    d = a._desc()
+    t = _lib.ArrayDesc()
+    dt = -1 if dtype is None else _CODE[_dtype(dtype)]
+    _call(L.ionp_plan_reduce, code, ctypes.byref(d), axes, naxes, int(bool(keepdims)), dt, ctypes.byref(t))
+    if out is None:
+        out = empty_strided(tuple(t.shape[: t.ndim]), tuple(t.strides[: t.ndim]), _NP_DTYPES[t.dtype])
+    do = out._desc()
+    _call(L.ionp_reduce, code, ctypes.byref(d), axes, naxes, int(bool(keepdims)), ctypes.byref(do), None,
+          _lib.stream_handle())
+    return out
+
+
+add = Ufunc("add", _lib.ADD, True)
+subtract = Ufunc("subtract", _lib.SUBTRACT, False)
+multiply = Ufunc("multiply", _lib.MULTIPLY, False)
+true_divide = Ufunc("divide", _lib.TRUE_DIVIDE, False)
+divide = true_divide
+maximum = Ufunc("maximum", _lib.MAXIMUM, True)
+minimum = Ufunc("minimum", _lib.MINIMUM, True)
+less = Ufunc("less", _lib.LESS, False)
+less_equal = Ufunc("less_equal", _lib.LESS_EQUAL, False)
+greater = Ufunc("greater", _lib.GREATER, False)
+greater_equal = Ufunc("greater_equal", _lib.GREATER_EQUAL, False)
+equal = Ufunc("equal", _lib.EQUAL, False)
+not_equal = Ufunc("not_equal", _lib.NOT_EQUAL, False)
+
+
+def sum(a, axis=None, dtype=None, keepdims=False):
+    """sum (ionp-py/src/reductions.rs:1032) = add.reduce with axis=None default."""
+    return add.reduce(a, axis=axis, dtype=dtype, keepdims=keepdims)
+
+
+def amax(a, axis=None, keepdims=False):
+    """max / amax (reductions.rs:1114) = maximum.reduce, axis=None default."""
+    return maximum.reduce(a, axis=axis, keepdims=keepdims)
+
+
+def amin(a, axis=None, keepdims=False):
+    return minimum.reduce(a, axis=axis, keepdims=keepdims)
+
+
+max = amax  # noqa: A001  (NumPy-named API)
+min = amin  # noqa: A001
+
+
+def _argext(ismax, a, axis, keepdims):
+    a = asarray(a)
+    L = _lib.load()
+    ax = _lib.AXIS_NONE if axis is None else int(axis)
+    d = a._desc()
+    t = _lib.ArrayDesc()
+    _call(L.ionp_plan_argext, int(ismax), ctypes.byref(d), ax, int(bool(keepdims)), ctypes.byref(t))
+    out = empty_strided(tuple(t.shape[: t.ndim]), tuple(t.strides[: t.ndim]), np.int64)
+    do = out._desc()
+    _call(L.ionp_argext, int(ismax), ctypes.byref(d), ax, int(bool(keepdims)), ctypes.byref(do), None,
+          _lib.stream_handle())
+    return out
+
+
+def argmax(a, axis=None, keepdims=False):
+    """argmax (reductions.rs:1186) -> argext (ionp-core/src/ufunc.rs:8131)."""
+    return _argext(True, a, axis, keepdims)
+
+
+def argmin(a, axis=None, keepdims=False):
+    return _argext(False, a, axis, keepdims)
+
+
+def where(condition, x, y):
+    """where (reductions.rs:3316) -> where_select (ionp-core/src/sort.rs:1869)."""
+    c = asarray(condition)
+    if _is_weak(x) and not _is_weak(y):
+        y = asarray(y)
+        x = asarray(np.array(x, dtype=_weak_dtype(x, y)))
+    elif _is_weak(y) and not _is_weak(x):
+        x = asarray(x)
+        y = asarray(np.array(y, dtype=_weak_dtype(y, x)))
+    else:
+        x, y = asarray(x), asarray(y)
+    L = _lib.load()
+    dc, dx, dy = c._desc(), x._desc(), y._desc()
+    t = _lib.ArrayDesc()
+    _call(L.ionp_plan_where, ctypes.byref(dc), ctypes.byref(dx), ctypes.byref(dy), ctypes.byref(t))
+    out = empty_strided(tuple(t.shape[: t.ndim]), tuple(t.strides[: t.ndim]), _NP_DTYPES[t.dtype])
+    do = out._desc()
+    _call(L.ionp_where, ctypes.byref(dc), ctypes.byref(dx), ctypes.byref(dy), ctypes.byref(do), None,
+          _lib.stream_handle())
+    return out

@@ -1,0 +1,122 @@
+"""ctypes binding of the C ABI in include/ionp_hip.h (libionp_hip.so).
+
+torch is imported first on purpose: it loads the HIP runtime
+(libamdhip64.so.7) that this library then binds to, so both share one
+runtime, one device context and torch's streams.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads the HIP runtime before libionp_hip.so)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libionp_hip.so")
+
+MAX_DIMS = 16
+AXIS_NONE = -1000000
+NFEATURES = 24
+
+BOOL, INT8, UINT8, INT16, UINT16, INT32, UINT32, INT64, UINT64, FLOAT32, FLOAT64 = range(11)
+ADD, SUBTRACT, MULTIPLY, TRUE_DIVIDE, MAXIMUM, MINIMUM, LESS, LESS_EQUAL, GREATER, GREATER_EQUAL, EQUAL, NOT_EQUAL = range(12)
+CALL_BINARY, CALL_WHERE, CALL_CAST, CALL_REDUCE = range(4)
+BATCH_PERSISTENT, BATCH_GRAPH, BATCH_PER_CALL = range(3)
+
+STATUS_NAMES = {0: "OK", 1: "BROADCAST", 2: "RESHAPE", 3: "INDEX", 4: "TYPE", 5: "VALUE",
+                6: "NO_LOOP", 7: "AXIS", 20: "DEVICE", 21: "UNSUPPORTED"}
+
+
+class ArrayDesc(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int32), ("ndim", ctypes.c_int32),
+                ("shape", ctypes.c_int64 * MAX_DIMS), ("strides", ctypes.c_int64 * MAX_DIMS),
+                ("offset", ctypes.c_int64), ("data", ctypes.c_void_p)]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int32), ("vec", ctypes.c_int32),
+                ("blocks_per_cu", ctypes.c_int32), ("split", ctypes.c_int32)]
+
+
+class Call(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("op", ctypes.c_int32), ("naxes", ctypes.c_int32),
+                ("keepdims", ctypes.c_int32), ("axes", ctypes.c_int32 * MAX_DIMS),
+                ("inp", ArrayDesc * 3), ("out", ArrayDesc)]
+
+
+class IonpError(Exception):
+    """Raised for a non-zero ionp_status; `.code` is the status."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+_P = ctypes.POINTER
+_D = _P(ArrayDesc)
+_C = _P(Config)
+_V = ctypes.c_void_p
+_I = ctypes.c_int32
+_L = ctypes.c_int64
+_F = _P(ctypes.c_float)
+
+# name -> argtypes (all return ionp_status unless listed in _RESTYPES)
+SIGNATURES = {
+    "ionp_init": [],
+    "ionp_last_error": [],
+    "ionp_version": [],
+    "ionp_set_bufsize": [_L],
+    "ionp_get_bufsize": [],
+    "ionp_result_type": [_I, _I, _P(_I)],
+    "ionp_binary_types": [_I, _I, _I, _P(_I), _P(_I)],
+    "ionp_plan_binary": [_I, _D, _D, _D],
+    "ionp_plan_where": [_D, _D, _D, _D],
+    "ionp_plan_cast": [_D, _I, _D],
+    "ionp_plan_reduce": [_I, _D, _P(_I), _I, _I, _I, _D],
+    "ionp_plan_argext": [_I, _D, _I, _I, _D],
+    "ionp_binary": [_I, _D, _D, _D, _C, _V],
+    "ionp_where": [_D, _D, _D, _D, _C, _V],
+    "ionp_cast": [_D, _D, _C, _V],
+    "ionp_reduce": [_I, _D, _P(_I), _I, _I, _D, _C, _V],
+    "ionp_argext": [_I, _D, _I, _I, _D, _C, _V],
+    "ionp_batch_create": [_P(Call), _I, _V, _P(_V)],
+    "ionp_batch_run": [_V, _I, _V],
+    "ionp_batch_info": [_V, _P(_L), _P(_L)],
+    "ionp_batch_destroy": [_V],
+    "ionp_call_features": [_P(Call), _F],
+    "ionp_mlp_linear_fwd": [_V, _V, _V, _V, _I, _I, _I, _I, _V],
+    "ionp_mlp_linear_bwd": [_V, _V, _V, _V, _V, _V, _V, _I, _I, _I, _I, _V],
+    "ionp_mlp_softmax_xent": [_V, _V, _V, _I, _I, _I, _V, _V, _V],
+    "ionp_mlp_adam": [_V, _V, _V, _V, _L, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                      ctypes.c_float, _I, _V],
+}
+_RESTYPES = {"ionp_last_error": ctypes.c_char_p, "ionp_version": _I, "ionp_get_bufsize": _L,
+             "ionp_batch_destroy": None}
+
+_lib = None
+
+
+def load():
+    """Load libionp_hip.so; raises ImportError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m anionpy_amd.build` "
+                          "(the HIP backend has no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, ctypes.c_int)
+    _lib = lib
+    return lib
+
+
+def check(code):
+    if code != 0:
+        msg = load().ionp_last_error().decode(errors="replace")
+        raise IonpError(code, msg)
+
+
+def stream_handle():
+    """hipStream_t of torch's current stream on the current device."""
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)

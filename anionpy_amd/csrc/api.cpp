@@ -1,0 +1,347 @@
+// api.cpp -- the extern "C" entry points declared in include/ionp_hip.h.
+//
+// Validation happens on the host before any launch, in the order the
+// reference validates (dtype loop before shape, ufunc.rs:3629-3640), and
+// errors carry NumPy's message text (error.rs:64-90).
+#include <atomic>
+#include <mutex>
+
+#include "elementwise.h"
+#include "planner.h"
+#include "reduce.h"
+
+using namespace ionp;
+
+namespace {
+
+std::atomic<int64_t> g_bufsize{8192};
+
+hipStream_t as_stream(void *s) { return (hipStream_t)s; }
+
+void fill_template(ionp_array_desc *t, int dtype, int ndim, const int64_t *shape, const int64_t *strides) {
+    std::memset(t, 0, sizeof *t);
+    t->dtype = dtype;
+    t->ndim = ndim;
+    for (int i = 0; i < ndim; i++) {
+        t->shape[i] = shape[i];
+        t->strides[i] = strides[i];
+    }
+}
+
+ionp_status same_shape(const ionp_array_desc *out, int ndim, const int64_t *shape, const char *what) {
+    bool ok = out->ndim == ndim;
+    for (int i = 0; ok && i < ndim; i++) ok = out->shape[i] == shape[i];
+    if (!ok)
+        return set_error(IONP_ERR_VALUE, std::string("non-broadcastable output operand with shape ") +
+                                             fmt_shape(out->shape, out->ndim) + " doesn't match the broadcast shape " +
+                                             fmt_shape(shape, ndim) + " (" + what + ")");
+    return IONP_OK;
+}
+
+ionp_status binary_common(int32_t op, const ionp_array_desc *a, const ionp_array_desc *b, int *loop, int *odt,
+                          int *nd, int64_t *shape) {
+    ionp_status st;
+    if ((st = check_desc(a, "x1")) || (st = check_desc(b, "x2"))) return st;
+    if ((st = binary_types(op, a->dtype, b->dtype, loop, odt))) return st;
+    const ionp_array_desc *ops[2] = {a, b};
+    return broadcast(2, ops, nd, shape);
+}
+
+bool is_mixed_su(int op, int a, int b) {
+    return op >= IONP_LESS && ((a == IONP_INT64 && b == IONP_UINT64) || (a == IONP_UINT64 && b == IONP_INT64));
+}
+
+}  // namespace
+
+extern "C" {
+
+ionp_status ionp_init(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_error(IONP_ERR_DEVICE, "no HIP device available");
+    (void)num_cus();
+    return IONP_OK;
+}
+
+const char *ionp_last_error(void) { return last_error(); }
+
+int32_t ionp_version(void) { return (1 << 16) | 0; }
+
+ionp_status ionp_set_bufsize(int64_t elements) {
+    if (elements < 16 || elements > 8192) return set_error(IONP_ERR_VALUE, "bufsize must be in [16, 8192]");
+    g_bufsize = elements;
+    return IONP_OK;
+}
+int64_t ionp_get_bufsize(void) { return g_bufsize; }
+
+ionp_status ionp_result_type(int32_t a, int32_t b, int32_t *out) {
+    if (a < 0 || a >= IONP_NTYPES || b < 0 || b >= IONP_NTYPES) return set_error(IONP_ERR_TYPE, "invalid dtype code");
+    *out = promote(a, b);
+    return IONP_OK;
+}
+
+ionp_status ionp_binary_types(int32_t op, int32_t a, int32_t b, int32_t *loop, int32_t *out) {
+    int l, o;
+    ionp_status st = binary_types(op, a, b, &l, &o);
+    if (st) return st;
+    *loop = l;
+    *out = o;
+    return IONP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plans
+// ---------------------------------------------------------------------------
+ionp_status ionp_plan_binary(int32_t op, const ionp_array_desc *a, const ionp_array_desc *b,
+                             ionp_array_desc *out_template) {
+    int loop, odt, nd;
+    int64_t shape[IONP_MAX_DIMS], st[2][IONP_MAX_DIMS], ost[IONP_MAX_DIMS];
+    ionp_status s = binary_common(op, a, b, &loop, &odt, &nd, shape);
+    if (s) return s;
+    broadcast_strides(a, nd, shape, st[0]);
+    broadcast_strides(b, nd, shape, st[1]);
+    korder_strides(2, st, nd, shape, ost);
+    fill_template(out_template, odt, nd, shape, ost);
+    return IONP_OK;
+}
+
+ionp_status ionp_plan_where(const ionp_array_desc *c, const ionp_array_desc *x, const ionp_array_desc *y,
+                            ionp_array_desc *out_template) {
+    ionp_status s;
+    if ((s = check_desc(c, "condition")) || (s = check_desc(x, "x")) || (s = check_desc(y, "y"))) return s;
+    const ionp_array_desc *ops[3] = {c, x, y};
+    int nd;
+    int64_t shape[IONP_MAX_DIMS], st[3][IONP_MAX_DIMS], ost[IONP_MAX_DIMS];
+    if ((s = broadcast(3, ops, &nd, shape))) return s;
+    for (int k = 0; k < 3; k++) broadcast_strides(ops[k], nd, shape, st[k]);
+    korder_strides(3, st, nd, shape, ost);
+    fill_template(out_template, promote(x->dtype, y->dtype), nd, shape, ost);
+    return IONP_OK;
+}
+
+ionp_status ionp_plan_cast(const ionp_array_desc *in, int32_t dtype, ionp_array_desc *out_template) {
+    ionp_status s;
+    if ((s = check_desc(in, "array"))) return s;
+    if (dtype < 0 || dtype >= IONP_NTYPES) return set_error(IONP_ERR_TYPE, "invalid dtype code");
+    int64_t st[1][IONP_MAX_DIMS], ost[IONP_MAX_DIMS];
+    std::memcpy(st[0], in->strides, sizeof(int64_t) * in->ndim);
+    korder_strides(1, st, in->ndim, in->shape, ost);
+    fill_template(out_template, dtype, in->ndim, in->shape, ost);
+    return IONP_OK;
+}
+
+static ionp_status reduce_dtype(int32_t op, int in_dt, int32_t dtype, int *out_dt) {
+    if (op != IONP_ADD && op != IONP_MAXIMUM && op != IONP_MINIMUM)
+        return set_error(IONP_ERR_VALUE, "reduce is only provided for add, maximum and minimum");
+    int dt = dtype < 0 ? reduce_default_dtype(op, in_dt) : dtype;
+    if (dt < 0 || dt >= IONP_NTYPES) return set_error(IONP_ERR_TYPE, "invalid dtype code");
+    if (op != IONP_ADD && dt != in_dt)
+        return set_error(IONP_ERR_UNSUPPORTED, "maximum/minimum.reduce with a dtype other than the input's");
+    // same_kind casting of the input to the loop dtype (a float can not feed an integer loop)
+    if (is_float_dt(in_dt) && !is_float_dt(dt))
+        return set_error(IONP_ERR_TYPE, std::string("Cannot cast ufunc '") + op_name(op) + "' input from dtype('" +
+                                            dtype_name(in_dt) + "') to dtype('" + dtype_name(dt) +
+                                            "') with casting rule 'same_kind'");
+    if (dt == IONP_BOOL && op == IONP_ADD) dt = IONP_BOOL;
+    *out_dt = dt;
+    return IONP_OK;
+}
+
+ionp_status ionp_plan_reduce(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes,
+                             int32_t keepdims, int32_t dtype, ionp_array_desc *out_template) {
+    ionp_status s;
+    if ((s = check_desc(in, "array"))) return s;
+    int odt;
+    if ((s = reduce_dtype(op, in->dtype, dtype, &odt))) return s;
+    bool mask[IONP_MAX_DIMS];
+    if ((s = normalize_axes(axes, naxes, in->ndim, mask))) return s;
+    // output layout: kept axes keep the input's iterator order (array.rs:910)
+    int64_t st[1][IONP_MAX_DIMS];
+    std::memcpy(st[0], in->strides, sizeof(int64_t) * in->ndim);
+    int perm[IONP_MAX_DIMS];
+    iter_axis_order(in->ndim, in->shape, 1, st, perm);
+    int64_t kst[IONP_MAX_DIMS] = {0};
+    int64_t acc = 1;
+    for (int k = 0; k < in->ndim; k++) {
+        const int ax = perm[k];
+        if (mask[ax]) continue;
+        kst[ax] = acc;
+        acc *= in->shape[ax] > 0 ? in->shape[ax] : 1;
+    }
+    int64_t shape[IONP_MAX_DIMS], ost[IONP_MAX_DIMS];
+    int nd = 0;
+    for (int k = 0; k < in->ndim; k++) {
+        if (mask[k]) {
+            if (keepdims) {
+                shape[nd] = 1;
+                ost[nd++] = 0;
+            }
+        } else {
+            shape[nd] = in->shape[k];
+            ost[nd++] = kst[k];
+        }
+    }
+    fill_template(out_template, odt, nd, shape, ost);
+    return IONP_OK;
+}
+
+static ionp_status arg_normalize(const ionp_array_desc *in, int32_t axis, int *ax) {
+    if (axis == IONP_AXIS_NONE) {
+        *ax = -1;
+        return IONP_OK;
+    }
+    const int eff = in->ndim > 1 ? in->ndim : 1;
+    const int norm = axis < 0 ? axis + eff : axis;
+    if (norm < 0 || norm >= eff)
+        return set_error(IONP_ERR_AXIS, "axis " + std::to_string(axis) + " is out of bounds for array of dimension " +
+                                            std::to_string(in->ndim));
+    *ax = in->ndim == 0 ? -1 : norm;
+    return IONP_OK;
+}
+
+ionp_status ionp_plan_argext(int32_t ismax, const ionp_array_desc *in, int32_t axis, int32_t keepdims,
+                             ionp_array_desc *out_template) {
+    ionp_status s;
+    if ((s = check_desc(in, "array"))) return s;
+    int ax;
+    if ((s = arg_normalize(in, axis, &ax))) return s;
+    int64_t shape[IONP_MAX_DIMS], ost[IONP_MAX_DIMS];
+    int nd = 0;
+    for (int k = 0; k < in->ndim; k++) {
+        if (ax < 0 || k == ax) {
+            if (keepdims) shape[nd++] = 1;
+        } else {
+            shape[nd++] = in->shape[k];
+        }
+    }
+    int64_t acc = 1;
+    for (int k = nd - 1; k >= 0; k--) {
+        ost[k] = shape[k] == 1 ? 0 : acc;
+        acc *= shape[k];
+    }
+    (void)ismax;
+    fill_template(out_template, IONP_INT64, nd, shape, ost);
+    return IONP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// execution
+// ---------------------------------------------------------------------------
+ionp_status ionp_binary(int32_t op, const ionp_array_desc *a, const ionp_array_desc *b, const ionp_array_desc *out,
+                        const ionp_config *cfg, void *stream) {
+    int loop, odt, nd;
+    int64_t shape[IONP_MAX_DIMS];
+    ionp_status s = binary_common(op, a, b, &loop, &odt, &nd, shape);
+    if (s) return s;
+    if ((s = check_desc(out, "out")) || (s = same_shape(out, nd, shape, op_name(op)))) return s;
+    if (op >= IONP_LESS && out->dtype != IONP_BOOL)
+        return set_error(IONP_ERR_TYPE, "comparison output must be bool");
+    const ionp_array_desc *ins[2] = {a, b};
+    EwArgs A;
+    if ((s = plan_elementwise(op, 2, ins, out, loop, &A))) return s;
+    A.mixed_su = is_mixed_su(op, a->dtype, b->dtype);
+    return run_elementwise(A, cfg, as_stream(stream));
+}
+
+ionp_status ionp_where(const ionp_array_desc *c, const ionp_array_desc *x, const ionp_array_desc *y,
+                       const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
+    ionp_array_desc t;
+    ionp_status s = ionp_plan_where(c, x, y, &t);
+    if (s) return s;
+    if ((s = check_desc(out, "out")) || (s = same_shape(out, t.ndim, t.shape, "where"))) return s;
+    const ionp_array_desc *ins[3] = {c, x, y};
+    EwArgs A;
+    if ((s = plan_elementwise(OP_WHERE, 3, ins, out, out->dtype, &A))) return s;
+    return run_elementwise(A, cfg, as_stream(stream));
+}
+
+ionp_status ionp_cast(const ionp_array_desc *in, const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
+    ionp_status s;
+    if ((s = check_desc(in, "array")) || (s = check_desc(out, "out"))) return s;
+    if ((s = same_shape(out, in->ndim, in->shape, "astype"))) return s;
+    const ionp_array_desc *ins[1] = {in};
+    EwArgs A;
+    if ((s = plan_elementwise(OP_CAST, 1, ins, out, out->dtype, &A))) return s;
+    return run_elementwise(A, cfg, as_stream(stream));
+}
+
+ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes, int32_t keepdims,
+                        const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
+    ionp_array_desc t;
+    ionp_status s;
+    if ((s = check_desc(out, "out"))) return s;
+    if ((s = ionp_plan_reduce(op, in, axes, naxes, keepdims, out->dtype, &t))) return s;
+    if ((s = same_shape(out, t.ndim, t.shape, "reduce"))) return s;
+    bool mask[IONP_MAX_DIMS];
+    normalize_axes(axes, naxes, in->ndim, mask);
+    ReducePlan p;
+    if ((s = plan_reduce(op, in, mask, out, &p))) return s;
+    return run_reduce(p, g_bufsize, cfg, as_stream(stream));
+}
+
+ionp_status ionp_argext(int32_t ismax, const ionp_array_desc *in, int32_t axis, int32_t keepdims,
+                        const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
+    ionp_array_desc t;
+    ionp_status s;
+    if ((s = check_desc(out, "out"))) return s;
+    if ((s = ionp_plan_argext(ismax, in, axis, keepdims, &t))) return s;
+    if ((s = same_shape(out, t.ndim, t.shape, "argext"))) return s;
+    if (out->dtype != IONP_INT64) return set_error(IONP_ERR_TYPE, "argmax/argmin output must be int64");
+    int ax;
+    arg_normalize(in, axis, &ax);
+    ArgArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.in_dt = in->dtype;
+    A.ismax = ismax ? 1 : 0;
+    A.in = elem0(in);
+    A.out = (char *)elem0(out);
+    const int64_t total = desc_size(in);
+    if (ax < 0) {
+        A.len = total;
+        A.nout = 1;
+        if (total == 0)
+            return set_error(IONP_ERR_VALUE, std::string("attempt to get ") + (ismax ? "argmax" : "argmin") +
+                                                 " of an empty sequence");
+        // C-order ravel, innermost first, coalesced where the strides allow
+        for (int k = in->ndim - 1; k >= 0; k--) {
+            if (in->shape[k] == 1) continue;
+            if (A.nl > 0 && in->strides[k] == A.l_st[A.nl - 1] * A.l_shape[A.nl - 1]) {
+                A.l_shape[A.nl - 1] *= in->shape[k];
+            } else {
+                A.l_shape[A.nl] = in->shape[k];
+                A.l_st[A.nl++] = in->strides[k];
+            }
+        }
+        if (A.nl == 0) {
+            A.nl = 1;
+            A.l_shape[0] = 1;
+            A.l_st[0] = 0;
+        }
+    } else {
+        A.len = in->shape[ax];
+        A.nl = 1;
+        A.l_shape[0] = A.len;
+        A.l_st[0] = in->strides[ax];
+        A.nout = 1;
+        // kept axes innermost first (reverse C order) with output strides
+        int j = out->ndim - 1;
+        for (int k = in->ndim - 1; k >= 0; k--) {
+            if (k == ax) {
+                if (keepdims) j--;
+                continue;
+            }
+            const int64_t n = in->shape[k];
+            const int64_t ost = out->strides[j--];
+            A.nout *= n;
+            if (n == 1) continue;
+            A.k_shape[A.nk] = n;
+            A.k_in_st[A.nk] = in->strides[k];
+            A.k_out_st[A.nk++] = ost;
+        }
+        if (A.len == 0 && A.nout > 0)
+            return set_error(IONP_ERR_VALUE, std::string("attempt to get ") + (ismax ? "argmax" : "argmin") +
+                                                 " of an empty sequence");
+    }
+    return run_argext(A, cfg, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,25 @@
+// elementwise.h -- launch interface of the elementwise kernels.
+#pragma once
+
+#include "planner.h"
+
+namespace ionp {
+
+// rows x inner fast-path arguments (dim 0 of the coalesced space is `inner`)
+struct RowArgs {
+    int64_t inner, rows, tiles_per_row;
+    FastDiv tpr;
+    int32_t nouter;
+    int64_t oshape[kMaxKDims];
+    FastDiv odiv[kMaxKDims];
+    int64_t ost[2][kMaxKDims], ost_out[kMaxKDims];
+    int64_t inner_st[2];
+    const char *in[2];
+    char *out;
+};
+
+int num_cus();
+ionp_status run_elementwise(EwArgs &A, const ionp_config *cfg, hipStream_t s);
+ionp_status launch_generic(EwArgs A, const ionp_config *cfg, hipStream_t s);
+
+}  // namespace ionp

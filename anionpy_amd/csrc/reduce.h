@@ -1,0 +1,51 @@
+// reduce.h -- launch interface of the reduction kernels.
+#pragma once
+
+#include "planner.h"
+
+namespace ionp {
+
+// Device-side view of a ReducePlan: coalesced kept / wide / outer-reduced
+// axes with fast dividers, buffer-window geometry and scratch pointers.
+struct RedArgs {
+    int32_t in_dt, out_dt, op;
+    int32_t nk, nw, nr;
+    int64_t nout, wsize, rsize, group;
+    int64_t bufsize, nwin;  // add/max windows: nwin = ceil(wsize / bufsize)
+    int64_t k_shape[kMaxKDims], k_in_st[kMaxKDims], k_out_st[kMaxKDims];
+    FastDiv k_div[kMaxKDims];
+    int64_t w_shape[kMaxKDims], w_st[kMaxKDims];
+    FastDiv w_div[kMaxKDims];
+    int64_t r_shape[kMaxKDims], r_st[kMaxKDims];
+    FastDiv r_div[kMaxKDims];
+    const char *in;
+    char *out;
+    void *scratch;  // partial results
+};
+
+ionp_status run_reduce(const ReducePlan &p, int64_t bufsize, const ionp_config *cfg, hipStream_t s);
+
+// argmax / argmin along one axis (C index order) or over the C-order ravel.
+struct ArgArgs {
+    int32_t in_dt, ismax;
+    int64_t nout, len;
+    // kept axes (C order of the input without `axis`), with input and output strides
+    int32_t nk;
+    int64_t k_shape[IONP_MAX_DIMS], k_in_st[IONP_MAX_DIMS], k_out_st[IONP_MAX_DIMS];
+    FastDiv k_div[IONP_MAX_DIMS];
+    // the scanned index space: one axis (nl == 1) or the C-order ravel
+    int32_t nl;
+    int64_t l_shape[IONP_MAX_DIMS], l_st[IONP_MAX_DIMS];
+    FastDiv l_div[IONP_MAX_DIMS];
+    int64_t seg, nseg;
+    const char *in;
+    char *out;
+    void *scratch;
+};
+
+ionp_status run_argext(ArgArgs &a, const ionp_config *cfg, hipStream_t s);
+
+// device scratch owned by the library (grown on demand, stream-ordered use)
+void *scratch_get(size_t bytes, hipStream_t s);
+
+}  // namespace ionp

@@ -1,0 +1,207 @@
+"""Benchmark of the MI355X ufunc / reduction backend.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
+
+A step is one pass of the large broadcast elementwise workload:
+    float64 X(8192,8192) * c(8192,1), float32 X * c,
+    float32 X.T[::2] * c[::2]  (non-contiguous view, (4096,8192)),
+    int64 I(8192,8192) + j(8192,1) (wrapping add),
+X ~ normal(0,1), c ~ uniform[0.5,2), I, j ~ uniform[-2^62, 2^62), all bit-exact
+with NumPy.  `value` is the step's algorithmic HBM bytes (every operand read
+once, the output written once) divided by the step time, inputs resident in
+HBM.  One JSON line is printed; `roofline` covers the dominant kernel (the
+float64 multiply), timed with HIP events on the stream it runs on, and
+`cpu_baseline` times the scalar C oracle (single thread) on a row sample of
+the same four operations.  `extra` carries the reduction workloads (float64
+add.reduce of 2^28 elements, float32 (4096,65536) sums, maximum/argmax).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PEAK_HBM = 8.0e12
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-extra", action="store_true")
+    return p.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this backend runs on one GPU; multi-GPU sharding is described "
+                                   "in DESIGN.md and not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    import anionpy_amd as ap
+
+    torch.cuda.set_device(0)
+    log("generating inputs")
+    X = np.random.default_rng(1).standard_normal((8192, 8192))
+    c = np.random.default_rng(2).uniform(0.5, 2, (8192, 1))
+    X64, c64 = ap.asarray(X), ap.asarray(c)
+    X32, c32 = ap.asarray(X.astype(np.float32)), ap.asarray(c.astype(np.float32))
+    del X
+    I = ap.asarray(np.random.default_rng(3).integers(-2 ** 62, 2 ** 62, (8192, 8192)))
+    j = ap.asarray(np.random.default_rng(8).integers(-2 ** 62, 2 ** 62, (8192, 1)))
+    V32, v32 = X32.T[::2], c32[::2]
+
+    ops = [("mul_f64", ap.multiply, X64, c64), ("mul_f32", ap.multiply, X32, c32),
+           ("view_f32", ap.multiply, V32, v32), ("add_i64", ap.add, I, j)]
+    outs = [f(a, b) for _, f, a, b in ops]  # preallocated outputs (K-order layouts)
+
+    def alg_bytes(a, b, o):
+        return a.nbytes + b.nbytes + o.nbytes
+
+    step_bytes = sum(alg_bytes(a, b, o) for (_, _, a, b), o in zip(ops, outs))
+
+    def step():
+        for (_, f, a, b), o in zip(ops, outs):
+            f(a, b, out=o)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt / args.steps * 1e3
+    value = step_bytes * args.steps / dt / 1e9
+
+    # per-op kernel time with HIP events on the launch stream
+    per_op = {}
+    for (name, f, a, b), o in zip(ops, outs):
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 10
+        e0.record(s)
+        for _ in range(reps):
+            f(a, b, out=o)
+        e1.record(s)
+        e1.synchronize()
+        t = e0.elapsed_time(e1) / reps * 1e-3
+        per_op[name] = {"us": round(t * 1e6, 1), "GB/s": round(alg_bytes(a, b, o) / t / 1e9, 1),
+                        "frac": round(alg_bytes(a, b, o) / t / PEAK_HBM, 4)}
+    dom = per_op["mul_f64"]
+    roofline = {"bound": "hbm", "achieved": dom["GB/s"], "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": dom["frac"], "traffic": None, "kernel": "ew_rows<double,multiply>",
+                "algorithmic_bytes_per_launch": alg_bytes(X64, c64, outs[0])}
+    traffic_file = os.path.join(HERE, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            roofline["traffic"] = json.load(f).get("mul_f64")
+
+    extra = {"per_op": per_op}
+    del X64, c64, X32, c32, I, j, V32, v32, outs
+    torch.cuda.empty_cache()
+    if not args.no_extra:
+        extra.update(reduction_extras(ap, torch))
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    line = {"metric": "HBM GB/s (% of peak) ufunc+reduce, bit-exact; batched vs per-call calls/s",
+            "value": round(value, 1), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64,f32,i64", "data": "synthetic (seeded NumPy generators)",
+            "config": {"workload": "broadcast elementwise: f64/f32 X(8192,8192)*c(8192,1), "
+                                   "f32 X.T[::2]*c[::2], i64 I(8192,8192)+j(8192,1)",
+                       "bytes_per_step": step_bytes, "frac_of_peak": round(value * 1e9 / PEAK_HBM, 4)},
+            "roofline": roofline, "cpu_baseline": cpu, "extra": extra}
+    print(json.dumps(line), flush=True)
+
+
+def timed(torch, fn, reps):
+    s = torch.cuda.current_stream()
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        fn()
+    e1.record(s)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps * 1e-3
+
+
+def reduction_extras(ap, torch):
+    log("reduction workloads")
+    out = {}
+    x = ap.asarray(np.random.default_rng(4).uniform(-1, 1, 2 ** 28))
+    nb = x.nbytes + 8
+    t = timed(torch, lambda: ap.add.reduce(x), 5)
+    out["sum_f64_2^28"] = {"us": round(t * 1e6, 1), "GB/s": round(nb / t / 1e9, 1), "frac": round(nb / t / PEAK_HBM, 4)}
+    t = timed(torch, lambda: ap.maximum.reduce(x), 5)
+    out["max_f64_2^28"] = {"us": round(t * 1e6, 1), "GB/s": round(nb / t / 1e9, 1), "frac": round(nb / t / PEAK_HBM, 4)}
+    t = timed(torch, lambda: ap.argmax(x), 5)
+    out["argmax_f64_2^28"] = {"us": round(t * 1e6, 1), "GB/s": round(nb / t / 1e9, 1),
+                              "frac": round(nb / t / PEAK_HBM, 4)}
+    del x
+    F = ap.asarray(np.random.default_rng(6).uniform(-1, 1, (4096, 65536)).astype(np.float32))
+    for ax, nbo in ((0, 65536 * 4), (1, 4096 * 4)):
+        nb = F.nbytes + nbo
+        t = timed(torch, lambda: ap.add.reduce(F, axis=ax), 5)
+        out[f"sum_f32_(4096,65536)_axis{ax}"] = {"us": round(t * 1e6, 1), "GB/s": round(nb / t / 1e9, 1),
+                                                 "frac": round(nb / t / PEAK_HBM, 4)}
+    del F
+    I = ap.asarray(np.random.default_rng(7).integers(-1000, 1000, 2 ** 28, dtype=np.int32))
+    nb = I.nbytes + 8
+    t = timed(torch, lambda: ap.sum(I), 5)
+    out["sum_i32_2^28"] = {"us": round(t * 1e6, 1), "GB/s": round(nb / t / 1e9, 1), "frac": round(nb / t / PEAK_HBM, 4)}
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline():
+    """The scalar C oracle (single thread) on a 512-row sample of each of the
+    four step operations."""
+    sys.path.insert(0, HERE)
+    import oracle as O
+    rows = 512
+    X = np.random.default_rng(1).standard_normal((rows, 8192))
+    c = np.random.default_rng(2).uniform(0.5, 2, (rows, 1))
+    X32, c32 = X.astype(np.float32), c.astype(np.float32)
+    I = np.random.default_rng(3).integers(-2 ** 62, 2 ** 62, (rows, 8192))
+    j = np.random.default_rng(8).integers(-2 ** 62, 2 ** 62, (rows, 1))
+    XT = np.random.default_rng(1).standard_normal((8192, 2 * rows)).astype(np.float32)
+    work = [("multiply", X, c), ("multiply", X32, c32), ("multiply", XT.T[::2], XT.T[::2][:, :1]), ("add", I, j)]
+    nbytes = 0
+    t0 = time.perf_counter()
+    for op, a, b in work:
+        o = O.binary(op, a, b)
+        nbytes += a.nbytes + b.nbytes + o.nbytes
+    dt = time.perf_counter() - t0
+    return {"value": round(nbytes / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"C oracle (scalar restatement) on {rows} rows of each step op "
+                      f"({nbytes / 1e6:.0f} MB algorithmic, {dt:.1f} s), host CPU: {cpu_name()}"}
+
+
+def cpu_name():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip() + f" x{os.cpu_count()}"
+    except OSError:
+        pass
+    return "unknown"
+
+
+if __name__ == "__main__":
+    main()

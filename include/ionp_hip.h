@@ -1,0 +1,250 @@
+/*
+ * ionp_hip.h -- C ABI of the MI355X (gfx950) ufunc / reduction backend.
+ *
+ * Plain C: fixed-size descriptors, device pointers, status codes.  No HIP,
+ * torch or C++ types cross this boundary; `stream` is a hipStream_t passed
+ * as void* (NULL = the default stream).  A Rust `ionp-hip` crate binds it
+ * with `extern "C"` verbatim; the Python package anionpy_amd binds it with
+ * ctypes.
+ *
+ * Each entry point names the reference function it replaces.  Paths are in
+ * the anionpy tree (Son0fSun/anionpy).
+ *
+ * Conventions
+ *   - Strides and offsets are in ELEMENTS (ionp-core/src/shape.rs:1-5,
+ *     ionp-core/src/array.rs:24-29); negative and zero strides are allowed.
+ *   - Callers allocate every output.  The plan_* calls fill an output
+ *     template (dtype, shape, NumPy K-order strides) without touching the
+ *     device, so the caller can allocate it.
+ *   - Results are bit-identical to NumPy 2.2.6 on x86-64 (the reference's
+ *     own oracle), including add.reduce's pairwise tree per 8192-element
+ *     iterator buffer window.
+ *   - Calls are stream-ordered and thread-safe; the last error message is
+ *     thread-local.
+ */
+#ifndef IONP_HIP_H
+#define IONP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IONP_MAX_DIMS 16
+#define IONP_AXIS_NONE (-1000000)
+
+/* dtype codes (ionp-core/src/dtype.rs DType, numeric subset) */
+typedef enum {
+    IONP_BOOL = 0,
+    IONP_INT8 = 1,
+    IONP_UINT8 = 2,
+    IONP_INT16 = 3,
+    IONP_UINT16 = 4,
+    IONP_INT32 = 5,
+    IONP_UINT32 = 6,
+    IONP_INT64 = 7,
+    IONP_UINT64 = 8,
+    IONP_FLOAT32 = 9,
+    IONP_FLOAT64 = 10,
+    IONP_NTYPES = 11
+} ionp_dtype;
+
+/* binary ufuncs (ionp-core/src/ufunc.rs:43-65 BinaryOp, subset) */
+typedef enum {
+    IONP_ADD = 0,
+    IONP_SUBTRACT = 1,
+    IONP_MULTIPLY = 2,
+    IONP_TRUE_DIVIDE = 3,
+    IONP_MAXIMUM = 4,
+    IONP_MINIMUM = 5,
+    IONP_LESS = 6,
+    IONP_LESS_EQUAL = 7,
+    IONP_GREATER = 8,
+    IONP_GREATER_EQUAL = 9,
+    IONP_EQUAL = 10,
+    IONP_NOT_EQUAL = 11,
+    IONP_NOPS = 12
+} ionp_op;
+
+/* status codes; 1..7 map 1:1 onto IonpError (ionp-core/src/error.rs:8-42) */
+typedef enum {
+    IONP_OK = 0,
+    IONP_ERR_BROADCAST = 1, /* ValueError: operands could not be broadcast ... */
+    IONP_ERR_RESHAPE = 2,   /* ValueError */
+    IONP_ERR_INDEX = 3,     /* IndexError */
+    IONP_ERR_TYPE = 4,      /* TypeError */
+    IONP_ERR_VALUE = 5,     /* ValueError */
+    IONP_ERR_NO_LOOP = 6,   /* UFuncTypeError (TypeError subclass) */
+    IONP_ERR_AXIS = 7,      /* AxisError (ValueError and IndexError) */
+    IONP_ERR_DEVICE = 20,   /* HIP runtime failure */
+    IONP_ERR_UNSUPPORTED = 21
+} ionp_status;
+
+/* Strided array view in device memory (NdArray, ionp-core/src/array.rs:24-29). */
+typedef struct {
+    int32_t dtype; /* ionp_dtype */
+    int32_t ndim;
+    int64_t shape[IONP_MAX_DIMS];
+    int64_t strides[IONP_MAX_DIMS]; /* elements */
+    int64_t offset;                 /* elements from `data` to element (0,...,0) */
+    void *data;                     /* device pointer */
+} ionp_array_desc;
+
+/* Launch configuration.  Zero fields mean "use the built-in heuristic".
+ * Every configuration computes bit-identical results: it changes only which
+ * thread handles which element, never the arithmetic order. */
+typedef struct {
+    int32_t block;         /* threads per workgroup: 64..1024 */
+    int32_t vec;           /* elements per vector access: 1, 2, 4, 8 */
+    int32_t blocks_per_cu; /* grid size = blocks_per_cu * CUs (grid-stride) */
+    int32_t split;         /* reduce: outputs per thread (narrow) / unroll */
+} ionp_config;
+
+/* ---- library state --------------------------------------------------- */
+
+/* Initialise on the current device (idempotent).  Optional: every entry
+ * point initialises lazily. */
+ionp_status ionp_init(void);
+
+/* Text of the last error on this thread, NumPy-verbatim where NumPy has a
+ * message (IonpError's Display, ionp-core/src/error.rs:64-90). */
+const char *ionp_last_error(void);
+
+/* ABI version (major << 16 | minor). */
+int32_t ionp_version(void);
+
+/* NumPy's iterator buffer size used to window add.reduce (np.getbufsize(),
+ * default 8192).  Values 16..8192 accepted. */
+ionp_status ionp_set_bufsize(int64_t elements);
+int64_t ionp_get_bufsize(void);
+
+/* ---- type resolution (host only) ------------------------------------- */
+
+/* promote_dtype (ionp-core/src/dtype.rs:297). */
+ionp_status ionp_result_type(int32_t a, int32_t b, int32_t *out);
+
+/* binary_out_dtype (ionp-core/src/ufunc.rs:1101): loop and output dtype. */
+ionp_status ionp_binary_types(int32_t op, int32_t a, int32_t b, int32_t *loop, int32_t *out);
+
+/* ---- planning (host only: fills an output template) ------------------ */
+
+/* binary_op's shape/dtype/layout half (ufunc.rs:3628-3695, shape.rs:54,
+ * lib.rs:1290 apply_ufunc_order 'K'). */
+ionp_status ionp_plan_binary(int32_t op, const ionp_array_desc *a, const ionp_array_desc *b,
+                             ionp_array_desc *out_template);
+
+/* where_select's shape/dtype/layout half (sort.rs:1869-1914). */
+ionp_status ionp_plan_where(const ionp_array_desc *cond, const ionp_array_desc *x,
+                            const ionp_array_desc *y, ionp_array_desc *out_template);
+
+/* NdArray::cast_to's layout half (array.rs:393-429), order 'K'. */
+ionp_status ionp_plan_cast(const ionp_array_desc *in, int32_t dtype, ionp_array_desc *out_template);
+
+/* reduce_axis's shape/dtype half (ufunc.rs:7551-7617 normalize_reduce_axes,
+ * 7634-7825, array.rs:910 reduction output layout).  naxes < 0 means
+ * axis=None; dtype < 0 means the default reduce dtype. */
+ionp_status ionp_plan_reduce(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes,
+                             int32_t keepdims, int32_t dtype, ionp_array_desc *out_template);
+
+/* argext's shape half (ufunc.rs:8131-8206); axis = IONP_AXIS_NONE for the
+ * flattened form. */
+ionp_status ionp_plan_argext(int32_t ismax, const ionp_array_desc *in, int32_t axis, int32_t keepdims,
+                             ionp_array_desc *out_template);
+
+/* ---- execution (device) ---------------------------------------------- */
+
+/* Replaces ionp_core::ufunc::binary_op (ufunc.rs:3628).  `out` must have
+ * the broadcast shape and the binary_out_dtype; any strides. */
+ionp_status ionp_binary(int32_t op, const ionp_array_desc *a, const ionp_array_desc *b,
+                        const ionp_array_desc *out, const ionp_config *cfg, void *stream);
+
+/* Replaces ionp_core::sort::where_select (sort.rs:1869). */
+ionp_status ionp_where(const ionp_array_desc *cond, const ionp_array_desc *x, const ionp_array_desc *y,
+                       const ionp_array_desc *out, const ionp_config *cfg, void *stream);
+
+/* Replaces NdArray::cast_to (array.rs:393) / Buffer::cast_to (buffer.rs:210). */
+ionp_status ionp_cast(const ionp_array_desc *in, const ionp_array_desc *out, const ionp_config *cfg,
+                      void *stream);
+
+/* Replaces ionp_core::ufunc::reduce_axis (ufunc.rs:7634) for op in
+ * {ADD, MAXIMUM, MINIMUM}: `out` as given by ionp_plan_reduce (its dtype
+ * selects the accumulation dtype). */
+ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes,
+                        int32_t keepdims, const ionp_array_desc *out, const ionp_config *cfg, void *stream);
+
+/* Replaces ionp_core::ufunc::argext (ufunc.rs:8131): int64 indices. */
+ionp_status ionp_argext(int32_t ismax, const ionp_array_desc *in, int32_t axis, int32_t keepdims,
+                        const ionp_array_desc *out, const ionp_config *cfg, void *stream);
+
+/* ---- batching: many small calls, one launch --------------------------- */
+/* No counterpart in the reference (it runs one call at a time). */
+
+typedef enum {
+    IONP_CALL_BINARY = 0,
+    IONP_CALL_WHERE = 1,
+    IONP_CALL_CAST = 2,
+    IONP_CALL_REDUCE = 3
+} ionp_call_kind;
+
+typedef struct {
+    int32_t kind;     /* ionp_call_kind */
+    int32_t op;       /* ionp_op for BINARY / REDUCE */
+    int32_t naxes;    /* REDUCE: < 0 means axis=None */
+    int32_t keepdims; /* REDUCE */
+    int32_t axes[IONP_MAX_DIMS];
+    ionp_array_desc in[3]; /* BINARY: a,b; WHERE: cond,x,y; CAST/REDUCE: in */
+    ionp_array_desc out;
+} ionp_call;
+
+typedef struct ionp_batch ionp_batch;
+
+typedef enum {
+    IONP_BATCH_PERSISTENT = 0, /* one persistent-kernel launch over a tile queue */
+    IONP_BATCH_GRAPH = 1,      /* per-call launches captured once into a hipGraph */
+    IONP_BATCH_PER_CALL = 2    /* one stream launch sequence per call */
+} ionp_batch_mode;
+
+/* Validate and plan `n` calls, pack their descriptors and tile table into
+ * device memory.  The calls' device buffers must outlive the batch. */
+ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, ionp_batch **out);
+ionp_status ionp_batch_run(ionp_batch *batch, int32_t mode, void *stream);
+/* number of tiles / bytes moved (algorithmic) by one run */
+ionp_status ionp_batch_info(const ionp_batch *batch, int64_t *ntiles, int64_t *bytes);
+void ionp_batch_destroy(ionp_batch *batch);
+
+/* ---- layout features for the cost model (host only) ------------------ */
+/* No counterpart in the reference. */
+
+#define IONP_NFEATURES 24
+
+/* Feature vector of a call's coalesced layout (rank, log2 dims,
+ * stride/contiguity/broadcast bits, dtype and op class one-hots). */
+ionp_status ionp_call_features(const ionp_call *call, float *features);
+
+/* ---- MLP cost model kernels (device) ---------------------------------- */
+/* No counterpart in the reference.  fp32, row-major, hidden width 64. */
+
+/* y[n, out] = relu?(x[n, in] @ w[in, out] + b[out]) */
+ionp_status ionp_mlp_linear_fwd(const float *x, const float *w, const float *b, float *y, int32_t n,
+                                int32_t in, int32_t out, int32_t relu, void *stream);
+/* Given dy (already masked by the ReLU derivative if relu was used), y
+ * (forward output, for the ReLU mask) -> dx, dw, db. dx may be NULL. */
+ionp_status ionp_mlp_linear_bwd(const float *x, const float *w, const float *y, const float *dy,
+                                float *dx, float *dw, float *db, int32_t n, int32_t in, int32_t out,
+                                int32_t relu, void *stream);
+/* Softmax cross-entropy over `heads` groups of logits: logits[n, total],
+ * head h occupies [off[h], off[h+1]); labels[n, heads]; writes dlogits and
+ * the summed loss (one float on device). */
+ionp_status ionp_mlp_softmax_xent(const float *logits, const int32_t *labels, const int32_t *head_off,
+                                  int32_t heads, int32_t n, int32_t total, float *dlogits, float *loss,
+                                  void *stream);
+/* Adam step over `count` parameters. */
+ionp_status ionp_mlp_adam(float *param, const float *grad, float *m, float *v, int64_t count, float lr,
+                          float beta1, float beta2, float eps, int32_t step, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IONP_HIP_H */

@@ -523,11 +523,11 @@ int ora_binary(int op, const ora_desc *a, const ora_desc *b, const ora_desc *out
             const char *pa = elem_ptr(a, idx, sa, n), *pb = elem_ptr(b, idx, sb, n);
             char *po = (char *)elem_ptr(out, idx, out->strides, n);
             if (mixed_su) {
+                /* uint64 op int64 == int64 (swapped op) uint64 */
+                int sw = op == OP_LESS ? OP_GREATER : op == OP_LESS_EQUAL ? OP_GREATER_EQUAL
+                       : op == OP_GREATER ? OP_LESS : op == OP_GREATER_EQUAL ? OP_LESS_EQUAL : op;
                 int r = a->dtype == DT_I64 ? cmp_su(op, *(const int64_t *)pa, *(const uint64_t *)pb)
-                                           : !cmp_su(op == OP_LESS ? OP_GREATER_EQUAL : op == OP_LESS_EQUAL ? OP_GREATER
-                                                     : op == OP_GREATER ? OP_LESS_EQUAL : op == OP_GREATER_EQUAL ? OP_LESS
-                                                     : op == OP_EQUAL ? OP_NOT_EQUAL : OP_EQUAL,
-                                                     *(const int64_t *)pb, *(const uint64_t *)pa);
+                                           : cmp_su(sw, *(const int64_t *)pb, *(const uint64_t *)pa);
                 *(uint8_t *)po = (uint8_t)r;
             } else {
                 cast_elem(a->dtype, pa, ldt, ca);

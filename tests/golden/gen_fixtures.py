@@ -1,0 +1,422 @@
+"""Write the golden vectors for the ufunc / reduction hot path.
+
+Run in a container with NumPy 2.2.6 on x86-64 -- the oracle the reference's
+differential suite compares against byte-for-byte
+(tests/differential/harness.py:571-630 `_bit_exact_equal`):
+
+    python tests/golden/gen_fixtures.py
+
+Outputs (committed):
+  tests/golden/fixtures.npz   input buffers and expected outputs
+  tests/golden/cases.json     case list: op, arguments, operand views
+                              (offset/shape/strides in elements into a
+                              stored buffer), expected dtype/shape/strides
+                              or expected exception
+  tests/golden/big.json       SHA-256 digests / scalars of the full-size
+                              workloads (inputs regenerated from their seeds)
+
+Cases mirror the reference's own tests: the Rust KATs of ufunc.rs
+(add_same_shape, add_broadcasts_row_vector_over_matrix, add_promotes_dtype,
+add_on_transposed_view_matches_manual_layout, multiply_bool_is_logical_and;
+ionp-core/src/ufunc.rs:10912-10986), the reduce length sweep of
+tests/differential/ufunc_cases.py:156-252 (lengths 2..1000 over 1-D,
+strided 1-D, (n,3) C and F, RandomState seeds as there), the corpus'
+special values (tests/differential/corpus.py:100-158: NaN, +-inf, +-0,
+denormals, integer extremes) and views (negative strides, transposes,
+F order, offset slices), plus the buffer-window boundaries (8191/8192/8193,
+2^20+13) that the reference's sweep never reaches.
+"""
+import ctypes
+import hashlib
+import json
+import os
+import sys
+import warnings
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+assert np.__version__ == "2.2.6", f"fixtures are pinned to NumPy 2.2.6, found {np.__version__}"
+assert np.getbufsize() == 8192
+
+warnings.simplefilter("ignore")
+
+DTYPES = [np.bool_, np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64, np.uint64,
+          np.float32, np.float64]
+
+
+class Gen:
+    """A large 1-D input regenerated from a seed instead of stored:
+    np.random.default_rng(seed).uniform(-1, 1, n).astype(dtype)[::step]."""
+
+    def __init__(self, seed, n, dtype, step=1):
+        self.seed, self.n, self.dtype, self.step = seed, n, np.dtype(dtype), step
+
+    def array(self):
+        return np.random.default_rng(self.seed).uniform(-1, 1, self.n).astype(self.dtype)[:: self.step]
+
+    def spec(self):
+        return {"gen": "uniform", "seed": self.seed, "n": self.n, "dtype": self.dtype.name, "step": self.step}
+
+
+class Store:
+    def __init__(self):
+        self.arrays = {}
+        self.cases = []
+        self.roots = {}
+        self.keep = []
+
+    def buf(self, a):
+        key = f"b{len(self.arrays)}"
+        self.arrays[key] = np.ascontiguousarray(a).reshape(-1)
+        return key
+
+    def view(self, v):
+        """Record an operand view as (stored root buffer, element offset,
+        shape, strides): views of one array share its stored buffer."""
+        if isinstance(v, Gen):
+            return v.spec()
+        v = np.asarray(v)
+        isz = v.itemsize
+        st = [s // isz for s in v.strides]
+        root = v
+        while isinstance(root.base, np.ndarray):
+            root = root.base
+        if root.dtype != v.dtype or not (root.flags.c_contiguous or root.flags.f_contiguous):
+            root = np.ascontiguousarray(v)
+            st = [s // isz for s in root.strides]
+            v = root
+        key = self.roots.get(id(root))
+        if key is None:
+            key = self.buf(root.ravel(order="K"))  # memory order
+            self.roots[id(root)] = key
+            self.keep.append(root)
+        off = (v.__array_interface__["data"][0] - root.__array_interface__["data"][0]) // isz
+        return {"buf": key, "offset": int(off), "shape": list(v.shape), "strides": st, "dtype": v.dtype.name}
+
+    def expect(self, r):
+        """Expected output: its values (C order) or, above 32 KB, their SHA-256."""
+        r = np.asarray(r)
+        isz = r.itemsize
+        e = {"dtype": r.dtype.name, "shape": list(r.shape), "strides": [s // isz for s in r.strides]}
+        if r.nbytes > 32768:
+            e["sha256"] = sha(r)
+        else:
+            e["buf"] = self.buf(np.ascontiguousarray(r))
+        return e
+
+    def add(self, name, kind, op, inputs, fn, args=None, check_strides=True):
+        try:
+            r = fn()
+            exp = self.expect(r)
+            exp["check_strides"] = check_strides
+        except Exception as e:  # noqa: BLE001 - expected exceptions are data here
+            exp = {"error": type(e).__name__, "msg": str(e)}
+        self.cases.append({"name": name, "kind": kind, "op": op, "args": args or {},
+                           "inputs": [self.view(x) for x in inputs], "expected": exp})
+
+
+def specials(dt, n, rng):
+    dt = np.dtype(dt)
+    if dt.kind == "f":
+        base = rng.standard_normal(n) * 10
+        sp = np.array([np.nan, np.inf, -np.inf, 0.0, -0.0, 1.0, -1.0, 0.5, 1e-310, -1e-310,
+                       1e300, -1e300, 3.4e38, 1e-45, 2.5, -2.5], dtype=np.float64)
+        base[: min(n, len(sp))] = sp[: min(n, len(sp))]
+        rng.shuffle(base)
+        return base.astype(dt)
+    if dt.kind == "b":
+        return rng.random(n) < 0.5
+    info = np.iinfo(dt)
+    base = rng.integers(info.min, info.max, n, dtype=dt, endpoint=True)
+    sp = np.array([info.min, info.max, 0, 1, info.max - 1, info.min + 1], dtype=dt)
+    base[: min(n, len(sp))] = sp[: min(n, len(sp))]
+    rng.shuffle(base)
+    return base
+
+
+def binary_cases(S, rng):
+    ops = ["add", "subtract", "multiply", "true_divide", "maximum", "minimum", "less", "less_equal",
+           "greater", "greater_equal", "equal", "not_equal"]
+    # reference Rust KATs (ufunc.rs:10912-10986)
+    a = np.array([1.0, 2.0, 3.0]); b = np.array([10.0, 20.0, 30.0])
+    S.add("kat/add_same_shape", "binary", "add", [a, b], lambda: np.add(a, b))
+    a = np.arange(6.0).reshape(2, 3); b = np.array([100.0, 200.0, 300.0])
+    S.add("kat/add_broadcasts_row_vector_over_matrix", "binary", "add", [a, b], lambda: np.add(a, b))
+    a = np.array([1, 2, 3], dtype=np.int32); b = np.array([0.5, 0.5, 0.5])
+    S.add("kat/add_promotes_dtype", "binary", "add", [a, b], lambda: np.add(a, b))
+    a = np.array([True, True, False]); b = np.array([True, False, False])
+    S.add("kat/multiply_bool_is_logical_and", "binary", "multiply", [a, b], lambda: np.multiply(a, b))
+    a = np.arange(6.0).reshape(2, 3).T; b = np.array([1.0, 1.0])
+    S.add("kat/add_on_transposed_view_matches_manual_layout", "binary", "add", [a, b], lambda: np.add(a, b))
+
+    # every op over same-dtype pairs with special values, several layouts
+    for op in ops:
+        f = getattr(np, op)
+        for dt in DTYPES:
+            if op == "subtract" and dt == np.bool_:
+                x = specials(dt, 4, rng)
+                S.add(f"binary/{op}/{np.dtype(dt).name}/error", "binary", op, [x, x], lambda: f(x, x))
+                continue
+            A = specials(dt, 7 * 9, rng).reshape(7, 9)
+            B = specials(dt, 9, rng)
+            C = specials(dt, 7, rng).reshape(7, 1)
+            nm = np.dtype(dt).name
+            S.add(f"binary/{op}/{nm}/contig", "binary", op, [A, A[::-1]], lambda: f(A, A[::-1]))
+            S.add(f"binary/{op}/{nm}/row", "binary", op, [A, B], lambda: f(A, B))
+            S.add(f"binary/{op}/{nm}/col", "binary", op, [A, C], lambda: f(A, C))
+            S.add(f"binary/{op}/{nm}/T", "binary", op, [A.T, A.T[:, ::-1]], lambda: f(A.T, A.T[:, ::-1]))
+            S.add(f"binary/{op}/{nm}/strided", "binary", op, [A[::2, 1::3], B[::3]], lambda: f(A[::2, 1::3], B[::3]))
+            S.add(f"binary/{op}/{nm}/0d", "binary", op, [A, np.array(B[2])], lambda: f(A, np.array(B[2])))
+    # mixed dtypes (promotion + fused casts), including int64 vs uint64 comparisons
+    pairs = [(a, b) for a in DTYPES for b in DTYPES if a != b]
+    for i, (da, db) in enumerate(pairs):
+        x = specials(da, 30, rng).reshape(5, 6)
+        y = specials(db, 6, rng)
+        for op in (["add", "multiply", "less", "maximum"] if i % 2 == 0 else ["subtract", "equal", "true_divide", "minimum", "greater_equal"]):
+            f = getattr(np, op)
+            S.add(f"mixed/{op}/{np.dtype(da).name}-{np.dtype(db).name}", "binary", op, [x, y], lambda: f(x, y))
+    # broadcast shapes that must fail (corpus must-fail pairs)
+    x = np.zeros(3); y = np.zeros((4, 5))
+    S.add("error/broadcast", "binary", "add", [x, y], lambda: np.add(x, y))
+    # larger layouts reaching the vectorised row path
+    for dt in (np.float32, np.float64, np.int64, np.int32, np.uint8):
+        A = specials(dt, 21 * 1029, rng).reshape(21, 1029)
+        b = specials(dt, 1029, rng)
+        c = specials(dt, 21, rng).reshape(21, 1)
+        nm = np.dtype(dt).name
+        for op in ("add", "multiply", "less", "maximum"):
+            f = getattr(np, op)
+            S.add(f"rows/{op}/{nm}/row", "binary", op, [A, b], lambda: f(A, b))
+            S.add(f"rows/{op}/{nm}/col", "binary", op, [A, c], lambda: f(A, c))
+        A2 = specials(dt, 16 * 2048, rng).reshape(16, 2048)
+        c2 = specials(dt, 16, rng).reshape(16, 1)
+        S.add(f"rows/multiply/{nm}/aligned", "binary", "multiply", [A2, c2], lambda: np.multiply(A2, c2))
+        S.add(f"rows/multiply/{nm}/offset1", "binary", "multiply", [A2[:, 1:], c2], lambda: np.multiply(A2[:, 1:], c2))
+        V2 = A2.T[::2]
+        S.add(f"rows/multiply/{nm}/Tview", "binary", "multiply", [V2, c2[::2]], lambda: np.multiply(V2, c2[::2]))
+
+
+def where_cast_cases(S, rng):
+    c = rng.random((4, 5)) < 0.5
+    x = rng.uniform(size=(5,)).astype(np.float32)
+    y = np.arange(4, dtype=np.int64)[:, None]
+    S.add("where/mixed", "where", "where", [c, x, y], lambda: np.where(c, x, y))
+    ci = rng.integers(0, 3, (6, 7)).astype(np.int32)
+    xf = specials(np.float64, 42, rng).reshape(6, 7)
+    S.add("where/intcond_T", "where", "where", [ci.T, xf.T, np.array(2.5)], lambda: np.where(ci.T, xf.T, np.array(2.5)))
+    for dx in (np.float32, np.float64, np.int64, np.uint8):
+        x = specials(dx, 3 * 2000, rng).reshape(3, 2000)
+        y = specials(dx, 2000, rng)
+        cc = rng.random((3, 2000)) < 0.3
+        S.add(f"where/{np.dtype(dx).name}", "where", "where", [cc, x, y], lambda: np.where(cc, x, y))
+    S.add("error/where_broadcast", "where", "where", [np.zeros(3, bool), np.zeros(4), np.zeros(3)],
+          lambda: np.where(np.zeros(3, bool), np.zeros(4), np.zeros(3)))
+    for s in DTYPES:
+        for d in DTYPES:
+            x = specials(s, 64, rng)
+            if np.dtype(s).kind == "f" and np.dtype(d) == np.uint32:
+                # float -> uint32 out of range takes different x86 paths in NumPy
+                # (vector and scalar loops disagree): only the defined range is pinned
+                x = np.abs(np.nan_to_num(x, nan=1.0, posinf=2.0, neginf=3.0)) % 4.0e9
+                x = x.astype(s)
+            S.add(f"cast/{np.dtype(s).name}->{np.dtype(d).name}", "cast", "astype", [x],
+                  lambda: x.astype(d), args={"dtype": np.dtype(d).name})
+            if s == np.float64:
+                xv = x[::-3]
+                S.add(f"cast/{np.dtype(s).name}->{np.dtype(d).name}/neg", "cast", "astype", [xv],
+                      lambda: xv.astype(d), args={"dtype": np.dtype(d).name})
+
+
+def reduce_cases(S, rng):
+    # the reference's reduce length sweep (ufunc_cases.py:156-252), add only
+    for dtype_i, dtype in enumerate(("float32", "float64")):
+        for n in (2, 4, 7, 8, 9, 16, 32, 64, 127, 128, 129, 1000):
+            seed = (0 * 10_000 + (dtype_i + 1) * 1_000 + n) % (2 ** 31)
+            mk = lambda k, sd: np.random.RandomState(sd).uniform(-4.0, 4.0, size=k).astype(np.float64)  # noqa: E731
+            flat = mk(n, seed).astype(dtype)
+            S.add(f"sweep/add/{dtype}/n{n}/1d", "reduce", "add", [flat], lambda: np.add.reduce(flat), {"axis": 0})
+            S.add(f"sweep/add/{dtype}/n{n}/1d/none", "reduce", "add", [flat], lambda: np.add.reduce(flat, axis=None),
+                  {"axis": None})
+            nc = mk(2 * n, seed + 2).astype(dtype)[::2]
+            S.add(f"sweep/add/{dtype}/n{n}/1d_noncontig", "reduce", "add", [nc], lambda: np.add.reduce(nc), {"axis": 0})
+            b2 = mk(n * 3, seed + 4).reshape(n, 3)
+            c2 = np.ascontiguousarray(b2.astype(dtype))
+            f2 = np.asfortranarray(b2.astype(dtype))
+            S.add(f"sweep/add/{dtype}/n{n}/2d_c", "reduce", "add", [c2], lambda: np.add.reduce(c2), {"axis": 0})
+            S.add(f"sweep/add/{dtype}/n{n}/2d_f", "reduce", "add", [f2], lambda: np.add.reduce(f2), {"axis": 0})
+            S.add(f"sweep/maximum/{dtype}/n{n}/1d", "reduce", "maximum", [flat], lambda: np.maximum.reduce(flat),
+                  {"axis": 0})
+    # buffer-window boundaries
+    for dt in (np.float32, np.float64):
+        nm = np.dtype(dt).name
+        for n in (8191, 8192, 8193, 12345, 16384, 20000, 2 ** 20 + 13):
+            g = Gen(n, n, dt)
+            x = g.array()
+            S.add(f"window/add/{nm}/n{n}", "reduce", "add", [g], lambda: np.add.reduce(x), {"axis": 0})
+            g2 = Gen(n + 1, 2 * n, dt, -2)
+            xs = g2.array()
+            S.add(f"window/add/{nm}/n{n}/neg2", "reduce", "add", [g2], lambda: np.add.reduce(xs), {"axis": 0})
+            S.add(f"window/maximum/{nm}/n{n}/neg2", "reduce", "maximum", [g2], lambda: np.maximum.reduce(xs), {"axis": 0})
+        for shp in ((2, 17000), (9000, 3), (64, 300), (5, 3000), (3, 9000)):
+            A = rng.uniform(-1, 1, shp).astype(dt)
+            for vn, V in (("C", A), ("F", np.asfortranarray(A)), ("T", A.T), ("sub", A[:, : shp[1] - 1]),
+                          ("rev", A[::-1, ::2])):
+                for ax in (0, 1, None, (0, 1)):
+                    S.add(f"layout/add/{nm}/{shp}/{vn}/{ax}", "reduce", "add", [V],
+                          lambda: np.add.reduce(V, axis=ax), {"axis": ax})
+        for shp in ((3, 60, 160), (4, 50, 200), (6, 7, 9)):
+            A = rng.uniform(-1, 1, shp).astype(dt)
+            for vn, V in (("C", A), ("perm", A.transpose(2, 0, 1)), ("sub", A[:, ::2, : shp[2] - 3])):
+                for ax in ((1, 2), (0, 2), (0, 1), None, 1, (0, 1, 2)):
+                    for kd in (False, True):
+                        if kd and ax not in ((0, 2), None):
+                            continue
+                        S.add(f"layout3/add/{nm}/{shp}/{vn}/{ax}/kd{int(kd)}", "reduce", "add", [V],
+                              lambda: np.add.reduce(V, axis=ax, keepdims=kd), {"axis": ax, "keepdims": kd})
+        # specials: NaN / inf / -0
+        x = rng.uniform(-1, 1, 9000).astype(dt)
+        x[17] = np.inf; x[4000] = -np.inf
+        S.add(f"special/add/{nm}/inf-inf", "reduce", "add", [x], lambda: np.add.reduce(x), {"axis": 0})
+        x = rng.uniform(-1, 1, 9000).astype(dt)
+        x[8500] = np.nan
+        S.add(f"special/add/{nm}/nan", "reduce", "add", [x], lambda: np.add.reduce(x), {"axis": 0})
+        x = np.full(5, -0.0, dtype=dt)
+        S.add(f"special/add/{nm}/negzero", "reduce", "add", [x], lambda: np.add.reduce(x), {"axis": 0})
+        A = rng.uniform(-1, 1, (300, 5)).astype(dt)
+        A[7, 1] = np.nan; A[9, 2] = np.inf; A[11, 2] = -np.inf
+        S.add(f"special/add/{nm}/narrow_nan", "reduce", "add", [A], lambda: np.add.reduce(A, axis=0), {"axis": 0})
+        # float sums with a wider dtype (buffered casts)
+        S.add(f"dtype/add/{nm}->float64", "reduce", "add", [A], lambda: np.add.reduce(A, axis=1, dtype=np.float64),
+              {"axis": 1, "dtype": "float64"})
+
+    # maximum / minimum with +-0 ties and NaN payloads (x86 AVX-512 loop)
+    def f64(u):
+        return np.array([u], dtype=np.uint64).view(np.float64)[0]
+
+    P = f64(0x7ff8000000000123); N = f64(0xfff8000000000456); Sn = f64(0x7ff4000000000789)
+    for trial in range(160):
+        dt = (np.float64, np.float32)[trial % 2]
+        nd = 1 + trial % 3
+        shp = tuple(int(v) for v in rng.integers(1, [9000 if trial % 8 else 17000, 100, 24][nd - 1], nd))
+        x = np.zeros(shp, dtype=dt)
+        m = rng.random(shp)
+        x[m < 0.5] = -0.0
+        x[m < 0.2] = -1
+        if trial % 4 == 1:
+            x[rng.random(shp) < 0.01] = [P, N, Sn][trial % 3]
+        if trial % 4 == 3:
+            x = rng.uniform(-1, 1, shp).astype(dt)
+            x[rng.random(shp) < 0.001] = np.nan
+        v = [x, x[..., ::2], x[::-1], x.T if nd > 1 else x[::3]][trial % 4 if x.shape[-1] > 1 else 0]
+        ax = [None] + list(range(v.ndim))
+        ax = ax[trial % len(ax)]
+        op = ("maximum", "minimum")[(trial // 2) % 2]
+        S.add(f"ext/{op}/{np.dtype(dt).name}/{trial}", "reduce", op, [v],
+              lambda: getattr(np, op).reduce(v, axis=ax), {"axis": ax})
+    S.add("error/max_empty", "reduce", "maximum", [np.zeros((3, 0))], lambda: np.maximum.reduce(np.zeros((3, 0)), axis=1),
+          {"axis": 1})
+    S.add("ext/max_empty_kept", "reduce", "maximum", [np.zeros((0, 3))], lambda: np.maximum.reduce(np.zeros((0, 3)), axis=1),
+          {"axis": 1})
+    S.add("reduce/add_empty", "reduce", "add", [np.zeros((3, 0))], lambda: np.add.reduce(np.zeros((3, 0)), axis=1),
+          {"axis": 1})
+    S.add("error/axis", "reduce", "add", [np.zeros((3, 4))], lambda: np.add.reduce(np.zeros((3, 4)), axis=2), {"axis": 2})
+    S.add("error/dup_axis", "reduce", "add", [np.zeros((3, 4))], lambda: np.add.reduce(np.zeros((3, 4)), axis=(1, -1)),
+          {"axis": [1, -1]})
+    # integer and bool reductions (reduce dtype widening)
+    for dt in (np.int32, np.int64, np.uint8, np.int8, np.uint32, np.bool_, np.int16, np.uint64):
+        x = specials(dt, 50 * 70, rng).reshape(50, 70)
+        nm = np.dtype(dt).name
+        for ax in (0, 1, None):
+            for op in ("add", "maximum", "minimum"):
+                S.add(f"int/{op}/{nm}/{ax}", "reduce", op, [x], lambda: getattr(np, op).reduce(x, axis=ax), {"axis": ax})
+        S.add(f"int/add/{nm}/T", "reduce", "add", [x.T[::2]], lambda: np.add.reduce(x.T[::2], axis=1), {"axis": 1})
+    x = rng.integers(-1000, 1000, 150000, dtype=np.int32)
+    S.add("int/add/int32/long", "reduce", "add", [x], lambda: np.add.reduce(x), {"axis": 0})
+
+
+def argext_cases(S, rng):
+    for dt in (np.float64, np.float32, np.int64, np.int32, np.uint8, np.bool_):
+        nm = np.dtype(dt).name
+        for shp in ((1000,), (30, 50), (7, 300, 4)) + (((70000,),) if dt in (np.float64, np.int32) else ()):
+            x = specials(dt, int(np.prod(shp)), rng).reshape(shp)
+            if np.dtype(dt).kind == "f":
+                x = np.nan_to_num(x, nan=0.0) if rng.random() < 0.5 else x
+                x.reshape(-1)[rng.integers(0, x.size)] = x.max() if not np.isnan(x).any() else np.nan
+            for V in (x, x[::-1] if x.ndim == 1 else x.transpose()):
+                for ax in [None] + list(range(V.ndim)):
+                    for fn in ("argmax", "argmin"):
+                        S.add(f"arg/{fn}/{nm}/{shp}/{ax}", "argext", fn, [V],
+                              lambda: getattr(np, fn)(V, axis=ax), {"axis": ax})
+    x = np.array([1.0, 3.0, 3.0, np.nan, 5.0, np.nan])
+    S.add("arg/argmax/nan_first", "argext", "argmax", [x], lambda: np.argmax(x), {"axis": None})
+    x = np.array([-0.0, 0.0, 0.0])
+    S.add("arg/argmax/zero_tie", "argext", "argmax", [x], lambda: np.argmax(x), {"axis": None})
+    x = np.zeros((3, 4))
+    S.add("arg/argmax/keepdims", "argext", "argmax", [x], lambda: np.argmax(x, axis=1, keepdims=True),
+          {"axis": 1, "keepdims": True})
+    S.add("error/argmax_empty", "argext", "argmax", [np.zeros((0,))], lambda: np.argmax(np.zeros((0,))), {"axis": None})
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def big_workloads():
+    """Digests of the full-size workloads (inputs regenerated from seeds)."""
+    out = {}
+    rng = np.random.default_rng(0)
+    A = rng.uniform(-1, 1, (1024, 1024))
+    b = rng.uniform(-1, 1, 1024)
+    C = np.add(A, b)
+    out["cfg1"] = {"C": sha(C), "sum0": sha(np.add.reduce(C, 0)), "sum1": sha(np.add.reduce(C, 1)),
+                   "sumNone": float(np.add.reduce(C, axis=None)).hex()}
+    X = np.random.default_rng(1).standard_normal((8192, 8192))
+    c = np.random.default_rng(2).uniform(0.5, 2, (8192, 1))
+    X32 = X.astype(np.float32)
+    c32 = c.astype(np.float32)
+    out["cfg2"] = {"mul_f64": sha(X * c), "mul_f32": sha(X32 * c32),
+                   "view_f32": sha(np.multiply(X32.T[::2], c32[::2])),
+                   "view_f64": sha(np.multiply(X.T[::2], c[::2]))}
+    del X, X32
+    I = np.random.default_rng(3).integers(-2 ** 62, 2 ** 62, (8192, 8192))
+    j = np.random.default_rng(8).integers(-2 ** 62, 2 ** 62, (8192, 1))
+    out["cfg2"]["add_i64"] = sha(I + j)
+    del I
+    rng4 = np.random.default_rng(4)
+    x = rng4.uniform(-1, 1, 2 ** 28)
+    out["cfg3"] = {"sum_f64": float(np.add.reduce(x)).hex()}
+    xn = x.copy()
+    xn[np.random.default_rng(5).random(2 ** 28) < 1e-6] = np.nan
+    out["cfg3"]["max_nan"] = hex(np.maximum.reduce(xn).view(np.uint64))
+    out["cfg3"]["argmax_nan"] = int(np.argmax(xn))
+    out["cfg3"]["max"] = hex(np.maximum.reduce(x).view(np.uint64))
+    out["cfg3"]["argmax"] = int(np.argmax(x))
+    del x, xn
+    F = np.random.default_rng(6).uniform(-1, 1, (4096, 65536)).astype(np.float32)
+    out["cfg3"]["f32_sum0"] = sha(np.add.reduce(F, axis=0))
+    out["cfg3"]["f32_sum1"] = sha(np.add.reduce(F, axis=1))
+    del F
+    I32 = np.random.default_rng(7).integers(-1000, 1000, 2 ** 28, dtype=np.int32)
+    out["cfg3"]["i32_sum"] = int(np.sum(I32))
+    return out
+
+
+def main():
+    rng = np.random.default_rng(20260731)  # the reference corpus seed (corpus.py:22)
+    S = Store()
+    binary_cases(S, rng)
+    where_cast_cases(S, rng)
+    reduce_cases(S, rng)
+    argext_cases(S, rng)
+    np.savez_compressed(os.path.join(HERE, "fixtures.npz"), **S.arrays)
+    with open(os.path.join(HERE, "cases.json"), "w") as f:
+        json.dump(S.cases, f)
+    print(f"{len(S.cases)} cases, {sum(a.nbytes for a in S.arrays.values()) / 1e6:.1f} MB raw")
+    if "--no-big" not in sys.argv:
+        with open(os.path.join(HERE, "big.json"), "w") as f:
+            json.dump(big_workloads(), f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

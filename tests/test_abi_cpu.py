@@ -1,0 +1,119 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol
+include/ionp_hip.h declares, and its host-only planning matches the oracle."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "anionpy_amd", "libionp_hip.so")
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "ionp_hip.h")).read()
+    return sorted(set(re.findall(r"\b(ionp_[a-z0-9_]+)\s*\(", txt)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.fail("libionp_hip.so is not built (python -m anionpy_amd.build)")
+    from anionpy_amd import _lib
+    return _lib.load()
+
+
+def test_every_header_symbol_is_exported(lib):
+    syms = header_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    from anionpy_amd import _lib
+    assert set(header_symbols()) == set(_lib.SIGNATURES)
+
+
+def _desc(a):
+    from anionpy_amd import _lib
+    import anionpy_amd as ap
+    d = _lib.ArrayDesc()
+    d.dtype = ap._CODE[a.dtype]
+    d.ndim = a.ndim
+    for i in range(a.ndim):
+        d.shape[i] = a.shape[i]
+        d.strides[i] = a.strides[i] // a.itemsize
+    d.offset = 0
+    d.data = 0
+    return d
+
+
+def test_result_type_table_matches_oracle(lib):
+    import oracle as O
+    out = ctypes.c_int32()
+    for i, a in enumerate(O.DTYPES):
+        for j, b in enumerate(O.DTYPES):
+            assert lib.ionp_result_type(i, j, ctypes.byref(out)) == 0
+            assert O.DTYPES[out.value] == O.result_type(a, b).type
+
+
+def test_plan_binary_korder_matches_oracle(lib):
+    import oracle as O
+    from anionpy_amd import _lib
+    rng = np.random.default_rng(0)
+    base = rng.uniform(size=(6, 7, 8))
+    views = [base, base.T, base[::-1, ::2], base.transpose(1, 0, 2), np.asfortranarray(base), base[:, :1, :]]
+    for a in views:
+        for b in (base[0, 0], base[:, 0, :1], a[::1]):
+            try:
+                shape = O.broadcast_shape(a.shape, b.shape)
+            except O.OracleError:
+                continue
+            t = _lib.ArrayDesc()
+            da, db = _desc(a), _desc(b)
+            assert lib.ionp_plan_binary(0, ctypes.byref(da), ctypes.byref(db), ctypes.byref(t)) == 0
+            assert tuple(t.shape[: t.ndim]) == shape
+            got = [t.strides[i] for i in range(t.ndim) if t.shape[i] > 1]
+            want = [s for s, n in zip(O.korder_strides([a, b], shape), shape) if n > 1]
+            assert got == want
+            np_out = np.add(a, b)
+            assert got == [s // 8 for s, n in zip(np_out.strides, np_out.shape) if n > 1]
+
+
+def test_errors_carry_numpy_messages(lib):
+    from anionpy_amd import _lib
+    t = _lib.ArrayDesc()
+    a, b = _desc(np.zeros(3)), _desc(np.zeros((4, 5)))
+    assert lib.ionp_plan_binary(0, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)) == 1
+    assert lib.ionp_last_error().decode() == "operands could not be broadcast together with shapes (3,) (4,5) "
+    x = _desc(np.zeros((3, 4)))
+    axes = (ctypes.c_int32 * 1)(2)
+    assert lib.ionp_plan_reduce(0, ctypes.byref(x), axes, 1, 0, -1, ctypes.byref(t)) == 7
+    assert lib.ionp_last_error().decode() == "axis 2 is out of bounds for array of dimension 2"
+    axes = (ctypes.c_int32 * 2)(1, -1)
+    assert lib.ionp_plan_reduce(0, ctypes.byref(x), axes, 2, 0, -1, ctypes.byref(t)) == 5
+    assert lib.ionp_last_error().decode() == "duplicate value in 'axis'"
+    bb = _desc(np.zeros(3, dtype=bool))
+    assert lib.ionp_plan_binary(1, ctypes.byref(bb), ctypes.byref(bb), ctypes.byref(t)) == 4
+    assert "boolean subtract" in lib.ionp_last_error().decode()
+
+
+def test_plan_reduce_layout_and_dtype(lib):
+    from anionpy_amd import _lib
+    t = _lib.ArrayDesc()
+    for a, axis, kd in [(np.zeros((4, 5, 6), np.int32), (0, 2), 0), (np.zeros((4, 5, 6)).T, (1,), 1),
+                        (np.zeros((3, 4), np.uint8), None, 0)]:
+        d = _desc(a)
+        if axis is None:
+            axes, n = None, -1
+        else:
+            axes, n = (ctypes.c_int32 * len(axis))(*axis), len(axis)
+        assert lib.ionp_plan_reduce(0, ctypes.byref(d), axes, n, kd, -1, ctypes.byref(t)) == 0
+        want = np.add.reduce(a, axis=axis, keepdims=bool(kd))
+        assert tuple(t.shape[: t.ndim]) == want.shape
+        import anionpy_amd as ap
+        assert ap._NP_DTYPES[t.dtype] == want.dtype.type
+        st = [t.strides[i] for i in range(t.ndim) if t.shape[i] > 1]
+        assert st == [s // want.itemsize for s, n in zip(want.strides, want.shape) if n > 1]

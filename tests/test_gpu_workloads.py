@@ -1,0 +1,77 @@
+"""Full-size workloads (inputs regenerated from their seeds) compared
+bit-for-bit with NumPy 2.2.6 digests recorded in tests/golden/big.json."""
+import hashlib
+
+import numpy as np
+import pytest
+import torch
+
+from fixture_cases import big
+
+pytestmark = pytest.mark.gpu
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def digests():
+    d = big()
+    if d is None:
+        pytest.fail("tests/golden/big.json missing")
+    return d
+
+
+def test_cfg1_broadcast_add_and_reduce(ap, digests):
+    rng = np.random.default_rng(0)
+    A = rng.uniform(-1, 1, (1024, 1024))
+    b = rng.uniform(-1, 1, 1024)
+    C = ap.add(ap.asarray(A), ap.asarray(b))
+    d = digests["cfg1"]
+    assert sha(C.get()) == d["C"]
+    assert sha(ap.add.reduce(C, axis=0).get()) == d["sum0"]
+    assert sha(ap.add.reduce(C, axis=1).get()) == d["sum1"]
+    assert float(ap.add.reduce(C, axis=None).get()).hex() == d["sumNone"]
+
+
+def test_cfg2_broadcast_multiply(ap, digests):
+    d = digests["cfg2"]
+    X = np.random.default_rng(1).standard_normal((8192, 8192))
+    c = np.random.default_rng(2).uniform(0.5, 2, (8192, 1))
+    Xd, cd = ap.asarray(X), ap.asarray(c)
+    assert sha(ap.multiply(Xd, cd).get()) == d["mul_f64"]
+    assert sha(ap.multiply(Xd.T[::2], cd[::2]).get()) == d["view_f64"]
+    del Xd
+    X32d, c32d = ap.asarray(X.astype(np.float32)), ap.asarray(c.astype(np.float32))
+    del X
+    assert sha(ap.multiply(X32d, c32d).get()) == d["mul_f32"]
+    assert sha(ap.multiply(X32d.T[::2], c32d[::2]).get()) == d["view_f32"]
+    del X32d
+    I = np.random.default_rng(3).integers(-2 ** 62, 2 ** 62, (8192, 8192))
+    j = np.random.default_rng(8).integers(-2 ** 62, 2 ** 62, (8192, 1))
+    assert sha(ap.add(ap.asarray(I), ap.asarray(j)).get()) == d["add_i64"]
+    torch.cuda.empty_cache()
+
+
+def test_cfg3_reductions(ap, digests):
+    d = digests["cfg3"]
+    x = np.random.default_rng(4).uniform(-1, 1, 2 ** 28)
+    xd = ap.asarray(x)
+    assert float(ap.add.reduce(xd).get()).hex() == d["sum_f64"]
+    assert hex(ap.maximum.reduce(xd).get().view(np.uint64)) == d["max"]
+    assert int(ap.argmax(xd).get()) == d["argmax"]
+    x[np.random.default_rng(5).random(2 ** 28) < 1e-6] = np.nan
+    xd = ap.asarray(x)
+    del x
+    assert hex(ap.maximum.reduce(xd).get().view(np.uint64)) == d["max_nan"]
+    assert int(ap.argmax(xd).get()) == d["argmax_nan"]
+    del xd
+    F = ap.asarray(np.random.default_rng(6).uniform(-1, 1, (4096, 65536)).astype(np.float32))
+    assert sha(ap.add.reduce(F, axis=0).get()) == d["f32_sum0"]
+    assert sha(ap.add.reduce(F, axis=1).get()) == d["f32_sum1"]
+    del F
+    I = ap.asarray(np.random.default_rng(7).integers(-1000, 1000, 2 ** 28, dtype=np.int32))
+    s = ap.sum(I)
+    assert s.dtype == np.int64 and int(s.get()) == d["i32_sum"]
+    torch.cuda.empty_cache()

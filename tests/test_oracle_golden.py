@@ -1,0 +1,120 @@
+"""The CPU oracle against the NumPy 2.2.6 golden vectors (pins the oracle)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from fixture_cases import args_axis, check_result, host_input, load
+
+KINDS = ["binary", "where", "cast", "reduce", "argext"]
+
+
+def run_oracle(case, npz):
+    ins = [host_input(s, npz) for s in case["inputs"]]
+    k, op, args = case["kind"], case["op"], case["args"]
+    if k == "binary":
+        return O.binary(op, ins[0], ins[1])
+    if k == "where":
+        return O.where(*ins)
+    if k == "cast":
+        return O.cast(ins[0], np.dtype(args["dtype"]))
+    if k == "reduce":
+        x = ins[0]
+        if args.get("dtype"):
+            x = O.cast(x, np.dtype(args["dtype"]))
+        r = O.reduce(op, x, args_axis(args))
+        if args.get("keepdims"):
+            ax = args_axis(args)
+            axes = range(x.ndim) if ax is None else ((ax,) if isinstance(ax, int) else ax)
+            r = np.expand_dims(r, tuple(a % x.ndim for a in axes))
+        return r
+    r = O.argext(op == "argmax", ins[0], args.get("axis"))
+    if args.get("keepdims"):
+        r = np.expand_dims(r, args["axis"])
+    return r
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_oracle_matches_numpy_fixtures(kind):
+    cases, npz = load()
+    bad = []
+    n = 0
+    for c in cases:
+        if c["kind"] != kind:
+            continue
+        n += 1
+        exp = c["expected"]
+        try:
+            got = run_oracle(c, npz)
+        except O.OracleError:
+            if "error" not in exp:
+                bad.append((c["name"], "oracle raised"))
+            continue
+        if "error" in exp:
+            bad.append((c["name"], "oracle did not raise " + exp["error"]))
+            continue
+        # keepdims views from expand_dims carry stride 0 on the new axes
+        if c["args"].get("keepdims"):
+            c = dict(c, expected=dict(exp, check_strides=False))
+        why = check_result(c, got, npz)
+        if why:
+            bad.append((c["name"], why))
+    assert n > 0
+    assert not bad, f"{len(bad)} of {n} mismatches, first: {bad[:5]}"
+
+
+def test_window_model_matters():
+    """A run longer than the 8192-element buffer: NumPy folds window partials
+    in order; one pairwise tree over the whole run (the reference's
+    pairwise_sum, ionp-core/src/ufunc.rs:6416) differs.  Pins that the oracle
+    applies the windows and that the fixtures would catch a dropped window."""
+    x = np.random.default_rng(3).uniform(-1, 1, 20000)
+    windowed = O.reduce("add", x)
+    whole = O.pairwise(x)
+    assert windowed.tobytes() == np.add.reduce(x).tobytes()
+    assert windowed != whole
+    # and a sequential fold (a dropped tree) differs from both
+    seq = np.float64(0)
+    for v in x:
+        seq = seq + v
+    assert seq != windowed
+
+
+def test_pairwise_closed_forms():
+    """Exact small cases: sums of integers are exact in any order; a leaf of
+    8 accumulators combines ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7))."""
+    x = np.arange(1000, dtype=np.float64)
+    assert O.pairwise(x) == 499500.0
+    # bracketing-sensitive leaf: 1e16 cancels only in the documented order
+    v = np.array([1.0, 1e16, -1e16, 1.0, 0.0, 0.0, 0.0, 0.0])
+    r = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]))
+    assert O.pairwise(v) == r
+    flat = 0.0
+    for t in v:
+        flat += t
+    assert r != flat  # a flat fold of the accumulators would be caught
+
+
+def test_signed_zero_seed():
+    """add.reduce seeds with +0: a sum of -0.0 is +0.0 (ufunc.rs:7137)."""
+    r = O.reduce("add", np.full(5, -0.0))
+    assert r == 0.0 and not np.signbit(r)
+
+
+def test_argmax_first_nan_and_first_max():
+    x = np.array([1.0, 5.0, 5.0, np.nan, 7.0, np.nan])
+    assert O.argmax(x) == 3
+    assert O.argmax(np.array([1.0, 5.0, 5.0])) == 1
+    assert O.argmin(np.array([2.0, -1.0, -1.0])) == 1
+
+
+def test_max_nan_rules():
+    """Contiguous runs: a NaN the vector loop sees becomes the canonical
+    quiet NaN; a strided run returns the first NaN's own bits."""
+    neg = np.array([0xFFF8000000000456], dtype=np.uint64).view(np.float64)[0]
+    x = np.arange(100, dtype=np.float64)
+    x[10] = neg
+    assert O.reduce("maximum", x).view(np.uint64) == 0x7FF8000000000000
+    y = np.zeros(200)
+    y[::2] = np.arange(100)
+    y[20] = neg
+    assert O.reduce("maximum", y[::2]).view(np.uint64) == 0xFFF8000000000456
