@@ -16,146 +16,9 @@
 //   ew_generic - any strides (rank <= 8 after coalescing), any dtype mix.
 #include "elementwise.h"
 
+#include "device_ops.h"
+
 namespace ionp {
-
-// ---------------------------------------------------------------------------
-// scalar ops on the loop dtype
-// ---------------------------------------------------------------------------
-template <typename T> struct Arith {
-    static constexpr bool is_f = std::is_floating_point<T>::value;
-    using U = typename std::make_unsigned<typename std::conditional<is_f, int, T>::type>::type;
-    __device__ static inline T add(T a, T b) {
-        if constexpr (is_f) return fadd(a, b);
-        else return (T)((U)a + (U)b);
-    }
-    __device__ static inline T sub(T a, T b) {
-        if constexpr (is_f) return fsub(a, b);
-        else return (T)((U)a - (U)b);
-    }
-    __device__ static inline T mul(T a, T b) {
-        if constexpr (is_f) return fmul(a, b);
-        else return (T)((U)a * (U)b);
-    }
-    __device__ static inline T div(T a, T b) {
-        if constexpr (is_f) return fdiv(a, b);
-        else return (T)0;  // integer true_divide resolves to a float loop
-    }
-    __device__ static inline T max(T a, T b) {
-        if constexpr (is_f) return fmaxn(a, b);
-        else return a > b ? a : b;
-    }
-    __device__ static inline T min(T a, T b) {
-        if constexpr (is_f) return fminn(a, b);
-        else return a < b ? a : b;
-    }
-};
-
-// bool loop (ufunc.rs bool kernels): add/maximum = or, multiply/minimum = and
-template <int LT, typename T = typename dt_type<LT>::T> __device__ inline T arith_op(int op, T a, T b) {
-    if constexpr (LT == IONP_BOOL) {
-        const uint8_t x = a != 0, y = b != 0;
-        return (op == IONP_ADD || op == IONP_MAXIMUM) ? (T)(x | y) : (T)(x & y);
-    } else {
-        switch (op) {
-            case IONP_ADD: return Arith<T>::add(a, b);
-            case IONP_SUBTRACT: return Arith<T>::sub(a, b);
-            case IONP_MULTIPLY: return Arith<T>::mul(a, b);
-            case IONP_TRUE_DIVIDE: return Arith<T>::div(a, b);
-            case IONP_MAXIMUM: return Arith<T>::max(a, b);
-            default: return Arith<T>::min(a, b);
-        }
-    }
-}
-
-template <typename T> __device__ inline bool cmp_op(int op, T a, T b) {
-    switch (op) {
-        case IONP_LESS: return a < b;
-        case IONP_LESS_EQUAL: return a <= b;
-        case IONP_GREATER: return a > b;
-        case IONP_GREATER_EQUAL: return a >= b;
-        case IONP_EQUAL: return a == b;
-        default: return a != b;
-    }
-}
-
-// exact int64 vs uint64 comparison (NumPy 2's mixed-sign comparison loop)
-__device__ inline bool cmp_su(int op, int64_t a, uint64_t b) {
-    const bool lt = a < 0 || (uint64_t)a < b, eq = a >= 0 && (uint64_t)a == b;
-    switch (op) {
-        case IONP_LESS: return lt;
-        case IONP_LESS_EQUAL: return lt || eq;
-        case IONP_GREATER: return !lt && !eq;
-        case IONP_GREATER_EQUAL: return !lt;
-        case IONP_EQUAL: return eq;
-        default: return !eq;
-    }
-}
-__device__ inline int swap_cmp(int op) {
-    switch (op) {
-        case IONP_LESS: return IONP_GREATER;
-        case IONP_LESS_EQUAL: return IONP_GREATER_EQUAL;
-        case IONP_GREATER: return IONP_LESS;
-        case IONP_GREATER_EQUAL: return IONP_LESS_EQUAL;
-        default: return op;
-    }
-}
-
-// dynamic-dtype load converted to the loop dtype
-template <int LT> __device__ inline typename dt_type<LT>::T load_as(int dt, const char *p) {
-    switch (dt) {
-        case IONP_BOOL: return cvt<LT>((uint8_t)(*(const uint8_t *)p != 0));
-        case IONP_INT8: return cvt<LT>(*(const int8_t *)p);
-        case IONP_UINT8: return cvt<LT>(*(const uint8_t *)p);
-        case IONP_INT16: return cvt<LT>(*(const int16_t *)p);
-        case IONP_UINT16: return cvt<LT>(*(const uint16_t *)p);
-        case IONP_INT32: return cvt<LT>(*(const int32_t *)p);
-        case IONP_UINT32: return cvt<LT>(*(const uint32_t *)p);
-        case IONP_INT64: return cvt<LT>(*(const int64_t *)p);
-        case IONP_UINT64: return cvt<LT>(*(const uint64_t *)p);
-        case IONP_FLOAT32: return cvt<LT>(*(const float *)p);
-        default: return cvt<LT>(*(const double *)p);
-    }
-}
-
-// dynamic-dtype store of a loop-dtype value
-template <typename T> __device__ inline void store_as(int dt, char *p, T v) {
-    switch (dt) {
-        case IONP_BOOL: *(uint8_t *)p = cvt<IONP_BOOL>(v); break;
-        case IONP_INT8: *(int8_t *)p = cvt<IONP_INT8>(v); break;
-        case IONP_UINT8: *(uint8_t *)p = cvt<IONP_UINT8>(v); break;
-        case IONP_INT16: *(int16_t *)p = cvt<IONP_INT16>(v); break;
-        case IONP_UINT16: *(uint16_t *)p = cvt<IONP_UINT16>(v); break;
-        case IONP_INT32: *(int32_t *)p = cvt<IONP_INT32>(v); break;
-        case IONP_UINT32: *(uint32_t *)p = cvt<IONP_UINT32>(v); break;
-        case IONP_INT64: *(int64_t *)p = cvt<IONP_INT64>(v); break;
-        case IONP_UINT64: *(uint64_t *)p = cvt<IONP_UINT64>(v); break;
-        case IONP_FLOAT32: *(float *)p = cvt<IONP_FLOAT32>(v); break;
-        default: *(double *)p = cvt<IONP_FLOAT64>(v); break;
-    }
-}
-
-// One output element of the generic kernel, operand addresses resolved.
-template <int LT>
-__device__ inline void ew_element(const EwArgs &A, const char *p0, const char *p1, const char *p2, char *po) {
-    using T = typename dt_type<LT>::T;
-    if (A.op == OP_CAST) {
-        store_as<T>(A.out_dt, po, load_as<LT>(A.in_dt[0], p0));
-    } else if (A.op == OP_WHERE) {
-        const bool c = load_as<IONP_BOOL>(A.in_dt[0], p0) != 0;
-        store_as<T>(A.out_dt, po, c ? load_as<LT>(A.in_dt[1], p1) : load_as<LT>(A.in_dt[2], p2));
-    } else if (A.op >= IONP_LESS) {
-        bool r;
-        if (A.mixed_su) {
-            r = A.in_dt[0] == IONP_INT64 ? cmp_su(A.op, *(const int64_t *)p0, *(const uint64_t *)p1)
-                                         : cmp_su(swap_cmp(A.op), *(const int64_t *)p1, *(const uint64_t *)p0);
-        } else {
-            r = cmp_op<T>(A.op, load_as<LT>(A.in_dt[0], p0), load_as<LT>(A.in_dt[1], p1));
-        }
-        *(uint8_t *)po = (uint8_t)r;
-    } else {
-        store_as<T>(A.out_dt, po, arith_op<LT>(A.op, load_as<LT>(A.in_dt[0], p0), load_as<LT>(A.in_dt[1], p1)));
-    }
-}
 
 // Generic strided kernel: element e of the coalesced space (dim 0
 // innermost) -> per-operand offsets by fast division.  A.total < 2^31.

@@ -27,6 +27,7 @@
 #include <mutex>
 
 #include "elementwise.h"
+#include "device_ops.h"
 
 namespace ionp {
 
@@ -56,161 +57,6 @@ void *scratch_get(size_t bytes, hipStream_t s) {
     return g_scratch;
 }
 
-// ---------------------------------------------------------------------------
-// index helpers
-// ---------------------------------------------------------------------------
-// decompose idx over dims (innermost first) and dot with strides
-__device__ inline int64_t dot_idx(uint32_t idx, int n, const int64_t *shape, const FastDiv *dv, const int64_t *st) {
-    int64_t off = 0;
-    for (int d = 0; d < n; d++) {
-        uint32_t c;
-        if (d + 1 < n) {
-            const uint32_t q = dv[d].div(idx);
-            c = idx - q * (uint32_t)shape[d];
-            idx = q;
-        } else {
-            c = idx;
-        }
-        off += (int64_t)c * st[d];
-    }
-    return off;
-}
-__device__ inline void out_offsets(const RedArgs &A, uint32_t o, int64_t *in_off, int64_t *out_off) {
-    int64_t a = 0, b = 0;
-    for (int d = 0; d < A.nk; d++) {
-        uint32_t c;
-        if (d + 1 < A.nk) {
-            const uint32_t q = A.k_div[d].div(o);
-            c = o - q * (uint32_t)A.k_shape[d];
-            o = q;
-        } else {
-            c = o;
-        }
-        a += (int64_t)c * A.k_in_st[d];
-        b += (int64_t)c * A.k_out_st[d];
-    }
-    *in_off = a;
-    *out_off = b;
-}
-__device__ inline int64_t w_offset(const RedArgs &A, int64_t q) {
-    if (A.nw == 1) return q * A.w_st[0];
-    return dot_idx((uint32_t)q, A.nw, A.w_shape, A.w_div, A.w_st);
-}
-__device__ inline int64_t r_offset(const RedArgs &A, int64_t r) {
-    if (A.nr == 0) return 0;
-    if (A.nr == 1) return r * A.r_st[0];
-    return dot_idx((uint32_t)r, A.nr, A.r_shape, A.r_div, A.r_st);
-}
-
-template <int LT> __device__ inline typename dt_type<LT>::T ld(int dt, const char *base, int64_t off) {
-    switch (dt) {
-        case IONP_BOOL: return cvt<LT>((uint8_t)(((const uint8_t *)base)[off] != 0));
-        case IONP_INT8: return cvt<LT>(((const int8_t *)base)[off]);
-        case IONP_UINT8: return cvt<LT>(((const uint8_t *)base)[off]);
-        case IONP_INT16: return cvt<LT>(((const int16_t *)base)[off]);
-        case IONP_UINT16: return cvt<LT>(((const uint16_t *)base)[off]);
-        case IONP_INT32: return cvt<LT>(((const int32_t *)base)[off]);
-        case IONP_UINT32: return cvt<LT>(((const uint32_t *)base)[off]);
-        case IONP_INT64: return cvt<LT>(((const int64_t *)base)[off]);
-        case IONP_UINT64: return cvt<LT>(((const uint64_t *)base)[off]);
-        case IONP_FLOAT32: return cvt<LT>(((const float *)base)[off]);
-        default: return cvt<LT>(((const double *)base)[off]);
-    }
-}
-
-template <typename T> struct FDT;
-template <> struct FDT<float> { static constexpr int v = IONP_FLOAT32; };
-template <> struct FDT<double> { static constexpr int v = IONP_FLOAT64; };
-
-// ---------------------------------------------------------------------------
-// NumPy's pairwise tree over an LDS window (ufunc.rs:6416-6448)
-// ---------------------------------------------------------------------------
-// Node split of pairwise_sum: n2 = n/2 rounded down to a multiple of 8.
-__device__ inline int pw_split(int n) { return (n >> 1) & ~7; }
-
-constexpr int kPwMaxWin = 8192;
-constexpr int kPwThreads = 512;
-
-// Evaluates pairwise(xs[0..L)) with the whole block; returns the result in
-// every thread.  Scratch: racc (8 per leaf), val (per octet), leaf table.
-template <typename T>
-__device__ T pairwise_block(const T *xs, int L, T *racc, T *val, int *leaf_lo, int *leaf_n, int *cnt) {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int noct = (L + 7) >> 3;
-    // 1. leaf starts: a leaf begins at octet o iff the descent for element
-    //    8o ends at a leaf whose first element is 8o (every leaf starts on a
-    //    multiple of 8).  The table order is irrelevant: chains address
-    //    leaves through it and leaf values are stored by first octet.
-    if (tid == 0) cnt[0] = 0;
-    __syncthreads();
-    for (int o = tid; o < noct; o += nt) {
-        int lo = 0, n = L;
-        while (n > 128) {
-            const int n2 = pw_split(n);
-            if (8 * o < lo + n2) n = n2;
-            else { lo += n2; n -= n2; }
-        }
-        if (lo == 8 * o) {
-            const int idx = atomicAdd(&cnt[0], 1);
-            leaf_lo[idx] = lo;
-            leaf_n[idx] = n;
-        }
-    }
-    __syncthreads();
-    const int nleaves = cnt[0];
-    // 2. the 8 accumulator chains of every leaf, each in element order
-    for (int c = tid; c < 8 * nleaves; c += nt) {
-        const int l = c >> 3, j = c & 7, lo = leaf_lo[l], n = leaf_n[l];
-        if (n < 8) {
-            if (j == 0) {
-                T r = T(0);
-                for (int i = 0; i < n; i++) r = fadd(r, xs[lo + i]);
-                racc[c] = r;
-            }
-        } else {
-            T r = xs[lo + j];
-            const int m = n >> 3;
-            for (int k = 1; k < m; k++) r = fadd(r, xs[lo + 8 * k + j]);
-            racc[c] = r;
-        }
-    }
-    __syncthreads();
-    // 3. leaf results: ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)), then the n%8 tail
-    for (int l = tid; l < nleaves; l += nt) {
-        const int lo = leaf_lo[l], n = leaf_n[l];
-        const T *r = racc + 8 * l;
-        T res;
-        if (n < 8) {
-            res = r[0];
-        } else {
-            res = fadd(fadd(fadd(r[0], r[1]), fadd(r[2], r[3])), fadd(fadd(r[4], r[5]), fadd(r[6], r[7])));
-            for (int i = lo + (n & ~7); i < lo + n; i++) res = fadd(res, xs[i]);
-        }
-        val[lo >> 3] = res;
-    }
-    __syncthreads();
-    // 4. internal nodes, deepest level first: node value = left + right,
-    //    stored at the node's first octet.
-    for (int d = 9; d >= 0; d--) {  // leaves hold >= 64 elements: depth <= 8 for L <= 8192
-        if (tid < (1 << d)) {
-            int lo = 0, n = L;
-            bool ok = true;
-            for (int lev = 0; lev < d; lev++) {
-                if (n <= 128) { ok = false; break; }
-                const int n2 = pw_split(n);
-                if ((tid >> (d - 1 - lev)) & 1) { lo += n2; n -= n2; }
-                else n = n2;
-            }
-            if (ok && n > 128) {
-                const int n2 = pw_split(n);
-                val[lo >> 3] = fadd(val[lo >> 3], val[(lo + n2) >> 3]);
-            }
-        }
-        __syncthreads();
-    }
-    return val[0];
-}
-
 // One workgroup per window g = (o, r, w), windows enumerated output-major.
 template <typename T>
 __global__ __launch_bounds__(kPwThreads) void pw_windows(const RedArgs A, int64_t nwindows, int direct) {
@@ -231,7 +77,7 @@ __global__ __launch_bounds__(kPwThreads) void pw_windows(const RedArgs A, int64_
         const int L = (int)min(A.bufsize, A.wsize - s0);
         for (int i = threadIdx.x; i < L; i += blockDim.x) xs[i] = ld<LT>(A.in_dt, A.in, ib + w_offset(A, s0 + i));
         __syncthreads();
-        const T res = pairwise_block<T>(xs, L, racc, val, leaf_lo, leaf_n, cnt);
+        const T res = pairwise_block<T>([&](int i) { return xs[i]; }, L, racc, val, leaf_lo, leaf_n, cnt);
         if (threadIdx.x == 0) {
             if (direct) ((T *)A.out)[ob] = fadd(T(0), res);
             else ((T *)A.scratch)[g] = res;
