@@ -38,6 +38,17 @@ ionp_status same_shape(const ionp_array_desc *out, int ndim, const int64_t *shap
     return IONP_OK;
 }
 
+ionp_status check_cfg(const ionp_config *cfg) {
+    if (!cfg) return IONP_OK;
+    if (cfg->block != 0 && (cfg->block < 64 || cfg->block > 256 || cfg->block % 64))
+        return set_error(IONP_ERR_VALUE, "ionp_config.block must be 64, 128, 192 or 256");
+    if (cfg->vec != 0 && cfg->vec != 1 && cfg->vec != 2 && cfg->vec != 4 && cfg->vec != 8)
+        return set_error(IONP_ERR_VALUE, "ionp_config.vec must be 1, 2, 4 or 8");
+    if (cfg->blocks_per_cu < 0 || cfg->blocks_per_cu > 64 || cfg->split < 0 || cfg->split > 64)
+        return set_error(IONP_ERR_VALUE, "ionp_config.blocks_per_cu and split must be in [0, 64]");
+    return IONP_OK;
+}
+
 ionp_status binary_common(int32_t op, const ionp_array_desc *a, const ionp_array_desc *b, int *loop, int *odt,
                           int *nd, int64_t *shape) {
     ionp_status st;
@@ -228,6 +239,7 @@ ionp_status ionp_plan_argext(int32_t ismax, const ionp_array_desc *in, int32_t a
 // ---------------------------------------------------------------------------
 ionp_status ionp_binary(int32_t op, const ionp_array_desc *a, const ionp_array_desc *b, const ionp_array_desc *out,
                         const ionp_config *cfg, void *stream) {
+    if (ionp_status cs = check_cfg(cfg)) return cs;
     int loop, odt, nd;
     int64_t shape[IONP_MAX_DIMS];
     ionp_status s = binary_common(op, a, b, &loop, &odt, &nd, shape);
@@ -244,6 +256,7 @@ ionp_status ionp_binary(int32_t op, const ionp_array_desc *a, const ionp_array_d
 
 ionp_status ionp_where(const ionp_array_desc *c, const ionp_array_desc *x, const ionp_array_desc *y,
                        const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
+    if (ionp_status cs = check_cfg(cfg)) return cs;
     ionp_array_desc t;
     ionp_status s = ionp_plan_where(c, x, y, &t);
     if (s) return s;
@@ -255,6 +268,7 @@ ionp_status ionp_where(const ionp_array_desc *c, const ionp_array_desc *x, const
 }
 
 ionp_status ionp_cast(const ionp_array_desc *in, const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
+    if (ionp_status cs = check_cfg(cfg)) return cs;
     ionp_status s;
     if ((s = check_desc(in, "array")) || (s = check_desc(out, "out"))) return s;
     if ((s = same_shape(out, in->ndim, in->shape, "astype"))) return s;
@@ -266,6 +280,7 @@ ionp_status ionp_cast(const ionp_array_desc *in, const ionp_array_desc *out, con
 
 ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes, int32_t keepdims,
                         const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
+    if (ionp_status cs = check_cfg(cfg)) return cs;
     ionp_array_desc t;
     ionp_status s;
     if ((s = check_desc(out, "out"))) return s;
@@ -280,6 +295,7 @@ ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *ax
 
 ionp_status ionp_argext(int32_t ismax, const ionp_array_desc *in, int32_t axis, int32_t keepdims,
                         const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
+    if (ionp_status cs = check_cfg(cfg)) return cs;
     ionp_array_desc t;
     ionp_status s;
     if ((s = check_desc(out, "out"))) return s;

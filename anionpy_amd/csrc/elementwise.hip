@@ -71,7 +71,9 @@ template <typename T, int OPK> __device__ inline auto row_apply(T a, T b) {
     }
 }
 
-template <typename T, int OPK, int VEC, int UNROLL>
+// GATHER: inputs whose inner stride is neither 0 nor 1 (e.g. X.T[::2]) are
+// read element by element at that stride; the output stays vectorised.
+template <typename T, int OPK, int VEC, int UNROLL, bool GATHER = false>
 __global__ __launch_bounds__(256) void ew_rows(const RowArgs R) {
     using OT = typename std::conditional<(OPK >= IONP_LESS), uint8_t, T>::type;
     const T *__restrict__ a = (const T *)R.in[0];
@@ -105,12 +107,21 @@ __global__ __launch_bounds__(256) void ew_rows(const RowArgs R) {
         const T a0 = sa ? T(0) : pa[0];
         const T b0 = sb ? T(0) : pb[0];
         VecT<T, VEC> va[UNROLL], vb[UNROLL];
+        const int64_t sta = R.inner_st[0], stb = R.inner_st[1];
 #pragma unroll
         for (int u = 0; u < UNROLL; u++) {
             const int64_t i = i0 + ((int64_t)u * blockDim.x + threadIdx.x) * VEC;
             if (i + VEC <= R.inner) {
-                if (sa) va[u] = *(const VecT<T, VEC> *)(pa + i);
-                if (sb) vb[u] = *(const VecT<T, VEC> *)(pb + i);
+                if constexpr (GATHER) {
+#pragma unroll
+                    for (int k = 0; k < VEC; k++) {
+                        if (sa) va[u][k] = pa[(i + k) * sta];
+                        if (sb) vb[u][k] = pb[(i + k) * stb];
+                    }
+                } else {
+                    if (sa) va[u] = *(const VecT<T, VEC> *)(pa + i);
+                    if (sb) vb[u] = *(const VecT<T, VEC> *)(pb + i);
+                }
             }
         }
 #pragma unroll
@@ -124,7 +135,7 @@ __global__ __launch_bounds__(256) void ew_rows(const RowArgs R) {
                 __builtin_nontemporal_store(r, (VecT<OT, VEC> *)(po + i));
             } else {
                 for (int64_t j = i; j < R.inner && j < i + VEC; j++)
-                    po[j] = row_apply<T, OPK>(sa ? pa[j] : a0, sb ? pb[j] : b0);
+                    po[j] = row_apply<T, OPK>(sa ? pa[j * sta] : a0, sb ? pb[j * stb] : b0);
             }
         }
     }
@@ -220,6 +231,33 @@ static RowFn row_kernel_v(int op, int unroll) {
 }
 
 template <typename T>
+static RowFn row_gather_t(int op) {
+    constexpr int V = 16 / sizeof(T) > 4 ? 4 : 16 / sizeof(T);
+    switch (op) {
+        case IONP_ADD: return ew_rows<T, IONP_ADD, V, 4, true>;
+        case IONP_SUBTRACT: return ew_rows<T, IONP_SUBTRACT, V, 4, true>;
+        case IONP_MULTIPLY: return ew_rows<T, IONP_MULTIPLY, V, 4, true>;
+        case IONP_MAXIMUM: return ew_rows<T, IONP_MAXIMUM, V, 4, true>;
+        case IONP_MINIMUM: return ew_rows<T, IONP_MINIMUM, V, 4, true>;
+        case IONP_LESS: return ew_rows<T, IONP_LESS, V, 4, true>;
+        case IONP_TRUE_DIVIDE:
+            if constexpr (std::is_floating_point<T>::value) return ew_rows<T, IONP_TRUE_DIVIDE, V, 4, true>;
+            else return nullptr;
+        default: return nullptr;
+    }
+}
+
+static RowFn row_gather(int dt, int op) {
+    switch (dt) {
+        case IONP_INT32: return row_gather_t<int32_t>(op);
+        case IONP_INT64: return row_gather_t<int64_t>(op);
+        case IONP_FLOAT32: return row_gather_t<float>(op);
+        case IONP_FLOAT64: return row_gather_t<double>(op);
+        default: return nullptr;
+    }
+}
+
+template <typename T>
 static RowFn row_kernel_t(int op, int vec, int unroll) {
     if (op == IONP_TRUE_DIVIDE && !std::is_floating_point<T>::value) return nullptr;
     if (op == IONP_TRUE_DIVIDE) {
@@ -261,12 +299,14 @@ bool try_rows(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_statu
     const int want_out = A.op >= IONP_LESS ? (int)IONP_BOOL : A.loop_dt;
     if (A.out_dt != want_out || A.loop_dt == IONP_BOOL) return false;
     if (A.out_st[0] != 1) return false;
+    bool gather = false;
     for (int k = 0; k < 2; k++)
-        if (A.in_st[k][0] != 0 && A.in_st[k][0] != 1) return false;
+        if (A.in_st[k][0] != 0 && A.in_st[k][0] != 1) gather = true;
     if (A.ndim > 4 || A.shape[0] < 64) return false;
     const int esz = itemsize(A.loop_dt);
     int vec = (cfg && cfg->vec > 0) ? cfg->vec : (16 / esz > 8 ? 8 : 16 / esz);
     if (A.op >= IONP_LESS && vec > 4) vec = 4;
+    if (gather) vec = 16 / esz > 4 ? 4 : 16 / esz;
     // vector accesses need every contiguous row base aligned to vec elements
     auto aligned = [&](const char *p, int isz, const int64_t *ost, int64_t inner_st) {
         if (inner_st == 0) return true;
@@ -281,12 +321,20 @@ bool try_rows(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_statu
         ost1[d] = A.in_st[1][d];
     }
     const int oes = itemsize(A.out_dt);
-    while (vec > 1 && (A.shape[0] % vec || !aligned(A.in[0], esz, ost0, A.in_st[0][0]) ||
-                       !aligned(A.in[1], esz, ost1, A.in_st[1][0]) || !aligned(A.out, oes, A.out_st, 1)))
-        vec /= 2;
+    int unroll = (cfg && cfg->split > 0) ? cfg->split : 4;
+    RowFn fn;
+    if (gather) {
+        // only the output is accessed by vector
+        if (A.shape[0] % vec || !aligned(A.out, oes, A.out_st, 1)) return false;
+        unroll = 4;
+        fn = row_gather(A.loop_dt, A.op);
+    } else {
+        while (vec > 1 && (A.shape[0] % vec || !aligned(A.in[0], esz, ost0, A.in_st[0][0]) ||
+                           !aligned(A.in[1], esz, ost1, A.in_st[1][0]) || !aligned(A.out, oes, A.out_st, 1)))
+            vec /= 2;
+        fn = row_kernel(A.loop_dt, A.op, vec, unroll);
+    }
     const int block = (cfg && cfg->block > 0) ? cfg->block : 256;
-    const int unroll = (cfg && cfg->split > 0) ? cfg->split : 4;
-    RowFn fn = row_kernel(A.loop_dt, A.op, vec, unroll);
     if (!fn) return false;
     RowArgs R;
     std::memset(&R, 0, sizeof R);

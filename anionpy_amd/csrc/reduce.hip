@@ -59,7 +59,9 @@ void *scratch_get(size_t bytes, hipStream_t s) {
 
 // One workgroup per window g = (o, r, w), windows enumerated output-major.
 template <typename T>
-__global__ __launch_bounds__(kPwThreads) void pw_windows(const RedArgs A, int64_t nwindows, int direct) {
+// tail_nwin > 0: only the last (partial) window of each output, written to
+// partial index o * tail_nwin + tail_nwin - 1 (the full windows go to pw_fast).
+__global__ __launch_bounds__(kPwThreads) void pw_windows(const RedArgs A, int64_t nwindows, int direct, int64_t tail_nwin) {
     __shared__ T xs[kPwMaxWin];
     __shared__ T racc[kPwMaxWin / 8];
     __shared__ T val[kPwMaxWin / 8];
@@ -68,8 +70,17 @@ __global__ __launch_bounds__(kPwThreads) void pw_windows(const RedArgs A, int64_
     constexpr int LT = FDT<T>::v;
     for (int64_t g = blockIdx.x; g < nwindows; g += gridDim.x) {
         const int64_t per_out = A.rsize * A.nwin;
-        const int64_t o = g / per_out, rem = g - o * per_out;
-        const int64_t r = rem / A.nwin, w = rem - r * A.nwin;
+        int64_t o, r, w;
+        if (tail_nwin) {
+            o = g;
+            r = 0;
+            w = tail_nwin - 1;
+        } else {
+            o = g / per_out;
+            const int64_t rem = g - o * per_out;
+            r = rem / A.nwin;
+            w = rem - r * A.nwin;
+        }
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
         ib += r_offset(A, r);
@@ -80,10 +91,239 @@ __global__ __launch_bounds__(kPwThreads) void pw_windows(const RedArgs A, int64_
         const T res = pairwise_block<T>([&](int i) { return xs[i]; }, L, racc, val, leaf_lo, leaf_n, cnt);
         if (threadIdx.x == 0) {
             if (direct) ((T *)A.out)[ob] = fadd(T(0), res);
-            else ((T *)A.scratch)[g] = res;
+            else ((T *)A.scratch)[tail_nwin ? o * tail_nwin + w : g] = res;
         }
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------
+// Full 8192-element windows of contiguous rows: NumPy's tree on a balanced
+// 64-leaf shape, evaluated from registers, with the ordered fold of window
+// partials running inside the same launch.
+// ---------------------------------------------------------------------------
+// Leaf l of a window holds elements [128 l, 128 l + 128); its accumulator j
+// sums elements 8k + j (k = 0..15) in order, and the window value is the
+// balanced tree over the 64 leaf values (ufunc.rs:6416-6448 with n = 8192:
+// every split is exact, every leaf has 128 elements).
+//
+// Producer workgroups take windows round-robin (window g = producer + k *
+// producers, o = g / nfull, w = g % nfull), so windows finish roughly in
+// order.  Each publishes partial[o][w] with one atomic store; a partial slot
+// starts as a signalling-NaN sentinel that no IEEE add can produce (every
+// NaN an add returns is quiet), so a slot that is not the sentinel holds its
+// final value and no flag, fence or counter is needed.  The fold
+// acc = 0; acc = acc + partial[w] (w order) is done
+//   - for few outputs: by a folder workgroup per output (blocks 0..nout-1)
+//     that polls the partial slots 256 at a time while the producers stream,
+//     so the inherently serial fold (32768 adds for 2^28 elements) overlaps
+//     the HBM traffic instead of following it;
+//   - otherwise: by pw_fold after the launch.
+// Producers never wait; a folder waits only for slots producers always fill
+// and gives up (writing NaN) after a bounded number of polls.
+struct PwFastState {
+    void *partials;    // nout * nwin of T, pre-filled with the sentinel
+    int64_t nfull, nwin;
+    int32_t folders;   // folder workgroups (0 or nout)
+};
+
+template <typename T> struct Sentinel;
+template <> struct Sentinel<double> { static constexpr uint64_t bits = 0x7ff0dead0000beefull; };
+template <> struct Sentinel<float> { static constexpr uint64_t bits = 0x7f80beefu; };
+
+template <typename T> __device__ inline uint64_t ld_bits(const T *p) {
+    if constexpr (sizeof(T) == 8) return __hip_atomic_load((uint64_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return __hip_atomic_load((uint32_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ inline void st_value(T *p, T v) {
+    if constexpr (sizeof(T) == 8)
+        __hip_atomic_store((uint64_t *)p, __builtin_bit_cast(uint64_t, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        __hip_atomic_store((uint32_t *)p, __builtin_bit_cast(uint32_t, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ inline T from_bits(uint64_t u) {
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, u);
+    else return __builtin_bit_cast(T, (uint32_t)u);
+}
+template <typename T> __device__ inline T lane_value(T v, int i) {
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, i), hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), i);
+        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, (uint32_t)__builtin_amdgcn_readlane(__builtin_bit_cast(uint32_t, v), i));
+    }
+}
+// pairwise combine across lanes at distance m: the lower lane's value is
+// the left operand, so both lanes compute left + right
+template <typename T> __device__ inline T xor_combine(T v, int m, int lane) {
+    const T o = __shfl_xor(v, m, 64);
+    return (lane & m) ? o + v : v + o;
+}
+
+// Folder (one wave): acc = acc + partial[w] for w = 0 .. nwin-1 as the
+// slots fill.  Slots are read 512 at a time (8 per lane, slot cur + 64 q +
+// lane in register q) and the next 512 are requested before the current
+// ones are folded, so load latency overlaps the dependent add chain.  A NaN
+// result is redone with x86 NaN rules.
+template <typename T> __device__ void pw_folder(const RedArgs &A, const PwFastState &F, int64_t o, T *stage) {
+    constexpr int Q = 8;
+    const int lane = threadIdx.x & 63;
+    const T *part = (const T *)F.partials + o * F.nwin;
+    T acc = T(0);
+    int64_t cur = 0;
+    uint32_t idle = 0;
+    bool failed = false;
+    uint64_t b[Q], nb[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int64_t idx = cur + 64 * q + lane;
+        b[q] = idx < F.nwin ? ld_bits(part + idx) : Sentinel<T>::bits;
+    }
+    while (cur < F.nwin) {
+        // request the following block before folding this one
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int64_t idx = cur + 64 * (Q + q) + lane;
+            nb[q] = idx < F.nwin ? ld_bits(part + idx) : Sentinel<T>::bits;
+        }
+        int n = 0;  // filled prefix of this block (stops at the first sentinel)
+        bool open = true;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int64_t idx0 = cur + 64 * q;
+            const uint64_t ok = __ballot(b[q] != Sentinel<T>::bits || idx0 + lane >= F.nwin);
+            stage[64 * q + lane] = from_bits<T>(b[q]);
+            if (open) {
+                const int m = ~ok == 0 ? 64 : __builtin_ctzll(~ok);
+                const int valid = (int)min<int64_t>(64, F.nwin - idx0 > 0 ? F.nwin - idx0 : 0);
+                const int take = m < valid ? m : valid;
+                n += take;
+                open = take == 64;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // the serial chain runs in lane 0 from LDS, 16 operands in flight
+        if (lane == 0) {
+            int i = 0;
+            for (; i + 16 <= n; i += 16) {
+                T t[16];
+#pragma unroll
+                for (int k = 0; k < 16; k++) t[k] = stage[i + k];
+#pragma unroll
+                for (int k = 0; k < 16; k++) acc = acc + t[k];
+            }
+            for (; i < n; i++) acc = acc + stage[i];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (n == 0) {
+            if (++idle > (1u << 26)) {  // producers always fill every slot; safety bound only
+                failed = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        } else {
+            idle = 0;
+        }
+        if (n == 64 * Q) {
+            cur += n;
+#pragma unroll
+            for (int q = 0; q < Q; q++) b[q] = nb[q];
+        } else {
+            cur += n;  // re-read from the first unfilled slot
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const int64_t idx = cur + 64 * q + lane;
+                b[q] = idx < F.nwin ? ld_bits(part + idx) : Sentinel<T>::bits;
+            }
+        }
+    }
+    if (lane == 0) {
+        if (failed) {
+            acc = canonical_nan(T(0));
+        } else if (acc != acc) {
+            acc = T(0);
+            for (int64_t i = 0; i < F.nwin; i++) acc = fadd(acc, from_bits<T>(ld_bits(part + i)));
+        }
+        int64_t ib, ob;
+        out_offsets(A, (uint32_t)o, &ib, &ob);
+        ((T *)A.out)[ob] = acc;
+    }
+}
+
+// Producer mapping: every lane owns CH = 16 / sizeof(T) consecutive
+// accumulators of one leaf and reads them with one 16-byte load per step k,
+// so 8 / CH lanes cover a leaf and a window takes 64 * 8 / CH threads
+// (f64: 256, f32: 128).
+template <typename T> constexpr int pw_fast_threads() { return sizeof(T) == 8 ? 256 : 128; }
+
+template <typename T>
+__global__ __launch_bounds__(256) void pw_fast(const RedArgs A, const PwFastState F, int64_t nwindows) {
+    constexpr int LT = FDT<T>::v;
+    constexpr int CH = 16 / sizeof(T);          // accumulators per lane
+    constexpr int LPL = 8 / CH;                 // lanes per leaf
+    constexpr int LEAVES_PER_WAVE = 64 / LPL;   // 16 (f64) or 32 (f32)
+    using V = T __attribute__((ext_vector_type(CH)));
+    __shared__ T leafv[64];
+    __shared__ int sh_nan;
+    __shared__ T racc[kPwMaxWin / 8];
+    __shared__ T val[kPwMaxWin / 8];
+    __shared__ int leaf_lo[kPwMaxWin / 64 + 2], leaf_n[kPwMaxWin / 64 + 2], cnt[1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lf = lane / LPL, jq = (lane % LPL) * CH;
+    if ((int64_t)blockIdx.x < F.folders) {
+        if (wave == 0) pw_folder<T>(A, F, blockIdx.x, racc);  // racc: 1024 T of LDS staging
+        return;
+    }
+    const int64_t nprod = (int64_t)gridDim.x - F.folders;
+    for (int64_t g = (int64_t)blockIdx.x - F.folders; g < nwindows; g += nprod) {
+        const int64_t o = g / F.nfull, w = g - o * F.nfull;
+        int64_t ib, ob;
+        out_offsets(A, (uint32_t)o, &ib, &ob);
+        const int leaf = LEAVES_PER_WAVE * wave + lf;
+        const T *x = (const T *)A.in + ib + w * 8192 + (int64_t)leaf * 128 + jq;
+        V v[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = *(const V *)(x + 8 * k);
+        // accumulators jq .. jq+CH-1 of this leaf, each over k in order
+        V r = v[0];
+#pragma unroll
+        for (int k = 1; k < 16; k++) r = r + v[k];
+        T a;
+        if constexpr (CH == 2) {
+            a = r[0] + r[1];              // r0+r1 (lane holds accumulators 2m, 2m+1)
+            a = xor_combine(a, 1, lane);  // (r0+r1)+(r2+r3)
+            a = xor_combine(a, 2, lane);  // leaf value
+        } else {
+            a = (r[0] + r[1]) + (r[2] + r[3]);
+            a = xor_combine(a, 1, lane);  // leaf value
+        }
+        if (lane % LPL == 0) leafv[leaf] = a;
+        __syncthreads();
+        T res = T(0);
+        if (wave == 0) {
+            T t = leafv[lane];
+#pragma unroll
+            for (int m = 1; m < 64; m <<= 1) t = xor_combine(t, m, lane);
+            res = t;
+            if (lane == 0) sh_nan = t != t;
+        }
+        __syncthreads();
+        if (sh_nan) {  // NaN somewhere in the window: x86 NaN rules, exactly
+            res = pairwise_block<T>([&](int i) { return ld<LT>(A.in_dt, A.in, ib + w * 8192 + i); }, 8192, racc, val,
+                                    leaf_lo, leaf_n, cnt);
+        }
+        if (threadIdx.x == 0) st_value((T *)F.partials + o * F.nwin + w, res);
+    }
+}
+
+template <typename T> __global__ void fill_sentinel(T *p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = from_bits<T>(Sentinel<T>::bits);
+}
+
+__global__ void zero_u32(uint32_t *p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
 }
 
 // Ordered fold per output: acc = 0; acc = acc + partial[i] (ufunc.rs:7137
@@ -117,19 +357,38 @@ template <typename T> __global__ __launch_bounds__(256) void pw_narrow(const Red
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
         T acc = T(0);
-        constexpr int U = 16;
         int64_t r = 0;
-        if (A.nr == 1) {
+        if (A.nr == 1 && A.in_dt == A.out_dt) {
+            // native dtype, one outer axis: U loads in flight ahead of the
+            // dependent add chain (double-buffered), plain IEEE adds; a NaN
+            // result is redone below with x86 NaN rules
+            constexpr int U = sizeof(T) == 8 ? 24 : 32;
+            const T *x = (const T *)A.in + ib;
             const int64_t st = A.r_st[0];
-            for (; r + U <= A.rsize; r += U) {
-                T v[U];
+            T cur[U], nxt[U];
+            if (A.rsize >= U) {
 #pragma unroll
-                for (int u = 0; u < U; u++) v[u] = ld<LT>(A.in_dt, A.in, ib + (r + u) * st);
+                for (int u = 0; u < U; u++) cur[u] = __builtin_nontemporal_load(x + (int64_t)u * st);
+                r = U;
+                for (; r + U <= A.rsize; r += U) {
 #pragma unroll
-                for (int u = 0; u < U; u++) acc = fadd(acc, v[u]);
+                    for (int u = 0; u < U; u++) nxt[u] = __builtin_nontemporal_load(x + (r + u) * st);
+#pragma unroll
+                    for (int u = 0; u < U; u++) acc = acc + cur[u];
+#pragma unroll
+                    for (int u = 0; u < U; u++) cur[u] = nxt[u];
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) acc = acc + cur[u];
             }
+            for (; r < A.rsize; r++) acc = acc + x[r * st];
+            if (acc != acc) {
+                acc = T(0);
+                for (int64_t q = 0; q < A.rsize; q++) acc = fadd(acc, x[q * st]);
+            }
+        } else {
+            for (; r < A.rsize; r++) acc = fadd(acc, ld<LT>(A.in_dt, A.in, ib + r_offset(A, r)));
         }
-        for (; r < A.rsize; r++) acc = fadd(acc, ld<LT>(A.in_dt, A.in, ib + r_offset(A, r)));
         ((T *)A.out)[ob] = acc;
     }
 }
@@ -201,86 +460,192 @@ template <typename T, bool MAX> __device__ inline void cand_merge(ExtCand<T> &a,
     a.vec_nan |= b.vec_nan;
 }
 
-template <typename T, bool MAX>
-__global__ __launch_bounds__(256) void ext_windows(const RedArgs A, int64_t nwindows) {
+// Summary of window g (block-cooperative; `red` is LDS scratch): every
+// element gets its position in the x86 loop's evaluation order, so ties
+// between -0 and +0 and NaN payloads come out as NumPy's.
+template <typename T, bool MAX> __device__ ExtSum<T> ext_window_summary(const RedArgs &A, int64_t g, ExtCand<T> *red) {
     constexpr int LT = FDT<T>::v;
     constexpr int V = sizeof(T) == 8 ? 8 : 16;
+    const int64_t per_out = A.rsize * A.nwin;
+    const int64_t o = g / per_out, rem = g - o * per_out;
+    const int64_t r = rem / A.nwin, w = rem - r * A.nwin;
+    int64_t ib, ob;
+    out_offsets(A, (uint32_t)o, &ib, &ob);
+    ib += r_offset(A, r);
+    const int64_t s0 = w * A.bufsize;
+    const int64_t L = min(A.bufsize, A.wsize - s0);
+    // a window inside W's first coalesced run is read in place with that
+    // run's stride; one spanning runs goes through NumPy's contiguous buffer
+    const bool one_run = (s0 / A.w_shape[0]) == ((s0 + L - 1) / A.w_shape[0]);
+    const bool simd = one_run ? (A.w_st[0] == 1) : true;
+    const int64_t skip = (r == 0 && w == 0) ? 1 : 0;  // first element seeds the fold
+    const int64_t len = L - skip;
+    const int64_t nv = simd ? len - len % V : (len >= 8 ? len - len % 8 : 0);
+    const int64_t per_lane = simd ? nv / V : nv / 8;
+    ExtCand<T> c;
+    c.has_m = c.has_nan = c.vec_nan = 0;
+    c.km = c.kn = 0;
+    c.m = c.nanv = T(0);
+    for (int64_t e = threadIdx.x; e < len; e += blockDim.x) {
+        const T v = ld<LT>(A.in_dt, A.in, ib + w_offset(A, s0 + skip + e));
+        int64_t key;
+        if (e < nv) {
+            const int lane = simd ? (int)(e % V) : (int)(e & 7);
+            const int rank = simd ? (V == 8 ? kLaneRank64[lane] : kLaneRank32[lane]) : lane;
+            key = (int64_t)rank * per_lane + e / (simd ? V : 8);
+        } else {
+            key = e;
+        }
+        if (v != v) {
+            if (!c.has_nan || key < c.kn) {
+                c.has_nan = 1;
+                c.kn = key;
+                c.nanv = v;
+            }
+            if (simd && e < nv) c.vec_nan = 1;
+        } else if (!c.has_m || beats<T, MAX>(v, c.m) || (v == c.m && key > c.km)) {
+            c.has_m = 1;
+            c.m = v;
+            c.km = key;
+        }
+    }
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+        if ((int)threadIdx.x < s2) {
+            ExtCand<T> a = red[threadIdx.x];
+            cand_merge<T, MAX>(a, red[threadIdx.x + s2]);
+            red[threadIdx.x] = a;
+        }
+        __syncthreads();
+    }
+    const ExtCand<T> b = red[0];
+    __syncthreads();
+    ExtSum<T> S;
+    S.simd = simd && len >= 1;
+    S.has_m = b.has_m;
+    S.m = b.m;
+    S.flag = simd ? (b.km >= (int64_t)(V - 1) * per_lane) : 1;
+    S.has_nan = b.has_nan;
+    S.nanv = (simd && b.vec_nan) ? canonical_nan(T(0)) : b.nanv;
+    return S;
+}
+
+template <typename T, bool MAX>
+__global__ __launch_bounds__(256) void ext_windows(const RedArgs A, int64_t nwindows) {
     __shared__ ExtCand<T> red[256];
     for (int64_t g = blockIdx.x; g < nwindows; g += gridDim.x) {
-        const int64_t per_out = A.rsize * A.nwin;
-        const int64_t o = g / per_out, rem = g - o * per_out;
-        const int64_t r = rem / A.nwin, w = rem - r * A.nwin;
+        const ExtSum<T> S = ext_window_summary<T, MAX>(A, g, red);
+        if (threadIdx.x == 0) ((ExtSum<T> *)A.scratch)[g] = S;
+    }
+}
+
+// Contiguous windows (W one run of stride 1, no outer axes): the window's
+// best value and NaN outcome from 16-byte loads, all issued before use.
+// Positions only matter when the best value is a zero (a -0/+0 tie picks a
+// sign); such windows are recomputed by ext_window_summary.  Workgroup g
+// takes `wpb` consecutive windows of one output and combines their
+// summaries in order, so ext_fold sees nwin / wpb summaries per output.
+template <typename T, bool MAX>
+__global__ __launch_bounds__(256) void ext_fast(const RedArgs A, int64_t wpb, int64_t nblk) {
+    constexpr int V = sizeof(T) == 8 ? 8 : 16;  // NumPy's AVX-512 lanes
+    constexpr int VEC = 16 / sizeof(T);
+    constexpr int NV = 8192 / (256 * VEC);      // vectors per thread in a full window
+    using VT = T __attribute__((ext_vector_type(VEC)));
+    __shared__ ExtCand<T> red[256];
+    __shared__ T sm[256];
+    __shared__ uint32_t sflags[256];
+    __shared__ int64_t stail[256];
+    __shared__ T stailv[256];
+    for (int64_t gb = blockIdx.x; gb < A.nout * nblk; gb += gridDim.x) {
+        const int64_t o = gb / nblk, bk = gb - o * nblk;
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
-        ib += r_offset(A, r);
-        const int64_t s0 = w * A.bufsize;
-        const int64_t L = min(A.bufsize, A.wsize - s0);
-        // a window inside W's first coalesced run is read in place with that
-        // run's stride; one spanning runs goes through NumPy's contiguous buffer
-        const bool one_run = (s0 / A.w_shape[0]) == ((s0 + L - 1) / A.w_shape[0]);
-        const bool simd = one_run ? (A.w_st[0] == 1) : true;
-        const int64_t skip = (r == 0 && w == 0) ? 1 : 0;  // first element seeds the fold
-        const int64_t len = L - skip;
-        const int64_t nv = simd ? len - len % V : (len >= 8 ? len - len % 8 : 0);
-        const int64_t per_lane = simd ? nv / V : nv / 8;
-        ExtCand<T> c;
-        c.has_m = c.has_nan = c.vec_nan = 0;
-        c.km = c.kn = 0;
-        c.m = c.nanv = T(0);
-        for (int64_t e = threadIdx.x; e < len; e += blockDim.x) {
-            const T v = ld<LT>(A.in_dt, A.in, ib + w_offset(A, s0 + skip + e));
-            int64_t key;
-            if (e < nv) {
-                const int lane = simd ? (int)(e % V) : (int)(e & 7);
-                const int rank = simd ? (V == 8 ? kLaneRank64[lane] : kLaneRank32[lane]) : lane;
-                key = (int64_t)rank * per_lane + e / (simd ? V : 8);
-            } else {
-                key = e;
-            }
-            if (v != v) {
-                if (!c.has_nan || key < c.kn) {
-                    c.has_nan = 1;
-                    c.kn = key;
-                    c.nanv = v;
+        ExtSum<T> acc;
+        acc.has_m = acc.has_nan = acc.simd = acc.flag = 0;
+        acc.m = acc.nanv = T(0);
+        const int64_t w_end = min(A.nwin, (bk + 1) * wpb);
+        for (int64_t w = bk * wpb; w < w_end; w++) {
+            const int64_t s0 = w * A.bufsize;
+            const int64_t L = min(A.bufsize, A.wsize - s0);
+            const int64_t skip = w == 0 ? 1 : 0;
+            const int64_t len = L - skip, nv = len - len % V;
+            const T *x = (const T *)A.in + ib + s0;
+            T m = T(0);
+            uint32_t has_m = 0, vnan = 0;
+            int64_t tk = INT64_MAX;  // first NaN in the tail
+            T tv = T(0);
+            auto visit = [&](T v, int64_t e) {  // e: index within the window
+                if (e < skip) return;
+                const int64_t rel = e - skip;
+                if (v != v) {
+                    if (rel < nv) vnan = 1;
+                    else if (rel < tk) {
+                        tk = rel;
+                        tv = v;
+                    }
+                } else if (!has_m || beats<T, MAX>(v, m)) {
+                    m = v;
+                    has_m = 1;
                 }
-                if (simd && e < nv) c.vec_nan = 1;
-            } else if (!c.has_m || beats<T, MAX>(v, c.m) || (v == c.m && key > c.km)) {
-                c.has_m = 1;
-                c.m = v;
-                c.km = key;
+            };
+            if (L == 8192 && ((uintptr_t)x % 16) == 0) {
+                VT v[NV];
+#pragma unroll
+                for (int k = 0; k < NV; k++)
+                    v[k] = __builtin_nontemporal_load((const VT *)(x + ((int64_t)k * 256 + threadIdx.x) * VEC));
+#pragma unroll
+                for (int k = 0; k < NV; k++)
+#pragma unroll
+                    for (int c = 0; c < VEC; c++) visit(v[k][c], ((int64_t)k * 256 + threadIdx.x) * VEC + c);
+            } else {
+                for (int64_t e = threadIdx.x; e < L; e += blockDim.x) visit(x[e], e);
             }
-        }
-        red[threadIdx.x] = c;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) {
-                ExtCand<T> a = red[threadIdx.x];
-                cand_merge<T, MAX>(a, red[threadIdx.x + s]);
-                red[threadIdx.x] = a;
-            }
+            sm[threadIdx.x] = m;
+            sflags[threadIdx.x] = has_m | (vnan << 1);
+            stail[threadIdx.x] = tk;
+            stailv[threadIdx.x] = tv;
             __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            const ExtCand<T> b = red[0];
+            for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+                if ((int)threadIdx.x < s2) {
+                    const uint32_t fa = sflags[threadIdx.x], fb = sflags[threadIdx.x + s2];
+                    if ((fb & 1) && (!(fa & 1) || beats<T, MAX>(sm[threadIdx.x + s2], sm[threadIdx.x])))
+                        sm[threadIdx.x] = sm[threadIdx.x + s2];
+                    sflags[threadIdx.x] = fa | fb;
+                    if (stail[threadIdx.x + s2] < stail[threadIdx.x]) {
+                        stail[threadIdx.x] = stail[threadIdx.x + s2];
+                        stailv[threadIdx.x] = stailv[threadIdx.x + s2];
+                    }
+                }
+                __syncthreads();
+            }
+            const T M = sm[0];
+            const uint32_t fl = sflags[0];
+            const int64_t TK = stail[0];
+            const T TV = stailv[0];
+            __syncthreads();
             ExtSum<T> S;
-            S.simd = simd && len >= 1;
-            S.has_m = b.has_m;
-            S.m = b.m;
-            S.flag = simd ? (b.km >= (int64_t)(V - 1) * per_lane) : 1;
-            S.has_nan = b.has_nan;
-            S.nanv = (simd && b.vec_nan) ? canonical_nan(T(0)) : b.nanv;
-            ((ExtSum<T> *)A.scratch)[g] = S;
+            if ((fl & 1) && M == T(0)) {
+                S = ext_window_summary<T, MAX>(A, o * A.nwin + w, red);  // a zero best: signs by position
+            } else {
+                S.simd = len >= 1;
+                S.has_m = fl & 1;
+                S.m = M;
+                S.flag = 1;
+                S.has_nan = (fl >> 1) | (TK != INT64_MAX);
+                S.nanv = (fl >> 1) ? canonical_nan(T(0)) : TV;
+            }
+            if (threadIdx.x == 0) acc = ext_combine<T, MAX>(acc, S);
         }
-        __syncthreads();
+        if (threadIdx.x == 0) ((ExtSum<T> *)A.scratch)[gb] = acc;
     }
 }
 
 // Per output: ordered combine of its window summaries (block-wide tree that
 // keeps left-to-right order), applied to the seed element.
-template <typename T, bool MAX> __global__ __launch_bounds__(256) void ext_fold(const RedArgs A) {
+template <typename T, bool MAX> __global__ __launch_bounds__(256) void ext_fold(const RedArgs A, int64_t per_out) {
     constexpr int LT = FDT<T>::v;
     __shared__ ExtSum<T> red[256];
-    const int64_t per_out = A.rsize * A.nwin;
     for (int64_t o = blockIdx.x; o < A.nout; o += gridDim.x) {
         const ExtSum<T> *S = (const ExtSum<T> *)A.scratch + o * per_out;
         const int64_t per_t = (per_out + blockDim.x - 1) / blockDim.x;
@@ -367,17 +732,68 @@ __global__ __launch_bounds__(256) void int_reduce(const RedArgs A, int64_t seg, 
         __syncthreads();
     }
 }
+// per output, a workgroup combines its segment partials (order-free)
 template <typename AT, int LT, int OP> __global__ __launch_bounds__(256) void int_reduce_fold(const RedArgs A, int64_t nseg) {
     using ST = typename dt_type<LT>::T;
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
+    __shared__ AT red[256];
+    for (int64_t o = blockIdx.x; o < A.nout; o += gridDim.x) {
         const AT *p = (const AT *)A.scratch + o * nseg;
-        AT acc = p[0];
-        for (int64_t i = 1; i < nseg; i++) acc = assoc<AT, OP>(acc, p[i]);
-        int64_t ib, ob;
-        out_offsets(A, (uint32_t)o, &ib, &ob);
-        ((ST *)A.out)[ob] = (ST)acc;
+        AT acc = OP == IONP_ADD ? AT(0) : p[0];
+        for (int64_t i = threadIdx.x; i < nseg; i += blockDim.x) acc = assoc<AT, OP>(acc, p[i]);
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+            if ((int)threadIdx.x < s2) red[threadIdx.x] = assoc<AT, OP>(red[threadIdx.x], red[threadIdx.x + s2]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            int64_t ib, ob;
+            out_offsets(A, (uint32_t)o, &ib, &ob);
+            ((ST *)A.out)[ob] = (ST)red[0];
+        }
+        __syncthreads();
     }
 }
+
+// contiguous groups (one run of stride 1 per output) in the input dtype:
+// 16-byte loads, values widened to the accumulator
+template <typename AT, typename IT, typename ST, int OP>
+__global__ __launch_bounds__(256) void int_reduce_contig(const RedArgs A, int64_t seg, int64_t nseg) {
+    constexpr int VEC = 16 / sizeof(IT);
+    using V = IT __attribute__((ext_vector_type(VEC)));
+    __shared__ AT red[256];
+    for (int64_t g = blockIdx.x; g < A.nout * nseg; g += gridDim.x) {
+        const int64_t o = g / nseg, sg = g - o * nseg;
+        int64_t ib, ob;
+        out_offsets(A, (uint32_t)o, &ib, &ob);
+        const IT *x = (const IT *)A.in + ib;
+        const int64_t q0 = sg * seg, q1 = min(A.group, q0 + seg);
+        AT acc = OP == IONP_ADD ? AT(0) : (AT)x[q0];
+        // aligned vector body
+        int64_t v0 = q0, v1 = q1;
+        while (v0 < q1 && ((uintptr_t)(x + v0) % 16)) v0++;
+        v1 = v0 + (q1 - v0) / VEC * VEC;
+        for (int64_t q = v0 + (int64_t)threadIdx.x * VEC; q < v1; q += (int64_t)blockDim.x * VEC) {
+            const V v = __builtin_nontemporal_load((const V *)(x + q));
+#pragma unroll
+            for (int k = 0; k < VEC; k++) acc = assoc<AT, OP>(acc, (AT)v[k]);
+        }
+        for (int64_t q = q0 + threadIdx.x; q < v0; q += blockDim.x) acc = assoc<AT, OP>(acc, (AT)x[q]);
+        for (int64_t q = v1 + threadIdx.x; q < q1; q += blockDim.x) acc = assoc<AT, OP>(acc, (AT)x[q]);
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+            if ((int)threadIdx.x < s2) red[threadIdx.x] = assoc<AT, OP>(red[threadIdx.x], red[threadIdx.x + s2]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            if (nseg == 1) ((ST *)A.out)[ob] = (ST)red[0];
+            else ((AT *)A.scratch)[g] = red[0];
+        }
+        __syncthreads();
+    }
+}
+
 // narrow layouts: one thread per output, elements in any order
 template <typename AT, int LT, int OP> __global__ __launch_bounds__(256) void int_reduce_cols(const RedArgs A) {
     using ST = typename dt_type<LT>::T;
@@ -429,9 +845,23 @@ static ionp_status run_int(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
         A.scratch = scratch_get((size_t)(A.nout * nseg) * sizeof(AT), s);
         if (!A.scratch) return set_error(IONP_ERR_DEVICE, "out of device memory for reduction scratch");
     }
-    hipLaunchKernelGGL((int_reduce<AT, LT, OP>), dim3(grid_cap(A.nout * nseg, bpc)), dim3(256), 0, s, A, seg, nseg);
-    if (nseg > 1)
-        hipLaunchKernelGGL((int_reduce_fold<AT, LT, OP>), dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A, nseg);
+    using ST = typename dt_type<LT>::T;
+    const dim3 grid(grid_cap(A.nout * nseg, bpc)), blk(256);
+    if (A.nw == 1 && A.w_st[0] == 1 && A.nr == 0 && A.in_dt != IONP_BOOL && A.group < (1ll << 40)) {
+        switch (A.in_dt) {  // one contiguous run per output: vector loads
+            case IONP_INT8: hipLaunchKernelGGL((int_reduce_contig<AT, int8_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
+            case IONP_UINT8: hipLaunchKernelGGL((int_reduce_contig<AT, uint8_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
+            case IONP_INT16: hipLaunchKernelGGL((int_reduce_contig<AT, int16_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
+            case IONP_UINT16: hipLaunchKernelGGL((int_reduce_contig<AT, uint16_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
+            case IONP_INT32: hipLaunchKernelGGL((int_reduce_contig<AT, int32_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
+            case IONP_UINT32: hipLaunchKernelGGL((int_reduce_contig<AT, uint32_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
+            case IONP_INT64: hipLaunchKernelGGL((int_reduce_contig<AT, int64_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
+            default: hipLaunchKernelGGL((int_reduce_contig<AT, uint64_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
+        }
+    } else {
+        hipLaunchKernelGGL((int_reduce<AT, LT, OP>), grid, blk, 0, s, A, seg, nseg);
+    }
+    if (nseg > 1) hipLaunchKernelGGL((int_reduce_fold<AT, LT, OP>), dim3(grid_cap(A.nout, bpc)), dim3(256), 0, s, A, nseg);
     return launched();
 }
 
@@ -457,19 +887,49 @@ template <int OP> static ionp_status run_int_dispatch(RedArgs &A, const ionp_con
     }
 }
 
+template <typename T> static ionp_status run_pw_fast(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
+    const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
+    PwFastState F;
+    F.nfull = A.wsize / 8192;
+    const int64_t has_tail = A.wsize % 8192 != 0;
+    F.nwin = F.nfull + has_tail;
+    F.folders = (A.nout <= 64 && F.nwin > 8) ? (int32_t)A.nout : 0;
+    const int64_t nw_total = A.nout * F.nwin;
+    T *part = (T *)scratch_get((size_t)nw_total * sizeof(T), s);
+    if (!part) return set_error(IONP_ERR_DEVICE, "out of device memory for reduction scratch");
+    F.partials = part;
+    A.scratch = part;
+    hipLaunchKernelGGL(fill_sentinel<T>, dim3((unsigned)std::min<int64_t>(1024, (nw_total + 255) / 256)), dim3(256), 0, s,
+                       part, nw_total);
+    if (has_tail)  // the last, partial window of each output (generic tree)
+        hipLaunchKernelGGL(pw_windows<T>, dim3(grid_cap(A.nout, bpc)), dim3(kPwThreads), 0, s, A, A.nout, 0, F.nwin);
+    const int64_t nwindows = A.nout * F.nfull;
+    hipLaunchKernelGGL(pw_fast<T>, dim3(F.folders + grid_cap(nwindows, bpc)), dim3(pw_fast_threads<T>()), 0, s, A, F,
+                       nwindows);
+    if (!F.folders) hipLaunchKernelGGL(pw_fold<T>, dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
+    return launched();
+}
+
 template <typename T> static ionp_status run_float_add(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
     const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
     if (A.nw == 0) {
         hipLaunchKernelGGL(pw_narrow<T>, dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
         return launched();
     }
+    // contiguous rows with at least one full 8192-element window, native dtype;
+    // either few windows per output or few outputs (one folder each)
+    const bool rows_ok = A.nw == 1 && A.w_st[0] == 1 && A.rsize == 1 && A.bufsize == 8192 && A.wsize >= 8192 &&
+                         A.in_dt == A.out_dt && ((uintptr_t)A.in % sizeof(T)) == 0 &&
+                         A.wsize / 8192 * A.nout < (1ll << 31);
+    if (rows_ok)
+        return run_pw_fast<T>(A, cfg, s);
     const int64_t nwindows = A.nout * A.rsize * A.nwin;
     const int direct = A.rsize == 1 && A.nwin == 1;
     if (!direct) {
         A.scratch = scratch_get((size_t)nwindows * sizeof(T), s);
         if (!A.scratch) return set_error(IONP_ERR_DEVICE, "out of device memory for reduction scratch");
     }
-    hipLaunchKernelGGL(pw_windows<T>, dim3(grid_cap(nwindows, bpc)), dim3(kPwThreads), 0, s, A, nwindows, direct);
+    hipLaunchKernelGGL(pw_windows<T>, dim3(grid_cap(nwindows, bpc)), dim3(kPwThreads), 0, s, A, nwindows, direct, (int64_t)0);
     if (!direct) hipLaunchKernelGGL(pw_fold<T>, dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
     return launched();
 }
@@ -483,8 +943,19 @@ template <typename T, bool MAX> static ionp_status run_float_ext(RedArgs &A, con
     const int64_t nwindows = A.nout * A.rsize * A.nwin;
     A.scratch = scratch_get((size_t)nwindows * sizeof(ExtSum<T>), s);
     if (!A.scratch) return set_error(IONP_ERR_DEVICE, "out of device memory for reduction scratch");
-    hipLaunchKernelGGL((ext_windows<T, MAX>), dim3(grid_cap(nwindows, bpc)), dim3(256), 0, s, A, nwindows);
-    hipLaunchKernelGGL((ext_fold<T, MAX>), dim3(grid_cap(A.nout, bpc)), dim3(256), 0, s, A);
+    if (A.nw == 1 && A.w_st[0] == 1 && A.nr == 0 && A.in_dt == A.out_dt) {
+        // enough workgroups to fill the chip, each folding a run of windows
+        const int64_t want = (int64_t)num_cus() * 8;
+        int64_t wpb = (nwindows + want - 1) / want;
+        if (wpb < 1) wpb = 1;
+        if (wpb > A.nwin) wpb = A.nwin;
+        const int64_t nblk = (A.nwin + wpb - 1) / wpb;
+        hipLaunchKernelGGL((ext_fast<T, MAX>), dim3(grid_cap(A.nout * nblk, bpc)), dim3(256), 0, s, A, wpb, nblk);
+        hipLaunchKernelGGL((ext_fold<T, MAX>), dim3(grid_cap(A.nout, bpc)), dim3(256), 0, s, A, nblk);
+    } else {
+        hipLaunchKernelGGL((ext_windows<T, MAX>), dim3(grid_cap(nwindows, bpc)), dim3(256), 0, s, A, nwindows);
+        hipLaunchKernelGGL((ext_fold<T, MAX>), dim3(grid_cap(A.nout, bpc)), dim3(256), 0, s, A, A.rsize * A.nwin);
+    }
     return launched();
 }
 
@@ -651,16 +1122,95 @@ template <typename T, bool MAX> __global__ __launch_bounds__(256) void arg_rows(
         __syncthreads();
     }
 }
+// per output, a workgroup merges its segment candidates (order-free merge)
 template <typename T, bool MAX> __global__ __launch_bounds__(256) void arg_fold(const ArgArgs A) {
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
+    __shared__ ArgBest<T, MAX> red[256];
+    for (int64_t o = blockIdx.x; o < A.nout; o += gridDim.x) {
         const ArgBest<T, MAX> *p = (const ArgBest<T, MAX> *)A.scratch + o * A.nseg;
-        ArgBest<T, MAX> b = p[0];
-        for (int64_t k = 1; k < A.nseg; k++) arg_merge(b, p[k]);
-        int64_t ib, ob;
-        arg_offsets(A, (uint32_t)o, &ib, &ob);
-        ((int64_t *)A.out)[ob] = b.i;
+        ArgBest<T, MAX> b;
+        b.has = 0;
+        b.nan = 0;
+        b.i = 0;
+        b.v = T(0);
+        for (int64_t k = threadIdx.x; k < A.nseg; k += blockDim.x) arg_merge(b, p[k]);
+        red[threadIdx.x] = b;
+        __syncthreads();
+        for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+            if ((int)threadIdx.x < s2) {
+                ArgBest<T, MAX> c = red[threadIdx.x];
+                arg_merge(c, red[threadIdx.x + s2]);
+                red[threadIdx.x] = c;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            int64_t ib, ob;
+            arg_offsets(A, (uint32_t)o, &ib, &ob);
+            ((int64_t *)A.out)[ob] = red[0].i;
+        }
+        __syncthreads();
     }
 }
+
+// contiguous scan (stride 1): 16-byte loads; each thread visits its indices
+// in increasing order, so a strict compare keeps the first extreme and the
+// first NaN ends its scan
+template <typename T, bool MAX> __global__ __launch_bounds__(256) void arg_contig(const ArgArgs A) {
+    constexpr int VEC = 16 / sizeof(T) > 0 ? 16 / sizeof(T) : 1;
+    using V = T __attribute__((ext_vector_type(VEC)));
+    __shared__ ArgBest<T, MAX> red[256];
+    for (int64_t g = blockIdx.x; g < A.nout * A.nseg; g += gridDim.x) {
+        const int64_t o = g / A.nseg, sg = g - o * A.nseg;
+        int64_t ib, ob;
+        arg_offsets(A, (uint32_t)o, &ib, &ob);
+        const T *x = (const T *)A.in + ib;
+        const int64_t i0 = sg * A.seg, i1 = min(A.len, i0 + A.seg);
+        int64_t v0 = i0;
+        while (v0 < i1 && ((uintptr_t)(x + v0) % 16)) v0++;
+        const int64_t v1 = v0 + (i1 - v0) / VEC * VEC;
+        ArgBest<T, MAX> b;
+        b.has = 0;
+        b.nan = 0;
+        b.i = 0;
+        b.v = T(0);
+        auto visit = [&](T v, int64_t i) {
+            if (b.nan) return;
+            if (is_nan_any(v)) {
+                b.v = v;
+                b.i = i;
+                b.nan = 1;
+                b.has = 1;
+            } else if (!b.has || beats<T, MAX>(v, b.v)) {
+                b.v = v;
+                b.i = i;
+                b.has = 1;
+            }
+        };
+        for (int64_t i = i0 + threadIdx.x; i < v0; i += blockDim.x) visit(x[i], i);
+        for (int64_t i = v0 + (int64_t)threadIdx.x * VEC; i < v1; i += (int64_t)blockDim.x * VEC) {
+            const V v = __builtin_nontemporal_load((const V *)(x + i));
+#pragma unroll
+            for (int k = 0; k < VEC; k++) visit(v[k], i + k);
+        }
+        for (int64_t i = v1 + threadIdx.x; i < i1; i += blockDim.x) visit(x[i], i);
+        red[threadIdx.x] = b;
+        __syncthreads();
+        for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+            if ((int)threadIdx.x < s2) {
+                ArgBest<T, MAX> c = red[threadIdx.x];
+                arg_merge(c, red[threadIdx.x + s2]);
+                red[threadIdx.x] = c;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            if (A.nseg == 1) ((int64_t *)A.out)[ob] = red[0].i;
+            else ((ArgBest<T, MAX> *)A.scratch)[g] = red[0];
+        }
+        __syncthreads();
+    }
+}
+
 template <typename T, bool MAX> __global__ __launch_bounds__(256) void arg_cols(const ArgArgs A) {
     const T *x = (const T *)A.in;
     for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
@@ -692,8 +1242,11 @@ template <typename T, bool MAX> static ionp_status run_arg_t(ArgArgs &A, const i
         A.scratch = scratch_get((size_t)(A.nout * A.nseg) * sizeof(ArgBest<T, MAX>), s);
         if (!A.scratch) return set_error(IONP_ERR_DEVICE, "out of device memory for argext scratch");
     }
-    hipLaunchKernelGGL((arg_rows<T, MAX>), dim3(grid_cap(A.nout * A.nseg, bpc)), dim3(256), 0, s, A);
-    if (A.nseg > 1) hipLaunchKernelGGL((arg_fold<T, MAX>), dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
+    if (A.nl == 1 && A.l_st[0] == 1 && sizeof(T) <= 16)
+        hipLaunchKernelGGL((arg_contig<T, MAX>), dim3(grid_cap(A.nout * A.nseg, bpc)), dim3(256), 0, s, A);
+    else
+        hipLaunchKernelGGL((arg_rows<T, MAX>), dim3(grid_cap(A.nout * A.nseg, bpc)), dim3(256), 0, s, A);
+    if (A.nseg > 1) hipLaunchKernelGGL((arg_fold<T, MAX>), dim3(grid_cap(A.nout, bpc)), dim3(256), 0, s, A);
     return launched();
 }
 

@@ -95,7 +95,7 @@ typedef struct {
  * Every configuration computes bit-identical results: it changes only which
  * thread handles which element, never the arithmetic order. */
 typedef struct {
-    int32_t block;         /* threads per workgroup: 64..1024 */
+    int32_t block;         /* threads per workgroup: 64, 128, 192 or 256 */
     int32_t vec;           /* elements per vector access: 1, 2, 4, 8 */
     int32_t blocks_per_cu; /* grid size = blocks_per_cu * CUs (grid-stride) */
     int32_t split;         /* reduce: outputs per thread (narrow) / unroll */
