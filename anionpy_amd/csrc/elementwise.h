@@ -14,6 +14,7 @@ struct RowArgs {
     FastDiv odiv[kMaxKDims];
     int64_t ost[2][kMaxKDims], ost_out[kMaxKDims];
     int64_t inner_st[2];
+    int32_t pair_ok[2];
     const char *in[2];
     char *out;
 };

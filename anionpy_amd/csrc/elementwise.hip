@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) void ew_rows(const RowArgs R) {
         const T *pa = a + ra;
         const T *pb = b + rb;
         OT *po = o + ro;
-        const bool sa = R.inner_st[0] != 0, sb = R.inner_st[1] != 0;
+        const int64_t sa = R.inner_st[0], sb = R.inner_st[1];
         const T a0 = sa ? T(0) : pa[0];
         const T b0 = sb ? T(0) : pb[0];
         VecT<T, VEC> va[UNROLL], vb[UNROLL];
@@ -113,10 +113,25 @@ __global__ __launch_bounds__(256) void ew_rows(const RowArgs R) {
             const int64_t i = i0 + ((int64_t)u * blockDim.x + threadIdx.x) * VEC;
             if (i + VEC <= R.inner) {
                 if constexpr (GATHER) {
+                    // stride 2 (x[::2] views): two 16-byte loads, keep the even
+                    // elements; other strides element by element
+                    if (sa == 2 && R.pair_ok[0]) {
+                        const VecT<T, VEC> lo = *(const VecT<T, VEC> *)(pa + 2 * i);
+                        const VecT<T, VEC> hi = *(const VecT<T, VEC> *)(pa + 2 * i + VEC);
 #pragma unroll
-                    for (int k = 0; k < VEC; k++) {
-                        if (sa) va[u][k] = pa[(i + k) * sta];
-                        if (sb) vb[u][k] = pb[(i + k) * stb];
+                        for (int k = 0; k < VEC; k++) va[u][k] = k < VEC / 2 ? lo[2 * k] : hi[2 * k - VEC];
+                    } else if (sa) {
+#pragma unroll
+                        for (int k = 0; k < VEC; k++) va[u][k] = pa[(i + k) * sta];
+                    }
+                    if (stb == 2 && R.pair_ok[1]) {
+                        const VecT<T, VEC> lo = *(const VecT<T, VEC> *)(pb + 2 * i);
+                        const VecT<T, VEC> hi = *(const VecT<T, VEC> *)(pb + 2 * i + VEC);
+#pragma unroll
+                        for (int k = 0; k < VEC; k++) vb[u][k] = k < VEC / 2 ? lo[2 * k] : hi[2 * k - VEC];
+                    } else if (sb) {
+#pragma unroll
+                        for (int k = 0; k < VEC; k++) vb[u][k] = pb[(i + k) * stb];
                     }
                 } else {
                     if (sa) va[u] = *(const VecT<T, VEC> *)(pa + i);
@@ -351,6 +366,11 @@ bool try_rows(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_statu
     }
     R.inner_st[0] = A.in_st[0][0];
     R.inner_st[1] = A.in_st[1][0];
+    for (int k = 0; k < 2; k++) {  // stride-2 inputs read as aligned pairs of vectors
+        bool ok = A.in_st[k][0] == 2 && ((uintptr_t)A.in[k]) % 16 == 0;
+        for (int d = 1; ok && d < A.ndim; d++) ok = (A.in_st[k][d] * esz) % 16 == 0;
+        R.pair_ok[k] = ok;
+    }
     const int64_t tile = (int64_t)block * vec * (unroll >= 4 ? 4 : 1);
     R.tiles_per_row = (R.inner + tile - 1) / tile;
     if (R.rows * R.tiles_per_row >= (1ll << 31) || R.rows >= (1ll << 31)) return false;

@@ -162,38 +162,38 @@ template <typename T> __device__ inline T xor_combine(T v, int m, int lane) {
 }
 
 // Folder (one wave): acc = acc + partial[w] for w = 0 .. nwin-1 as the
-// slots fill.  Slots are read 512 at a time (8 per lane, slot cur + 64 q +
-// lane in register q) and the next 512 are requested before the current
-// ones are folded, so load latency overlaps the dependent add chain.  A NaN
-// result is redone with x86 NaN rules.
+// slots fill.  Slots are read 1024 at a time (16 per lane, slot cur + 64 q +
+// lane in register q) and the next two blocks are requested before the
+// current one is folded, so the load latency of a saturated HBM overlaps
+// the dependent add chain.  The chain itself runs in lane 0 from LDS.  A
+// NaN result is redone with x86 NaN rules.
 template <typename T> __device__ void pw_folder(const RedArgs &A, const PwFastState &F, int64_t o, T *stage) {
-    constexpr int Q = 8;
+    constexpr int Q = 16, BLK = 64 * Q;
     const int lane = threadIdx.x & 63;
     const T *part = (const T *)F.partials + o * F.nwin;
     T acc = T(0);
     int64_t cur = 0;
     uint32_t idle = 0;
     bool failed = false;
-    uint64_t b[Q], nb[Q];
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        const int64_t idx = cur + 64 * q + lane;
-        b[q] = idx < F.nwin ? ld_bits(part + idx) : Sentinel<T>::bits;
-    }
-    while (cur < F.nwin) {
-        // request the following block before folding this one
+    uint64_t b0[Q], b1[Q], b2[Q];
+    auto poll = [&](uint64_t *b, int64_t base) {
 #pragma unroll
         for (int q = 0; q < Q; q++) {
-            const int64_t idx = cur + 64 * (Q + q) + lane;
-            nb[q] = idx < F.nwin ? ld_bits(part + idx) : Sentinel<T>::bits;
+            const int64_t idx = base + 64 * q + lane;
+            b[q] = idx < F.nwin ? ld_bits(part + idx) : Sentinel<T>::bits;
         }
-        int n = 0;  // filled prefix of this block (stops at the first sentinel)
+    };
+    poll(b0, cur);
+    poll(b1, cur + BLK);
+    poll(b2, cur + 2 * BLK);
+    while (cur < F.nwin) {
+        int n = 0;  // filled prefix of block b0 (stops at the first sentinel)
         bool open = true;
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             const int64_t idx0 = cur + 64 * q;
-            const uint64_t ok = __ballot(b[q] != Sentinel<T>::bits || idx0 + lane >= F.nwin);
-            stage[64 * q + lane] = from_bits<T>(b[q]);
+            const uint64_t ok = __ballot(b0[q] != Sentinel<T>::bits || idx0 + lane >= F.nwin);
+            stage[64 * q + lane] = from_bits<T>(b0[q]);
             if (open) {
                 const int m = ~ok == 0 ? 64 : __builtin_ctzll(~ok);
                 const int valid = (int)min<int64_t>(64, F.nwin - idx0 > 0 ? F.nwin - idx0 : 0);
@@ -203,39 +203,51 @@ template <typename T> __device__ void pw_folder(const RedArgs &A, const PwFastSt
             }
         }
         __builtin_amdgcn_wave_barrier();
-        // the serial chain runs in lane 0 from LDS, 16 operands in flight
         if (lane == 0) {
+            // double-buffered: the next 16 operands are read while the
+            // current 16 are added
             int i = 0;
-            for (; i + 16 <= n; i += 16) {
-                T t[16];
+            T t[16], u[16];
+            if (n >= 16) {
 #pragma unroll
-                for (int k = 0; k < 16; k++) t[k] = stage[i + k];
+                for (int k = 0; k < 16; k++) t[k] = stage[k];
+                for (i = 16; i + 16 <= n; i += 16) {
+#pragma unroll
+                    for (int k = 0; k < 16; k++) u[k] = stage[i + k];
+#pragma unroll
+                    for (int k = 0; k < 16; k++) acc = acc + t[k];
+#pragma unroll
+                    for (int k = 0; k < 16; k++) t[k] = u[k];
+                }
 #pragma unroll
                 for (int k = 0; k < 16; k++) acc = acc + t[k];
             }
             for (; i < n; i++) acc = acc + stage[i];
         }
         __builtin_amdgcn_wave_barrier();
-        if (n == 0) {
-            if (++idle > (1u << 26)) {  // producers always fill every slot; safety bound only
-                failed = true;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        } else {
-            idle = 0;
-        }
-        if (n == 64 * Q) {
-            cur += n;
-#pragma unroll
-            for (int q = 0; q < Q; q++) b[q] = nb[q];
-        } else {
-            cur += n;  // re-read from the first unfilled slot
+        cur += n;
+        if (n == BLK) {
 #pragma unroll
             for (int q = 0; q < Q; q++) {
-                const int64_t idx = cur + 64 * q + lane;
-                b[q] = idx < F.nwin ? ld_bits(part + idx) : Sentinel<T>::bits;
+                b0[q] = b1[q];
+                b1[q] = b2[q];
             }
+            poll(b2, cur + 2 * BLK);
+            idle = 0;
+        } else {
+            if (n == 0) {
+                if (++idle > (1u << 26)) {  // producers always fill every slot; safety bound only
+                    failed = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            } else {
+                idle = 0;
+            }
+            // re-read from the first unfilled slot
+            poll(b0, cur);
+            poll(b1, cur + BLK);
+            poll(b2, cur + 2 * BLK);
         }
     }
     if (lane == 0) {
@@ -904,7 +916,10 @@ template <typename T> static ionp_status run_pw_fast(RedArgs &A, const ionp_conf
     if (has_tail)  // the last, partial window of each output (generic tree)
         hipLaunchKernelGGL(pw_windows<T>, dim3(grid_cap(A.nout, bpc)), dim3(kPwThreads), 0, s, A, A.nout, 0, F.nwin);
     const int64_t nwindows = A.nout * F.nfull;
-    hipLaunchKernelGGL(pw_fast<T>, dim3(F.folders + grid_cap(nwindows, bpc)), dim3(pw_fast_threads<T>()), 0, s, A, F,
+    // a folder shares its CU with producers: with one producer workgroup per
+    // CU the serial fold keeps pace with HBM (measured on MI355X)
+    const int pbpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : (F.folders ? 1 : 8);
+    hipLaunchKernelGGL(pw_fast<T>, dim3(F.folders + grid_cap(nwindows, pbpc)), dim3(pw_fast_threads<T>()), 0, s, A, F,
                        nwindows);
     if (!F.folders) hipLaunchKernelGGL(pw_fold<T>, dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
     return launched();
