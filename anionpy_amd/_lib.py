@@ -81,6 +81,7 @@ SIGNATURES = {
     "ionp_batch_run": [_V, _I, _V],
     "ionp_batch_info": [_V, _P(_L), _P(_L)],
     "ionp_batch_destroy": [_V],
+    "ionp_autotune": [_P(Call), _I, _C, _I, _I, _F, _V],
     "ionp_call_features": [_P(Call), _F],
     "ionp_mlp_linear_fwd": [_V, _V, _V, _V, _I, _I, _I, _I, _V],
     "ionp_mlp_linear_bwd": [_V, _V, _V, _V, _V, _V, _V, _I, _I, _I, _I, _V],

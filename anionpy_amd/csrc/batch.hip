@@ -8,8 +8,8 @@
 //                indices from an atomic counter, find the owning call by
 //                binary search over the tile prefix, and run the tile with
 //                the same device code as the single-call kernels.  Reductions
-//                whose outputs span several tiles write partials; the tile
-//                that completes a call last folds them in NumPy's order.
+//                whose outputs span several tiles write partials, which a
+//                second small launch folds in NumPy's order.
 //   GRAPH      - the single-call launch sequence of every call, captured
 //                once into a hipGraph and replayed.
 //   PER_CALL   - the single-call entry points, one call after another.
@@ -50,7 +50,9 @@ struct ionp_batch {
     int64_t ntiles, bytes, part_bytes;
     ionp::BatchCall *d_calls;   // device call table
     int64_t *d_tile_begin;      // device: first tile of each call (prefix), n + 1 entries
-    uint32_t *d_counters;       // device: [0] tile counter, [1 + i] tiles done per call
+    uint32_t *d_counters;       // device: [0] tile counter
+    int32_t *d_fold_list;       // device: calls whose partials need a fold
+    int32_t nfold;
     char *d_partials;           // device: reduction partials
     std::vector<ionp_call> calls;  // host copy for GRAPH / PER_CALL modes
     hipGraphExec_t graph;
@@ -218,7 +220,6 @@ __global__ __launch_bounds__(kBatchThreads) void batch_persistent(const BatchCal
     __shared__ uint64_t ired[kBatchThreads];
     __shared__ int64_t sh_tile;
     __shared__ int32_t sh_call;
-    __shared__ int32_t sh_last;
     for (;;) {
         if (threadIdx.x == 0) {
             const int64_t t = (int64_t)atomicAdd(&counters[0], 1u);
@@ -241,13 +242,11 @@ __global__ __launch_bounds__(kBatchThreads) void batch_persistent(const BatchCal
         if (c < 0) break;  // every tile has been handed out
         const BatchCall &C = calls[c];
         const int64_t lt = t - tile_begin[c];
-        bool folds = false;
         switch (C.kind) {
             case BK_ELEMENTWISE: ew_tile(C.ew, lt); break;
             case BK_FADD_WIDE:
                 if (C.red.out_dt == IONP_FLOAT64) fadd_window<double>(C, partials, lt, S);
                 else fadd_window<float>(C, partials, lt, S);
-                folds = !C.direct;
                 break;
             case BK_FADD_NARROW:
                 if (C.red.out_dt == IONP_FLOAT64) fadd_narrow<double>(C, lt);
@@ -258,32 +257,28 @@ __global__ __launch_bounds__(kBatchThreads) void batch_persistent(const BatchCal
                     case IONP_INT64: iadd_tile<IONP_INT64>(C, partials, lt, ired); break;
                     default: iadd_tile<IONP_UINT64>(C, partials, lt, ired); break;
                 }
-                folds = !C.direct;
                 break;
             default: fill_zero_tile(C, lt); break;
         }
-        if (folds) {
-            // the workgroup that finishes a call's last tile folds its partials
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                __threadfence();
-                const uint32_t prev = atomicAdd(&counters[1 + c], 1u);
-                sh_last = prev + 1 == (uint32_t)C.tiles;
-                __threadfence();
-            }
-            __syncthreads();
-            if (sh_last) {
-                if (C.kind == BK_FADD_WIDE) {
-                    if (C.red.out_dt == IONP_FLOAT64) fadd_fold<double>(C, partials);
-                    else fadd_fold<float>(C, partials);
-                } else if (C.red.out_dt == IONP_INT64) {
-                    iadd_fold<IONP_INT64>(C, partials);
-                } else {
-                    iadd_fold<IONP_UINT64>(C, partials);
-                }
-            }
-        }
         __syncthreads();
+    }
+}
+
+// Second launch: calls whose outputs span several tiles fold their partials
+// (float: acc = 0; acc = acc + partial in window order; integer: any order).
+// Stream order makes every partial of the first launch visible.
+__global__ __launch_bounds__(kBatchThreads) void batch_fold(const BatchCall *calls, const int32_t *fold_list,
+                                                            int32_t nfold, const char *partials) {
+    for (int32_t f = blockIdx.x; f < nfold; f += gridDim.x) {
+        const BatchCall &C = calls[fold_list[f]];
+        if (C.kind == BK_FADD_WIDE) {
+            if (C.red.out_dt == IONP_FLOAT64) fadd_fold<double>(C, partials);
+            else fadd_fold<float>(C, partials);
+        } else if (C.red.out_dt == IONP_INT64) {
+            iadd_fold<IONP_INT64>(C, partials);
+        } else {
+            iadd_fold<IONP_UINT64>(C, partials);
+        }
     }
 }
 
@@ -392,12 +387,12 @@ ionp_status plan_call(const ionp_call &c, BatchCall *B, int64_t *bytes) {
     return IONP_OK;
 }
 
-static ionp_status run_one(const ionp_call &c, hipStream_t s) {
+static ionp_status run_one(const ionp_call &c, hipStream_t s, const ionp_config *cfg = nullptr) {
     switch (c.kind) {
-        case IONP_CALL_BINARY: return ionp_binary(c.op, &c.in[0], &c.in[1], &c.out, nullptr, s);
-        case IONP_CALL_WHERE: return ionp_where(&c.in[0], &c.in[1], &c.in[2], &c.out, nullptr, s);
-        case IONP_CALL_CAST: return ionp_cast(&c.in[0], &c.out, nullptr, s);
-        default: return ionp_reduce(c.op, &c.in[0], c.axes, c.naxes, c.keepdims, &c.out, nullptr, s);
+        case IONP_CALL_BINARY: return ionp_binary(c.op, &c.in[0], &c.in[1], &c.out, cfg, s);
+        case IONP_CALL_WHERE: return ionp_where(&c.in[0], &c.in[1], &c.in[2], &c.out, cfg, s);
+        case IONP_CALL_CAST: return ionp_cast(&c.in[0], &c.out, cfg, s);
+        default: return ionp_reduce(c.op, &c.in[0], c.axes, c.naxes, c.keepdims, &c.out, cfg, s);
     }
 }
 
@@ -412,6 +407,7 @@ ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, i
     *out = nullptr;
     std::vector<BatchCall> table(n > 0 ? n : 1);
     std::vector<int64_t> begin(n + 1, 0);
+    std::vector<int32_t> fold_list;
     int64_t bytes = 0, part = 0;
     size_t scratch_need = 0;
     for (int32_t i = 0; i < n; i++) {
@@ -425,6 +421,7 @@ ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, i
             B.part_off = part;
             part += ((B.tiles * 8) + 255) / 256 * 256;
             scratch_need = std::max(scratch_need, (size_t)(B.tiles * 16));
+            fold_list.push_back(i);
         }
         begin[i + 1] = begin[i] + B.tiles;
         bytes += b;
@@ -438,12 +435,16 @@ ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, i
     bt->graph_ready = false;
     bt->scratch_need = scratch_need;
     hipStream_t s = (hipStream_t)stream;
+    bt->nfold = (int32_t)fold_list.size();
     bool ok = hipMalloc(&bt->d_calls, sizeof(BatchCall) * table.size()) == hipSuccess &&
               hipMalloc(&bt->d_tile_begin, sizeof(int64_t) * (n + 1)) == hipSuccess &&
-              hipMalloc(&bt->d_counters, sizeof(uint32_t) * (n + 1)) == hipSuccess &&
+              hipMalloc(&bt->d_counters, sizeof(uint32_t) * 64) == hipSuccess &&
+              hipMalloc(&bt->d_fold_list, sizeof(int32_t) * (fold_list.size() + 1)) == hipSuccess &&
               hipMalloc(&bt->d_partials, part > 0 ? part : 256) == hipSuccess;
     ok = ok && hipMemcpyAsync(bt->d_calls, table.data(), sizeof(BatchCall) * table.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
          hipMemcpyAsync(bt->d_tile_begin, begin.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
+         (fold_list.empty() || hipMemcpyAsync(bt->d_fold_list, fold_list.data(), sizeof(int32_t) * fold_list.size(),
+                                              hipMemcpyHostToDevice, s) == hipSuccess) &&
          hipStreamSynchronize(s) == hipSuccess;
     if (!ok) {
         ionp_batch_destroy(bt);
@@ -465,18 +466,32 @@ ionp_status ionp_batch_run(ionp_batch *bt, int32_t mode, void *stream) {
     }
     if (mode == IONP_BATCH_GRAPH) {
         if (!bt->graph_ready) {
-            // scratch must exist before capture: nothing may allocate inside it
-            if (bt->scratch_need && !scratch_get(bt->scratch_need, s))
-                return set_error(IONP_ERR_DEVICE, "out of device memory for reduction scratch");
+            // one eager pass first: it sizes the library scratch, so nothing
+            // allocates inside the capture
+            for (const ionp_call &c : bt->calls) {
+                ionp_status st0 = run_one(c, s);
+                if (st0) return st0;
+            }
             (void)num_cus();
-            hipGraph_t g;
-            if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            // capture on a private stream (the legacy default stream can not be
+            // captured); the graph is then launched on the caller's stream
+            hipStream_t cs;
+            if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+                return set_error(IONP_ERR_DEVICE, "hipStreamCreate failed");
+            if (hipStreamSynchronize(s) != hipSuccess || hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                (void)hipStreamDestroy(cs);
                 return set_error(IONP_ERR_DEVICE, "hipStreamBeginCapture failed");
+            }
             ionp_status st = IONP_OK;
             for (const ionp_call &c : bt->calls)
-                if (!st) st = run_one(c, s);
-            const hipError_t e = hipStreamEndCapture(s, &g);
-            if (st) return st;
+                if (!st) st = run_one(c, cs);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(cs, &g);
+            (void)hipStreamDestroy(cs);
+            if (st) {
+                if (g) (void)hipGraphDestroy(g);
+                return st;
+            }
             if (e != hipSuccess) return set_error(IONP_ERR_DEVICE, "hipStreamEndCapture failed");
             if (hipGraphInstantiate(&bt->graph, g, nullptr, nullptr, 0) != hipSuccess) {
                 (void)hipGraphDestroy(g);
@@ -489,13 +504,15 @@ ionp_status ionp_batch_run(ionp_batch *bt, int32_t mode, void *stream) {
     }
     if (mode != IONP_BATCH_PERSISTENT) return set_error(IONP_ERR_VALUE, "unknown batch mode");
     if (bt->n == 0 || bt->ntiles == 0) return IONP_OK;
-    hipLaunchKernelGGL(batch_zero, dim3(std::max<int64_t>(1, std::min<int64_t>(1024, (bt->n + 256) / 256))), dim3(256), 0, s,
-                       bt->d_counters, (int64_t)bt->n + 1);
+    hipLaunchKernelGGL(batch_zero, dim3(1), dim3(64), 0, s, bt->d_counters, (int64_t)64);
     // persistent grid: a few workgroups per CU, never more than the tiles
     int64_t grid = (int64_t)num_cus() * 4;
     if (grid > bt->ntiles) grid = bt->ntiles;
     hipLaunchKernelGGL(batch_persistent, dim3((unsigned)grid), dim3(kBatchThreads), 0, s, bt->d_calls, bt->d_tile_begin,
                        bt->n, bt->ntiles, bt->d_counters, bt->d_partials);
+    if (bt->nfold)
+        hipLaunchKernelGGL(batch_fold, dim3((unsigned)std::min<int32_t>(bt->nfold, num_cus() * 4)), dim3(kBatchThreads), 0, s,
+                           bt->d_calls, bt->d_fold_list, bt->nfold, bt->d_partials);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, hipGetErrorString(e));
 }
@@ -507,12 +524,51 @@ ionp_status ionp_batch_info(const ionp_batch *bt, int64_t *ntiles, int64_t *byte
     return IONP_OK;
 }
 
+ionp_status ionp_autotune(const ionp_call *calls, int32_t n, const ionp_config *cfgs, int32_t ncfg, int32_t reps,
+                          float *times_us, void *stream) {
+    if (n < 0 || ncfg <= 0 || reps <= 0 || (n > 0 && (!calls || !cfgs || !times_us)))
+        return set_error(IONP_ERR_VALUE, "bad autotune arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const int nev = ncfg * reps;
+    std::vector<hipEvent_t> ev(2 * (size_t)nev);
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return set_error(IONP_ERR_DEVICE, "hipEventCreate failed");
+    ionp_status st = IONP_OK;
+    for (int32_t i = 0; i < n && !st; i++) {
+        // warm the call (first-touch, scratch growth) with the default configuration
+        if ((st = run_one(calls[i], s))) break;
+        std::vector<char> usable(ncfg, 1);
+        for (int32_t j = 0; j < ncfg; j++)
+            for (int32_t r = 0; r < reps; r++) {
+                (void)hipEventRecord(ev[2 * (j * reps + r)], s);
+                if (run_one(calls[i], s, &cfgs[j]) != IONP_OK) usable[j] = 0;
+                (void)hipEventRecord(ev[2 * (j * reps + r) + 1], s);
+            }
+        if (hipStreamSynchronize(s) != hipSuccess) {
+            st = set_error(IONP_ERR_DEVICE, "autotune run failed");
+            break;
+        }
+        for (int32_t j = 0; j < ncfg; j++) {
+            float best = -1.0f;
+            for (int32_t r = 0; r < reps && usable[j]; r++) {
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, ev[2 * (j * reps + r)], ev[2 * (j * reps + r) + 1]);
+                if (best < 0.0f || ms * 1000.0f < best) best = ms * 1000.0f;
+            }
+            times_us[(size_t)i * ncfg + j] = best;
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return st;
+}
+
 void ionp_batch_destroy(ionp_batch *bt) {
     if (!bt) return;
     if (bt->graph_ready) (void)hipGraphExecDestroy(bt->graph);
     if (bt->d_calls) (void)hipFree(bt->d_calls);
     if (bt->d_tile_begin) (void)hipFree(bt->d_tile_begin);
     if (bt->d_counters) (void)hipFree(bt->d_counters);
+    if (bt->d_fold_list) (void)hipFree(bt->d_fold_list);
     if (bt->d_partials) (void)hipFree(bt->d_partials);
     delete bt;
 }

@@ -213,6 +213,16 @@ ionp_status ionp_batch_run(ionp_batch *batch, int32_t mode, void *stream);
 ionp_status ionp_batch_info(const ionp_batch *batch, int64_t *ntiles, int64_t *bytes);
 void ionp_batch_destroy(ionp_batch *batch);
 
+/* ---- autotuning (device) ----------------------------------------------- */
+/* No counterpart in the reference (its CPU loops have no launch knobs). */
+
+/* Times every call under every configuration with HIP events on `stream`:
+ * times_us[i * ncfg + j] is the fastest of `reps` runs of call i with
+ * cfgs[j] (a configuration a call cannot use is reported as a negative
+ * time).  The calls' outputs are overwritten. */
+ionp_status ionp_autotune(const ionp_call *calls, int32_t n, const ionp_config *cfgs, int32_t ncfg, int32_t reps,
+                          float *times_us, void *stream);
+
 /* ---- layout features for the cost model (host only) ------------------ */
 /* No counterpart in the reference. */
 
@@ -234,8 +244,9 @@ ionp_status ionp_mlp_linear_bwd(const float *x, const float *w, const float *y, 
                                 float *dx, float *dw, float *db, int32_t n, int32_t in, int32_t out,
                                 int32_t relu, void *stream);
 /* Softmax cross-entropy over `heads` groups of logits: logits[n, total],
- * head h occupies [off[h], off[h+1]); labels[n, heads]; writes dlogits and
- * the summed loss (one float on device). */
+ * head h occupies [off[h], off[h+1]) (off and labels[n, heads] in device
+ * memory); writes dlogits (gradient of the mean loss) and loss, a device
+ * buffer of n + 1 floats: loss[0] = mean loss, loss[1..n] per sample. */
 ionp_status ionp_mlp_softmax_xent(const float *logits, const int32_t *labels, const int32_t *head_off,
                                   int32_t heads, int32_t n, int32_t total, float *dlogits, float *loss,
                                   void *stream);
