@@ -114,6 +114,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_extra:
         extra.update(reduction_extras(ap, torch))
+        extra["batch"] = batch_extras(ap, torch)
+        extra["learned_config"] = learned_extras()
 
     cpu = None if args.no_cpu else cpu_baseline()
     line = {"metric": "HBM GB/s (% of peak) ufunc+reduce, bit-exact; batched vs per-call calls/s",
@@ -166,6 +168,51 @@ def reduction_extras(ap, torch):
     out["sum_i32_2^28"] = {"us": round(t * 1e6, 1), "GB/s": round(nb / t / 1e9, 1), "frac": round(nb / t / PEAK_HBM, 4)}
     torch.cuda.empty_cache()
     return out
+
+
+def batch_extras(ap, torch):
+    """The uneven batch: 10,000 calls (seed 5), one persistent launch vs a
+    captured hipGraph vs per-call launches; outputs compared bytewise."""
+    from anionpy_amd.batch import Batch
+    from anionpy_amd.layouts import uneven_batch
+    log("uneven batch")
+    specs = uneven_batch(seed=5, n=10000)
+    t0 = time.perf_counter()
+    bt = Batch(specs)
+    create_s = time.perf_counter() - t0
+    info = bt.info()
+    res = {"calls": len(specs), "tiles": info["tiles"], "alg_bytes": info["bytes"],
+           "create_ms": round(create_s * 1e3, 2)}
+    outs = {}
+    for mode in ("per_call", "graph", "persistent"):
+        bt.run(mode)  # warm (the graph is captured here)
+        torch.cuda.synchronize()
+        reps = 3 if mode == "per_call" else 10
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            bt.run(mode)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        res[mode] = {"ms": round(dt * 1e3, 3), "calls_per_s": round(len(specs) / dt, 1),
+                     "GB/s": round(info["bytes"] / dt / 1e9, 1)}
+        outs[mode] = [s.out._buf.clone() for s in specs]
+    same = all(torch.equal(a, b) and torch.equal(a, c)
+               for a, b, c in zip(outs["per_call"], outs["graph"], outs["persistent"]))
+    res["identical_outputs"] = bool(same)
+    res["speedup_persistent_vs_per_call"] = round(res["per_call"]["ms"] / res["persistent"]["ms"], 2)
+    res["speedup_graph_vs_per_call"] = round(res["per_call"]["ms"] / res["graph"]["ms"], 2)
+    bt.close()
+    del specs, outs
+    torch.cuda.empty_cache()
+    return res
+
+
+def learned_extras():
+    path = os.path.join(HERE, "anionpy_amd", "costmodel.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("meta")
 
 
 def cpu_baseline():
