@@ -22,6 +22,7 @@ from .autotune import GRID, HEADS, config_index, knob_labels
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "costmodel.json")
+SWEEP_DATA = os.path.join(HERE, "costmodel_sweep.npz")
 HEAD_SIZES = [len(v) for v in HEADS.values()]
 HEAD_OFF = np.concatenate([[0], np.cumsum(HEAD_SIZES)]).astype(np.int32)
 NOUT = int(HEAD_OFF[-1])

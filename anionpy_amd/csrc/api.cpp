@@ -145,14 +145,15 @@ static ionp_status reduce_dtype(int32_t op, int in_dt, int32_t dtype, int *out_d
         return set_error(IONP_ERR_VALUE, "reduce is only provided for add, maximum and minimum");
     int dt = dtype < 0 ? reduce_default_dtype(op, in_dt) : dtype;
     if (dt < 0 || dt >= IONP_NTYPES) return set_error(IONP_ERR_TYPE, "invalid dtype code");
-    if (op != IONP_ADD && dt != in_dt)
-        return set_error(IONP_ERR_UNSUPPORTED, "maximum/minimum.reduce with a dtype other than the input's");
+    // integer maximum/minimum cast every element to the loop dtype first;
+    // the float x86 loop model covers the input's own dtype only
+    if (op != IONP_ADD && dt != in_dt && (is_float_dt(dt) || is_float_dt(in_dt)))
+        return set_error(IONP_ERR_UNSUPPORTED, "float maximum/minimum.reduce with a dtype other than the input's");
     // same_kind casting of the input to the loop dtype (a float can not feed an integer loop)
     if (is_float_dt(in_dt) && !is_float_dt(dt))
         return set_error(IONP_ERR_TYPE, std::string("Cannot cast ufunc '") + op_name(op) + "' input from dtype('" +
                                             dtype_name(in_dt) + "') to dtype('" + dtype_name(dt) +
                                             "') with casting rule 'same_kind'");
-    if (dt == IONP_BOOL && op == IONP_ADD) dt = IONP_BOOL;
     *out_dt = dt;
     return IONP_OK;
 }

@@ -14,6 +14,8 @@
 //                once into a hipGraph and replayed.
 //   PER_CALL   - the single-call entry points, one call after another.
 // All three produce bit-identical outputs.
+#include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "device_ops.h"
@@ -24,12 +26,15 @@ enum : int32_t {
     BK_ELEMENTWISE = 0,  // binary / where / cast
     BK_FADD_WIDE = 1,    // float add.reduce, windows over W
     BK_FADD_NARROW = 2,  // float add.reduce, W empty
-    BK_IADD = 3,         // integer / bool add.reduce (order-free)
+    BK_IADD = 3,         // integer / bool add.reduce (order-free), segments of a group
     BK_FILL_ZERO = 4,    // empty reduction groups
+    BK_FADD_SMALL = 5,   // float add.reduce, windows of <= 128: one thread per output
+    BK_IADD_SMALL = 6,   // integer / bool add.reduce: one thread per output
+    BK_FADD_NARROW_WAVE = 7,  // float add.reduce, W empty, long chains: one wavefront per output
 };
 
 constexpr int kBatchThreads = 256;
-constexpr int64_t kEwTile = kBatchThreads * 16;  // elements per elementwise tile
+constexpr int64_t kEwTile = kBatchThreads * 64;  // elements per elementwise tile
 constexpr int64_t kIntSeg = 8192;                // elements per integer-reduce tile
 constexpr int64_t kNarrowOutputs = kBatchThreads;  // outputs per narrow tile
 
@@ -49,7 +54,7 @@ struct ionp_batch {
     int32_t n;
     int64_t ntiles, bytes, part_bytes;
     ionp::BatchCall *d_calls;   // device call table
-    int64_t *d_tile_begin;      // device: first tile of each call (prefix), n + 1 entries
+    int32_t *d_tile_map;        // device: per tile, (owning call, tile index within the call)
     uint32_t *d_counters;       // device: [0] tile counter
     int32_t *d_fold_list;       // device: calls whose partials need a fold
     int32_t nfold;
@@ -66,44 +71,156 @@ __global__ void batch_zero(uint32_t *p, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
 }
 
-template <int LT> __device__ inline void ew_range(const EwArgs &A, uint32_t e0, uint32_t e1) {
-    const int i0 = itemsize(A.in_dt[0]), i1 = itemsize(A.in_dt[1]), i2 = itemsize(A.in_dt[2]);
-    const int io = itemsize(A.out_dt);
-    for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-        uint32_t rem = e;
-        int64_t o0 = 0, o1 = 0, o2 = 0, oo = 0;
-        for (int d = 0; d < A.ndim; d++) {
-            uint32_t c;
-            if (d + 1 < A.ndim) {
-                const uint32_t q = A.div[d].div(rem);
-                c = rem - q * (uint32_t)A.shape[d];
-                rem = q;
-            } else {
-                c = rem;
+// K loads of one operand (dtype dt, converted to the loop dtype LT), issued
+// back to back: the dtype switch sits outside the loads.
+template <int LT, int K>
+__device__ inline void load_k(int dt, const char *p, const int64_t (&off)[K], typename dt_type<LT>::T (&v)[K]) {
+    in_type_dispatch(dt, [&](auto it) {
+        using IT = decltype(it);
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = ld_t<LT, IT>(p, off[k]);
+        return 0;
+    });
+}
+
+template <typename T, int K>
+__device__ inline void store_k(int dt, char *p, const int64_t (&off)[K], const T (&v)[K], const bool (&ok)[K]) {
+    auto st = [&](auto ot, auto cv) {
+        using OT = decltype(ot);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (ok[k]) ((OT *)p)[off[k]] = cv(v[k]);
+        return 0;
+    };
+    switch (dt) {
+        case IONP_BOOL: st(uint8_t(), [](T x) { return cvt<IONP_BOOL>(x); }); break;
+        case IONP_INT8: st(int8_t(), [](T x) { return cvt<IONP_INT8>(x); }); break;
+        case IONP_UINT8: st(uint8_t(), [](T x) { return cvt<IONP_UINT8>(x); }); break;
+        case IONP_INT16: st(int16_t(), [](T x) { return cvt<IONP_INT16>(x); }); break;
+        case IONP_UINT16: st(uint16_t(), [](T x) { return cvt<IONP_UINT16>(x); }); break;
+        case IONP_INT32: st(int32_t(), [](T x) { return cvt<IONP_INT32>(x); }); break;
+        case IONP_UINT32: st(uint32_t(), [](T x) { return cvt<IONP_UINT32>(x); }); break;
+        case IONP_INT64: st(int64_t(), [](T x) { return cvt<IONP_INT64>(x); }); break;
+        case IONP_UINT64: st(uint64_t(), [](T x) { return cvt<IONP_UINT64>(x); }); break;
+        case IONP_FLOAT32: st(float(), [](T x) { return cvt<IONP_FLOAT32>(x); }); break;
+        default: st(double(), [](T x) { return cvt<IONP_FLOAT64>(x); }); break;
+    }
+}
+
+// Elements [e0, e1) of an elementwise call.  Thread t takes elements
+// e0 + t + j * blockDim.x; K of them at a time have their offsets computed,
+// their loads issued together, then their results stored (K loads per
+// operand in flight instead of one).  A slot past e1 re-reads element e0 and
+// stores nothing.
+//
+// ND > 0: the call has exactly ND dims, so the index loop unrolls and the
+// call's shape / stride fields stay in scalar registers for the whole tile.
+template <int LT, int ND> __device__ void ew_range(const EwArgs &A, uint32_t e0, uint32_t e1) {
+    using T = typename dt_type<LT>::T;
+    constexpr int K = 4;
+    const int nin = A.nin;
+    const int nd = ND > 0 ? ND : A.ndim;
+    for (uint32_t base = e0 + threadIdx.x; base < e1; base += K * blockDim.x) {
+        int64_t off[3][K], oo[K];
+        bool ok[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            uint32_t e = base + k * blockDim.x;
+            ok[k] = e < e1;
+            uint32_t rem = ok[k] ? e : e0;
+            int64_t o0 = 0, o1 = 0, o2 = 0, ou = 0;
+#pragma unroll
+            for (int d = 0; d < (ND > 0 ? ND : kMaxKDims); d++) {
+                if (ND == 0 && d >= nd) break;
+                uint32_t c;
+                if (d + 1 < nd) {
+                    const uint32_t q = A.div[d].div(rem);
+                    c = rem - q * (uint32_t)A.shape[d];
+                    rem = q;
+                } else {
+                    c = rem;
+                }
+                o0 += (int64_t)c * A.in_st[0][d];
+                if (nin > 1) o1 += (int64_t)c * A.in_st[1][d];
+                if (nin > 2) o2 += (int64_t)c * A.in_st[2][d];
+                ou += (int64_t)c * A.out_st[d];
             }
-            o0 += (int64_t)c * A.in_st[0][d];
-            o1 += (int64_t)c * A.in_st[1][d];
-            o2 += (int64_t)c * A.in_st[2][d];
-            oo += (int64_t)c * A.out_st[d];
+            off[0][k] = o0;
+            off[1][k] = o1;
+            off[2][k] = o2;
+            oo[k] = ou;
         }
-        ew_element<LT>(A, A.in[0] + o0 * i0, A.in[1] + o1 * i1, A.in[2] + o2 * i2, A.out + oo * io);
+        T r[K];
+        if (A.op == OP_CAST) {
+            load_k<LT, K>(A.in_dt[0], A.in[0], off[0], r);
+        } else if (A.op == OP_WHERE) {
+            uint8_t c[K];
+            T x[K], y[K];
+            load_k<IONP_BOOL, K>(A.in_dt[0], A.in[0], off[0], c);
+            load_k<LT, K>(A.in_dt[1], A.in[1], off[1], x);
+            load_k<LT, K>(A.in_dt[2], A.in[2], off[2], y);
+#pragma unroll
+            for (int k = 0; k < K; k++) r[k] = c[k] ? x[k] : y[k];
+        } else if (A.op >= IONP_LESS) {
+            uint8_t cr[K];
+            if (A.mixed_su) {
+                const bool first = A.in_dt[0] == IONP_INT64;
+                int64_t sv[K], so[K], uo[K];
+                uint64_t uv[K];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    so[k] = first ? off[0][k] : off[1][k];
+                    uo[k] = first ? off[1][k] : off[0][k];
+                }
+                load_k<IONP_INT64, K>(IONP_INT64, A.in[first ? 0 : 1], so, sv);
+                load_k<IONP_UINT64, K>(IONP_UINT64, A.in[first ? 1 : 0], uo, uv);
+                const int op = first ? A.op : swap_cmp(A.op);
+#pragma unroll
+                for (int k = 0; k < K; k++) cr[k] = (uint8_t)cmp_su(op, sv[k], uv[k]);
+            } else {
+                T a[K], b[K];
+                load_k<LT, K>(A.in_dt[0], A.in[0], off[0], a);
+                load_k<LT, K>(A.in_dt[1], A.in[1], off[1], b);
+#pragma unroll
+                for (int k = 0; k < K; k++) cr[k] = (uint8_t)cmp_op<T>(A.op, a[k], b[k]);
+            }
+            store_k<uint8_t, K>(A.out_dt, A.out, oo, cr, ok);
+            continue;
+        } else {
+            T a[K], b[K];
+            load_k<LT, K>(A.in_dt[0], A.in[0], off[0], a);
+            load_k<LT, K>(A.in_dt[1], A.in[1], off[1], b);
+#pragma unroll
+            for (int k = 0; k < K; k++) r[k] = arith_op<LT>(A.op, a[k], b[k]);
+        }
+        store_k<T, K>(A.out_dt, A.out, oo, r, ok);
+    }
+}
+
+template <int LT> __device__ inline void ew_range_nd(const EwArgs &A, uint32_t e0, uint32_t e1) {
+    switch (A.ndim) {
+        case 1: ew_range<LT, 1>(A, e0, e1); break;
+        case 2: ew_range<LT, 2>(A, e0, e1); break;
+        case 3: ew_range<LT, 3>(A, e0, e1); break;
+        case 4: ew_range<LT, 4>(A, e0, e1); break;
+        default: ew_range<LT, 0>(A, e0, e1); break;
     }
 }
 
 __device__ void ew_tile(const EwArgs &A, int64_t t) {
-    const int64_t e0 = t * kEwTile, e1 = min(A.total, e0 + kEwTile);
-    switch (A.loop_dt) {
-        case IONP_BOOL: ew_range<IONP_BOOL>(A, (uint32_t)e0, (uint32_t)e1); break;
-        case IONP_INT8: ew_range<IONP_INT8>(A, (uint32_t)e0, (uint32_t)e1); break;
-        case IONP_UINT8: ew_range<IONP_UINT8>(A, (uint32_t)e0, (uint32_t)e1); break;
-        case IONP_INT16: ew_range<IONP_INT16>(A, (uint32_t)e0, (uint32_t)e1); break;
-        case IONP_UINT16: ew_range<IONP_UINT16>(A, (uint32_t)e0, (uint32_t)e1); break;
-        case IONP_INT32: ew_range<IONP_INT32>(A, (uint32_t)e0, (uint32_t)e1); break;
-        case IONP_UINT32: ew_range<IONP_UINT32>(A, (uint32_t)e0, (uint32_t)e1); break;
-        case IONP_INT64: ew_range<IONP_INT64>(A, (uint32_t)e0, (uint32_t)e1); break;
-        case IONP_UINT64: ew_range<IONP_UINT64>(A, (uint32_t)e0, (uint32_t)e1); break;
-        case IONP_FLOAT32: ew_range<IONP_FLOAT32>(A, (uint32_t)e0, (uint32_t)e1); break;
-        default: ew_range<IONP_FLOAT64>(A, (uint32_t)e0, (uint32_t)e1); break;
+    const uint32_t e0 = (uint32_t)(t * kEwTile), e1 = (uint32_t)min(A.total, t * kEwTile + kEwTile);
+    switch (A.loop_dt) {  // the common loop dtypes get rank-specialised index code
+        case IONP_FLOAT64: ew_range_nd<IONP_FLOAT64>(A, e0, e1); break;
+        case IONP_FLOAT32: ew_range_nd<IONP_FLOAT32>(A, e0, e1); break;
+        case IONP_INT64: ew_range_nd<IONP_INT64>(A, e0, e1); break;
+        case IONP_BOOL: ew_range<IONP_BOOL, 0>(A, e0, e1); break;
+        case IONP_INT8: ew_range<IONP_INT8, 0>(A, e0, e1); break;
+        case IONP_UINT8: ew_range<IONP_UINT8, 0>(A, e0, e1); break;
+        case IONP_INT16: ew_range<IONP_INT16, 0>(A, e0, e1); break;
+        case IONP_UINT16: ew_range<IONP_UINT16, 0>(A, e0, e1); break;
+        case IONP_INT32: ew_range<IONP_INT32, 0>(A, e0, e1); break;
+        case IONP_UINT32: ew_range<IONP_UINT32, 0>(A, e0, e1); break;
+        default: ew_range<IONP_UINT64, 0>(A, e0, e1); break;
     }
 }
 
@@ -148,14 +265,37 @@ template <typename T> __device__ void fadd_fold(const BatchCall &C, const char *
 }
 
 template <typename T> __device__ void fadd_narrow(const BatchCall &C, int64_t t) {
+    const RedArgs &A = C.red;
+    const int64_t o = t * kNarrowOutputs + threadIdx.x;
+    if (o >= A.nout) return;
+    int64_t ib, ob;
+    out_offsets(A, (uint32_t)o, &ib, &ob);
+    ((T *)A.out)[ob] = narrow_sum<T, 8>(A, ib);  // few registers: the persistent kernel runs every tile kind
+}
+
+template <typename T> __device__ void fadd_narrow_wave(const BatchCall &C, int64_t t) {
+    const RedArgs &A = C.red;
+    const int64_t o = t * (kBatchThreads / 64) + (threadIdx.x >> 6);
+    if (o >= A.nout) return;
+    int64_t ib, ob;
+    out_offsets(A, (uint32_t)o, &ib, &ob);
+    const T acc = narrow_wave_sum<T>(A, ib);
+    if ((threadIdx.x & 63) == 0) ((T *)A.out)[ob] = acc;
+}
+
+template <typename T> __device__ void fadd_small(const BatchCall &C, int64_t t) {
     constexpr int LT = FDT<T>::v;
     const RedArgs &A = C.red;
     const int64_t o = t * kNarrowOutputs + threadIdx.x;
     if (o >= A.nout) return;
     int64_t ib, ob;
     out_offsets(A, (uint32_t)o, &ib, &ob);
+    const int L = (int)A.wsize;
     T acc = T(0);
-    for (int64_t r = 0; r < A.rsize; r++) acc = fadd(acc, ld<LT>(A.in_dt, A.in, ib + r_offset(A, r)));
+    for (int64_t r = 0; r < A.rsize; r++) {
+        const int64_t base = ib + r_offset(A, r);
+        acc = fadd(acc, pw_leaf<T>([&](int i) { return ld<LT>(A.in_dt, A.in, base + w_offset(A, i)); }, L));
+    }
     ((T *)A.out)[ob] = acc;
 }
 
@@ -163,6 +303,18 @@ __device__ inline int64_t group_off(const RedArgs &A, int64_t q) {
     if (A.nr == 0) return w_offset(A, q);
     const int64_t r = q / A.wsize, w = q - r * A.wsize;
     return w_offset(A, w) + r_offset(A, r);
+}
+
+template <int LT> __device__ void iadd_small(const BatchCall &C, int64_t t) {
+    using ST = typename dt_type<LT>::T;
+    const RedArgs &A = C.red;
+    const int64_t o = t * kNarrowOutputs + threadIdx.x;
+    if (o >= A.nout) return;
+    int64_t ib, ob;
+    out_offsets(A, (uint32_t)o, &ib, &ob);
+    uint64_t acc = 0;
+    for (int64_t q = 0; q < A.group; q++) acc += (uint64_t)ld<LT>(A.in_dt, A.in, ib + group_off(A, q));
+    ((ST *)A.out)[ob] = int_result<LT>(acc);
 }
 
 template <int LT> __device__ void iadd_tile(const BatchCall &C, char *partials, int64_t g, uint64_t *red) {
@@ -181,7 +333,7 @@ template <int LT> __device__ void iadd_tile(const BatchCall &C, char *partials, 
     }
     if (threadIdx.x == 0) {
         using ST = typename dt_type<LT>::T;
-        if (C.direct) ((ST *)A.out)[ob] = (ST)red[0];
+        if (C.direct) ((ST *)A.out)[ob] = int_result<LT>(red[0]);
         else ((uint64_t *)(partials + C.part_off))[g] = red[0];
     }
 }
@@ -195,7 +347,7 @@ template <int LT> __device__ void iadd_fold(const BatchCall &C, const char *part
         for (int64_t i = 0; i < C.nseg; i++) acc += p[i];
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
-        ((ST *)A.out)[ob] = (ST)acc;
+        ((ST *)A.out)[ob] = int_result<LT>(acc);
     }
 }
 
@@ -213,50 +365,72 @@ __device__ void fill_zero_tile(const BatchCall &C, int64_t t) {
     }
 }
 
-__global__ __launch_bounds__(kBatchThreads) void batch_persistent(const BatchCall *calls, const int64_t *tile_begin,
-                                                                   int32_t ncalls, int64_t ntiles, uint32_t *counters,
-                                                                   char *partials) {
+// f(std::integral_constant<int, LT>) for an integer / bool output dtype
+template <typename F> __device__ inline void int_dispatch(int dt, const F &f) {
+    switch (dt) {
+        case IONP_BOOL: f(std::integral_constant<int, IONP_BOOL>()); break;
+        case IONP_INT8: f(std::integral_constant<int, IONP_INT8>()); break;
+        case IONP_UINT8: f(std::integral_constant<int, IONP_UINT8>()); break;
+        case IONP_INT16: f(std::integral_constant<int, IONP_INT16>()); break;
+        case IONP_UINT16: f(std::integral_constant<int, IONP_UINT16>()); break;
+        case IONP_INT32: f(std::integral_constant<int, IONP_INT32>()); break;
+        case IONP_UINT32: f(std::integral_constant<int, IONP_UINT32>()); break;
+        case IONP_INT64: f(std::integral_constant<int, IONP_INT64>()); break;
+        default: f(std::integral_constant<int, IONP_UINT64>()); break;
+    }
+}
+
+__global__ __launch_bounds__(kBatchThreads) void batch_persistent(
+    const BatchCall *__restrict__ calls, const int32_t *__restrict__ tile_map, uint32_t ntiles, uint32_t *counters,
+    char *partials) {
     __shared__ PwScratch S;
     __shared__ uint64_t ired[kBatchThreads];
-    __shared__ int64_t sh_tile;
-    __shared__ int32_t sh_call;
+    __shared__ int32_t sh_call, sh_local;
+    __shared__ BatchCall sC;  // the current call's record: field reads are LDS reads
+    static_assert(sizeof(BatchCall) % 8 == 0, "call record copied in 8-byte words");
+    int32_t cur = -1;
     for (;;) {
         if (threadIdx.x == 0) {
-            const int64_t t = (int64_t)atomicAdd(&counters[0], 1u);
-            int32_t c = -1;
-            if (t < ntiles) {  // owning call: last c with tile_begin[c] <= t
-                int32_t lo = 0, hi = ncalls - 1;
-                while (lo < hi) {
-                    const int32_t mid = (lo + hi + 1) >> 1;
-                    if (tile_begin[mid] <= t) lo = mid;
-                    else hi = mid - 1;
-                }
-                c = lo;
-            }
-            sh_tile = t;
-            sh_call = c;
+            const uint32_t t = atomicAdd(&counters[0], 1u);
+            sh_call = t < ntiles ? tile_map[2 * (size_t)t] : -1;
+            sh_local = t < ntiles ? tile_map[2 * (size_t)t + 1] : 0;
         }
         __syncthreads();
-        const int64_t t = sh_tile;
-        const int32_t c = sh_call;
+        // uniform across the workgroup: the call record is read through
+        // scalar loads
+        const int32_t c = __builtin_amdgcn_readfirstlane(sh_call);
         if (c < 0) break;  // every tile has been handed out
-        const BatchCall &C = calls[c];
-        const int64_t lt = t - tile_begin[c];
+        const int64_t lt = (int64_t)__builtin_amdgcn_readfirstlane(sh_local);
+        if (c != cur) {  // consecutive tiles of one call reuse the copy
+            const uint64_t *src = (const uint64_t *)&calls[c];
+            for (int i = threadIdx.x; i < (int)(sizeof(BatchCall) / 8); i += blockDim.x) ((uint64_t *)&sC)[i] = src[i];
+            __syncthreads();
+            cur = c;
+        }
+        const BatchCall &C = sC;
         switch (C.kind) {
             case BK_ELEMENTWISE: ew_tile(C.ew, lt); break;
             case BK_FADD_WIDE:
                 if (C.red.out_dt == IONP_FLOAT64) fadd_window<double>(C, partials, lt, S);
                 else fadd_window<float>(C, partials, lt, S);
                 break;
+            case BK_FADD_SMALL:
+                if (C.red.out_dt == IONP_FLOAT64) fadd_small<double>(C, lt);
+                else fadd_small<float>(C, lt);
+                break;
             case BK_FADD_NARROW:
                 if (C.red.out_dt == IONP_FLOAT64) fadd_narrow<double>(C, lt);
                 else fadd_narrow<float>(C, lt);
                 break;
+            case BK_FADD_NARROW_WAVE:
+                if (C.red.out_dt == IONP_FLOAT64) fadd_narrow_wave<double>(C, lt);
+                else fadd_narrow_wave<float>(C, lt);
+                break;
             case BK_IADD:
-                switch (C.red.out_dt) {
-                    case IONP_INT64: iadd_tile<IONP_INT64>(C, partials, lt, ired); break;
-                    default: iadd_tile<IONP_UINT64>(C, partials, lt, ired); break;
-                }
+                int_dispatch(C.red.out_dt, [&](auto lt_) { iadd_tile<decltype(lt_)::value>(C, partials, lt, ired); });
+                break;
+            case BK_IADD_SMALL:
+                int_dispatch(C.red.out_dt, [&](auto lt_) { iadd_small<decltype(lt_)::value>(C, lt); });
                 break;
             default: fill_zero_tile(C, lt); break;
         }
@@ -267,17 +441,16 @@ __global__ __launch_bounds__(kBatchThreads) void batch_persistent(const BatchCal
 // Second launch: calls whose outputs span several tiles fold their partials
 // (float: acc = 0; acc = acc + partial in window order; integer: any order).
 // Stream order makes every partial of the first launch visible.
-__global__ __launch_bounds__(kBatchThreads) void batch_fold(const BatchCall *calls, const int32_t *fold_list,
-                                                            int32_t nfold, const char *partials) {
+__global__ __launch_bounds__(kBatchThreads) void batch_fold(const BatchCall *__restrict__ calls,
+                                                            const int32_t *__restrict__ fold_list, int32_t nfold,
+                                                            const char *partials) {
     for (int32_t f = blockIdx.x; f < nfold; f += gridDim.x) {
         const BatchCall &C = calls[fold_list[f]];
         if (C.kind == BK_FADD_WIDE) {
             if (C.red.out_dt == IONP_FLOAT64) fadd_fold<double>(C, partials);
             else fadd_fold<float>(C, partials);
-        } else if (C.red.out_dt == IONP_INT64) {
-            iadd_fold<IONP_INT64>(C, partials);
         } else {
-            iadd_fold<IONP_UINT64>(C, partials);
+            int_dispatch(C.red.out_dt, [&](auto lt_) { iadd_fold<decltype(lt_)::value>(C, partials); });
         }
     }
 }
@@ -307,6 +480,8 @@ ionp_status plan_call(const ionp_call &c, BatchCall *B, int64_t *bytes) {
             loop = c.out.dtype;
         }
         if ((s = check_desc(&c.out, "out"))) return s;
+        if (c.kind == IONP_CALL_BINARY && c.op >= IONP_LESS && c.out.dtype != IONP_BOOL)
+            return set_error(IONP_ERR_TYPE, "comparison output must be bool");
         bool same = c.out.ndim == t.ndim;
         for (int i = 0; same && i < t.ndim; i++) same = c.out.shape[i] == t.shape[i];
         if (!same) return set_error(IONP_ERR_VALUE, "batch call output has the wrong shape");
@@ -370,14 +545,23 @@ ionp_status plan_call(const ionp_call &c, BatchCall *B, int64_t *bytes) {
         B->kind = BK_FILL_ZERO;
         B->tiles = (p.nout + kNarrowOutputs - 1) / kNarrowOutputs;
     } else if (is_float_dt(p.out_dt)) {
-        if (p.nw == 0) {
+        if (p.nw == 0 && narrow_wave_ok(A)) {
+            B->kind = BK_FADD_NARROW_WAVE;
+            B->tiles = (p.nout + kBatchThreads / 64 - 1) / (kBatchThreads / 64);
+        } else if (p.nw == 0) {
             B->kind = BK_FADD_NARROW;
+            B->tiles = (p.nout + kNarrowOutputs - 1) / kNarrowOutputs;
+        } else if (pw_small_ok(A)) {
+            B->kind = BK_FADD_SMALL;
             B->tiles = (p.nout + kNarrowOutputs - 1) / kNarrowOutputs;
         } else {
             B->kind = BK_FADD_WIDE;
             B->tiles = p.nout * p.rsize * A.nwin;
             B->direct = p.rsize == 1 && A.nwin == 1;
         }
+    } else if (int_cols_ok(A)) {
+        B->kind = BK_IADD_SMALL;
+        B->tiles = (p.nout + kNarrowOutputs - 1) / kNarrowOutputs;
     } else {
         B->kind = BK_IADD;
         B->nseg = (p.group + kIntSeg - 1) / kIntSeg;
@@ -405,17 +589,31 @@ extern "C" {
 ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, ionp_batch **out) {
     if (!out || (n > 0 && !calls) || n < 0) return set_error(IONP_ERR_VALUE, "bad batch arguments");
     *out = nullptr;
-    std::vector<BatchCall> table(n > 0 ? n : 1);
-    std::vector<int64_t> begin(n + 1, 0);
-    std::vector<int32_t> fold_list;
+    std::vector<BatchCall> planned(n > 0 ? n : 1), table(n > 0 ? n : 1);
+    std::vector<int64_t> begin(n + 1, 0), serial(n > 0 ? n : 1, 0);
+    std::vector<int32_t> fold_list, order(n);
     int64_t bytes = 0, part = 0;
     size_t scratch_need = 0;
     for (int32_t i = 0; i < n; i++) {
         int64_t b = 0;
-        ionp_status s = plan_call(calls[i], &table[i], &b);
+        ionp_status s = plan_call(calls[i], &planned[i], &b);
         if (s) {
             return set_error(s, "batch call " + std::to_string(i) + ": " + last_error());
         }
+        bytes += b;
+        const BatchCall &B = planned[i];
+        // length of the dependent chain one thread runs in a tile of this call
+        serial[i] = B.kind == BK_FADD_NARROW || B.kind == BK_FADD_NARROW_WAVE ? B.red.rsize
+                    : B.kind == BK_FADD_SMALL ? B.red.rsize * B.red.wsize
+                    : B.kind == BK_IADD_SMALL ? B.red.group
+                                              : 0;
+        order[i] = i;
+    }
+    // tiles are handed out in table order: long serial tiles first, so they do
+    // not finish the launch alone (the order of calls never changes a result)
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return serial[a] > serial[b]; });
+    for (int32_t i = 0; i < n; i++) {
+        table[i] = planned[order[i]];
         BatchCall &B = table[i];
         if ((B.kind == BK_FADD_WIDE || B.kind == BK_IADD) && !B.direct) {
             B.part_off = part;
@@ -424,7 +622,6 @@ ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, i
             fold_list.push_back(i);
         }
         begin[i + 1] = begin[i] + B.tiles;
-        bytes += b;
     }
     ionp_batch *bt = new ionp_batch();
     bt->n = n;
@@ -436,13 +633,23 @@ ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, i
     bt->scratch_need = scratch_need;
     hipStream_t s = (hipStream_t)stream;
     bt->nfold = (int32_t)fold_list.size();
+    if (begin[n] >= (1ll << 31)) {
+        delete bt;
+        return set_error(IONP_ERR_UNSUPPORTED, "batch has 2^31 tiles or more");
+    }
+    std::vector<int32_t> tmap(2 * (size_t)std::max<int64_t>(begin[n], 1));
+    for (int32_t i = 0; i < n; i++)
+        for (int64_t t = begin[i]; t < begin[i + 1]; t++) {
+            tmap[2 * t] = i;
+            tmap[2 * t + 1] = (int32_t)(t - begin[i]);
+        }
     bool ok = hipMalloc(&bt->d_calls, sizeof(BatchCall) * table.size()) == hipSuccess &&
-              hipMalloc(&bt->d_tile_begin, sizeof(int64_t) * (n + 1)) == hipSuccess &&
+              hipMalloc(&bt->d_tile_map, sizeof(int32_t) * tmap.size()) == hipSuccess &&
               hipMalloc(&bt->d_counters, sizeof(uint32_t) * 64) == hipSuccess &&
               hipMalloc(&bt->d_fold_list, sizeof(int32_t) * (fold_list.size() + 1)) == hipSuccess &&
               hipMalloc(&bt->d_partials, part > 0 ? part : 256) == hipSuccess;
     ok = ok && hipMemcpyAsync(bt->d_calls, table.data(), sizeof(BatchCall) * table.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
-         hipMemcpyAsync(bt->d_tile_begin, begin.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(bt->d_tile_map, tmap.data(), sizeof(int32_t) * tmap.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
          (fold_list.empty() || hipMemcpyAsync(bt->d_fold_list, fold_list.data(), sizeof(int32_t) * fold_list.size(),
                                               hipMemcpyHostToDevice, s) == hipSuccess) &&
          hipStreamSynchronize(s) == hipSuccess;
@@ -505,11 +712,19 @@ ionp_status ionp_batch_run(ionp_batch *bt, int32_t mode, void *stream) {
     if (mode != IONP_BATCH_PERSISTENT) return set_error(IONP_ERR_VALUE, "unknown batch mode");
     if (bt->n == 0 || bt->ntiles == 0) return IONP_OK;
     hipLaunchKernelGGL(batch_zero, dim3(1), dim3(64), 0, s, bt->d_counters, (int64_t)64);
-    // persistent grid: a few workgroups per CU, never more than the tiles
-    int64_t grid = (int64_t)num_cus() * 4;
+    // persistent grid: as many workgroups as fit on the chip at once (the
+    // kernel's register and LDS use decide), never more than the tiles
+    static int per_cu = 0;
+    if (!per_cu) {
+        int b = 0;
+        per_cu = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, batch_persistent, kBatchThreads, 0) == hipSuccess && b > 0
+                     ? b
+                     : 2;
+    }
+    int64_t grid = (int64_t)num_cus() * per_cu;
     if (grid > bt->ntiles) grid = bt->ntiles;
-    hipLaunchKernelGGL(batch_persistent, dim3((unsigned)grid), dim3(kBatchThreads), 0, s, bt->d_calls, bt->d_tile_begin,
-                       bt->n, bt->ntiles, bt->d_counters, bt->d_partials);
+    hipLaunchKernelGGL(batch_persistent, dim3((unsigned)grid), dim3(kBatchThreads), 0, s, bt->d_calls, bt->d_tile_map,
+                       (uint32_t)bt->ntiles, bt->d_counters, bt->d_partials);
     if (bt->nfold)
         hipLaunchKernelGGL(batch_fold, dim3((unsigned)std::min<int32_t>(bt->nfold, num_cus() * 4)), dim3(kBatchThreads), 0, s,
                            bt->d_calls, bt->d_fold_list, bt->nfold, bt->d_partials);
@@ -566,7 +781,7 @@ void ionp_batch_destroy(ionp_batch *bt) {
     if (!bt) return;
     if (bt->graph_ready) (void)hipGraphExecDestroy(bt->graph);
     if (bt->d_calls) (void)hipFree(bt->d_calls);
-    if (bt->d_tile_begin) (void)hipFree(bt->d_tile_begin);
+    if (bt->d_tile_map) (void)hipFree(bt->d_tile_map);
     if (bt->d_counters) (void)hipFree(bt->d_counters);
     if (bt->d_fold_list) (void)hipFree(bt->d_fold_list);
     if (bt->d_partials) (void)hipFree(bt->d_partials);

@@ -211,15 +211,204 @@ template <int LT> __device__ inline typename dt_type<LT>::T ld(int dt, const cha
     }
 }
 
+// Integer / bool reduce result in the output dtype: wrapped to its width;
+// add on bool is logical or (bool_same, ufunc.rs:2341).
+template <int LT, typename AT> __device__ inline typename dt_type<LT>::T int_result(AT acc) {
+    using ST = typename dt_type<LT>::T;
+    if constexpr (LT == IONP_BOOL) return (ST)(acc != 0);
+    else return (ST)acc;
+}
+
+// Element `off` of a buffer of input type IT, cast to loop dtype LT (the
+// conversion ld<LT> applies for the matching dtype code).
+template <int LT, typename IT> __device__ inline typename dt_type<LT>::T ld_t(const char *base, int64_t off) {
+    if constexpr (std::is_same<IT, bool>::value) return cvt<LT>((uint8_t)(((const uint8_t *)base)[off] != 0));
+    else return cvt<LT>(((const IT *)base)[off]);
+}
+
+// f(IT()) for the element type of dtype code dt (bool for IONP_BOOL)
+template <typename F> __device__ inline auto in_type_dispatch(int dt, const F &f) {
+    switch (dt) {
+        case IONP_BOOL: return f(bool());
+        case IONP_INT8: return f(int8_t());
+        case IONP_UINT8: return f(uint8_t());
+        case IONP_INT16: return f(int16_t());
+        case IONP_UINT16: return f(uint16_t());
+        case IONP_INT32: return f(int32_t());
+        case IONP_UINT32: return f(uint32_t());
+        case IONP_INT64: return f(int64_t());
+        case IONP_UINT64: return f(uint64_t());
+        case IONP_FLOAT32: return f(float());
+        default: return f(double());
+    }
+}
+
 template <typename T> struct FDT;
 template <> struct FDT<float> { static constexpr int v = IONP_FLOAT32; };
 template <> struct FDT<double> { static constexpr int v = IONP_FLOAT64; };
+
+// ---------------------------------------------------------------------------
+// Narrow float add (W empty): acc = 0; acc = acc + x_r for the reduced
+// elements in iterator order, starting at input offset ib.  The loads do not
+// depend on acc, so U of them are issued ahead of the dependent add chain.
+// The chain uses plain IEEE adds: they equal fadd whenever no NaN arises,
+// and a NaN, once produced, stays NaN to the end; an output that ends NaN
+// is recomputed with fadd's x86 NaN rules.
+// ---------------------------------------------------------------------------
+template <typename T, typename IT, int U> __device__ inline T narrow_chain(const RedArgs &A, int64_t ib) {
+    constexpr int LT = FDT<T>::v;
+    T acc = T(0);
+    int64_t r = 0;
+    if (A.nr == 1) {
+        const int64_t st = A.r_st[0];
+        const char *x = A.in;
+        T cur[U], nxt[U];
+        if (A.rsize >= U) {
+#pragma unroll
+            for (int u = 0; u < U; u++) cur[u] = ld_t<LT, IT>(x, ib + (int64_t)u * st);
+            r = U;
+            for (; r + U <= A.rsize; r += U) {
+#pragma unroll
+                for (int u = 0; u < U; u++) nxt[u] = ld_t<LT, IT>(x, ib + (r + u) * st);
+#pragma unroll
+                for (int u = 0; u < U; u++) acc = acc + cur[u];
+#pragma unroll
+                for (int u = 0; u < U; u++) cur[u] = nxt[u];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) acc = acc + cur[u];
+        }
+        for (; r < A.rsize; r++) acc = acc + ld_t<LT, IT>(x, ib + r * st);
+        if (acc != acc) {
+            acc = T(0);
+            for (int64_t q = 0; q < A.rsize; q++) acc = fadd(acc, ld_t<LT, IT>(x, ib + q * st));
+        }
+        return acc;
+    }
+    constexpr int V = 8;
+    for (; r + V <= A.rsize; r += V) {
+        T v[V];
+#pragma unroll
+        for (int u = 0; u < V; u++) v[u] = ld_t<LT, IT>(A.in, ib + r_offset(A, r + u));
+#pragma unroll
+        for (int u = 0; u < V; u++) acc = acc + v[u];
+    }
+    for (; r < A.rsize; r++) acc = acc + ld_t<LT, IT>(A.in, ib + r_offset(A, r));
+    if (acc != acc) {
+        acc = T(0);
+        for (int64_t q = 0; q < A.rsize; q++) acc = fadd(acc, ld_t<LT, IT>(A.in, ib + r_offset(A, q)));
+    }
+    return acc;
+}
+// U: loads in flight per thread (registers: 2 U values)
+template <typename T, int U = (sizeof(T) == 8 ? 24 : 32)> __device__ inline T narrow_sum(const RedArgs &A, int64_t ib) {
+    return in_type_dispatch(A.in_dt, [&](auto it) { return narrow_chain<T, decltype(it), U>(A, ib); });
+}
+
+// Value of v in lane `lane` (uniform) of the wavefront.
+template <typename T> __device__ inline T readlane_t(T v, int lane) {
+    if constexpr (sizeof(T) == 8) {
+        uint64_t b;
+        std::memcpy(&b, &v, 8);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+        const uint64_t r = ((uint64_t)hi << 32) | lo;
+        T out;
+        std::memcpy(&out, &r, 8);
+        return out;
+    } else {
+        uint32_t b;
+        std::memcpy(&b, &v, 4);
+        const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)b, lane);
+        T out;
+        std::memcpy(&out, &r, 4);
+        return out;
+    }
+}
+
+// Narrow float add of one output by a whole wavefront: the 64 lanes load
+// blocks of 64 * B consecutive reduced elements (the next block in flight
+// while the current one is added), and the add chain, the same in every
+// lane, reads the block lane by lane in element order.  Plain IEEE adds with
+// the NaN redo of narrow_chain.  Result in every lane.
+template <typename T, typename IT> __device__ inline T narrow_wave_chain(const RedArgs &A, int64_t ib) {
+    constexpr int LT = FDT<T>::v;
+    constexpr int B = 4, BLK = 64 * B;
+    const int lane = threadIdx.x & 63;
+    const int64_t n = A.rsize;
+    auto off = [&](int64_t r) { return A.nr == 1 ? r * A.r_st[0] : r_offset(A, r); };
+    T cur[B], nxt[B];
+#pragma unroll
+    for (int k = 0; k < B; k++) {
+        const int64_t r = k * 64 + lane;
+        cur[k] = r < n ? ld_t<LT, IT>(A.in, ib + off(r)) : T(0);
+    }
+    T acc = T(0);
+    for (int64_t rb = 0; rb < n; rb += BLK) {
+        const int64_t nb = rb + BLK;
+        if (nb < n) {
+#pragma unroll
+            for (int k = 0; k < B; k++) {
+                const int64_t r = nb + k * 64 + lane;
+                nxt[k] = r < n ? ld_t<LT, IT>(A.in, ib + off(r)) : T(0);
+            }
+        }
+        if (nb <= n) {
+#pragma unroll
+            for (int k = 0; k < B; k++)
+#pragma unroll
+                for (int j = 0; j < 64; j++) acc = acc + readlane_t(cur[k], j);
+        } else {
+            const int cnt = (int)(n - rb);
+            for (int e = 0; e < cnt; e++) {
+                const int k = e >> 6;
+                const T v = k == 0 ? cur[0] : k == 1 ? cur[1] : k == 2 ? cur[2] : cur[3];
+                acc = acc + readlane_t(v, e & 63);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < B; k++) cur[k] = nxt[k];
+    }
+    if (acc != acc) {
+        acc = T(0);
+        for (int64_t q = 0; q < n; q++) acc = fadd(acc, ld_t<LT, IT>(A.in, ib + off(q)));
+    }
+    return acc;
+}
+template <typename T> __device__ inline T narrow_wave_sum(const RedArgs &A, int64_t ib) {
+    return in_type_dispatch(A.in_dt, [&](auto it) { return narrow_wave_chain<T, decltype(it)>(A, ib); });
+}
 
 // ---------------------------------------------------------------------------
 // NumPy's pairwise tree over an LDS window (ufunc.rs:6416-6448)
 // ---------------------------------------------------------------------------
 // Node split of pairwise_sum: n2 = n/2 rounded down to a multiple of 8.
 __device__ inline int pw_split(int n) { return (n >> 1) & ~7; }
+
+// pairwise_sum of x[0..n) for n <= 128 (a single leaf) in one thread: n < 8
+// adds in order from +0; otherwise 8 interleaved accumulators combined as
+// ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)), then the n % 8 tail in order.
+template <typename T, typename Get> __device__ inline T pw_leaf(const Get &get, int n) {
+    if (n < 8) {
+        T r = T(0);
+        for (int i = 0; i < n; i++) r = fadd(r, get(i));
+        return r;
+    }
+    T r[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) r[j] = get(j);
+    int i = 8;
+    for (; i + 8 <= n; i += 8) {
+        T x[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) x[j] = get(i + j);
+#pragma unroll
+        for (int j = 0; j < 8; j++) r[j] = fadd(r[j], x[j]);
+    }
+    T res = fadd(fadd(fadd(r[0], r[1]), fadd(r[2], r[3])), fadd(fadd(r[4], r[5]), fadd(r[6], r[7])));
+    for (; i < n; i++) res = fadd(res, get(i));
+    return res;
+}
 
 constexpr int kPwMaxWin = 8192;
 constexpr int kPwThreads = 512;

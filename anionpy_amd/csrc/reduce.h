@@ -23,6 +23,22 @@ struct RedArgs {
     void *scratch;  // partial results
 };
 
+// Float add: every window is one pairwise leaf (<= 128 elements) and there
+// are outputs enough (or no outer reduced loop) for one thread per output.
+inline bool pw_small_ok(const RedArgs &A) {
+    return A.nw > 0 && A.nwin == 1 && A.wsize <= 128 && (A.rsize == 1 || A.nout >= 2048);
+}
+
+// Narrow float add (W empty) with long chains and few outputs: a wavefront
+// per output keeps enough loads in flight for the serial add chain.
+inline bool narrow_wave_ok(const RedArgs &A) { return A.nw == 0 && A.rsize >= 256 && A.nout <= 8192; }
+
+// Integer reductions: one thread per output (elements in any order) when
+// groups are short, or outputs plenty and groups moderate.
+inline bool int_cols_ok(const RedArgs &A) {
+    return A.group <= 64 || (A.nout >= 16384 && A.group <= 4096) || (A.nw == 0 && A.nout >= 2048 && A.group <= 65536);
+}
+
 ionp_status run_reduce(const ReducePlan &p, int64_t bufsize, const ionp_config *cfg, hipStream_t s);
 
 // argmax / argmin along one axis (C index order) or over the C-order ravel.

@@ -361,47 +361,43 @@ template <typename T> __global__ __launch_bounds__(256) void pw_fold(const RedAr
     }
 }
 
-// W empty: per output, acc = 0; acc = acc + x for each reduced element in
-// iterator order.  Loads are issued U at a time ahead of the add chain.
-template <typename T> __global__ __launch_bounds__(256) void pw_narrow(const RedArgs A) {
+// Windows of at most 128 elements (one pairwise leaf each) and many
+// outputs: one thread per output runs acc = 0; acc = acc + pw(window r) over
+// the outer reduced index r in order (the same value pw_windows + pw_fold
+// produce, without a workgroup per window).
+template <typename T> __global__ __launch_bounds__(256) void pw_small(const RedArgs A) {
     constexpr int LT = FDT<T>::v;
+    const int L = (int)A.wsize;
     for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
         T acc = T(0);
-        int64_t r = 0;
-        if (A.nr == 1 && A.in_dt == A.out_dt) {
-            // native dtype, one outer axis: U loads in flight ahead of the
-            // dependent add chain (double-buffered), plain IEEE adds; a NaN
-            // result is redone below with x86 NaN rules
-            constexpr int U = sizeof(T) == 8 ? 24 : 32;
-            const T *x = (const T *)A.in + ib;
-            const int64_t st = A.r_st[0];
-            T cur[U], nxt[U];
-            if (A.rsize >= U) {
-#pragma unroll
-                for (int u = 0; u < U; u++) cur[u] = __builtin_nontemporal_load(x + (int64_t)u * st);
-                r = U;
-                for (; r + U <= A.rsize; r += U) {
-#pragma unroll
-                    for (int u = 0; u < U; u++) nxt[u] = __builtin_nontemporal_load(x + (r + u) * st);
-#pragma unroll
-                    for (int u = 0; u < U; u++) acc = acc + cur[u];
-#pragma unroll
-                    for (int u = 0; u < U; u++) cur[u] = nxt[u];
-                }
-#pragma unroll
-                for (int u = 0; u < U; u++) acc = acc + cur[u];
-            }
-            for (; r < A.rsize; r++) acc = acc + x[r * st];
-            if (acc != acc) {
-                acc = T(0);
-                for (int64_t q = 0; q < A.rsize; q++) acc = fadd(acc, x[q * st]);
-            }
-        } else {
-            for (; r < A.rsize; r++) acc = fadd(acc, ld<LT>(A.in_dt, A.in, ib + r_offset(A, r)));
+        for (int64_t r = 0; r < A.rsize; r++) {
+            const int64_t base = ib + r_offset(A, r);
+            acc = fadd(acc, pw_leaf<T>([&](int i) { return ld<LT>(A.in_dt, A.in, base + w_offset(A, i)); }, L));
         }
         ((T *)A.out)[ob] = acc;
+    }
+}
+
+// W empty: per output, acc = 0; acc = acc + x for each reduced element in
+// iterator order (narrow_sum).
+template <typename T> __global__ __launch_bounds__(256) void pw_narrow(const RedArgs A) {
+    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
+        int64_t ib, ob;
+        out_offsets(A, (uint32_t)o, &ib, &ob);
+        ((T *)A.out)[ob] = narrow_sum<T>(A, ib);
+    }
+}
+
+// the same with a wavefront per output (narrow_wave_ok)
+template <typename T> __global__ __launch_bounds__(256) void pw_narrow_wave(const RedArgs A) {
+    const int64_t w0 = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+    for (int64_t o = w0; o < A.nout; o += (int64_t)gridDim.x * (blockDim.x >> 6)) {
+        int64_t ib, ob;
+        out_offsets(A, (uint32_t)o, &ib, &ob);
+        const T acc = narrow_wave_sum<T>(A, ib);
+        if ((threadIdx.x & 63) == 0) ((T *)A.out)[ob] = acc;
     }
 }
 
@@ -738,7 +734,7 @@ __global__ __launch_bounds__(256) void int_reduce(const RedArgs A, int64_t seg, 
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            if (nseg == 1) ((ST *)A.out)[ob] = (ST)red[0];
+            if (nseg == 1) ((ST *)A.out)[ob] = int_result<LT>(red[0]);
             else ((AT *)A.scratch)[g] = red[0];
         }
         __syncthreads();
@@ -761,7 +757,7 @@ template <typename AT, int LT, int OP> __global__ __launch_bounds__(256) void in
         if (threadIdx.x == 0) {
             int64_t ib, ob;
             out_offsets(A, (uint32_t)o, &ib, &ob);
-            ((ST *)A.out)[ob] = (ST)red[0];
+            ((ST *)A.out)[ob] = int_result<LT>(red[0]);
         }
         __syncthreads();
     }
@@ -799,7 +795,7 @@ __global__ __launch_bounds__(256) void int_reduce_contig(const RedArgs A, int64_
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            if (nseg == 1) ((ST *)A.out)[ob] = (ST)red[0];
+            if (nseg == 1) ((ST *)A.out)[ob] = (ST)red[0];  // never a bool output (see run_int)
             else ((AT *)A.scratch)[g] = red[0];
         }
         __syncthreads();
@@ -815,7 +811,7 @@ template <typename AT, int LT, int OP> __global__ __launch_bounds__(256) void in
         AT acc = OP == IONP_ADD ? AT(0) : (AT)load_acc<ST, LT>(A.in_dt, A.in, ib);
         for (int64_t q = 0; q < A.group; q++)
             acc = assoc<AT, OP>(acc, (AT)load_acc<ST, LT>(A.in_dt, A.in, ib + group_offset(A, q)));
-        ((ST *)A.out)[ob] = (ST)acc;
+        ((ST *)A.out)[ob] = int_result<LT>(acc);
     }
 }
 
@@ -845,9 +841,7 @@ static ionp_status launched() {
 template <typename AT, int LT, int OP>
 static ionp_status run_int(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
     const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
-    // column form when the reduced elements are strided and outputs plenty
-    const bool cols = A.nw == 0 || (A.nout >= 16384 && A.group <= 4096);
-    if (cols) {
+    if (int_cols_ok(A)) {
         hipLaunchKernelGGL((int_reduce_cols<AT, LT, OP>), dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
         return launched();
     }
@@ -859,7 +853,11 @@ static ionp_status run_int(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
     }
     using ST = typename dt_type<LT>::T;
     const dim3 grid(grid_cap(A.nout * nseg, bpc)), blk(256);
-    if (A.nw == 1 && A.w_st[0] == 1 && A.nr == 0 && A.in_dt != IONP_BOOL && A.group < (1ll << 40)) {
+    // raw input values only where that equals casting each to the output
+    // dtype first: integer add (wraps modulo the output width either way) or
+    // no cast; bool outputs (logical or) take int_reduce
+    if (A.nw == 1 && A.w_st[0] == 1 && A.nr == 0 && A.in_dt != IONP_BOOL && A.out_dt != IONP_BOOL &&
+        A.group < (1ll << 40) && (OP == IONP_ADD || A.in_dt == A.out_dt)) {
         switch (A.in_dt) {  // one contiguous run per output: vector loads
             case IONP_INT8: hipLaunchKernelGGL((int_reduce_contig<AT, int8_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
             case IONP_UINT8: hipLaunchKernelGGL((int_reduce_contig<AT, uint8_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
@@ -928,7 +926,10 @@ template <typename T> static ionp_status run_pw_fast(RedArgs &A, const ionp_conf
 template <typename T> static ionp_status run_float_add(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
     const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
     if (A.nw == 0) {
-        hipLaunchKernelGGL(pw_narrow<T>, dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
+        if (narrow_wave_ok(A))
+            hipLaunchKernelGGL(pw_narrow_wave<T>, dim3(grid_cap((A.nout + 3) / 4, bpc)), dim3(256), 0, s, A);
+        else
+            hipLaunchKernelGGL(pw_narrow<T>, dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
         return launched();
     }
     // contiguous rows with at least one full 8192-element window, native dtype;
@@ -938,6 +939,10 @@ template <typename T> static ionp_status run_float_add(RedArgs &A, const ionp_co
                          A.wsize / 8192 * A.nout < (1ll << 31);
     if (rows_ok)
         return run_pw_fast<T>(A, cfg, s);
+    if (pw_small_ok(A)) {
+        hipLaunchKernelGGL(pw_small<T>, dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
+        return launched();
+    }
     const int64_t nwindows = A.nout * A.rsize * A.nwin;
     const int direct = A.rsize == 1 && A.nwin == 1;
     if (!direct) {

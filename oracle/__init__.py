@@ -190,12 +190,12 @@ def _norm_axes(axis, ndim):
     return [] if ndim == 0 else out
 
 
-def reduce(op, x, axis=0, bufsize=BUFSIZE):
-    """np.<op>.reduce(x, axis) for op in add/maximum/minimum."""
+def reduce(op, x, axis=0, bufsize=BUFSIZE, dtype=None):
+    """np.<op>.reduce(x, axis, dtype) for op in add/maximum/minimum."""
     x = np.asarray(x)
     axes = _norm_axes(axis, x.ndim)
     mask = bytes(1 if k in axes else 0 for k in range(x.ndim)) or b"\0"
-    odt = DTYPES[lib().ora_reduce_dtype(OP[op], _CODE[x.dtype])]
+    odt = DTYPES[lib().ora_reduce_dtype(OP[op], _CODE[x.dtype])] if dtype is None else np.dtype(dtype)
     kept = [k for k in range(x.ndim) if k not in axes]
     shape = tuple(x.shape[k] for k in kept)
     # K-order of the kept axes, following the input's iterator order

@@ -781,8 +781,11 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
     int64_t *woff, okidx[ORA_MAXDIM], nout, o, gsize = 1;
     char *buf = NULL;
     if (op != OP_ADD && op != OP_MAXIMUM && op != OP_MINIMUM) return ORA_ENOLOOP;
-    cdt = ora_reduce_dtype(op, in->dtype);
-    if (out->dtype != cdt) return ORA_ETYPE;
+    /* the loop runs in the output dtype (the default of ora_reduce_dtype or
+       an explicit dtype=); elements are cast to it as they are read */
+    cdt = out->dtype;
+    if (is_float(in->dtype) && !is_float(cdt)) return ORA_ETYPE;
+    if (is_float(cdt) && in->dtype != cdt) return ORA_EVALUE; /* float loops read the input as is: cast it first */
     make_red_plan(in, axmask, &p);
     for (k = 0; k < n; k++) {
         if (!axmask[k]) kept[nk++] = k;
@@ -843,6 +846,8 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
             }
             {
                 uint64_t acc = uns ? uacc : (uint64_t)sacc;
+                /* add on bool is logical or (bool_same, ufunc.rs:2341) */
+                if (cdt == DT_BOOL) acc = acc != 0;
                 switch (ITEMSIZE[cdt]) {
                 case 1: *(uint8_t *)po = (uint8_t)acc; break;
                 case 2: *(uint16_t *)po = (uint16_t)acc; break;

@@ -333,6 +333,30 @@ def reduce_cases(S, rng):
         S.add(f"int/add/{nm}/T", "reduce", "add", [x.T[::2]], lambda: np.add.reduce(x.T[::2], axis=1), {"axis": 1})
     x = rng.integers(-1000, 1000, 150000, dtype=np.int32)
     S.add("int/add/int32/long", "reduce", "add", [x], lambda: np.add.reduce(x), {"axis": 0})
+    # explicit narrow result dtypes: add on bool is logical or (ufunc.rs:2341),
+    # integer results wrap in the requested width (own generator: the cases
+    # above keep their draws)
+    r2 = np.random.default_rng(77)
+    b = r2.random((300, 700)) < 0.6
+    b[:, 5] = False
+    for ax in (0, 1, None):
+        S.add(f"int/add/bool->bool/{ax}", "reduce", "add", [b], lambda: np.add.reduce(b, axis=ax, dtype=np.bool_),
+              {"axis": ax, "dtype": "bool"})
+    x = r2.integers(-2 ** 31, 2 ** 31, (40, 900), dtype=np.int32)
+    for ax in (0, 1):
+        S.add(f"int/add/int32->int8/{ax}", "reduce", "add", [x], lambda: np.add.reduce(x, axis=ax, dtype=np.int8),
+              {"axis": ax, "dtype": "int8"})
+    u = r2.integers(0, 256, (90, 333), dtype=np.uint8)
+    S.add("int/add/uint8->int16/1", "reduce", "add", [u], lambda: np.add.reduce(u, axis=1, dtype=np.int16),
+          {"axis": 1, "dtype": "int16"})
+    i8 = r2.integers(-5, 5, (4, 6), dtype=np.int8)
+    S.add("int/add/int8->bool/1", "reduce", "add", [i8], lambda: np.add.reduce(i8, axis=1, dtype=np.bool_),
+          {"axis": 1, "dtype": "bool"})
+    for op in ("maximum", "minimum"):
+        S.add(f"int/{op}/int32->int8/1", "reduce", op, [x], lambda: getattr(np, op).reduce(x, axis=1, dtype=np.int8),
+              {"axis": 1, "dtype": "int8"})
+        S.add(f"int/{op}/uint8->int8/1", "reduce", op, [u], lambda: getattr(np, op).reduce(u, axis=1, dtype=np.int8),
+              {"axis": 1, "dtype": "int8"})
 
 
 def argext_cases(S, rng):
