@@ -226,6 +226,14 @@ template <int LT, typename IT> __device__ inline typename dt_type<LT>::T ld_t(co
     else return cvt<LT>(((const IT *)base)[off]);
 }
 
+// ld_t through a streaming (non-temporal) load
+template <int LT, typename IT> __device__ inline typename dt_type<LT>::T ld_t_nt(const char *base, int64_t off) {
+    if constexpr (std::is_same<IT, bool>::value)
+        return cvt<LT>((uint8_t)(__builtin_nontemporal_load((const uint8_t *)base + off) != 0));
+    else
+        return cvt<LT>(__builtin_nontemporal_load((const IT *)base + off));
+}
+
 // f(IT()) for the element type of dtype code dt (bool for IONP_BOOL)
 template <typename F> __device__ inline auto in_type_dispatch(int dt, const F &f) {
     switch (dt) {
@@ -265,11 +273,11 @@ template <typename T, typename IT, int U> __device__ inline T narrow_chain(const
         T cur[U], nxt[U];
         if (A.rsize >= U) {
 #pragma unroll
-            for (int u = 0; u < U; u++) cur[u] = ld_t<LT, IT>(x, ib + (int64_t)u * st);
+            for (int u = 0; u < U; u++) cur[u] = ld_t_nt<LT, IT>(x, ib + (int64_t)u * st);
             r = U;
             for (; r + U <= A.rsize; r += U) {
 #pragma unroll
-                for (int u = 0; u < U; u++) nxt[u] = ld_t<LT, IT>(x, ib + (r + u) * st);
+                for (int u = 0; u < U; u++) nxt[u] = ld_t_nt<LT, IT>(x, ib + (r + u) * st);
 #pragma unroll
                 for (int u = 0; u < U; u++) acc = acc + cur[u];
 #pragma unroll

@@ -263,41 +263,101 @@ template <typename T> __device__ void pw_folder(const RedArgs &A, const PwFastSt
     }
 }
 
-// Producer mapping: every lane owns CH = 16 / sizeof(T) consecutive
-// accumulators of one leaf and reads them with one 16-byte load per step k,
-// so 8 / CH lanes cover a leaf and a window takes 64 * 8 / CH threads
-// (f64: 256, f32: 128).
+// One wavefront evaluates a whole 8192-element window (NumPy's tree with 64
+// leaves of 128; pw_fast uses it to redo a window whose plain-IEEE value
+// came out NaN, with EXACT set).  Every
+// lane owns CH = 16 / sizeof(T) consecutive accumulators of one leaf and
+// reads them with one 16-byte load per step k (k = 0..15, element
+// 128 leaf + 8 k + j), so 8 / CH lanes cover a leaf, a pass covers 64 * CH / 8
+// leaves and NP passes cover the window's 64 leaves.  Leaf values are
+// gathered one per lane and combined by the balanced tree over lanes.
+// EXACT: every add with fadd's x86 NaN rules (plain IEEE adds otherwise;
+// the two agree unless a NaN arises).
+template <typename T, bool EXACT> __device__ inline T pw_window_wave(const T *x, int lane) {
+    constexpr int CH = 16 / sizeof(T);  // accumulators per lane
+    constexpr int LPL = 8 / CH;         // lanes per leaf
+    constexpr int LPW = 64 / LPL;       // leaves per pass
+    constexpr int NP = 64 / LPW;        // passes per window
+    using V = T __attribute__((ext_vector_type(CH)));
+    auto add = [](T a, T b) { return EXACT ? fadd(a, b) : a + b; };
+    auto comb = [&](T v, int m) {  // pairwise combine at lane distance m, left operand = lower lane
+        const T o = __shfl_xor(v, m, 64);
+        return (lane & m) ? add(o, v) : add(v, o);
+    };
+    const int lf = lane / LPL, jq = (lane % LPL) * CH;
+    T leafv[NP];
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        const T *px = x + (int64_t)(p * LPW + lf) * 128 + jq;
+        V v[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = *(const V *)(px + 8 * k);
+        T r[CH];
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            r[c] = v[0][c];
+#pragma unroll
+            for (int k = 1; k < 16; k++) r[c] = add(r[c], v[k][c]);
+        }
+        T a;
+        if constexpr (CH == 2) {
+            a = add(r[0], r[1]);  // r0+r1 (lane holds accumulators 2m, 2m+1)
+            a = comb(a, 1);       // (r0+r1)+(r2+r3)
+            a = comb(a, 2);       // leaf value
+        } else {
+            a = add(add(r[0], r[1]), add(r[2], r[3]));
+            a = comb(a, 1);  // leaf value
+        }
+        leafv[p] = a;
+    }
+    // lane L takes leaf L: pass L / LPW, held by lane (L % LPW) * LPL
+    T t = leafv[0];
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        const T s = __shfl(leafv[p], (lane % LPW) * LPL, 64);
+        if (lane / LPW == p) t = s;
+    }
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) t = comb(t, m);
+    return t;  // the same in every lane
+}
+
+// Producers: a workgroup of 64 * 8 / CH lanes evaluates one window at a time
+// (every lane owns CH accumulators of one leaf and reads them with one
+// 16-byte load per step k), the 64 leaf values meet in LDS and wavefront 0
+// combines them by the balanced tree over lanes.  (Measured on MI355X: one
+// wavefront per window, or one workgroup per CU, are both slower.)
 template <typename T> constexpr int pw_fast_threads() { return sizeof(T) == 8 ? 256 : 128; }
 
 template <typename T>
 __global__ __launch_bounds__(256) void pw_fast(const RedArgs A, const PwFastState F, int64_t nwindows) {
-    constexpr int LT = FDT<T>::v;
-    constexpr int CH = 16 / sizeof(T);          // accumulators per lane
-    constexpr int LPL = 8 / CH;                 // lanes per leaf
-    constexpr int LEAVES_PER_WAVE = 64 / LPL;   // 16 (f64) or 32 (f32)
+    constexpr int CH = 16 / sizeof(T);
+    constexpr int LPL = 8 / CH;
     using V = T __attribute__((ext_vector_type(CH)));
+    __shared__ T stage[1024];  // folder staging
     __shared__ T leafv[64];
-    __shared__ int sh_nan;
-    __shared__ T racc[kPwMaxWin / 8];
-    __shared__ T val[kPwMaxWin / 8];
-    __shared__ int leaf_lo[kPwMaxWin / 64 + 2], leaf_n[kPwMaxWin / 64 + 2], cnt[1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int lf = lane / LPL, jq = (lane % LPL) * CH;
     if ((int64_t)blockIdx.x < F.folders) {
-        if (wave == 0) pw_folder<T>(A, F, blockIdx.x, racc);  // racc: 1024 T of LDS staging
+        if (wave == 0) {
+            // the serial fold must keep pace with the producers that share
+            // its CU: its wavefront issues first
+            __builtin_amdgcn_s_setprio(3);
+            pw_folder<T>(A, F, blockIdx.x, stage);
+        }
         return;
     }
+    const int lf = threadIdx.x / LPL, jq = (lane % LPL) * CH;
     const int64_t nprod = (int64_t)gridDim.x - F.folders;
     for (int64_t g = (int64_t)blockIdx.x - F.folders; g < nwindows; g += nprod) {
         const int64_t o = g / F.nfull, w = g - o * F.nfull;
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
-        const int leaf = LEAVES_PER_WAVE * wave + lf;
-        const T *x = (const T *)A.in + ib + w * 8192 + (int64_t)leaf * 128 + jq;
+        const T *xw = (const T *)A.in + ib + w * 8192;
+        const T *x = xw + (int64_t)lf * 128 + jq;
         V v[16];
 #pragma unroll
         for (int k = 0; k < 16; k++) v[k] = *(const V *)(x + 8 * k);
-        // accumulators jq .. jq+CH-1 of this leaf, each over k in order
+        // accumulators jq .. jq+CH-1 of leaf lf, each over k in order
         V r = v[0];
 #pragma unroll
         for (int k = 1; k < 16; k++) r = r + v[k];
@@ -310,22 +370,16 @@ __global__ __launch_bounds__(256) void pw_fast(const RedArgs A, const PwFastStat
             a = (r[0] + r[1]) + (r[2] + r[3]);
             a = xor_combine(a, 1, lane);  // leaf value
         }
-        if (lane % LPL == 0) leafv[leaf] = a;
+        if (lane % LPL == 0) leafv[lf] = a;
         __syncthreads();
-        T res = T(0);
         if (wave == 0) {
             T t = leafv[lane];
 #pragma unroll
             for (int m = 1; m < 64; m <<= 1) t = xor_combine(t, m, lane);
-            res = t;
-            if (lane == 0) sh_nan = t != t;
+            if (t != t) t = pw_window_wave<T, true>(xw, lane);  // NaN: x86 NaN rules, exactly
+            if (lane == 0) st_value((T *)F.partials + o * F.nwin + w, t);
         }
         __syncthreads();
-        if (sh_nan) {  // NaN somewhere in the window: x86 NaN rules, exactly
-            res = pairwise_block<T>([&](int i) { return ld<LT>(A.in_dt, A.in, ib + w * 8192 + i); }, 8192, racc, val,
-                                    leaf_lo, leaf_n, cnt);
-        }
-        if (threadIdx.x == 0) st_value((T *)F.partials + o * F.nwin + w, res);
     }
 }
 
@@ -381,25 +435,28 @@ template <typename T> __global__ __launch_bounds__(256) void pw_small(const RedA
 }
 
 // W empty: per output, acc = 0; acc = acc + x for each reduced element in
-// iterator order (narrow_sum).
-template <typename T> __global__ __launch_bounds__(256) void pw_narrow(const RedArgs A) {
+// iterator order (narrow_chain); IT is the input element type (the host
+// dispatches on it, so each kernel is one straight loop).
+template <typename T, typename IT> __global__ __launch_bounds__(256) void pw_narrow(const RedArgs A) {
     for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
-        ((T *)A.out)[ob] = narrow_sum<T>(A, ib);
+        ((T *)A.out)[ob] = narrow_chain<T, IT, (sizeof(T) == 8 ? 24 : 32)>(A, ib);
     }
 }
 
 // the same with a wavefront per output (narrow_wave_ok)
-template <typename T> __global__ __launch_bounds__(256) void pw_narrow_wave(const RedArgs A) {
+template <typename T, typename IT> __global__ __launch_bounds__(256) void pw_narrow_wave(const RedArgs A) {
     const int64_t w0 = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
     for (int64_t o = w0; o < A.nout; o += (int64_t)gridDim.x * (blockDim.x >> 6)) {
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
-        const T acc = narrow_wave_sum<T>(A, ib);
+        const T acc = narrow_wave_chain<T, IT>(A, ib);
         if ((threadIdx.x & 63) == 0) ((T *)A.out)[ob] = acc;
     }
 }
+
+template <typename T> static void launch_narrow(const RedArgs &A, int bpc, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // maximum / minimum on floats
@@ -548,12 +605,18 @@ __global__ __launch_bounds__(256) void ext_windows(const RedArgs A, int64_t nwin
     }
 }
 
-// Contiguous windows (W one run of stride 1, no outer axes): the window's
-// best value and NaN outcome from 16-byte loads, all issued before use.
-// Positions only matter when the best value is a zero (a -0/+0 tie picks a
-// sign); such windows are recomputed by ext_window_summary.  Workgroup g
-// takes `wpb` consecutive windows of one output and combines their
-// summaries in order, so ext_fold sees nwin / wpb summaries per output.
+// Contiguous windows (W one run of stride 1, no outer axes), `wpb`
+// consecutive windows of one output per workgroup, summarised in one pass:
+// with every window running NumPy's SIMD loop, the ordered combine of their
+// summaries reduces to
+//   best value   - the best non-NaN element of the run (equal non-zero
+//                  values have equal bits, so tie order is invisible);
+//   NaN outcome  - canonical NaN if a NaN occurs before the run's last
+//                  window or in the vector part of the last window, else the
+//                  first NaN of the last window's scalar tail.
+// Only a run whose best value is a zero (a -0/+0 tie picks a sign by
+// position) is recomputed window by window with ext_window_summary.  Each
+// thread keeps its candidate across the run; the workgroup combines once.
 template <typename T, bool MAX>
 __global__ __launch_bounds__(256) void ext_fast(const RedArgs A, int64_t wpb, int64_t nblk) {
     constexpr int V = sizeof(T) == 8 ? 8 : 16;  // NumPy's AVX-512 lanes
@@ -561,33 +624,32 @@ __global__ __launch_bounds__(256) void ext_fast(const RedArgs A, int64_t wpb, in
     constexpr int NV = 8192 / (256 * VEC);      // vectors per thread in a full window
     using VT = T __attribute__((ext_vector_type(VEC)));
     __shared__ ExtCand<T> red[256];
-    __shared__ T sm[256];
-    __shared__ uint32_t sflags[256];
-    __shared__ int64_t stail[256];
-    __shared__ T stailv[256];
+    __shared__ T wm[4], wtv[4];
+    __shared__ uint32_t wfl[4];
+    __shared__ int64_t wtk[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int64_t gb = blockIdx.x; gb < A.nout * nblk; gb += gridDim.x) {
         const int64_t o = gb / nblk, bk = gb - o * nblk;
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
-        ExtSum<T> acc;
-        acc.has_m = acc.has_nan = acc.simd = acc.flag = 0;
-        acc.m = acc.nanv = T(0);
-        const int64_t w_end = min(A.nwin, (bk + 1) * wpb);
-        for (int64_t w = bk * wpb; w < w_end; w++) {
+        const int64_t w0 = bk * wpb, w1 = min(A.nwin, (bk + 1) * wpb);
+        T m = T(0), tv = T(0);
+        uint32_t has_m = 0, nan_early = 0, vnan_last = 0, any_simd = 0;
+        int64_t tk = INT64_MAX;  // first NaN in the last window's tail
+        for (int64_t w = w0; w < w1; w++) {
+            const bool last = w == w1 - 1;
             const int64_t s0 = w * A.bufsize;
             const int64_t L = min(A.bufsize, A.wsize - s0);
             const int64_t skip = w == 0 ? 1 : 0;
             const int64_t len = L - skip, nv = len - len % V;
+            any_simd |= len >= 1;
             const T *x = (const T *)A.in + ib + s0;
-            T m = T(0);
-            uint32_t has_m = 0, vnan = 0;
-            int64_t tk = INT64_MAX;  // first NaN in the tail
-            T tv = T(0);
             auto visit = [&](T v, int64_t e) {  // e: index within the window
                 if (e < skip) return;
                 const int64_t rel = e - skip;
                 if (v != v) {
-                    if (rel < nv) vnan = 1;
+                    if (!last) nan_early = 1;
+                    else if (rel < nv) vnan_last = 1;
                     else if (rel < tk) {
                         tk = rel;
                         tv = v;
@@ -609,41 +671,58 @@ __global__ __launch_bounds__(256) void ext_fast(const RedArgs A, int64_t wpb, in
             } else {
                 for (int64_t e = threadIdx.x; e < L; e += blockDim.x) visit(x[e], e);
             }
-            sm[threadIdx.x] = m;
-            sflags[threadIdx.x] = has_m | (vnan << 1);
-            stail[threadIdx.x] = tk;
-            stailv[threadIdx.x] = tv;
-            __syncthreads();
-            for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
-                if ((int)threadIdx.x < s2) {
-                    const uint32_t fa = sflags[threadIdx.x], fb = sflags[threadIdx.x + s2];
-                    if ((fb & 1) && (!(fa & 1) || beats<T, MAX>(sm[threadIdx.x + s2], sm[threadIdx.x])))
-                        sm[threadIdx.x] = sm[threadIdx.x + s2];
-                    sflags[threadIdx.x] = fa | fb;
-                    if (stail[threadIdx.x + s2] < stail[threadIdx.x]) {
-                        stail[threadIdx.x] = stail[threadIdx.x + s2];
-                        stailv[threadIdx.x] = stailv[threadIdx.x + s2];
-                    }
-                }
-                __syncthreads();
+        }
+        // workgroup combine: wavefront shuffles, then the 4 wavefronts
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const T om = __shfl_xor(m, off, 64);
+            const uint32_t oh = __shfl_xor(has_m, off, 64);
+            if (oh && (!has_m || beats<T, MAX>(om, m))) {
+                m = om;
+                has_m = 1;
             }
-            const T M = sm[0];
-            const uint32_t fl = sflags[0];
-            const int64_t TK = stail[0];
-            const T TV = stailv[0];
-            __syncthreads();
-            ExtSum<T> S;
-            if ((fl & 1) && M == T(0)) {
-                S = ext_window_summary<T, MAX>(A, o * A.nwin + w, red);  // a zero best: signs by position
-            } else {
-                S.simd = len >= 1;
-                S.has_m = fl & 1;
-                S.m = M;
-                S.flag = 1;
-                S.has_nan = (fl >> 1) | (TK != INT64_MAX);
-                S.nanv = (fl >> 1) ? canonical_nan(T(0)) : TV;
+            const int64_t otk = __shfl_xor(tk, off, 64);
+            const T otv = __shfl_xor(tv, off, 64);
+            if (otk < tk) {
+                tk = otk;
+                tv = otv;
             }
-            if (threadIdx.x == 0) acc = ext_combine<T, MAX>(acc, S);
+        }
+        const uint32_t fl = (__ballot(nan_early) ? 2u : 0u) | (__ballot(vnan_last) ? 4u : 0u) | has_m;
+        if (lane == 0) {
+            wm[wave] = m;
+            wfl[wave] = fl;
+            wtk[wave] = tk;
+            wtv[wave] = tv;
+        }
+        __syncthreads();
+        T M = T(0), TV = T(0);
+        uint32_t FL = 0;
+        int64_t TK = INT64_MAX;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
+            if ((wfl[q] & 1) && (!(FL & 1) || beats<T, MAX>(wm[q], M))) M = wm[q];
+            FL |= wfl[q];
+            if (wtk[q] < TK) {
+                TK = wtk[q];
+                TV = wtv[q];
+            }
+        }
+        __syncthreads();
+        ExtSum<T> acc;
+        if ((FL & 1) && M == T(0)) {  // a zero best: signs by position, window by window
+            acc.has_m = acc.has_nan = acc.simd = acc.flag = 0;
+            acc.m = acc.nanv = T(0);
+            for (int64_t w = w0; w < w1; w++) {
+                const ExtSum<T> S = ext_window_summary<T, MAX>(A, o * A.nwin + w, red);
+                acc = ext_combine<T, MAX>(acc, S);
+            }
+        } else {
+            acc.simd = any_simd;
+            acc.has_m = FL & 1;
+            acc.m = M;
+            acc.flag = 1;
+            acc.has_nan = (FL & 6) || TK != INT64_MAX;
+            acc.nanv = (FL & 6) ? canonical_nan(T(0)) : TV;
         }
         if (threadIdx.x == 0) ((ExtSum<T> *)A.scratch)[gb] = acc;
     }
@@ -833,6 +912,28 @@ static int grid_cap(int64_t items, int bpc) {
     return (int)(g > 0 ? g : 1);
 }
 
+template <typename T, typename IT> static void launch_narrow_t(const RedArgs &A, int bpc, hipStream_t s) {
+    if (narrow_wave_ok(A))
+        hipLaunchKernelGGL((pw_narrow_wave<T, IT>), dim3(grid_cap((A.nout + 3) / 4, bpc)), dim3(256), 0, s, A);
+    else
+        hipLaunchKernelGGL((pw_narrow<T, IT>), dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
+}
+template <typename T> static void launch_narrow(const RedArgs &A, int bpc, hipStream_t s) {
+    switch (A.in_dt) {
+        case IONP_BOOL: launch_narrow_t<T, bool>(A, bpc, s); break;
+        case IONP_INT8: launch_narrow_t<T, int8_t>(A, bpc, s); break;
+        case IONP_UINT8: launch_narrow_t<T, uint8_t>(A, bpc, s); break;
+        case IONP_INT16: launch_narrow_t<T, int16_t>(A, bpc, s); break;
+        case IONP_UINT16: launch_narrow_t<T, uint16_t>(A, bpc, s); break;
+        case IONP_INT32: launch_narrow_t<T, int32_t>(A, bpc, s); break;
+        case IONP_UINT32: launch_narrow_t<T, uint32_t>(A, bpc, s); break;
+        case IONP_INT64: launch_narrow_t<T, int64_t>(A, bpc, s); break;
+        case IONP_UINT64: launch_narrow_t<T, uint64_t>(A, bpc, s); break;
+        case IONP_FLOAT32: launch_narrow_t<T, float>(A, bpc, s); break;
+        default: launch_narrow_t<T, double>(A, bpc, s); break;
+    }
+}
+
 static ionp_status launched() {
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
@@ -916,7 +1017,7 @@ template <typename T> static ionp_status run_pw_fast(RedArgs &A, const ionp_conf
     const int64_t nwindows = A.nout * F.nfull;
     // a folder shares its CU with producers: with one producer workgroup per
     // CU the serial fold keeps pace with HBM (measured on MI355X)
-    const int pbpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : (F.folders ? 1 : 8);
+    const int pbpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : (F.folders ? 1 : 2);
     hipLaunchKernelGGL(pw_fast<T>, dim3(F.folders + grid_cap(nwindows, pbpc)), dim3(pw_fast_threads<T>()), 0, s, A, F,
                        nwindows);
     if (!F.folders) hipLaunchKernelGGL(pw_fold<T>, dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
@@ -926,10 +1027,7 @@ template <typename T> static ionp_status run_pw_fast(RedArgs &A, const ionp_conf
 template <typename T> static ionp_status run_float_add(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
     const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
     if (A.nw == 0) {
-        if (narrow_wave_ok(A))
-            hipLaunchKernelGGL(pw_narrow_wave<T>, dim3(grid_cap((A.nout + 3) / 4, bpc)), dim3(256), 0, s, A);
-        else
-            hipLaunchKernelGGL(pw_narrow<T>, dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
+        launch_narrow<T>(A, bpc, s);
         return launched();
     }
     // contiguous rows with at least one full 8192-element window, native dtype;
