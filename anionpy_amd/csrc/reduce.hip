@@ -329,7 +329,7 @@ template <typename T, bool EXACT> __device__ inline T pw_window_wave(const T *x,
 // wavefront per window, or one workgroup per CU, are both slower.)
 template <typename T> constexpr int pw_fast_threads() { return sizeof(T) == 8 ? 256 : 128; }
 
-template <typename T>
+template <typename T, bool PREFETCH>
 __global__ __launch_bounds__(256) void pw_fast(const RedArgs A, const PwFastState F, int64_t nwindows) {
     constexpr int CH = 16 / sizeof(T);
     constexpr int LPL = 8 / CH;
@@ -348,15 +348,36 @@ __global__ __launch_bounds__(256) void pw_fast(const RedArgs A, const PwFastStat
     }
     const int lf = threadIdx.x / LPL, jq = (lane % LPL) * CH;
     const int64_t nprod = (int64_t)gridDim.x - F.folders;
-    for (int64_t g = (int64_t)blockIdx.x - F.folders; g < nwindows; g += nprod) {
+    auto window_base = [&](int64_t g) {
         const int64_t o = g / F.nfull, w = g - o * F.nfull;
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
-        const T *xw = (const T *)A.in + ib + w * 8192;
-        const T *x = xw + (int64_t)lf * 128 + jq;
-        V v[16];
+        return (const T *)A.in + ib + w * 8192;
+    };
+    V v[16], nv[16];
+    int64_t g = (int64_t)blockIdx.x - F.folders;
+    if (PREFETCH && g < nwindows) {
+        const T *x = window_base(g) + (int64_t)lf * 128 + jq;
 #pragma unroll
-        for (int k = 0; k < 16; k++) v[k] = *(const V *)(x + 8 * k);
+        for (int k = 0; k < 16; k++) nv[k] = *(const V *)(x + 8 * k);
+    }
+    for (; g < nwindows; g += nprod) {
+        const int64_t o = g / F.nfull, w = g - o * F.nfull;
+        const T *xw = window_base(g);
+        if constexpr (PREFETCH) {
+            // this window was loaded one iteration ago; request the next one
+#pragma unroll
+            for (int k = 0; k < 16; k++) v[k] = nv[k];
+            if (g + nprod < nwindows) {
+                const T *xn = window_base(g + nprod) + (int64_t)lf * 128 + jq;
+#pragma unroll
+                for (int k = 0; k < 16; k++) nv[k] = *(const V *)(xn + 8 * k);
+            }
+        } else {
+            const T *x = xw + (int64_t)lf * 128 + jq;
+#pragma unroll
+            for (int k = 0; k < 16; k++) v[k] = *(const V *)(x + 8 * k);
+        }
         // accumulators jq .. jq+CH-1 of leaf lf, each over k in order
         V r = v[0];
 #pragma unroll
@@ -1018,8 +1039,16 @@ template <typename T> static ionp_status run_pw_fast(RedArgs &A, const ionp_conf
     // a folder shares its CU with producers: with one producer workgroup per
     // CU the serial fold keeps pace with HBM (measured on MI355X)
     const int pbpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : (F.folders ? 1 : 2);
-    hipLaunchKernelGGL(pw_fast<T>, dim3(F.folders + grid_cap(nwindows, pbpc)), dim3(pw_fast_threads<T>()), 0, s, A, F,
-                       nwindows);
+    // with one producer per CU (folders) the next window's loads are issued
+    // before the current one is combined; with several per CU they are not
+    // (measured on MI355X both ways); split 1 / 2 forces either
+    const bool prefetch = (cfg && cfg->split > 0) ? cfg->split == 2 : F.folders > 0;
+    if (prefetch)
+        hipLaunchKernelGGL((pw_fast<T, true>), dim3(F.folders + grid_cap(nwindows, pbpc)), dim3(pw_fast_threads<T>()), 0, s,
+                           A, F, nwindows);
+    else
+        hipLaunchKernelGGL((pw_fast<T, false>), dim3(F.folders + grid_cap(nwindows, pbpc)), dim3(pw_fast_threads<T>()), 0,
+                           s, A, F, nwindows);
     if (!F.folders) hipLaunchKernelGGL(pw_fold<T>, dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
     return launched();
 }

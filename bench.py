@@ -34,7 +34,34 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extra", action="store_true")
+    p.add_argument("--roofline-pmc", action="store_true",
+                   help="only run the dominant kernel and a plain copy of known size, 5 times each "
+                        "(the workload of the rocprofv3 FETCH_SIZE / WRITE_SIZE passes)")
     return p.parse_args()
+
+
+def roofline_pmc_workload():
+    """ew_rows<double,multiply> on X(8192,8192) * c(8192,1), then torch's
+    device copy of 2^26 float64 (536,870,912 B read and written), 5 times
+    each: the counter passes scale the multiply's FETCH_SIZE / WRITE_SIZE by
+    what the same counters read for the copy."""
+    import torch
+    import anionpy_amd as ap
+    torch.cuda.set_device(0)
+    X = ap.asarray(np.random.default_rng(1).standard_normal((8192, 8192)))
+    c = ap.asarray(np.random.default_rng(2).uniform(0.5, 2, (8192, 1)))
+    out = ap.multiply(X, c)
+    src = torch.ones(2 ** 26, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        ap.multiply(X, c, out=out)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"multiply_bytes": X.nbytes + c.nbytes + out.nbytes, "copy_bytes_each_way": src.numel() * 8,
+                      "launches": 5}), flush=True)
 
 
 def log(*a):
@@ -43,6 +70,9 @@ def log(*a):
 
 def main():
     args = parse()
+    if args.roofline_pmc:
+        roofline_pmc_workload()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this backend runs on one GPU; multi-GPU sharding is described "
                                    "in DESIGN.md and not launched", "n_gpus": args.gpus}))
@@ -215,28 +245,35 @@ def learned_extras():
         return json.load(f).get("meta")
 
 
-def cpu_baseline():
-    """The scalar C oracle (single thread) on a 512-row sample of each of the
-    four step operations."""
+def cpu_baseline(min_seconds=10.0):
+    """The scalar C oracle (single thread) on the step workload itself, at
+    full size, repeated until at least `min_seconds` of CPU work; NumPy on
+    the same host and arrays is timed beside it."""
     sys.path.insert(0, HERE)
     import oracle as O
-    rows = 512
-    X = np.random.default_rng(1).standard_normal((rows, 8192))
-    c = np.random.default_rng(2).uniform(0.5, 2, (rows, 1))
+    X = np.random.default_rng(1).standard_normal((8192, 8192))
+    c = np.random.default_rng(2).uniform(0.5, 2, (8192, 1))
     X32, c32 = X.astype(np.float32), c.astype(np.float32)
-    I = np.random.default_rng(3).integers(-2 ** 62, 2 ** 62, (rows, 8192))
-    j = np.random.default_rng(8).integers(-2 ** 62, 2 ** 62, (rows, 1))
-    XT = np.random.default_rng(1).standard_normal((8192, 2 * rows)).astype(np.float32)
-    work = [("multiply", X, c), ("multiply", X32, c32), ("multiply", XT.T[::2], XT.T[::2][:, :1]), ("add", I, j)]
-    nbytes = 0
+    I = np.random.default_rng(3).integers(-2 ** 62, 2 ** 62, (8192, 8192))
+    j = np.random.default_rng(8).integers(-2 ** 62, 2 ** 62, (8192, 1))
+    work = [("multiply", X, c), ("multiply", X32, c32), ("multiply", X32.T[::2], c32[::2]), ("add", I, j)]
+    nbytes, reps = 0, 0
     t0 = time.perf_counter()
-    for op, a, b in work:
-        o = O.binary(op, a, b)
-        nbytes += a.nbytes + b.nbytes + o.nbytes
+    while reps == 0 or time.perf_counter() - t0 < min_seconds:
+        for op, a, b in work:
+            o = O.binary(op, a, b)
+            nbytes += a.nbytes + b.nbytes + o.nbytes  # X.T[::2]: its own 128 MiB, as on the GPU
+            del o
+        reps += 1
     dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    for op, a, b in work:
+        getattr(np, op)(a, b)
+    dnp = time.perf_counter() - t1
     return {"value": round(nbytes / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"C oracle (scalar restatement) on {rows} rows of each step op "
-                      f"({nbytes / 1e6:.0f} MB algorithmic, {dt:.1f} s), host CPU: {cpu_name()}"}
+            "sample": f"C oracle (scalar restatement of the reference loop) over the full step workload, "
+                      f"{reps} pass(es), {nbytes / 1e9:.1f} GB algorithmic in {dt:.1f} s; host CPU: {cpu_name()}",
+            "numpy_same_host_GBps": round(nbytes / reps / dnp / 1e9, 3)}
 
 
 def cpu_name():
