@@ -26,7 +26,8 @@ from ._lib import IonpError
 __all__ = ["ndarray", "asarray", "empty", "zeros", "Ufunc", "add", "subtract", "multiply", "divide",
            "true_divide", "maximum", "minimum", "less", "less_equal", "greater", "greater_equal", "equal",
            "not_equal", "where", "sum", "max", "amax", "min", "amin", "argmax", "argmin", "result_type",
-           "AxisError", "UFuncTypeError", "IonpError", "setbufsize", "getbufsize"]
+           "AxisError", "UFuncTypeError", "IonpError", "setbufsize", "getbufsize", "set_config_policy",
+           "get_config_policy"]
 
 _NP_DTYPES = [np.bool_, np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64,
               np.uint64, np.float32, np.float64]
@@ -230,7 +231,8 @@ class ndarray:
         strides = _c_strides(self.shape) if order == "C" else list(t.strides)[: self.ndim]
         out = empty_strided(self.shape, strides, dtype)
         o = out._desc()
-        _call(_lib.load().ionp_cast, ctypes.byref(d), ctypes.byref(o), None, _lib.stream_handle())
+        _call(_lib.load().ionp_cast, ctypes.byref(d), ctypes.byref(o), _cfg(_lib.CALL_CAST, 0, (d,), o),
+              _lib.stream_handle())
         return out
 
     # ---- transfer ---------------------------------------------------------------------
@@ -416,6 +418,43 @@ def _finish(out):
     return out
 
 
+# ---- launch-configuration policy ---------------------------------------------
+_advisor = None
+
+
+def set_config_policy(policy):
+    """"heuristic" (default): every call uses the built-in launch heuristic.
+    "learned": the cost model (anionpy_amd/costmodel.json) proposes a launch
+    configuration per call layout; results are bit-identical either way."""
+    global _advisor
+    if policy == "heuristic":
+        _advisor = None
+    elif policy == "learned":
+        from . import costmodel
+        model, _ = costmodel.load()
+        _advisor = costmodel.Advisor(model)
+    else:
+        raise ValueError(f"unknown configuration policy {policy!r}")
+
+
+def get_config_policy():
+    return "heuristic" if _advisor is None else "learned"
+
+
+def _cfg(kind, op, ins, out, axes=None, naxes=0, keepdims=False):
+    """ctypes pointer to the configuration of this call (None: heuristic)."""
+    if _advisor is None:
+        return None
+    rec = _lib.Call()
+    rec.kind, rec.op, rec.naxes, rec.keepdims = kind, op, naxes, int(bool(keepdims))
+    for i in range(builtins.max(naxes, 0)):
+        rec.axes[i] = axes[i]
+    for i, d in enumerate(ins):
+        rec.inp[i] = d
+    rec.out = out
+    return ctypes.byref(_advisor.config(rec))
+
+
 class Ufunc:
     """Binary ufunc object: __call__ and .reduce (ionp-py/src/lib.rs:8697-10227)."""
 
@@ -443,8 +482,8 @@ class Ufunc:
             _call(L.ionp_plan_binary, self._code, ctypes.byref(da), ctypes.byref(db), ctypes.byref(t))
             out = empty_strided(tuple(t.shape[: t.ndim]), tuple(t.strides[: t.ndim]), _NP_DTYPES[t.dtype])
         do = out._desc()
-        _call(L.ionp_binary, self._code, ctypes.byref(da), ctypes.byref(db), ctypes.byref(do), None,
-              _lib.stream_handle())
+        _call(L.ionp_binary, self._code, ctypes.byref(da), ctypes.byref(db), ctypes.byref(do),
+              _cfg(_lib.CALL_BINARY, self._code, (da, db), do), _lib.stream_handle())
         return out
 
     def reduce(self, a, axis=0, dtype=None, out=None, keepdims=False):
@@ -474,8 +513,8 @@ def _reduce(code, a, axis, dtype, keepdims, out=None):
     if out is None:
         out = empty_strided(tuple(t.shape[: t.ndim]), tuple(t.strides[: t.ndim]), _NP_DTYPES[t.dtype])
     do = out._desc()
-    _call(L.ionp_reduce, code, ctypes.byref(d), axes, naxes, int(bool(keepdims)), ctypes.byref(do), None,
-          _lib.stream_handle())
+    _call(L.ionp_reduce, code, ctypes.byref(d), axes, naxes, int(bool(keepdims)), ctypes.byref(do),
+          _cfg(_lib.CALL_REDUCE, code, (d,), do, axes, naxes, keepdims), _lib.stream_handle())
     return out
 
 
@@ -552,6 +591,6 @@ def where(condition, x, y):
     _call(L.ionp_plan_where, ctypes.byref(dc), ctypes.byref(dx), ctypes.byref(dy), ctypes.byref(t))
     out = empty_strided(tuple(t.shape[: t.ndim]), tuple(t.strides[: t.ndim]), _NP_DTYPES[t.dtype])
     do = out._desc()
-    _call(L.ionp_where, ctypes.byref(dc), ctypes.byref(dx), ctypes.byref(dy), ctypes.byref(do), None,
-          _lib.stream_handle())
+    _call(L.ionp_where, ctypes.byref(dc), ctypes.byref(dx), ctypes.byref(dy), ctypes.byref(do),
+          _cfg(_lib.CALL_WHERE, 0, (dc, dx, dy), do), _lib.stream_handle())
     return out

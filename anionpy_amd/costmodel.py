@@ -171,3 +171,25 @@ def load(path=WEIGHTS):
     with open(path) as f:
         st = json.load(f)
     return MLP.from_state(st), st.get("meta", {})
+
+
+class Advisor:
+    """Per-call launch configuration proposed by the model, cached by the
+    call's layout features (ionp_call_features): the MLP's HIP forward runs
+    once per distinct layout.  Grid entry 0 is the built-in heuristic."""
+
+    def __init__(self, model):
+        self.model = model
+        self.cache = {}
+
+    def config(self, rec):
+        f = (ctypes.c_float * _lib.NFEATURES)()
+        _call(_lib.load().ionp_call_features, ctypes.byref(rec), f)
+        key = bytes(f)
+        cfg = self.cache.get(key)
+        if cfg is None:
+            idx = int(self.model.predict(np.frombuffer(f, dtype=np.float32)[None])[0])
+            cfg = _lib.Config(*GRID[idx])
+            self.cache[key] = cfg
+        return cfg
+

@@ -61,7 +61,7 @@ def test_learned_config_beats_heuristic_on_held_out(ap):
     from anionpy_amd import costmodel as CM
     from anionpy_amd.train_costmodel import fit, load_data
     f_train, best_train, f_test, t_test = load_data(CM.SWEEP_DATA)
-    m, meta = fit(f_train, best_train, f_test, t_test, epochs=20)
+    m, meta = fit(f_train, best_train, f_test, t_test, epochs=40)
     print(meta)
     assert meta["loss_last"] < meta["loss_first"]
     assert meta["regret_learned_aggregate"] < meta["regret_heuristic_aggregate"]
@@ -83,3 +83,31 @@ def test_committed_weights_on_live_sweep(ap):
     heuristic, _ = CM.regret(t, np.zeros(len(specs), dtype=int))
     print(f"live aggregate regret: learned {learned:.4f}, heuristic {heuristic:.4f}")
     assert learned < 0.10
+
+
+def test_learned_policy_is_bit_identical(ap):
+    """Calls under the learned policy (a configuration per call layout)
+    write the same bytes as under the heuristic."""
+    rng = np.random.default_rng(31)
+    X = ap.asarray(rng.standard_normal((300, 700)))
+    c = ap.asarray(rng.uniform(0.5, 2, (300, 1)))
+    I = ap.asarray(rng.integers(-1000, 1000, (300, 700), dtype=np.int32))
+    cond = ap.asarray(rng.random((300, 700)) < 0.5)
+
+    def run():
+        return [ap.multiply(X, c).get(), ap.add(X.T[::2], 1.5).get(), ap.add.reduce(X, axis=1).get(),
+                ap.add.reduce(X, axis=0).get(), ap.maximum.reduce(X, axis=None).get(), ap.sum(I, axis=1).get(),
+                ap.where(cond, X, c).get(), X.astype(np.float32).get()]
+
+    ap.set_config_policy("heuristic")
+    want = run()
+    ap.set_config_policy("learned")
+    try:
+        assert ap.get_config_policy() == "learned"
+        got = run()
+        got2 = run()  # second pass: proposals from the cache
+    finally:
+        ap.set_config_policy("heuristic")
+    for w, g, g2 in zip(want, got, got2):
+        assert w.dtype == g.dtype and w.shape == g.shape
+        assert w.tobytes() == g.tobytes() == g2.tobytes()
