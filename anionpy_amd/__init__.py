@@ -220,6 +220,14 @@ class ndarray:
             acc *= n
         return True
 
+    def flags_f_contiguous(self):
+        acc = 1
+        for n, s in zip(self.shape, self.strides_el):
+            if n != 1 and s != acc:
+                return False
+            acc *= n
+        return True
+
     def copy(self):
         return self.astype(self.dtype, order="C")
 
@@ -325,6 +333,60 @@ def empty_strided(shape, strides_el, dtype):
         nel = hi - lo + 1
     buf = torch.empty(builtins.max(nel, 1) * dtype.itemsize, dtype=torch.uint8, device=_device())
     return ndarray(buf, dtype, shape, strides_el, -lo)
+
+
+def _f_strides(shape):
+    st, acc = [0] * len(shape), 1
+    for i in range(len(shape)):
+        st[i] = acc
+        acc *= builtins.max(shape[i], 1)
+    return st
+
+
+def _byte_range(x):
+    """[first, last] byte addresses the view touches (None when empty)."""
+    if x.size == 0 or x._buf is None:
+        return None
+    lo, hi = _span(x.shape, x.strides_el)
+    base = x._buf.data_ptr()
+    return base + (x.offset + lo) * x.itemsize, base + (x.offset + hi + 1) * x.itemsize - 1
+
+
+def _needs_temp(out, inputs):
+    """True when `out` shares memory with an input other than as the very
+    same view: writing results straight into it could overwrite elements
+    before they are read (NumPy then computes into a temporary)."""
+    ro = _byte_range(out)
+    if ro is None:
+        return False
+    for x in inputs:
+        rx = _byte_range(x)
+        if rx is None or rx[1] < ro[0] or ro[1] < rx[0]:
+            continue
+        same = (x._buf.data_ptr() == out._buf.data_ptr() and x.offset == out.offset and x.itemsize == out.itemsize
+                and tuple(x.shape) == tuple(out.shape) and tuple(x.strides_el) == tuple(out.strides_el))
+        if not same:
+            return True
+    return False
+
+
+def _layout(shape, k_strides, order, inputs):
+    """Output strides for order= K / C / F / A."""
+    if order in (None, "K"):
+        return list(k_strides)
+    if order == "A":
+        order = "F" if inputs and all(x.flags_f_contiguous() and not x.flags_c_contiguous() for x in inputs) else "C"
+    if order == "C":
+        return _c_strides(shape)
+    if order == "F":
+        return _f_strides(shape)
+    raise ValueError(f"order must be one of 'C', 'F', 'A', or 'K' (got {order!r})")
+
+
+def _copy_into(src, dst):
+    """dst[...] = src (same shape; a cast kernel, exact for equal dtypes)."""
+    d, o = src._desc(), dst._desc()
+    _call(_lib.load().ionp_cast, ctypes.byref(d), ctypes.byref(o), None, _lib.stream_handle())
 
 
 def empty(shape, dtype=np.float64):
@@ -469,21 +531,47 @@ class Ufunc:
     def __repr__(self):
         return f"<ufunc '{self.__name__}'>"
 
-    def __call__(self, a, b, out=None, dtype=None):
-        """ufunc.__call__ (lib.rs:9204) -> binary_op (ionp-core/src/ufunc.rs:3628)."""
+    def __call__(self, a, b, out=None, where=True, dtype=None, order="K"):
+        """ufunc.__call__ (lib.rs:9204) -> binary_op (ionp-core/src/ufunc.rs:3628).
+
+        out=: any strides; an `out` that overlaps an input other than as the
+        same view gets the result through a temporary (write_out,
+        ufunc.rs:10112).  where=: elements where the mask is False keep
+        out's previous values.  dtype=: the loop dtype (inputs cast first).
+        order=: K (NumPy's default), C, F or A for a freshly allocated out."""
+        if isinstance(out, tuple):
+            if len(out) != 1:
+                raise ValueError("The 'out' tuple must have exactly one entry per ufunc output")
+            out = out[0]
         a, b = _operands(a, b)
         if dtype is not None:
             dt = _dtype(dtype)
             a, b = a.astype(dt), b.astype(dt)
         L = _lib.load()
         da, db = a._desc(), b._desc()
+        t = _lib.ArrayDesc()
+        _call(L.ionp_plan_binary, self._code, ctypes.byref(da), ctypes.byref(db), ctypes.byref(t))
+        shape = tuple(t.shape[: t.ndim])
+        masked = not (where is True)
         if out is None:
-            t = _lib.ArrayDesc()
-            _call(L.ionp_plan_binary, self._code, ctypes.byref(da), ctypes.byref(db), ctypes.byref(t))
-            out = empty_strided(tuple(t.shape[: t.ndim]), tuple(t.strides[: t.ndim]), _NP_DTYPES[t.dtype])
-        do = out._desc()
+            out = empty_strided(shape, _layout(shape, t.strides[: t.ndim], order, [a, b]), _NP_DTYPES[t.dtype])
+            target = out
+        elif masked or _needs_temp(out, [a, b]):
+            target = empty_strided(shape, list(t.strides[: t.ndim]), out.dtype)
+        else:
+            target = out
+        do = target._desc()
         _call(L.ionp_binary, self._code, ctypes.byref(da), ctypes.byref(db), ctypes.byref(do),
               _cfg(_lib.CALL_BINARY, self._code, (da, db), do), _lib.stream_handle())
+        if masked:
+            mask = asarray(where)
+            if mask.dtype != np.bool_:
+                raise TypeError("where= must be a boolean array")
+            dm, dr, dout = mask._desc(), target._desc(), out._desc()
+            _call(L.ionp_where, ctypes.byref(dm), ctypes.byref(dr), ctypes.byref(dout), ctypes.byref(dout), None,
+                  _lib.stream_handle())
+        elif target is not out:
+            _copy_into(target, out)
         return out
 
     def reduce(self, a, axis=0, dtype=None, out=None, keepdims=False):
@@ -510,11 +598,16 @@ def _reduce(code, a, axis, dtype, keepdims, out=None):
     t = _lib.ArrayDesc()
     dt = -1 if dtype is None else _CODE[_dtype(dtype)]
     _call(L.ionp_plan_reduce, code, ctypes.byref(d), axes, naxes, int(bool(keepdims)), dt, ctypes.byref(t))
-    if out is None:
-        out = empty_strided(tuple(t.shape[: t.ndim]), tuple(t.strides[: t.ndim]), _NP_DTYPES[t.dtype])
+    final = out
+    if out is None or _needs_temp(out, [a]):
+        out = empty_strided(tuple(t.shape[: t.ndim]), tuple(t.strides[: t.ndim]),
+                            _NP_DTYPES[t.dtype] if final is None else final.dtype)
     do = out._desc()
     _call(L.ionp_reduce, code, ctypes.byref(d), axes, naxes, int(bool(keepdims)), ctypes.byref(do),
           _cfg(_lib.CALL_REDUCE, code, (d,), do, axes, naxes, keepdims), _lib.stream_handle())
+    if final is not None and final is not out:
+        _copy_into(out, final)
+        return final
     return out
 
 
