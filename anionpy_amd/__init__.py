@@ -275,14 +275,14 @@ class ndarray:
         return bool(self.item())
 
     # ---- reductions as methods (ndarray_attrs.rs mirrors do_reduce_axis) --------------
-    def sum(self, axis=None, dtype=None, keepdims=False):
-        return sum(self, axis=axis, dtype=dtype, keepdims=keepdims)
+    def sum(self, axis=None, dtype=None, keepdims=False, initial=None):
+        return sum(self, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial)
 
-    def max(self, axis=None, keepdims=False):
-        return amax(self, axis=axis, keepdims=keepdims)
+    def max(self, axis=None, keepdims=False, initial=None):
+        return amax(self, axis=axis, keepdims=keepdims, initial=initial)
 
-    def min(self, axis=None, keepdims=False):
-        return amin(self, axis=axis, keepdims=keepdims)
+    def min(self, axis=None, keepdims=False, initial=None):
+        return amin(self, axis=axis, keepdims=keepdims, initial=initial)
 
     def argmax(self, axis=None, keepdims=False):
         return argmax(self, axis=axis, keepdims=keepdims)
@@ -574,11 +574,13 @@ class Ufunc:
             _copy_into(target, out)
         return out
 
-    def reduce(self, a, axis=0, dtype=None, out=None, keepdims=False):
-        """ufunc.reduce (lib.rs:9958) -> reduce_axis (ionp-core/src/ufunc.rs:7634)."""
+    def reduce(self, a, axis=0, dtype=None, out=None, keepdims=False, initial=None):
+        """ufunc.reduce (lib.rs:9958) -> reduce_axis (ionp-core/src/ufunc.rs:7634).
+        initial=: the fold's starting value (also the result of an empty
+        reduction)."""
         if not self._reducible:
             raise NotImplementedError(f"{self.__name__}.reduce is not provided by the MI355X backend")
-        return _reduce(self._code, asarray(a), axis, dtype, keepdims, out)
+        return _reduce(self._code, asarray(a), axis, dtype, keepdims, out, initial)
 
 
 def _axes_arg(axis):
@@ -591,7 +593,7 @@ def _axes_arg(axis):
     return arr, len(axis)
 
 
-def _reduce(code, a, axis, dtype, keepdims, out=None):
+def _reduce(code, a, axis, dtype, keepdims, out=None, initial=None):
     L = _lib.load()
     axes, naxes = _axes_arg(axis)
     d = a._desc()
@@ -602,8 +604,12 @@ def _reduce(code, a, axis, dtype, keepdims, out=None):
     if out is None or _needs_temp(out, [a]):
         out = empty_strided(tuple(t.shape[: t.ndim]), tuple(t.strides[: t.ndim]),
                             _NP_DTYPES[t.dtype] if final is None else final.dtype)
+    init = None
+    if initial is not None:  # one element of the output dtype, on the host
+        init = np.array(initial).astype(out.dtype).reshape(1)  # unsafe cast, as NumPy does
     do = out._desc()
-    _call(L.ionp_reduce, code, ctypes.byref(d), axes, naxes, int(bool(keepdims)), ctypes.byref(do),
+    _call(L.ionp_reduce, code, ctypes.byref(d), axes, naxes, int(bool(keepdims)),
+          None if init is None else init.ctypes.data, ctypes.byref(do),
           _cfg(_lib.CALL_REDUCE, code, (d,), do, axes, naxes, keepdims), _lib.stream_handle())
     if final is not None and final is not out:
         _copy_into(out, final)
@@ -626,18 +632,18 @@ equal = Ufunc("equal", _lib.EQUAL, False)
 not_equal = Ufunc("not_equal", _lib.NOT_EQUAL, False)
 
 
-def sum(a, axis=None, dtype=None, keepdims=False):
+def sum(a, axis=None, dtype=None, keepdims=False, initial=None):
     """sum (ionp-py/src/reductions.rs:1032) = add.reduce with axis=None default."""
-    return add.reduce(a, axis=axis, dtype=dtype, keepdims=keepdims)
+    return add.reduce(a, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial)
 
 
-def amax(a, axis=None, keepdims=False):
+def amax(a, axis=None, keepdims=False, initial=None):
     """max / amax (reductions.rs:1114) = maximum.reduce, axis=None default."""
-    return maximum.reduce(a, axis=axis, keepdims=keepdims)
+    return maximum.reduce(a, axis=axis, keepdims=keepdims, initial=initial)
 
 
-def amin(a, axis=None, keepdims=False):
-    return minimum.reduce(a, axis=axis, keepdims=keepdims)
+def amin(a, axis=None, keepdims=False, initial=None):
+    return minimum.reduce(a, axis=axis, keepdims=keepdims, initial=initial)
 
 
 max = amax  # noqa: A001  (NumPy-named API)

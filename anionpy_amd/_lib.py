@@ -75,7 +75,7 @@ SIGNATURES = {
     "ionp_binary": [_I, _D, _D, _D, _C, _V],
     "ionp_where": [_D, _D, _D, _D, _C, _V],
     "ionp_cast": [_D, _D, _C, _V],
-    "ionp_reduce": [_I, _D, _P(_I), _I, _I, _D, _C, _V],
+    "ionp_reduce": [_I, _D, _P(_I), _I, _I, _V, _D, _C, _V],
     "ionp_argext": [_I, _D, _I, _I, _D, _C, _V],
     "ionp_batch_create": [_P(Call), _I, _V, _P(_V)],
     "ionp_batch_run": [_V, _I, _V],

@@ -280,7 +280,7 @@ ionp_status ionp_cast(const ionp_array_desc *in, const ionp_array_desc *out, con
 }
 
 ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes, int32_t keepdims,
-                        const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
+                        const void *initial, const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
     if (ionp_status cs = check_cfg(cfg)) return cs;
     ionp_array_desc t;
     ionp_status s;
@@ -291,7 +291,7 @@ ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *ax
     normalize_axes(axes, naxes, in->ndim, mask);
     ReducePlan p;
     if ((s = plan_reduce(op, in, mask, out, &p))) return s;
-    return run_reduce(p, g_bufsize, cfg, as_stream(stream));
+    return run_reduce(p, g_bufsize, initial, cfg, as_stream(stream));
 }
 
 ionp_status ionp_argext(int32_t ismax, const ionp_array_desc *in, int32_t axis, int32_t keepdims,

@@ -246,7 +246,7 @@ template <typename T> __device__ void fadd_window(const BatchCall &C, char *part
     const T res = pairwise_block<T>([&](int i) { return ld<LT>(A.in_dt, A.in, ib + w_offset(A, s0 + i)); }, L,
                                     (T *)S.racc, (T *)S.val, S.leaf_lo, S.leaf_n, S.cnt);
     if (threadIdx.x == 0) {
-        if (C.direct) ((T *)A.out)[ob] = fadd(T(0), res);
+        if (C.direct) ((T *)A.out)[ob] = fadd(fseed<T>(A), res);
         else ((T *)(partials + C.part_off))[g] = res;
     }
 }
@@ -256,7 +256,7 @@ template <typename T> __device__ void fadd_fold(const BatchCall &C, const char *
     const int64_t per_out = A.rsize * A.nwin;
     for (int64_t o = threadIdx.x; o < A.nout; o += blockDim.x) {
         const T *p = (const T *)(partials + C.part_off) + o * per_out;
-        T acc = T(0);
+        T acc = fseed<T>(A);
         for (int64_t i = 0; i < per_out; i++) acc = fadd(acc, p[i]);
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
@@ -291,7 +291,7 @@ template <typename T> __device__ void fadd_small(const BatchCall &C, int64_t t) 
     int64_t ib, ob;
     out_offsets(A, (uint32_t)o, &ib, &ob);
     const int L = (int)A.wsize;
-    T acc = T(0);
+    T acc = fseed<T>(A);
     for (int64_t r = 0; r < A.rsize; r++) {
         const int64_t base = ib + r_offset(A, r);
         acc = fadd(acc, pw_leaf<T>([&](int i) { return ld<LT>(A.in_dt, A.in, base + w_offset(A, i)); }, L));
@@ -314,7 +314,7 @@ template <int LT> __device__ void iadd_small(const BatchCall &C, int64_t t) {
     out_offsets(A, (uint32_t)o, &ib, &ob);
     uint64_t acc = 0;
     for (int64_t q = 0; q < A.group; q++) acc += (uint64_t)ld<LT>(A.in_dt, A.in, ib + group_off(A, q));
-    ((ST *)A.out)[ob] = int_result<LT>(acc);
+    ((ST *)A.out)[ob] = int_result<LT>(acc + (A.has_init ? A.init_bits : 0));
 }
 
 template <int LT> __device__ void iadd_tile(const BatchCall &C, char *partials, int64_t g, uint64_t *red) {
@@ -333,7 +333,7 @@ template <int LT> __device__ void iadd_tile(const BatchCall &C, char *partials, 
     }
     if (threadIdx.x == 0) {
         using ST = typename dt_type<LT>::T;
-        if (C.direct) ((ST *)A.out)[ob] = int_result<LT>(red[0]);
+        if (C.direct) ((ST *)A.out)[ob] = int_result<LT>(red[0] + (A.has_init ? A.init_bits : 0));
         else ((uint64_t *)(partials + C.part_off))[g] = red[0];
     }
 }
@@ -347,7 +347,7 @@ template <int LT> __device__ void iadd_fold(const BatchCall &C, const char *part
         for (int64_t i = 0; i < C.nseg; i++) acc += p[i];
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
-        ((ST *)A.out)[ob] = int_result<LT>(acc);
+        ((ST *)A.out)[ob] = int_result<LT>(acc + (A.has_init ? A.init_bits : 0));
     }
 }
 
@@ -576,7 +576,7 @@ static ionp_status run_one(const ionp_call &c, hipStream_t s, const ionp_config 
         case IONP_CALL_BINARY: return ionp_binary(c.op, &c.in[0], &c.in[1], &c.out, cfg, s);
         case IONP_CALL_WHERE: return ionp_where(&c.in[0], &c.in[1], &c.in[2], &c.out, cfg, s);
         case IONP_CALL_CAST: return ionp_cast(&c.in[0], &c.out, cfg, s);
-        default: return ionp_reduce(c.op, &c.in[0], c.axes, c.naxes, c.keepdims, &c.out, cfg, s);
+        default: return ionp_reduce(c.op, &c.in[0], c.axes, c.naxes, c.keepdims, nullptr, &c.out, cfg, s);
     }
 }
 

@@ -251,6 +251,13 @@ template <typename F> __device__ inline auto in_type_dispatch(int dt, const F &f
     }
 }
 
+// Starting value of a float add fold: initial= when given, else 0.
+template <typename T> __device__ inline T fseed(const RedArgs &A) {
+    if (!A.has_init) return T(0);
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, A.init_bits);
+    else return __builtin_bit_cast(T, (uint32_t)A.init_bits);
+}
+
 template <typename T> struct FDT;
 template <> struct FDT<float> { static constexpr int v = IONP_FLOAT32; };
 template <> struct FDT<double> { static constexpr int v = IONP_FLOAT64; };
@@ -265,7 +272,7 @@ template <> struct FDT<double> { static constexpr int v = IONP_FLOAT64; };
 // ---------------------------------------------------------------------------
 template <typename T, typename IT, int U> __device__ inline T narrow_chain(const RedArgs &A, int64_t ib) {
     constexpr int LT = FDT<T>::v;
-    T acc = T(0);
+    T acc = fseed<T>(A);
     int64_t r = 0;
     if (A.nr == 1) {
         const int64_t st = A.r_st[0];
@@ -288,7 +295,7 @@ template <typename T, typename IT, int U> __device__ inline T narrow_chain(const
         }
         for (; r < A.rsize; r++) acc = acc + ld_t<LT, IT>(x, ib + r * st);
         if (acc != acc) {
-            acc = T(0);
+            acc = fseed<T>(A);
             for (int64_t q = 0; q < A.rsize; q++) acc = fadd(acc, ld_t<LT, IT>(x, ib + q * st));
         }
         return acc;
@@ -303,7 +310,7 @@ template <typename T, typename IT, int U> __device__ inline T narrow_chain(const
     }
     for (; r < A.rsize; r++) acc = acc + ld_t<LT, IT>(A.in, ib + r_offset(A, r));
     if (acc != acc) {
-        acc = T(0);
+        acc = fseed<T>(A);
         for (int64_t q = 0; q < A.rsize; q++) acc = fadd(acc, ld_t<LT, IT>(A.in, ib + r_offset(A, q)));
     }
     return acc;
@@ -351,7 +358,7 @@ template <typename T, typename IT> __device__ inline T narrow_wave_chain(const R
         const int64_t r = k * 64 + lane;
         cur[k] = r < n ? ld_t<LT, IT>(A.in, ib + off(r)) : T(0);
     }
-    T acc = T(0);
+    T acc = fseed<T>(A);
     for (int64_t rb = 0; rb < n; rb += BLK) {
         const int64_t nb = rb + BLK;
         if (nb < n) {
@@ -378,7 +385,7 @@ template <typename T, typename IT> __device__ inline T narrow_wave_chain(const R
         for (int k = 0; k < B; k++) cur[k] = nxt[k];
     }
     if (acc != acc) {
-        acc = T(0);
+        acc = fseed<T>(A);
         for (int64_t q = 0; q < n; q++) acc = fadd(acc, ld_t<LT, IT>(A.in, ib + off(q)));
     }
     return acc;
@@ -394,11 +401,11 @@ template <typename T> __device__ inline T narrow_wave_sum(const RedArgs &A, int6
 __device__ inline int pw_split(int n) { return (n >> 1) & ~7; }
 
 // pairwise_sum of x[0..n) for n <= 128 (a single leaf) in one thread: n < 8
-// adds in order from +0; otherwise 8 interleaved accumulators combined as
+// adds in order from -0.0; otherwise 8 interleaved accumulators combined as
 // ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)), then the n % 8 tail in order.
 template <typename T, typename Get> __device__ inline T pw_leaf(const Get &get, int n) {
     if (n < 8) {
-        T r = T(0);
+        T r = T(-0.0);  // NumPy 2.2.6's short leaf: a sum of -0.0 stays -0.0
         for (int i = 0; i < n; i++) r = fadd(r, get(i));
         return r;
     }
@@ -454,7 +461,7 @@ __device__ T pairwise_block(const Get &get, int L, T *racc, T *val, int *leaf_lo
         const int l = c >> 3, j = c & 7, lo = leaf_lo[l], n = leaf_n[l];
         if (n < 8) {
             if (j == 0) {
-                T r = T(0);
+                T r = T(-0.0);  // short leaf: starts from -0.0 as NumPy's does
                 for (int i = 0; i < n; i++) r = fadd(r, get(lo + i));
                 racc[c] = r;
             }

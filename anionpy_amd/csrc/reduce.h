@@ -21,6 +21,9 @@ struct RedArgs {
     const char *in;
     char *out;
     void *scratch;  // partial results
+    int32_t has_init;    // initial= given: the fold starts from it (and max/min feed every element)
+    int32_t pad_;
+    uint64_t init_bits;  // its value in the output dtype: float bits, or the integer sign-/zero-extended
 };
 
 // Float add: every window is one pairwise leaf (<= 128 elements) and there
@@ -39,7 +42,11 @@ inline bool int_cols_ok(const RedArgs &A) {
     return A.group <= 64 || (A.nout >= 16384 && A.group <= 4096) || (A.nw == 0 && A.nout >= 2048 && A.group <= 65536);
 }
 
-ionp_status run_reduce(const ReducePlan &p, int64_t bufsize, const ionp_config *cfg, hipStream_t s);
+// initial: one element of the output dtype (host memory), or null
+ionp_status run_reduce(const ReducePlan &p, int64_t bufsize, const void *initial, const ionp_config *cfg, hipStream_t s);
+
+// RedArgs::init_bits of one output-dtype element at `p`
+uint64_t init_bits_of(int out_dt, const void *p);
 
 // argmax / argmin along one axis (C index order) or over the C-order ravel.
 struct ArgArgs {

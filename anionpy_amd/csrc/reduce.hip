@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kPwThreads) void pw_windows(const RedArgs A, int64_
         __syncthreads();
         const T res = pairwise_block<T>([&](int i) { return xs[i]; }, L, racc, val, leaf_lo, leaf_n, cnt);
         if (threadIdx.x == 0) {
-            if (direct) ((T *)A.out)[ob] = fadd(T(0), res);
+            if (direct) ((T *)A.out)[ob] = fadd(fseed<T>(A), res);
             else ((T *)A.scratch)[tail_nwin ? o * tail_nwin + w : g] = res;
         }
         __syncthreads();
@@ -171,7 +171,7 @@ template <typename T> __device__ void pw_folder(const RedArgs &A, const PwFastSt
     constexpr int Q = 16, BLK = 64 * Q;
     const int lane = threadIdx.x & 63;
     const T *part = (const T *)F.partials + o * F.nwin;
-    T acc = T(0);
+    T acc = fseed<T>(A);
     int64_t cur = 0;
     uint32_t idle = 0;
     bool failed = false;
@@ -254,7 +254,7 @@ template <typename T> __device__ void pw_folder(const RedArgs &A, const PwFastSt
         if (failed) {
             acc = canonical_nan(T(0));
         } else if (acc != acc) {
-            acc = T(0);
+            acc = fseed<T>(A);
             for (int64_t i = 0; i < F.nwin; i++) acc = fadd(acc, from_bits<T>(ld_bits(part + i)));
         }
         int64_t ib, ob;
@@ -419,7 +419,7 @@ template <typename T> __global__ __launch_bounds__(256) void pw_fold(const RedAr
     const int64_t per_out = A.rsize * A.nwin;
     for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
         const T *p = (const T *)A.scratch + o * per_out;
-        T acc = T(0);
+        T acc = fseed<T>(A);
         int64_t i = 0;
         constexpr int U = 16;
         for (; i + U <= per_out; i += U) {
@@ -446,7 +446,7 @@ template <typename T> __global__ __launch_bounds__(256) void pw_small(const RedA
     for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
-        T acc = T(0);
+        T acc = fseed<T>(A);
         for (int64_t r = 0; r < A.rsize; r++) {
             const int64_t base = ib + r_offset(A, r);
             acc = fadd(acc, pw_leaf<T>([&](int i) { return ld<LT>(A.in_dt, A.in, base + w_offset(A, i)); }, L));
@@ -564,7 +564,7 @@ template <typename T, bool MAX> __device__ ExtSum<T> ext_window_summary(const Re
     // run's stride; one spanning runs goes through NumPy's contiguous buffer
     const bool one_run = (s0 / A.w_shape[0]) == ((s0 + L - 1) / A.w_shape[0]);
     const bool simd = one_run ? (A.w_st[0] == 1) : true;
-    const int64_t skip = (r == 0 && w == 0) ? 1 : 0;  // first element seeds the fold
+    const int64_t skip = (!A.has_init && r == 0 && w == 0) ? 1 : 0;  // first element seeds the fold
     const int64_t len = L - skip;
     const int64_t nv = simd ? len - len % V : (len >= 8 ? len - len % 8 : 0);
     const int64_t per_lane = simd ? nv / V : nv / 8;
@@ -661,7 +661,7 @@ __global__ __launch_bounds__(256) void ext_fast(const RedArgs A, int64_t wpb, in
             const bool last = w == w1 - 1;
             const int64_t s0 = w * A.bufsize;
             const int64_t L = min(A.bufsize, A.wsize - s0);
-            const int64_t skip = w == 0 ? 1 : 0;
+            const int64_t skip = (!A.has_init && w == 0) ? 1 : 0;  // without initial= element 0 is the seed
             const int64_t len = L - skip, nv = len - len % V;
             any_simd |= len >= 1;
             const T *x = (const T *)A.in + ib + s0;
@@ -772,7 +772,7 @@ template <typename T, bool MAX> __global__ __launch_bounds__(256) void ext_fold(
         if (threadIdx.x == 0) {
             int64_t ib, ob;
             out_offsets(A, (uint32_t)o, &ib, &ob);
-            const T seed = ld<LT>(A.in_dt, A.in, ib);
+            const T seed = A.has_init ? fseed<T>(A) : ld<LT>(A.in_dt, A.in, ib);
             ((T *)A.out)[ob] = ext_apply<T, MAX>(red[0], seed);
         }
         __syncthreads();
@@ -785,8 +785,8 @@ template <typename T, bool MAX> __global__ __launch_bounds__(256) void ext_narro
     for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
-        T acc = ld<LT>(A.in_dt, A.in, ib);
-        for (int64_t r = 1; r < A.rsize; r++) {
+        T acc = A.has_init ? fseed<T>(A) : ld<LT>(A.in_dt, A.in, ib);
+        for (int64_t r = A.has_init ? 0 : 1; r < A.rsize; r++) {
             const T v = ld<LT>(A.in_dt, A.in, ib + r_offset(A, r));
             acc = MAX ? fmaxn(acc, v) : fminn(acc, v);
         }
@@ -801,6 +801,10 @@ template <typename AT, int OP> __device__ inline AT assoc(AT a, AT b) {
     if constexpr (OP == IONP_ADD) return (AT)((uint64_t)a + (uint64_t)b);  // two's-complement wrap
     else if constexpr (OP == IONP_MAXIMUM) return a > b ? a : b;
     else return a < b ? a : b;
+}
+// a group's result combined with initial= (order-free: once, at the end)
+template <typename AT, int OP> __device__ inline AT with_init(const RedArgs &A, AT acc) {
+    return A.has_init ? assoc<AT, OP>(acc, (AT)A.init_bits) : acc;
 }
 
 // group element q -> input offset: q = w + wsize * r
@@ -834,7 +838,7 @@ __global__ __launch_bounds__(256) void int_reduce(const RedArgs A, int64_t seg, 
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            if (nseg == 1) ((ST *)A.out)[ob] = int_result<LT>(red[0]);
+            if (nseg == 1) ((ST *)A.out)[ob] = int_result<LT>(with_init<AT, OP>(A, red[0]));
             else ((AT *)A.scratch)[g] = red[0];
         }
         __syncthreads();
@@ -857,7 +861,7 @@ template <typename AT, int LT, int OP> __global__ __launch_bounds__(256) void in
         if (threadIdx.x == 0) {
             int64_t ib, ob;
             out_offsets(A, (uint32_t)o, &ib, &ob);
-            ((ST *)A.out)[ob] = int_result<LT>(red[0]);
+            ((ST *)A.out)[ob] = int_result<LT>(with_init<AT, OP>(A, red[0]));
         }
         __syncthreads();
     }
@@ -895,7 +899,7 @@ __global__ __launch_bounds__(256) void int_reduce_contig(const RedArgs A, int64_
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            if (nseg == 1) ((ST *)A.out)[ob] = (ST)red[0];  // never a bool output (see run_int)
+            if (nseg == 1) ((ST *)A.out)[ob] = (ST)with_init<AT, OP>(A, red[0]);  // never a bool output (see run_int)
             else ((AT *)A.scratch)[g] = red[0];
         }
         __syncthreads();
@@ -911,16 +915,16 @@ template <typename AT, int LT, int OP> __global__ __launch_bounds__(256) void in
         AT acc = OP == IONP_ADD ? AT(0) : (AT)load_acc<ST, LT>(A.in_dt, A.in, ib);
         for (int64_t q = 0; q < A.group; q++)
             acc = assoc<AT, OP>(acc, (AT)load_acc<ST, LT>(A.in_dt, A.in, ib + group_offset(A, q)));
-        ((ST *)A.out)[ob] = int_result<LT>(acc);
+        ((ST *)A.out)[ob] = int_result<LT>(with_init<AT, OP>(A, acc));
     }
 }
 
-// zero-size groups: the identity (add) in every output element
+// zero-size groups: initial= if given, else add's identity 0
 template <typename ST> __global__ void fill_zero(const RedArgs A) {
     for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
-        ((ST *)A.out)[ob] = ST(0);
+        ((ST *)A.out)[ob] = A.has_init ? (ST)A.init_bits : ST(0);  // initial= or add's identity
     }
 }
 
@@ -1106,7 +1110,22 @@ template <typename T, bool MAX> static ionp_status run_float_ext(RedArgs &A, con
     return launched();
 }
 
-ionp_status run_reduce(const ReducePlan &p, int64_t bufsize, const ionp_config *cfg, hipStream_t s) {
+uint64_t init_bits_of(int out_dt, const void *p) {
+    switch (out_dt) {
+        case IONP_BOOL: return *(const uint8_t *)p != 0;
+        case IONP_INT8: return (uint64_t)(int64_t)*(const int8_t *)p;
+        case IONP_UINT8: return *(const uint8_t *)p;
+        case IONP_INT16: return (uint64_t)(int64_t)*(const int16_t *)p;
+        case IONP_UINT16: return *(const uint16_t *)p;
+        case IONP_INT32: return (uint64_t)(int64_t)*(const int32_t *)p;
+        case IONP_UINT32: return *(const uint32_t *)p;
+        case IONP_FLOAT32: return *(const uint32_t *)p;
+        default: return *(const uint64_t *)p;  // int64, uint64, float64 bits
+    }
+}
+
+ionp_status run_reduce(const ReducePlan &p, int64_t bufsize, const void *initial, const ionp_config *cfg,
+                       hipStream_t s) {
     if (p.nout == 0) return IONP_OK;
     RedArgs A;
     std::memset(&A, 0, sizeof A);
@@ -1124,6 +1143,8 @@ ionp_status run_reduce(const ReducePlan &p, int64_t bufsize, const ionp_config *
     A.nwin = p.wsize > 0 ? (p.wsize + bufsize - 1) / bufsize : 0;
     A.in = p.in;
     A.out = p.out;
+    A.has_init = initial != nullptr;
+    A.init_bits = initial ? init_bits_of(p.out_dt, initial) : 0;
     if (p.nout >= (1ll << 31) || p.wsize >= (1ll << 31) * 8 || p.rsize >= (1ll << 31))
         return set_error(IONP_ERR_UNSUPPORTED, "reduction extent too large");
     for (int d = 0; d < p.nk; d++) {
@@ -1145,7 +1166,8 @@ ionp_status run_reduce(const ReducePlan &p, int64_t bufsize, const ionp_config *
     }
     const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
     if (p.group == 0) {
-        if (p.op != IONP_ADD) return set_error(IONP_ERR_VALUE, std::string("zero-size array to reduction operation ") +
+        if (p.op != IONP_ADD && !initial)
+            return set_error(IONP_ERR_VALUE, std::string("zero-size array to reduction operation ") +
                                                                op_name(p.op) + " which has no identity");
         const int g = grid_cap((p.nout + 255) / 256, bpc);
         switch (itemsize(p.out_dt)) {

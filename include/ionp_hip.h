@@ -169,9 +169,14 @@ ionp_status ionp_cast(const ionp_array_desc *in, const ionp_array_desc *out, con
 
 /* Replaces ionp_core::ufunc::reduce_axis (ufunc.rs:7634) for op in
  * {ADD, MAXIMUM, MINIMUM}: `out` as given by ionp_plan_reduce (its dtype
- * selects the accumulation dtype). */
+ * selects the accumulation dtype).  `initial`: NULL, or one element of the
+ * output dtype in host memory (reduce_axis's `initial`): the fold starts
+ * from it (add: acc = initial; acc += pairwise(window) per window), maximum
+ * and minimum feed every element to the loop instead of seeding with the
+ * first, and an empty reduction returns it. */
 ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes,
-                        int32_t keepdims, const ionp_array_desc *out, const ionp_config *cfg, void *stream);
+                        int32_t keepdims, const void *initial, const ionp_array_desc *out, const ionp_config *cfg,
+                        void *stream);
 
 /* Replaces ionp_core::ufunc::argext (ufunc.rs:8131): int64 indices. */
 ionp_status ionp_argext(int32_t ismax, const ionp_array_desc *in, int32_t axis, int32_t keepdims,

@@ -60,7 +60,7 @@ def lib():
         for name, args in [("ora_binary", [ctypes.c_int, P, P, P]),
                            ("ora_where", [P, P, P, P]),
                            ("ora_cast", [P, P]),
-                           ("ora_reduce", [ctypes.c_int, P, ctypes.c_char_p, P, ctypes.c_int64]),
+                           ("ora_reduce", [ctypes.c_int, P, ctypes.c_char_p, P, ctypes.c_int64, ctypes.c_void_p]),
                            ("ora_argext", [ctypes.c_int, P, ctypes.c_int, P])]:
             getattr(L, name).argtypes = args
             getattr(L, name).restype = ctypes.c_int
@@ -190,8 +190,8 @@ def _norm_axes(axis, ndim):
     return [] if ndim == 0 else out
 
 
-def reduce(op, x, axis=0, bufsize=BUFSIZE, dtype=None):
-    """np.<op>.reduce(x, axis, dtype) for op in add/maximum/minimum."""
+def reduce(op, x, axis=0, bufsize=BUFSIZE, dtype=None, initial=None):
+    """np.<op>.reduce(x, axis, dtype, initial) for op in add/maximum/minimum."""
     x = np.asarray(x)
     axes = _norm_axes(axis, x.ndim)
     mask = bytes(1 if k in axes else 0 for k in range(x.ndim)) or b"\0"
@@ -212,7 +212,9 @@ def reduce(op, x, axis=0, bufsize=BUFSIZE, dtype=None):
         acc *= max(x.shape[a], 1)
     out = _empty_strided(shape, ostr, odt)
     dx, do = _desc(x), _desc(out)
-    _check(lib().ora_reduce(OP[op], ctypes.byref(dx), mask, ctypes.byref(do), bufsize))
+    init = None if initial is None else np.array(initial, dtype=odt).reshape(1)
+    _check(lib().ora_reduce(OP[op], ctypes.byref(dx), mask, ctypes.byref(do), bufsize,
+                            None if init is None else init.ctypes.data))
     return out[()] if out.ndim == 0 else out
 
 

@@ -598,8 +598,8 @@ int ora_cast(const ora_desc *in, const ora_desc *out)
 static double pw_f64(const char *base, const int64_t *off, int64_t n)
 {
     if (n < 8) {
-        double res = 0.0; /* NumPy's leaf starts from 0. (ufunc.rs:6424 starts from get(base); the 0 seed of
-                             reduce_axis_generic makes these agree) */
+        double res = -0.0; /* NumPy 2.2.6's short leaf starts from -0.0, so a sum of -0.0 stays -0.0
+                              (ufunc.rs:6424 starts from get(base): the same value) */
         int64_t i;
         for (i = 0; i < n; i++) res = f64_arith(OP_ADD, res, *(const double *)(base + off[i]));
         return res;
@@ -623,7 +623,7 @@ static double pw_f64(const char *base, const int64_t *off, int64_t n)
 static float pw_f32(const char *base, const int64_t *off, int64_t n)
 {
     if (n < 8) {
-        float res = 0.0f;
+        float res = -0.0f;
         int64_t i;
         for (i = 0; i < n; i++) res = f32_arith(OP_ADD, res, *(const float *)(base + off[i]));
         return res;
@@ -773,7 +773,8 @@ static void w_offsets(const ora_desc *in, const red_plan *p, int64_t *off)
 }
 
 /* reduce op codes for ora_reduce: OP_ADD, OP_MAXIMUM, OP_MINIMUM */
-int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc *out, int64_t bufsize)
+int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc *out, int64_t bufsize,
+               const void *initial)
 {
     red_plan p;
     int n = in->ndim, k, nk = 0, kept[ORA_MAXDIM];
@@ -793,7 +794,7 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
     }
     nout = 1;
     for (k = 0; k < nk; k++) nout *= in->shape[kept[k]];
-    if (gsize == 0 && op != OP_ADD && nout > 0) return ORA_EVALUE; /* no identity */
+    if (gsize == 0 && op != OP_ADD && nout > 0 && !initial) return ORA_EVALUE; /* no identity */
     woff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(p.wsize > 0 ? p.wsize : 1));
     buf = (char *)malloc((size_t)(bufsize > 0 ? bufsize : 1) * 8);
     if (p.nw) w_offsets(in, &p, woff);
@@ -811,12 +812,18 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
         for (k = 0; k < p.nouter; k++) noutr *= in->shape[p.outer_axes[k]];
         memset(oidx, 0, sizeof oidx);
         if (gsize == 0) {
-            memset(po, 0, ITEMSIZE[cdt]);
+            if (initial) memcpy(po, initial, ITEMSIZE[cdt]);
+            else memset(po, 0, ITEMSIZE[cdt]);
         } else if (!is_float(cdt)) {
             /* sequential_group (ufunc.rs:6546) on exact dtypes: order-free */
             uint64_t uacc = 0;
             int64_t sacc = 0, e;
             int first = 1, uns = cdt == DT_BOOL || is_unsigned_int(cdt);
+            if (initial) { /* initial= seeds the fold (reduce_axis, ufunc.rs:7634 `initial`) */
+                uacc = (uint64_t)load_i(cdt, (const char *)initial);
+                sacc = load_i(cdt, (const char *)initial);
+                first = 0;
+            }
             for (r = 0; r < noutr; r++) {
                 int64_t ob = base;
                 for (k = 0; k < p.nouter; k++) ob += oidx[k] * in->strides[p.outer_axes[k]];
@@ -856,8 +863,9 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
                 }
             }
         } else if (op == OP_ADD) {
-            double acc64 = 0.0;
-            float acc32 = 0.0f;
+            /* acc = initial (default 0); acc = acc + pw(window) per window */
+            double acc64 = initial && cdt == DT_F64 ? *(const double *)initial : 0.0;
+            float acc32 = initial && cdt == DT_F32 ? *(const float *)initial : 0.0f;
             for (r = 0; r < noutr; r++) {
                 int64_t ob = base, s;
                 const char *pb;
@@ -882,9 +890,11 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
             else *(float *)po = acc32;
         } else {
             /* maximum / minimum on floats */
-            int ismax = op == OP_MAXIMUM, first = 1;
-            double acc64 = 0.0;
-            float acc32 = 0.0f;
+            /* without initial= the first element seeds the fold and is not
+               fed to the loop; with it every element is */
+            int ismax = op == OP_MAXIMUM, first = !initial;
+            double acc64 = initial && cdt == DT_F64 ? *(const double *)initial : 0.0;
+            float acc32 = initial && cdt == DT_F32 ? *(const float *)initial : 0.0f;
             for (r = 0; r < noutr; r++) {
                 int64_t ob = base, s;
                 const char *pb;

@@ -357,6 +357,64 @@ def reduce_cases(S, rng):
               {"axis": 1, "dtype": "int8"})
         S.add(f"int/{op}/uint8->int8/1", "reduce", op, [u], lambda: getattr(np, op).reduce(u, axis=1, dtype=np.int8),
               {"axis": 1, "dtype": "int8"})
+    initial_cases(S)
+
+
+def initial_cases(S):
+    """reduce with initial= (reduce_axis's `initial`, ufunc.rs:7634): add
+    starts the window fold from it; maximum/minimum feed every element to the
+    loop; an empty reduction returns it.  Own generator (seed 78)."""
+    r3 = np.random.default_rng(78)
+
+    def red(op, x, ax, init, **kw):
+        return getattr(np, op).reduce(x, axis=ax, initial=init, **kw)
+
+    for dt in (np.float64, np.float32):
+        nm = np.dtype(dt).name
+        x = r3.uniform(-1, 1, 20000).astype(dt)
+        S.add(f"init/add/{nm}/wide", "reduce", "add", [x], lambda: red("add", x, 0, 1e3), {"axis": 0, "initial": 1e3})
+        g = Gen(78, 2 ** 20 + 13, dt)
+        xg = g.array()
+        S.add(f"init/add/{nm}/2^20+13", "reduce", "add", [g], lambda: red("add", xg, 0, 7.0),
+              {"axis": 0, "initial": 7.0})
+        A = r3.uniform(-1, 1, (3000, 4)).astype(dt) * 1e4
+        S.add(f"init/add/{nm}/narrow", "reduce", "add", [A], lambda: red("add", A, 0, 0.25), {"axis": 0, "initial": 0.25})
+        B = r3.uniform(-1, 1, (500, 50)).astype(dt)
+        S.add(f"init/add/{nm}/small", "reduce", "add", [B], lambda: red("add", B, 1, -2.5), {"axis": 1, "initial": -2.5})
+        S.add(f"init/add/{nm}/all", "reduce", "add", [B], lambda: red("add", B, None, 1.5), {"axis": None, "initial": 1.5})
+        for n in (1, 5, 20):
+            z = np.full(n, -0.0, dtype=dt)
+            S.add(f"init/add/{nm}/negzero{n}", "reduce", "add", [z], lambda: red("add", z, 0, -0.0),
+                  {"axis": 0, "initial": -0.0})
+        for op in ("maximum", "minimum"):
+            y = r3.uniform(-1, 1, 17000).astype(dt)
+            S.add(f"init/{op}/{nm}/contig", "reduce", op, [y], lambda: red(op, y, 0, 0.5), {"axis": 0, "initial": 0.5})
+            S.add(f"init/{op}/{nm}/strided", "reduce", op, [y[::3]], lambda: red(op, y[::3], 0, -0.5),
+                  {"axis": 0, "initial": -0.5})
+            yn = y.copy()
+            yn[r3.random(17000) < 0.001] = np.nan
+            S.add(f"init/{op}/{nm}/nan", "reduce", op, [yn], lambda: red(op, yn, 0, 0.5), {"axis": 0, "initial": 0.5})
+            N = r3.uniform(-1, 1, (4000, 3)).astype(dt)
+            S.add(f"init/{op}/{nm}/narrow", "reduce", op, [N], lambda: red(op, N, 0, 0.9), {"axis": 0, "initial": 0.9})
+            zz = np.where(r3.random(100) < 0.5, -0.0, 0.0).astype(dt)
+            for iv in (0.0, -0.0):
+                S.add(f"init/{op}/{nm}/zeros/{iv}", "reduce", op, [zz], lambda: red(op, zz, 0, iv),
+                      {"axis": 0, "initial": iv})
+            E = np.zeros((0, 3), dtype=dt)
+            S.add(f"init/{op}/{nm}/empty", "reduce", op, [E], lambda: red(op, E, 0, 1.5), {"axis": 0, "initial": 1.5})
+    E = np.zeros((3, 0))
+    S.add("init/add/empty", "reduce", "add", [E], lambda: red("add", E, 1, 2.0), {"axis": 1, "initial": 2.0})
+    I = r3.integers(-1000, 1000, (40, 900), dtype=np.int32)
+    S.add("init/add/int32", "reduce", "add", [I], lambda: red("add", I, 1, 5), {"axis": 1, "initial": 5})
+    S.add("init/maximum/int32", "reduce", "maximum", [I], lambda: red("maximum", I, 0, 990), {"axis": 0, "initial": 990})
+    S.add("init/minimum/int32/all", "reduce", "minimum", [I], lambda: red("minimum", I, None, -2000),
+          {"axis": None, "initial": -2000})
+    U = r3.integers(0, 256, (30, 200), dtype=np.uint8)
+    S.add("init/add/uint8->uint8", "reduce", "add", [U], lambda: red("add", U, 1, 250, dtype=np.uint8),
+          {"axis": 1, "initial": 250, "dtype": "uint8"})
+    Bo = r3.random((20, 30)) < 0.05
+    S.add("init/add/bool->bool", "reduce", "add", [Bo], lambda: red("add", Bo, 1, True, dtype=np.bool_),
+          {"axis": 1, "initial": True, "dtype": "bool"})
 
 
 def argext_cases(S, rng):

@@ -35,7 +35,7 @@ def run_device(ap, case, npz, cache):
         return ins[0].astype(np.dtype(args["dtype"]))
     if k == "reduce":
         return getattr(ap, op).reduce(ins[0], axis=args_axis(args), dtype=args.get("dtype"),
-                                      keepdims=bool(args.get("keepdims", False)))
+                                      keepdims=bool(args.get("keepdims", False)), initial=args.get("initial"))
     return getattr(ap, op)(ins[0], axis=args.get("axis"), keepdims=bool(args.get("keepdims", False)))
 
 

@@ -21,7 +21,7 @@ def run_oracle(case, npz):
         x = ins[0]
         if args.get("dtype") and np.dtype(args["dtype"]).kind == "f":
             x = O.cast(x, np.dtype(args["dtype"]))  # NumPy's buffered cast, then the float loop
-        r = O.reduce(op, x, args_axis(args), dtype=args.get("dtype"))
+        r = O.reduce(op, x, args_axis(args), dtype=args.get("dtype"), initial=args.get("initial"))
         if args.get("keepdims"):
             ax = args_axis(args)
             axes = range(x.ndim) if ax is None else ((ax,) if isinstance(ax, int) else ax)
