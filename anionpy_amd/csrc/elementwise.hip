@@ -8,7 +8,9 @@
 // are fused into the loads, broadcasts are stride-0 reads and the planner
 // coalesces the iteration space first.
 //
-// Two kernels:
+// Three kernels:
+//   ew_tiled   - one operand contiguous along an axis other than the output's
+//                contiguous one (A + B.T): 64 x 64 tiles staged through LDS.
 //   ew_rows    - "rows x inner" layouts whose innermost axis is contiguous
 //                in the output and contiguous or broadcast (stride 0) in each
 //                input, all operands of one dtype: 16-byte vector loads and
@@ -390,9 +392,189 @@ bool try_rows(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_statu
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged transpose: one operand (Y) is contiguous along a dim d* other
+// than the output's contiguous dim d0 (e.g. A + B.T).  A 64 x 64 tile of Y
+// is read along d*, where its elements are adjacent, into LDS, and read back
+// along d0; the other operand (X) and the output are accessed along d0.
+// Every global access of a wavefront is 64 consecutive elements.
+// ---------------------------------------------------------------------------
+struct TileArgs {
+    int64_t n0, n1;        // extents of d0 and d*
+    int64_t x_st0, x_st1;  // X strides along d0 and d*
+    int64_t y_st0;         // Y stride along d0 (its stride along d* is 1)
+    int64_t o_st1;         // output stride along d* (along d0 it is 1)
+    int64_t t0, t1;        // tiles along d0, d*
+    FastDiv t0div, t1div;
+    int32_t nb;            // remaining ("batch") dims
+    int64_t bshape[kMaxKDims];
+    FastDiv bdiv[kMaxKDims];
+    int64_t bx[kMaxKDims], by[kMaxKDims], bo[kMaxKDims];
+    int64_t ntiles;
+    const char *x, *y;
+    char *out;
+    int32_t y_first;       // Y is the op's left operand
+};
+
+constexpr int kTile = 64;
+
+template <typename T, int OPK> __global__ __launch_bounds__(256) void ew_tiled(const TileArgs P) {
+    using OT = typename std::conditional<(OPK >= IONP_LESS), uint8_t, T>::type;
+    __shared__ T tile[kTile][kTile + 1];  // [d0][d*], padded against bank conflicts
+    const T *__restrict__ x = (const T *)P.x;
+    const T *__restrict__ y = (const T *)P.y;
+    OT *__restrict__ o = (OT *)P.out;
+    const int lo = threadIdx.x & 63, hi = threadIdx.x >> 6;  // 4 wavefronts
+    for (int64_t t = blockIdx.x; t < P.ntiles; t += gridDim.x) {
+        // t = (batch, tile along d*, tile along d0), d0 fastest
+        uint32_t rem = (uint32_t)t;
+        const uint32_t q0 = P.t0div.div(rem);
+        const int64_t i0 = (int64_t)(rem - q0 * (uint32_t)P.t0) * kTile;
+        rem = q0;
+        const uint32_t q1 = P.t1div.div(rem);
+        const int64_t i1 = (int64_t)(rem - q1 * (uint32_t)P.t1) * kTile;
+        rem = q1;
+        int64_t bx = 0, by = 0, bo = 0;
+        for (int d = 0; d < P.nb; d++) {
+            uint32_t c;
+            if (d + 1 < P.nb) {
+                const uint32_t q = P.bdiv[d].div(rem);
+                c = rem - q * (uint32_t)P.bshape[d];
+                rem = q;
+            } else {
+                c = rem;
+            }
+            bx += (int64_t)c * P.bx[d];
+            by += (int64_t)c * P.by[d];
+            bo += (int64_t)c * P.bo[d];
+        }
+        const int64_t m0 = min((int64_t)kTile, P.n0 - i0), m1 = min((int64_t)kTile, P.n1 - i1);
+        // Y tile: lanes walk d* (adjacent in Y)
+#pragma unroll
+        for (int k = 0; k < kTile / 4; k++) {
+            const int r = hi + 4 * k;  // d0 within the tile
+            if (r < m0 && lo < m1) tile[r][lo] = y[by + (i0 + r) * P.y_st0 + (i1 + lo)];
+        }
+        // X along d0 (issued before the barrier)
+        T xv[kTile / 4];
+#pragma unroll
+        for (int k = 0; k < kTile / 4; k++) {
+            const int c = hi + 4 * k;  // d* within the tile
+            xv[k] = (lo < m0 && c < m1) ? x[bx + (i0 + lo) * P.x_st0 + (i1 + c) * P.x_st1] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTile / 4; k++) {
+            const int c = hi + 4 * k;
+            if (lo < m0 && c < m1) {
+                const T yv = tile[lo][c];
+                o[bo + (i0 + lo) + (i1 + c) * P.o_st1] = P.y_first ? row_apply<T, OPK>(yv, xv[k]) : row_apply<T, OPK>(xv[k], yv);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+typedef void (*TileFn)(const TileArgs);
+
+template <typename T> static TileFn tiled_t(int op) {
+#define IONP_TILE_CASE(OPK) \
+    case OPK: return ew_tiled<T, OPK>;
+    switch (op) {
+        IONP_TILE_CASE(IONP_ADD)
+        IONP_TILE_CASE(IONP_SUBTRACT)
+        IONP_TILE_CASE(IONP_MULTIPLY)
+        IONP_TILE_CASE(IONP_MAXIMUM)
+        IONP_TILE_CASE(IONP_MINIMUM)
+        IONP_TILE_CASE(IONP_LESS)
+        IONP_TILE_CASE(IONP_LESS_EQUAL)
+        IONP_TILE_CASE(IONP_GREATER)
+        IONP_TILE_CASE(IONP_GREATER_EQUAL)
+        IONP_TILE_CASE(IONP_EQUAL)
+        IONP_TILE_CASE(IONP_NOT_EQUAL)
+        case IONP_TRUE_DIVIDE:
+            if constexpr (std::is_floating_point<T>::value) return ew_tiled<T, IONP_TRUE_DIVIDE>;
+            else return nullptr;
+        default: return nullptr;
+    }
+#undef IONP_TILE_CASE
+}
+
+static TileFn tiled_kernel(int dt, int op) {
+    switch (dt) {
+        case IONP_INT32: return tiled_t<int32_t>(op);
+        case IONP_INT64: return tiled_t<int64_t>(op);
+        case IONP_FLOAT32: return tiled_t<float>(op);
+        case IONP_FLOAT64: return tiled_t<double>(op);
+        default: return nullptr;
+    }
+}
+
+// Binary calls where one operand is contiguous along a dim other than the
+// output's contiguous dim 0 (the other operand is read along dim 0 at
+// whatever stride it has).
+bool try_tiled(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_status *st) {
+    if (A.nin != 2 || A.op < 0 || A.op >= IONP_NOPS || A.mixed_su || A.ndim < 2) return false;
+    if (A.in_dt[0] != A.loop_dt || A.in_dt[1] != A.loop_dt) return false;
+    if (A.out_dt != (A.op >= IONP_LESS ? (int)IONP_BOOL : A.loop_dt) || A.out_st[0] != 1) return false;
+    int yk = -1, dstar = -1;
+    for (int k = 0; k < 2 && yk < 0; k++) {
+        const int64_t s0 = A.in_st[k][0];
+        if (s0 == 0 || s0 == 1 || s0 == -1) continue;
+        for (int d = 1; d < A.ndim; d++)
+            if (A.in_st[k][d] == 1) {
+                yk = k;
+                dstar = d;
+                break;
+            }
+    }
+    if (yk < 0) return false;
+    const int xk = 1 - yk;
+    const TileFn fn = tiled_kernel(A.loop_dt, A.op);
+    if (!fn) return false;
+    TileArgs P;
+    std::memset(&P, 0, sizeof P);
+    P.n0 = A.shape[0];
+    P.n1 = A.shape[dstar];
+    P.x_st0 = A.in_st[xk][0];
+    P.x_st1 = A.in_st[xk][dstar];
+    P.y_st0 = A.in_st[yk][0];
+    P.o_st1 = A.out_st[dstar];
+    P.t0 = (P.n0 + kTile - 1) / kTile;
+    P.t1 = (P.n1 + kTile - 1) / kTile;
+    int64_t nbatch = 1;
+    for (int d = 1; d < A.ndim; d++) {
+        if (d == dstar) continue;
+        P.bshape[P.nb] = A.shape[d];
+        P.bdiv[P.nb] = FastDiv((uint32_t)A.shape[d]);
+        P.bx[P.nb] = A.in_st[xk][d];
+        P.by[P.nb] = A.in_st[yk][d];
+        P.bo[P.nb] = A.out_st[d];
+        P.nb++;
+        nbatch *= A.shape[d];
+    }
+    P.ntiles = P.t0 * P.t1 * nbatch;
+    if (P.ntiles >= (1ll << 31) || P.t0 >= (1ll << 31) || P.t1 >= (1ll << 31)) return false;
+    P.t0div = FastDiv((uint32_t)P.t0);
+    P.t1div = FastDiv((uint32_t)P.t1);
+    P.x = A.in[xk];
+    P.y = A.in[yk];
+    P.out = A.out;
+    P.y_first = yk == 0;
+    // measured on MI355X (8192 x 8192, A + B.T): 4 workgroups per CU for
+    // 8-byte elements, 16 for 4-byte ones (their LDS tile is half the size)
+    const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : (itemsize(A.loop_dt) == 8 ? 4 : 16);
+    int64_t grid = (int64_t)bpc * num_cus();
+    if (grid > P.ntiles) grid = P.ntiles;
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), 0, s, P);
+    *st = hipGetLastError() == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, "kernel launch failed");
+    return true;
+}
+
 ionp_status run_elementwise(EwArgs &A, const ionp_config *cfg, hipStream_t s) {
     if (A.total == 0) return IONP_OK;
     ionp_status st;
+    if (try_tiled(A, cfg, s, &st)) return st;
     if (try_rows(A, cfg, s, &st)) return st;
     return launch_generic(A, cfg, s);
 }

@@ -195,6 +195,38 @@ def binary_cases(S, rng):
         S.add(f"rows/multiply/{nm}/offset1", "binary", "multiply", [A2[:, 1:], c2], lambda: np.multiply(A2[:, 1:], c2))
         V2 = A2.T[::2]
         S.add(f"rows/multiply/{nm}/Tview", "binary", "multiply", [V2, c2[::2]], lambda: np.multiply(V2, c2[::2]))
+    transpose_cases(S)
+
+
+def transpose_cases(S):
+    """Operands whose contiguous axes differ (A + B.T, a 3-D batch of
+    transposes, negative and stepped strides, a broadcast partner), with
+    extents that are not multiples of 64.  Own generator (seed 79)."""
+    r4 = np.random.default_rng(79)
+    ops = ["add", "subtract", "multiply", "true_divide", "maximum", "minimum", "less", "less_equal", "greater",
+           "greater_equal", "equal", "not_equal"]
+    for dt in (np.float64, np.float32, np.int64, np.int32):
+        nm = np.dtype(dt).name
+        A = specials(dt, 70 * 90, r4).reshape(70, 90)
+        B = specials(dt, 90 * 70, r4).reshape(90, 70)
+        C = specials(dt, 3 * 66 * 130, r4).reshape(3, 66, 130)
+        D = specials(dt, 3 * 130 * 66, r4).reshape(3, 130, 66)
+        row = specials(dt, 90, r4).reshape(1, 90)
+        for k, op in enumerate(ops):
+            if op == "true_divide" and dt in (np.int64, np.int32):
+                continue
+            f = getattr(np, op)
+            S.add(f"transpose/{op}/{nm}/A+B.T", "binary", op, [A, B.T], lambda: f(A, B.T))
+            S.add(f"transpose/{op}/{nm}/B.T+A", "binary", op, [B.T, A], lambda: f(B.T, A))
+            if k % 3 == 0:
+                S.add(f"transpose/{op}/{nm}/batchT", "binary", op, [C, D.transpose(0, 2, 1)],
+                      lambda: f(C, D.transpose(0, 2, 1)))
+            if k % 3 == 1:
+                S.add(f"transpose/{op}/{nm}/negT", "binary", op, [A[::-1], B.T[:, ::-1]], lambda: f(A[::-1], B.T[:, ::-1]))
+            if k % 3 == 2:
+                S.add(f"transpose/{op}/{nm}/rowT", "binary", op, [row, B.T], lambda: f(row, B.T))
+                S.add(f"transpose/{op}/{nm}/stepT", "binary", op, [A[::2, 1::2], B.T[::2, 1::2]],
+                      lambda: f(A[::2, 1::2], B.T[::2, 1::2]))
 
 
 def where_cast_cases(S, rng):
