@@ -25,7 +25,7 @@ from ._lib import IonpError
 
 __all__ = ["ndarray", "asarray", "empty", "zeros", "Ufunc", "add", "subtract", "multiply", "divide",
            "true_divide", "maximum", "minimum", "less", "less_equal", "greater", "greater_equal", "equal",
-           "not_equal", "where", "sum", "max", "amax", "min", "amin", "argmax", "argmin", "result_type",
+           "not_equal", "where", "sum", "cumsum", "max", "amax", "min", "amin", "argmax", "argmin", "result_type",
            "AxisError", "UFuncTypeError", "IonpError", "setbufsize", "getbufsize", "set_config_policy",
            "get_config_policy"]
 
@@ -277,6 +277,9 @@ class ndarray:
     # ---- reductions as methods (ndarray_attrs.rs mirrors do_reduce_axis) --------------
     def sum(self, axis=None, dtype=None, keepdims=False, initial=None):
         return sum(self, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial)
+
+    def cumsum(self, axis=None, dtype=None, out=None):
+        return cumsum(self, axis=axis, dtype=dtype, out=out)
 
     def max(self, axis=None, keepdims=False, initial=None):
         return amax(self, axis=axis, keepdims=keepdims, initial=initial)
@@ -574,6 +577,13 @@ class Ufunc:
             _copy_into(target, out)
         return out
 
+    def accumulate(self, a, axis=0, dtype=None, out=None):
+        """ufunc.accumulate -> accumulate_axis (ionp-core/src/ufunc.rs:7967):
+        out[0] = a[0], out[i] = op(out[i-1], a[i]) along `axis`."""
+        if not self._reducible:
+            raise NotImplementedError(f"{self.__name__}.accumulate is not provided by the MI355X backend")
+        return _accumulate(self._code, asarray(a), axis, dtype, out)
+
     def reduce(self, a, axis=0, dtype=None, out=None, keepdims=False, initial=None):
         """ufunc.reduce (lib.rs:9958) -> reduce_axis (ionp-core/src/ufunc.rs:7634).
         initial=: the fold's starting value (also the result of an empty
@@ -581,6 +591,24 @@ class Ufunc:
         if not self._reducible:
             raise NotImplementedError(f"{self.__name__}.reduce is not provided by the MI355X backend")
         return _reduce(self._code, asarray(a), axis, dtype, keepdims, out, initial)
+
+
+def _accumulate(code, a, axis, dtype, out):
+    L = _lib.load()
+    d = a._desc()
+    t = _lib.ArrayDesc()
+    dt = -1 if dtype is None else _CODE[_dtype(dtype)]
+    _call(L.ionp_plan_accumulate, code, ctypes.byref(d), int(axis), dt, ctypes.byref(t))
+    final = out
+    if out is None or _needs_temp(out, [a]):
+        out = empty_strided(tuple(t.shape[: t.ndim]), tuple(t.strides[: t.ndim]),
+                            _NP_DTYPES[t.dtype] if final is None else final.dtype)
+    do = out._desc()
+    _call(L.ionp_accumulate, code, ctypes.byref(d), int(axis), ctypes.byref(do), None, _lib.stream_handle())
+    if final is not None and final is not out:
+        _copy_into(out, final)
+        return final
+    return out
 
 
 def _axes_arg(axis):
@@ -635,6 +663,16 @@ not_equal = Ufunc("not_equal", _lib.NOT_EQUAL, False)
 def sum(a, axis=None, dtype=None, keepdims=False, initial=None):
     """sum (ionp-py/src/reductions.rs:1032) = add.reduce with axis=None default."""
     return add.reduce(a, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial)
+
+
+def cumsum(a, axis=None, dtype=None, out=None):
+    """cumsum (reductions.rs) = add.accumulate; axis=None scans the C-order
+    ravel."""
+    a = asarray(a)
+    if axis is None:
+        a = a.reshape(-1)
+        axis = 0
+    return add.accumulate(a, axis=axis, dtype=dtype, out=out)
 
 
 def amax(a, axis=None, keepdims=False, initial=None):

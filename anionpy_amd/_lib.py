@@ -77,6 +77,8 @@ SIGNATURES = {
     "ionp_cast": [_D, _D, _C, _V],
     "ionp_reduce": [_I, _D, _P(_I), _I, _I, _V, _D, _C, _V],
     "ionp_argext": [_I, _D, _I, _I, _D, _C, _V],
+    "ionp_plan_accumulate": [_I, _D, _I, _I, _D],
+    "ionp_accumulate": [_I, _D, _I, _D, _C, _V],
     "ionp_batch_create": [_P(Call), _I, _V, _P(_V)],
     "ionp_batch_run": [_V, _I, _V],
     "ionp_batch_info": [_V, _P(_L), _P(_L)],

@@ -19,7 +19,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libionp_hip.so")
 ARCH = os.environ.get("IONP_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["planner.cpp", "api.cpp", "elementwise.hip", "reduce.hip", "batch.hip", "mlp.hip", "features.cpp"]
+SOURCES = ["planner.cpp", "api.cpp", "elementwise.hip", "reduce.hip", "accumulate.hip", "batch.hip", "mlp.hip",
+           "features.cpp"]
 
 FLAGS = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contract=off",

@@ -1,0 +1,349 @@
+// accumulate.hip -- ufunc.accumulate for add / maximum / minimum
+// (accumulate_axis, ionp-core/src/ufunc.rs:7967 -> accumulate_axis_generic
+// 7830): along one axis, out[0] = x[0] and out[i] = op(out[i-1], x[i]) in
+// the output dtype.  The scan is sequential per fiber (NumPy's float add
+// accumulate is the plain running sum, no pairwise tree), so the GPU runs
+// fibers in parallel:
+//   acc_thread - a thread per fiber, loads issued U ahead of the chain
+//                (fibers adjacent in memory);
+//   acc_tiled  - fibers contiguous along the axis: 16 per wavefront,
+//                chunks of 64 elements transposed through LDS;
+//   acc_wave   - few long fibers: a wavefront per fiber stages blocks in
+//                LDS and one lane runs the chain over them.
+// Float add runs plain IEEE adds and redoes a fiber that ends in NaN with
+// fadd's x86 NaN rules (they agree until a NaN appears, and a NaN stays).
+// An input of another dtype is first cast into scratch memory, as the
+// reference casts before scanning.
+#include "accumulate.h"
+
+#include "device_ops.h"
+#include "elementwise.h"
+
+namespace ionp {
+
+struct AccArgs {
+    int64_t len;              // extent of the scanned axis
+    int64_t in_st, out_st;    // strides along it (elements)
+    int64_t nfib;             // fibers (product of the other extents)
+    int32_t nf;               // fiber dims, innermost first
+    int64_t fshape[IONP_MAX_DIMS];
+    FastDiv fdiv[IONP_MAX_DIMS];
+    int64_t fin[IONP_MAX_DIMS], fout[IONP_MAX_DIMS];
+    const char *in;
+    char *out;
+};
+
+__device__ inline void fiber_offsets(const AccArgs &A, uint32_t f, int64_t *ib, int64_t *ob) {
+    int64_t a = 0, b = 0;
+    for (int d = 0; d < A.nf; d++) {
+        uint32_t c;
+        if (d + 1 < A.nf) {
+            const uint32_t q = A.fdiv[d].div(f);
+            c = f - q * (uint32_t)A.fshape[d];
+            f = q;
+        } else {
+            c = f;
+        }
+        a += (int64_t)c * A.fin[d];
+        b += (int64_t)c * A.fout[d];
+    }
+    *ib = a;
+    *ob = b;
+}
+
+// one step of the scan in dtype LT (float add: plain IEEE add, see above)
+template <int LT, int OP> __device__ inline typename dt_type<LT>::T acc_step(typename dt_type<LT>::T a,
+                                                                            typename dt_type<LT>::T b) {
+    using T = typename dt_type<LT>::T;
+    if constexpr (LT == IONP_BOOL) {
+        return OP == IONP_MINIMUM ? (T)((a != 0) & (b != 0)) : (T)((a != 0) | (b != 0));
+    } else if constexpr (std::is_floating_point<T>::value) {
+        if constexpr (OP == IONP_ADD) return a + b;
+        else if constexpr (OP == IONP_MAXIMUM) return fmaxn(a, b);
+        else return fminn(a, b);
+    } else {
+        if constexpr (OP == IONP_ADD) return (T)((uint64_t)a + (uint64_t)b);  // two's-complement wrap
+        else if constexpr (OP == IONP_MAXIMUM) return a > b ? a : b;
+        else return a < b ? a : b;
+    }
+}
+
+template <int LT, int OP> __device__ inline bool acc_redo(typename dt_type<LT>::T acc) {
+    if constexpr (std::is_floating_point<typename dt_type<LT>::T>::value && OP == IONP_ADD) return acc != acc;
+    else return false;
+}
+
+// the fiber again with fadd (x86 NaN rules), every output rewritten
+template <typename T>
+__device__ inline void acc_redo_fiber(const AccArgs &A, const T *x, T *o) {
+    T acc = x[0];
+    o[0] = acc;
+    for (int64_t i = 1; i < A.len; i++) {
+        acc = fadd(acc, x[i * A.in_st]);
+        o[i * A.out_st] = acc;
+    }
+}
+
+template <int LT, int OP> __global__ __launch_bounds__(256) void acc_thread(const AccArgs A) {
+    using T = typename dt_type<LT>::T;
+    constexpr int U = 16;
+    for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < A.nfib; f += (int64_t)gridDim.x * blockDim.x) {
+        int64_t ib, ob;
+        fiber_offsets(A, (uint32_t)f, &ib, &ob);
+        const T *x = (const T *)A.in + ib;
+        T *o = (T *)A.out + ob;
+        T acc = x[0];
+        o[0] = acc;
+        int64_t i = 1;
+        // double-buffered: the next U loads are in flight while U steps run
+        T cur[U], nxt[U];
+        if (i + U <= A.len) {
+#pragma unroll
+            for (int u = 0; u < U; u++) cur[u] = x[(i + u) * A.in_st];
+        }
+        for (; i + U <= A.len; i += U) {
+            if (i + 2 * U <= A.len) {
+#pragma unroll
+                for (int u = 0; u < U; u++) nxt[u] = x[(i + U + u) * A.in_st];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                acc = acc_step<LT, OP>(acc, cur[u]);
+                o[(i + u) * A.out_st] = acc;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) cur[u] = nxt[u];
+        }
+        for (; i < A.len; i++) {
+            acc = acc_step<LT, OP>(acc, x[i * A.in_st]);
+            o[i * A.out_st] = acc;
+        }
+        if constexpr (std::is_floating_point<T>::value && OP == IONP_ADD)
+            if (acc_redo<LT, OP>(acc)) acc_redo_fiber<T>(A, x, o);
+    }
+}
+
+// Few long fibers: one wavefront per fiber.  Blocks of 1024 elements are
+// loaded by all 64 lanes into LDS (16 each, all in flight), lane 0 runs the
+// chain over the block in LDS (reads issued 16 ahead of the dependent op,
+// results written back in place), and the lanes store the block.
+template <int LT, int OP> __global__ __launch_bounds__(64) void acc_wave(const AccArgs A) {
+    using T = typename dt_type<LT>::T;
+    constexpr int Q = 16, BLK = 64 * Q;
+    __shared__ T buf[BLK];
+    const int lane = threadIdx.x;
+    for (int64_t f = blockIdx.x; f < A.nfib; f += gridDim.x) {
+        int64_t ib, ob;
+        fiber_offsets(A, (uint32_t)f, &ib, &ob);
+        const T *x = (const T *)A.in + ib;
+        T *o = (T *)A.out + ob;
+        const int64_t n = A.len;
+        T acc = T(0);
+        for (int64_t rb = 0; rb < n; rb += BLK) {
+            const int cnt = (int)min((int64_t)BLK, n - rb);
+            T v[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const int e = q * 64 + lane;
+                v[q] = e < cnt ? x[(rb + e) * A.in_st] : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < Q; q++) buf[q * 64 + lane] = v[q];
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (lane == 0) {
+                int e = 0;
+                if (rb == 0) {  // out[0] = x[0]
+                    acc = buf[0];
+                    e = 1;
+                }
+                for (; e + 16 <= cnt; e += 16) {
+                    T t[16];
+#pragma unroll
+                    for (int u = 0; u < 16; u++) t[u] = buf[e + u];
+#pragma unroll
+                    for (int u = 0; u < 16; u++) {
+                        acc = acc_step<LT, OP>(acc, t[u]);
+                        buf[e + u] = acc;
+                    }
+                }
+                for (; e < cnt; e++) {
+                    acc = acc_step<LT, OP>(acc, buf[e]);
+                    buf[e] = acc;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const int e = q * 64 + lane;
+                if (e < cnt) o[(rb + e) * A.out_st] = buf[e];
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        acc = readlane_t(acc, 0);
+        if constexpr (std::is_floating_point<T>::value && OP == IONP_ADD)
+            if (acc_redo<LT, OP>(acc)) {
+                __threadfence();  // the wavefront's plain stores land before lane 0 rewrites them
+                if (lane == 0) acc_redo_fiber<T>(A, x, o);
+            }
+    }
+}
+
+// Many fibers contiguous along the axis: a wavefront takes FPW fibers and
+// walks them in chunks of 64 elements.  A chunk is loaded with the lanes
+// along the axis (every load 64 adjacent elements), transposed through LDS,
+// each lane scans its own fiber's 64 elements (running value carried across
+// chunks), and the chunk goes back through LDS to coalesced stores.
+constexpr int kAccFpw = 16;  // fibers per wavefront in acc_tiled
+
+template <int LT, int OP> __global__ __launch_bounds__(64) void acc_tiled(const AccArgs A) {
+    using T = typename dt_type<LT>::T;
+    constexpr int FPW = kAccFpw;
+    __shared__ T tile[FPW][65];  // [fiber][element], padded against bank conflicts
+    const int lane = threadIdx.x;
+    const int64_t ngroups = (A.nfib + FPW - 1) / FPW;
+    for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        const int64_t f0 = g * FPW;
+        const int nf = (int)min((int64_t)FPW, A.nfib - f0);
+        // lane r < nf: fiber f0 + r (its scan, and its base offsets for the copies)
+        int64_t myib = 0, myob = 0;
+        if (lane < nf) fiber_offsets(A, (uint32_t)(f0 + lane), &myib, &myob);
+        int64_t ibr[FPW], obr[FPW];
+#pragma unroll
+        for (int r = 0; r < FPW; r++) {
+            ibr[r] = __shfl(myib, r, 64);
+            obr[r] = __shfl(myob, r, 64);
+        }
+        T acc = T(0);
+        T v[FPW], nv[FPW];
+        auto load = [&](int64_t c0, T (&dst)[FPW]) {
+            const int m = (int)min((int64_t)64, A.len - c0);
+#pragma unroll
+            for (int r = 0; r < FPW; r++)
+                dst[r] = (r < nf && lane < m) ? ((const T *)A.in)[ibr[r] + (c0 + lane) * A.in_st] : T(0);
+        };
+        load(0, v);
+        for (int64_t c0 = 0; c0 < A.len; c0 += 64) {
+            const int m = (int)min((int64_t)64, A.len - c0);
+#pragma unroll
+            for (int r = 0; r < FPW; r++) tile[r][lane] = v[r];
+            if (c0 + 64 < A.len) load(c0 + 64, nv);  // next chunk in flight during the scan
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (lane < nf) {
+                int e = 0;
+                if (c0 == 0) {
+                    acc = tile[lane][0];
+                    e = 1;
+                }
+                for (; e < m; e++) {
+                    acc = acc_step<LT, OP>(acc, tile[lane][e]);
+                    tile[lane][e] = acc;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int r = 0; r < FPW; r++)
+                if (r < nf && lane < m) ((T *)A.out)[obr[r] + (c0 + lane) * A.out_st] = tile[r][lane];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int r = 0; r < FPW; r++) v[r] = nv[r];
+        }
+        if constexpr (std::is_floating_point<T>::value && OP == IONP_ADD) {
+            const bool redo = lane < nf && acc != acc;
+            if (__ballot(redo)) {
+                __threadfence();  // the plain stores land before they are rewritten
+                if (redo) acc_redo_fiber<T>(A, (const T *)A.in + myib, (T *)A.out + myob);
+            }
+        }
+    }
+}
+
+typedef void (*AccFn)(const AccArgs);
+
+enum { ACC_THREAD = 0, ACC_WAVE = 1, ACC_TILED = 2 };
+
+template <int LT, int OP> static AccFn acc_pick(int kind) {
+    return kind == ACC_WAVE ? acc_wave<LT, OP> : kind == ACC_TILED ? acc_tiled<LT, OP> : acc_thread<LT, OP>;
+}
+
+template <int LT> static AccFn acc_fn(int op, int kind) {
+    switch (op) {
+        case IONP_ADD: return acc_pick<LT, IONP_ADD>(kind);
+        case IONP_MAXIMUM: return acc_pick<LT, IONP_MAXIMUM>(kind);
+        default: return acc_pick<LT, IONP_MINIMUM>(kind);
+    }
+}
+
+static AccFn acc_kernel(int dt, int op, int wave) {
+    switch (dt) {
+        case IONP_BOOL: return acc_fn<IONP_BOOL>(op, wave);
+        case IONP_INT8: return acc_fn<IONP_INT8>(op, wave);
+        case IONP_UINT8: return acc_fn<IONP_UINT8>(op, wave);
+        case IONP_INT16: return acc_fn<IONP_INT16>(op, wave);
+        case IONP_UINT16: return acc_fn<IONP_UINT16>(op, wave);
+        case IONP_INT32: return acc_fn<IONP_INT32>(op, wave);
+        case IONP_UINT32: return acc_fn<IONP_UINT32>(op, wave);
+        case IONP_INT64: return acc_fn<IONP_INT64>(op, wave);
+        case IONP_UINT64: return acc_fn<IONP_UINT64>(op, wave);
+        case IONP_FLOAT32: return acc_fn<IONP_FLOAT32>(op, wave);
+        default: return acc_fn<IONP_FLOAT64>(op, wave);
+    }
+}
+
+// in (already in the output dtype) -> out along `axis`
+ionp_status run_accumulate(int op, const ionp_array_desc *in, int axis, const ionp_array_desc *out,
+                           const ionp_config *cfg, hipStream_t s) {
+    const int n = in->ndim;
+    AccArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.len = in->shape[axis];
+    A.in_st = in->strides[axis];
+    A.out_st = out->strides[axis];
+    // fiber dims ordered innermost first by the operands' strides, so
+    // neighbouring threads touch neighbouring memory
+    int64_t st[2][IONP_MAX_DIMS];
+    std::memcpy(st[0], in->strides, sizeof(int64_t) * n);
+    std::memcpy(st[1], out->strides, sizeof(int64_t) * n);
+    int perm[IONP_MAX_DIMS];
+    iter_axis_order(n, in->shape, 2, st, perm);
+    A.nfib = 1;
+    for (int k = 0; k < n; k++) {
+        const int d = perm[k];
+        if (d == axis || in->shape[d] == 1) continue;
+        A.fshape[A.nf] = in->shape[d];
+        A.fdiv[A.nf] = FastDiv((uint32_t)in->shape[d]);
+        A.fin[A.nf] = in->strides[d];
+        A.fout[A.nf] = out->strides[d];
+        A.nf++;
+        A.nfib *= in->shape[d];
+    }
+    if (A.len == 0 || A.nfib == 0) return IONP_OK;
+    if (A.nfib >= (1ll << 31)) return set_error(IONP_ERR_UNSUPPORTED, "accumulate with 2^31 fibers or more");
+    const int esz = itemsize(out->dtype);
+    A.in = (const char *)in->data + in->offset * esz;
+    A.out = (char *)out->data + out->offset * esz;
+    // contiguous along the axis: 64 fibers per wavefront through LDS when
+    // there are enough of them, else the LDS chain per fiber; otherwise
+    // (neighbouring fibers adjacent in memory) a thread per fiber
+    int kind;
+    if (A.in_st == 1 && A.nfib >= 256) kind = ACC_TILED;
+    else if (A.nfib < 4096 && A.len >= 256) kind = ACC_WAVE;
+    else kind = ACC_THREAD;
+    const AccFn fn = acc_kernel(out->dtype, op, kind);
+    const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
+    const int threads = kind == ACC_THREAD ? 256 : 64;
+    const int64_t units = kind == ACC_THREAD ? (A.nfib + 255) / 256
+                          : kind == ACC_TILED ? (A.nfib + kAccFpw - 1) / kAccFpw
+                                              : A.nfib;
+    int64_t grid = (int64_t)bpc * num_cus() * (threads == 64 ? 4 : 1);
+    if (grid > units) grid = units;
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(threads), 0, s, A);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, hipGetErrorString(e));
+}
+
+}  // namespace ionp

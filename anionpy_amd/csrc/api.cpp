@@ -9,6 +9,7 @@
 #include "elementwise.h"
 #include "planner.h"
 #include "reduce.h"
+#include "accumulate.h"
 
 using namespace ionp;
 
@@ -292,6 +293,57 @@ ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *ax
     ReducePlan p;
     if ((s = plan_reduce(op, in, mask, out, &p))) return s;
     return run_reduce(p, g_bufsize, initial, cfg, as_stream(stream));
+}
+
+static ionp_status acc_axis(const ionp_array_desc *in, int32_t axis, int *ax) {
+    if (in->ndim == 0) return set_error(IONP_ERR_VALUE, "cannot accumulate on a scalar");
+    const int norm = axis < 0 ? axis + in->ndim : axis;
+    if (norm < 0 || norm >= in->ndim)
+        return set_error(IONP_ERR_AXIS, "axis " + std::to_string(axis) + " is out of bounds for array of dimension " +
+                                            std::to_string(in->ndim));
+    *ax = norm;
+    return IONP_OK;
+}
+
+ionp_status ionp_plan_accumulate(int32_t op, const ionp_array_desc *in, int32_t axis, int32_t dtype,
+                                 ionp_array_desc *out_template) {
+    ionp_status s;
+    if ((s = check_desc(in, "array"))) return s;
+    int odt, ax;
+    if ((s = reduce_dtype(op, in->dtype, dtype, &odt))) return s;
+    if ((s = acc_axis(in, axis, &ax))) return s;
+    // the input's K order (the ordinary one-operand layout rule, as for casts)
+    int64_t st[1][IONP_MAX_DIMS], ost[IONP_MAX_DIMS];
+    std::memcpy(st[0], in->strides, sizeof(int64_t) * in->ndim);
+    korder_strides(1, st, in->ndim, in->shape, ost);
+    fill_template(out_template, odt, in->ndim, in->shape, ost);
+    return IONP_OK;
+}
+
+ionp_status ionp_accumulate(int32_t op, const ionp_array_desc *in, int32_t axis, const ionp_array_desc *out,
+                            const ionp_config *cfg, void *stream) {
+    if (ionp_status cs = check_cfg(cfg)) return cs;
+    ionp_status s;
+    ionp_array_desc t;
+    if ((s = check_desc(out, "out"))) return s;
+    if ((s = ionp_plan_accumulate(op, in, axis, out->dtype, &t))) return s;
+    if ((s = same_shape(out, t.ndim, t.shape, "accumulate"))) return s;
+    int ax = 0;
+    acc_axis(in, axis, &ax);
+    hipStream_t st = as_stream(stream);
+    if (in->dtype == out->dtype) return run_accumulate(op, in, ax, out, cfg, st);
+    // another input dtype: cast into scratch first (dense, K order), then scan
+    const int64_t nel = desc_size(in);
+    void *tmp = scratch_get((size_t)(nel > 0 ? nel : 1) * itemsize(out->dtype), st);
+    if (!tmp) return set_error(IONP_ERR_DEVICE, "out of device memory for accumulate scratch");
+    ionp_array_desc c = t;
+    c.data = tmp;
+    c.offset = 0;
+    const ionp_array_desc *ins[1] = {in};
+    EwArgs E;
+    if ((s = plan_elementwise(OP_CAST, 1, ins, &c, out->dtype, &E))) return s;
+    if ((s = run_elementwise(E, nullptr, st))) return s;
+    return run_accumulate(op, &c, ax, out, cfg, st);
 }
 
 ionp_status ionp_argext(int32_t ismax, const ionp_array_desc *in, int32_t axis, int32_t keepdims,

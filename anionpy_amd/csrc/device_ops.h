@@ -331,13 +331,15 @@ template <typename T> __device__ inline T readlane_t(T v, int lane) {
         T out;
         std::memcpy(&out, &r, 8);
         return out;
-    } else {
+    } else if constexpr (sizeof(T) == 4) {
         uint32_t b;
         std::memcpy(&b, &v, 4);
         const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)b, lane);
         T out;
         std::memcpy(&out, &r, 4);
         return out;
+    } else {  // 1- and 2-byte integers
+        return (T)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
     }
 }
 

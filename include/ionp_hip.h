@@ -182,6 +182,17 @@ ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *ax
 ionp_status ionp_argext(int32_t ismax, const ionp_array_desc *in, int32_t axis, int32_t keepdims,
                         const ionp_array_desc *out, const ionp_config *cfg, void *stream);
 
+/* Replaces ionp_core::ufunc::accumulate_axis (ufunc.rs:7967) for op in
+ * {ADD, MAXIMUM, MINIMUM}: along `axis` (negative counts from the end),
+ * out[0] = x[0], out[i] = op(out[i-1], x[i]) in the output dtype (the
+ * reduce dtype rule: small integers and bool add in int64 / uint64; or an
+ * explicit dtype).  Float add is the sequential running sum.  The output
+ * template has the input's shape in its K order. */
+ionp_status ionp_plan_accumulate(int32_t op, const ionp_array_desc *in, int32_t axis, int32_t dtype,
+                                 ionp_array_desc *out_template);
+ionp_status ionp_accumulate(int32_t op, const ionp_array_desc *in, int32_t axis, const ionp_array_desc *out,
+                            const ionp_config *cfg, void *stream);
+
 /* ---- batching: many small calls, one launch --------------------------- */
 /* No counterpart in the reference (it runs one call at a time). */
 

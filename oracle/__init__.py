@@ -61,7 +61,8 @@ def lib():
                            ("ora_where", [P, P, P, P]),
                            ("ora_cast", [P, P]),
                            ("ora_reduce", [ctypes.c_int, P, ctypes.c_char_p, P, ctypes.c_int64, ctypes.c_void_p]),
-                           ("ora_argext", [ctypes.c_int, P, ctypes.c_int, P])]:
+                           ("ora_argext", [ctypes.c_int, P, ctypes.c_int, P]),
+                           ("ora_accumulate", [ctypes.c_int, P, ctypes.c_int, P])]:
             getattr(L, name).argtypes = args
             getattr(L, name).restype = ctypes.c_int
         L.ora_result_type.argtypes = [ctypes.c_int, ctypes.c_int]
@@ -216,6 +217,20 @@ def reduce(op, x, axis=0, bufsize=BUFSIZE, dtype=None, initial=None):
     _check(lib().ora_reduce(OP[op], ctypes.byref(dx), mask, ctypes.byref(do), bufsize,
                             None if init is None else init.ctypes.data))
     return out[()] if out.ndim == 0 else out
+
+
+def accumulate(op, x, axis=0, dtype=None):
+    """np.<op>.accumulate(x, axis, dtype) for op in add/maximum/minimum, with
+    the output laid out in the input's K order."""
+    x = np.asarray(x)
+    if x.ndim == 0:
+        raise OracleError(6)
+    ax = axis + x.ndim if axis < 0 else axis
+    odt = DTYPES[lib().ora_reduce_dtype(OP[op], _CODE[x.dtype])] if dtype is None else np.dtype(dtype)
+    out = _empty_strided(x.shape, korder_strides([x], x.shape), odt)
+    dx, do = _desc(x), _desc(out)
+    _check(lib().ora_accumulate(OP[op], ctypes.byref(dx), ax, ctypes.byref(do)))
+    return out
 
 
 def argext(ismax, x, axis=None):

@@ -996,6 +996,79 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
     return ORA_OK;
 }
 
+/* accumulate_axis (ufunc.rs:7967) -> accumulate_axis_generic: along `axis`,
+ * out[0] = x[0] and out[i] = op(out[i-1], x[i]), in the output dtype (the
+ * reduce-family compute dtype, or an explicit dtype=); elements are cast to
+ * it as they are read.  Float add is the plain sequential sum (no pairwise
+ * tree); maximum/minimum use the scalar rules of f64_max/f64_min; integers
+ * wrap; on bool, add/maximum are logical or and minimum is logical and. */
+int ora_accumulate(int op, const ora_desc *in, int axis, const ora_desc *out)
+{
+    int n = in->ndim, k, cdt = out->dtype, osz;
+    int64_t idx[ORA_MAXDIM], len, nfib = 1, f, i;
+    if (op != OP_ADD && op != OP_MAXIMUM && op != OP_MINIMUM) return ORA_ENOLOOP;
+    if (axis < 0 || axis >= (n > 0 ? n : 1)) return ORA_EAXIS;
+    if (is_float(in->dtype) && !is_float(cdt)) return ORA_ETYPE;
+    if (out->ndim != n) return ORA_EVALUE;
+    for (k = 0; k < n; k++)
+        if (out->shape[k] != in->shape[k]) return ORA_EVALUE;
+    len = n > 0 ? in->shape[axis] : 1;
+    for (k = 0; k < n; k++)
+        if (k != axis) nfib *= in->shape[k];
+    osz = ITEMSIZE[cdt];
+    memset(idx, 0, sizeof idx);
+    for (f = 0; f < nfib; f++) {
+        int64_t ib = 0, ob = 0;
+        char acc[8], v[8];
+        for (k = 0; k < n; k++) {
+            ib += idx[k] * in->strides[k];
+            ob += idx[k] * out->strides[k];
+        }
+        for (i = 0; i < len; i++) {
+            const char *pi = (const char *)in->data + (ib + i * (n > 0 ? in->strides[axis] : 0)) * ITEMSIZE[in->dtype];
+            char *po = (char *)out->data + (ob + i * (n > 0 ? out->strides[axis] : 0)) * osz;
+            cast_elem(in->dtype, pi, cdt, v);
+            if (i == 0) {
+                memcpy(acc, v, osz);
+            } else if (cdt == DT_F64) {
+                double a = *(double *)acc, b = *(double *)v;
+                a = op == OP_ADD ? f64_arith(OP_ADD, a, b) : op == OP_MAXIMUM ? f64_max(a, b) : f64_min(a, b);
+                memcpy(acc, &a, 8);
+            } else if (cdt == DT_F32) {
+                float a = *(float *)acc, b = *(float *)v;
+                a = op == OP_ADD ? f32_arith(OP_ADD, a, b) : op == OP_MAXIMUM ? f32_max(a, b) : f32_min(a, b);
+                memcpy(acc, &a, 4);
+            } else if (cdt == DT_BOOL) {
+                uint8_t a = *(uint8_t *)acc != 0, b = *(uint8_t *)v != 0;
+                a = op == OP_MINIMUM ? (a & b) : (a | b);
+                memcpy(acc, &a, 1);
+            } else {
+                int64_t sa = load_i(cdt, acc), sb = load_i(cdt, v);
+                uint64_t r;
+                if (op == OP_ADD) {
+                    r = (uint64_t)sa + (uint64_t)sb;
+                } else {
+                    int gt = is_unsigned_int(cdt) ? (uint64_t)sa > (uint64_t)sb : sa > sb;
+                    r = (uint64_t)((op == OP_MAXIMUM) == gt ? sa : sb);
+                }
+                switch (osz) {
+                case 1: { uint8_t t = (uint8_t)r; memcpy(acc, &t, 1); break; }
+                case 2: { uint16_t t = (uint16_t)r; memcpy(acc, &t, 2); break; }
+                case 4: { uint32_t t = (uint32_t)r; memcpy(acc, &t, 4); break; }
+                default: memcpy(acc, &r, 8); break;
+                }
+            }
+            memcpy(po, acc, osz);
+        }
+        for (k = n - 1; k >= 0; k--) {
+            if (k == axis) continue;
+            if (++idx[k] < in->shape[k]) break;
+            idx[k] = 0;
+        }
+    }
+    return ORA_OK;
+}
+
 /* argext (ufunc.rs:8131) -> reduce_axis_argext (ufunc.rs:8037): for each
  * kept index (C order), scan the axis in index order; the first NaN wins,
  * otherwise the first strict extreme.  axis < 0 scans the C-order ravel. */

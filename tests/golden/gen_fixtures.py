@@ -95,11 +95,11 @@ class Store:
         return {"buf": key, "offset": int(off), "shape": list(v.shape), "strides": st, "dtype": v.dtype.name}
 
     def expect(self, r):
-        """Expected output: its values (C order) or, above 32 KB, their SHA-256."""
+        """Expected output: its values (C order) or, above 8 KB, their SHA-256."""
         r = np.asarray(r)
         isz = r.itemsize
         e = {"dtype": r.dtype.name, "shape": list(r.shape), "strides": [s // isz for s in r.strides]}
-        if r.nbytes > 32768:
+        if r.nbytes > 8192:
             e["sha256"] = sha(r)
         else:
             e["buf"] = self.buf(np.ascontiguousarray(r))
@@ -449,6 +449,66 @@ def initial_cases(S):
           {"axis": 1, "initial": True, "dtype": "bool"})
 
 
+def accumulate_cases(S):
+    """ufunc.accumulate (accumulate_axis, ufunc.rs:7967) and cumsum: the
+    sequential scan per fiber, reduce-family dtypes, K-order outputs.  Own
+    generator (seed 80)."""
+    r5 = np.random.default_rng(80)
+    for dt in (np.float64, np.float32, np.int64, np.int32, np.int8, np.uint8, np.bool_):
+        nm = np.dtype(dt).name
+        for shp in ((300,), (37, 41), (5, 20, 33)):
+            x = specials(dt, int(np.prod(shp)), r5).reshape(shp)
+            views = [("C", x), ("rev", x[::-1])]
+            if len(shp) > 1:
+                views += [("T", x.T), ("step", x[:, ::2])]
+            for vn, v in views:
+                for ax in range(v.ndim):
+                    for op in ("add", "maximum", "minimum"):
+                        S.add(f"acc/{op}/{nm}/{shp}/{vn}/{ax}", "accumulate", op, [v],
+                              lambda: getattr(np, op).accumulate(v, axis=ax), {"axis": ax})
+    # long fibers (few outputs) and many short ones, at sizes past one block
+    for dt in (np.float64, np.float32):
+        nm = np.dtype(dt).name
+        g = Gen(80, 20000, dt)
+        xg = g.array()
+        for op in ("add", "maximum"):
+            S.add(f"acc/{op}/{nm}/long1d", "accumulate", op, [g], lambda: getattr(np, op).accumulate(xg),
+                  {"axis": 0})
+        A = r5.uniform(-1, 1, (9000, 3)).astype(dt)
+        A[r5.random(A.shape) < 1e-3] = np.nan
+        for ax in (0, 1):
+            S.add(f"acc/add/{nm}/9000x3/{ax}", "accumulate", "add", [A], lambda: np.add.accumulate(A, axis=ax),
+                  {"axis": ax})
+            S.add(f"acc/minimum/{nm}/9000x3/{ax}", "accumulate", "minimum", [A],
+                  lambda: np.minimum.accumulate(A, axis=ax), {"axis": ax})
+        B = r5.uniform(-1, 1, (300, 700)).astype(dt)
+        for ax in (0, 1):
+            S.add(f"acc/add/{nm}/300x700/{ax}", "accumulate", "add", [B], lambda: np.add.accumulate(B, axis=ax),
+                  {"axis": ax})
+        T = r5.uniform(-1, 1, (2000, 150)).astype(dt)
+        T[r5.random(T.shape) < 2e-4] = np.nan
+        for op in ("add", "maximum"):
+            S.add(f"acc/{op}/{nm}/2000x150/1", "accumulate", op, [T], lambda: getattr(np, op).accumulate(T, axis=1),
+                  {"axis": 1})
+    W = r5.integers(-2 ** 62, 2 ** 62, (1500, 130))
+    S.add("acc/add/int64/1500x130/1", "accumulate", "add", [W], lambda: np.add.accumulate(W, axis=1), {"axis": 1})
+    I = r5.integers(-2 ** 62, 2 ** 62, (2000, 5))
+    S.add("acc/add/int64/wrap", "accumulate", "add", [I], lambda: np.add.accumulate(I, axis=0), {"axis": 0})
+    J = r5.integers(-1000, 1000, (40, 50), dtype=np.int32)
+    S.add("acc/add/int32->float64", "accumulate", "add", [J], lambda: np.add.accumulate(J, axis=1, dtype=np.float64),
+          {"axis": 1, "dtype": "float64"})
+    S.add("acc/add/int32->int16", "accumulate", "add", [J], lambda: np.add.accumulate(J, axis=0, dtype=np.int16),
+          {"axis": 0, "dtype": "int16"})
+    S.add("error/acc_axis", "accumulate", "add", [J], lambda: np.add.accumulate(J, axis=2), {"axis": 2})
+    E = np.zeros((0, 4))
+    S.add("acc/add/empty", "accumulate", "add", [E], lambda: np.add.accumulate(E, axis=0), {"axis": 0})
+    # cumsum: add.accumulate over the C-order ravel when axis is None
+    K = r5.standard_normal((30, 40))
+    S.add("cumsum/float64/None", "accumulate", "cumsum", [K.T], lambda: np.cumsum(K.T), {"axis": None})
+    S.add("cumsum/float64/1", "accumulate", "cumsum", [K], lambda: np.cumsum(K, axis=1), {"axis": 1})
+    S.add("cumsum/int32/None", "accumulate", "cumsum", [J], lambda: np.cumsum(J), {"axis": None})
+
+
 def argext_cases(S, rng):
     for dt in (np.float64, np.float32, np.int64, np.int32, np.uint8, np.bool_):
         nm = np.dtype(dt).name
@@ -523,6 +583,7 @@ def main():
     where_cast_cases(S, rng)
     reduce_cases(S, rng)
     argext_cases(S, rng)
+    accumulate_cases(S)
     np.savez_compressed(os.path.join(HERE, "fixtures.npz"), **S.arrays)
     with open(os.path.join(HERE, "cases.json"), "w") as f:
         json.dump(S.cases, f)

@@ -7,7 +7,7 @@ from fixture_cases import args_axis, check_result, host_input, load
 
 pytestmark = pytest.mark.gpu
 
-KINDS = ["binary", "where", "cast", "reduce", "argext"]
+KINDS = ["binary", "where", "cast", "reduce", "argext", "accumulate"]
 
 EXC = {"ValueError": ValueError, "TypeError": TypeError, "IndexError": IndexError,
        "AxisError": ValueError, "UFuncTypeError": TypeError}
@@ -36,6 +36,10 @@ def run_device(ap, case, npz, cache):
     if k == "reduce":
         return getattr(ap, op).reduce(ins[0], axis=args_axis(args), dtype=args.get("dtype"),
                                       keepdims=bool(args.get("keepdims", False)), initial=args.get("initial"))
+    if k == "accumulate":
+        if op == "cumsum":
+            return ap.cumsum(ins[0], axis=args.get("axis"))
+        return getattr(ap, op).accumulate(ins[0], axis=args["axis"], dtype=args.get("dtype"))
     return getattr(ap, op)(ins[0], axis=args.get("axis"), keepdims=bool(args.get("keepdims", False)))
 
 

@@ -5,7 +5,7 @@ import pytest
 import oracle as O
 from fixture_cases import args_axis, check_result, host_input, load
 
-KINDS = ["binary", "where", "cast", "reduce", "argext"]
+KINDS = ["binary", "where", "cast", "reduce", "argext", "accumulate"]
 
 
 def run_oracle(case, npz):
@@ -27,6 +27,13 @@ def run_oracle(case, npz):
             axes = range(x.ndim) if ax is None else ((ax,) if isinstance(ax, int) else ax)
             r = np.expand_dims(r, tuple(a % x.ndim for a in axes))
         return r
+    if k == "accumulate":
+        if op == "cumsum":
+            x, ax = ins[0], args.get("axis")
+            if ax is None:
+                x, ax = np.ravel(x), 0  # C-order ravel (a host copy), then the scan
+            return O.accumulate("add", x, ax)
+        return O.accumulate(op, ins[0], args["axis"], dtype=args.get("dtype"))
     r = O.argext(op == "argmax", ins[0], args.get("axis"))
     if args.get("keepdims"):
         r = np.expand_dims(r, args["axis"])
