@@ -236,11 +236,16 @@ template <typename T> __device__ void pw_folder(const RedArgs &A, const PwFastSt
             idle = 0;
         } else {
             if (n == 0) {
-                if (++idle > (1u << 26)) {  // producers always fill every slot; safety bound only
-                    failed = true;
-                    break;
+                // wait on the first unfilled slot alone (one load per poll,
+                // not three blocks), then re-read the blocks
+                while (ld_bits(part + cur) == Sentinel<T>::bits) {
+                    if (++idle > (1u << 26)) {  // producers always fill every slot; safety bound only
+                        failed = true;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
                 }
-                __builtin_amdgcn_s_sleep(1);
+                if (failed) break;
             } else {
                 idle = 0;
             }
@@ -327,15 +332,16 @@ template <typename T, bool EXACT> __device__ inline T pw_window_wave(const T *x,
 // 16-byte load per step k), the 64 leaf values meet in LDS and wavefront 0
 // combines them by the balanced tree over lanes.  (Measured on MI355X: one
 // wavefront per window, or one workgroup per CU, are both slower.)
-template <typename T> constexpr int pw_fast_threads() { return sizeof(T) == 8 ? 256 : 128; }
+template <typename T> constexpr int pw_fast_threads() { return sizeof(T) == 8 ? 256 : 128; }  // one window
 
 template <typename T, bool PREFETCH>
-__global__ __launch_bounds__(256) void pw_fast(const RedArgs A, const PwFastState F, int64_t nwindows) {
+__global__ __launch_bounds__(512) void pw_fast(const RedArgs A, const PwFastState F, int64_t nwindows) {
     constexpr int CH = 16 / sizeof(T);
     constexpr int LPL = 8 / CH;
+    constexpr int TPW = 64 * LPL;  // threads per window
     using V = T __attribute__((ext_vector_type(CH)));
     __shared__ T stage[1024];  // folder staging
-    __shared__ T leafv[64];
+    __shared__ T leafv[4][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if ((int64_t)blockIdx.x < F.folders) {
         if (wave == 0) {
@@ -346,7 +352,10 @@ __global__ __launch_bounds__(256) void pw_fast(const RedArgs A, const PwFastStat
         }
         return;
     }
-    const int lf = threadIdx.x / LPL, jq = (lane % LPL) * CH;
+    // WPB windows side by side per workgroup (slot `sub`), taken round-robin
+    // in groups of WPB consecutive windows
+    const int wpb = blockDim.x / TPW, sub = threadIdx.x / TPW, tin = threadIdx.x % TPW;
+    const int lf = tin / LPL, jq = (lane % LPL) * CH;
     const int64_t nprod = (int64_t)gridDim.x - F.folders;
     auto window_base = [&](int64_t g) {
         const int64_t o = g / F.nfull, w = g - o * F.nfull;
@@ -355,46 +364,50 @@ __global__ __launch_bounds__(256) void pw_fast(const RedArgs A, const PwFastStat
         return (const T *)A.in + ib + w * 8192;
     };
     V v[16], nv[16];
-    int64_t g = (int64_t)blockIdx.x - F.folders;
-    if (PREFETCH && g < nwindows) {
-        const T *x = window_base(g) + (int64_t)lf * 128 + jq;
+    int64_t gb = ((int64_t)blockIdx.x - F.folders) * wpb;  // first window of this round
+    if (PREFETCH && gb + sub < nwindows) {
+        const T *x = window_base(gb + sub) + (int64_t)lf * 128 + jq;
 #pragma unroll
         for (int k = 0; k < 16; k++) nv[k] = *(const V *)(x + 8 * k);
     }
-    for (; g < nwindows; g += nprod) {
-        const int64_t o = g / F.nfull, w = g - o * F.nfull;
-        const T *xw = window_base(g);
+    for (; gb < nwindows; gb += nprod * wpb) {
+        const int64_t g = gb + sub;
+        const bool live = g < nwindows;
+        const int64_t o = live ? g / F.nfull : 0, w = live ? g - o * F.nfull : 0;
+        const T *xw = live ? window_base(g) : (const T *)A.in;
         if constexpr (PREFETCH) {
-            // this window was loaded one iteration ago; request the next one
+            // this window was loaded one round ago; request the next one
 #pragma unroll
             for (int k = 0; k < 16; k++) v[k] = nv[k];
-            if (g + nprod < nwindows) {
-                const T *xn = window_base(g + nprod) + (int64_t)lf * 128 + jq;
+            if (g + nprod * wpb < nwindows) {
+                const T *xn = window_base(g + nprod * wpb) + (int64_t)lf * 128 + jq;
 #pragma unroll
                 for (int k = 0; k < 16; k++) nv[k] = *(const V *)(xn + 8 * k);
             }
-        } else {
+        } else if (live) {
             const T *x = xw + (int64_t)lf * 128 + jq;
 #pragma unroll
             for (int k = 0; k < 16; k++) v[k] = *(const V *)(x + 8 * k);
         }
-        // accumulators jq .. jq+CH-1 of leaf lf, each over k in order
-        V r = v[0];
+        if (live) {
+            // accumulators jq .. jq+CH-1 of leaf lf, each over k in order
+            V r = v[0];
 #pragma unroll
-        for (int k = 1; k < 16; k++) r = r + v[k];
-        T a;
-        if constexpr (CH == 2) {
-            a = r[0] + r[1];              // r0+r1 (lane holds accumulators 2m, 2m+1)
-            a = xor_combine(a, 1, lane);  // (r0+r1)+(r2+r3)
-            a = xor_combine(a, 2, lane);  // leaf value
-        } else {
-            a = (r[0] + r[1]) + (r[2] + r[3]);
-            a = xor_combine(a, 1, lane);  // leaf value
+            for (int k = 1; k < 16; k++) r = r + v[k];
+            T a;
+            if constexpr (CH == 2) {
+                a = r[0] + r[1];              // r0+r1 (lane holds accumulators 2m, 2m+1)
+                a = xor_combine(a, 1, lane);  // (r0+r1)+(r2+r3)
+                a = xor_combine(a, 2, lane);  // leaf value
+            } else {
+                a = (r[0] + r[1]) + (r[2] + r[3]);
+                a = xor_combine(a, 1, lane);  // leaf value
+            }
+            if (lane % LPL == 0) leafv[sub][lf] = a;
         }
-        if (lane % LPL == 0) leafv[lf] = a;
         __syncthreads();
-        if (wave == 0) {
-            T t = leafv[lane];
+        if (live && tin < 64) {  // the slot's first wavefront: the tree over its 64 leaves
+            T t = leafv[sub][lane];
 #pragma unroll
             for (int m = 1; m < 64; m <<= 1) t = xor_combine(t, m, lane);
             if (t != t) t = pw_window_wave<T, true>(xw, lane);  // NaN: x86 NaN rules, exactly
@@ -1047,12 +1060,12 @@ template <typename T> static ionp_status run_pw_fast(RedArgs &A, const ionp_conf
     // before the current one is combined; with several per CU they are not
     // (measured on MI355X both ways); split 1 / 2 forces either
     const bool prefetch = (cfg && cfg->split > 0) ? cfg->split == 2 : F.folders > 0;
-    if (prefetch)
-        hipLaunchKernelGGL((pw_fast<T, true>), dim3(F.folders + grid_cap(nwindows, pbpc)), dim3(pw_fast_threads<T>()), 0, s,
-                           A, F, nwindows);
-    else
-        hipLaunchKernelGGL((pw_fast<T, false>), dim3(F.folders + grid_cap(nwindows, pbpc)), dim3(pw_fast_threads<T>()), 0,
-                           s, A, F, nwindows);
+    // vec: windows side by side per workgroup (1, 2 or 4; 512 threads at most)
+    int wpb = (cfg && cfg->vec > 0) ? cfg->vec : 1;
+    while (wpb > 1 && pw_fast_threads<T>() * wpb > 512) wpb >>= 1;
+    const dim3 grid(F.folders + grid_cap((nwindows + wpb - 1) / wpb, pbpc)), blk(pw_fast_threads<T>() * wpb);
+    if (prefetch) hipLaunchKernelGGL((pw_fast<T, true>), grid, blk, 0, s, A, F, nwindows);
+    else hipLaunchKernelGGL((pw_fast<T, false>), grid, blk, 0, s, A, F, nwindows);
     if (!F.folders) hipLaunchKernelGGL(pw_fold<T>, dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
     return launched();
 }
