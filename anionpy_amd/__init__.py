@@ -25,7 +25,7 @@ from ._lib import IonpError
 
 __all__ = ["ndarray", "asarray", "empty", "zeros", "Ufunc", "add", "subtract", "multiply", "divide",
            "true_divide", "maximum", "minimum", "less", "less_equal", "greater", "greater_equal", "equal",
-           "not_equal", "where", "sum", "cumsum", "max", "amax", "min", "amin", "argmax", "argmin", "result_type",
+           "not_equal", "where", "sum", "prod", "cumsum", "cumprod", "max", "amax", "min", "amin", "argmax", "argmin", "result_type",
            "AxisError", "UFuncTypeError", "IonpError", "setbufsize", "getbufsize", "set_config_policy",
            "get_config_policy"]
 
@@ -280,6 +280,12 @@ class ndarray:
 
     def cumsum(self, axis=None, dtype=None, out=None):
         return cumsum(self, axis=axis, dtype=dtype, out=out)
+
+    def prod(self, axis=None, dtype=None, keepdims=False, initial=None):
+        return prod(self, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial)
+
+    def cumprod(self, axis=None, dtype=None, out=None):
+        return cumprod(self, axis=axis, dtype=dtype, out=out)
 
     def max(self, axis=None, keepdims=False, initial=None):
         return amax(self, axis=axis, keepdims=keepdims, initial=initial)
@@ -647,7 +653,7 @@ def _reduce(code, a, axis, dtype, keepdims, out=None, initial=None):
 
 add = Ufunc("add", _lib.ADD, True)
 subtract = Ufunc("subtract", _lib.SUBTRACT, False)
-multiply = Ufunc("multiply", _lib.MULTIPLY, False)
+multiply = Ufunc("multiply", _lib.MULTIPLY, True)
 true_divide = Ufunc("divide", _lib.TRUE_DIVIDE, False)
 divide = true_divide
 maximum = Ufunc("maximum", _lib.MAXIMUM, True)
@@ -665,14 +671,30 @@ def sum(a, axis=None, dtype=None, keepdims=False, initial=None):
     return add.reduce(a, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial)
 
 
-def cumsum(a, axis=None, dtype=None, out=None):
-    """cumsum (reductions.rs) = add.accumulate; axis=None scans the C-order
-    ravel."""
+def prod(a, axis=None, dtype=None, keepdims=False, initial=None):
+    """prod (ionp-py/src/reductions.rs:1053) = multiply.reduce with axis=None
+    default."""
+    return multiply.reduce(a, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial)
+
+
+def _scan(uf, a, axis, dtype, out):
     a = asarray(a)
     if axis is None:
         a = a.reshape(-1)
         axis = 0
-    return add.accumulate(a, axis=axis, dtype=dtype, out=out)
+    return uf.accumulate(a, axis=axis, dtype=dtype, out=out)
+
+
+def cumsum(a, axis=None, dtype=None, out=None):
+    """cumsum (reductions.rs:1312) = add.accumulate; axis=None scans the
+    C-order ravel."""
+    return _scan(add, a, axis, dtype, out)
+
+
+def cumprod(a, axis=None, dtype=None, out=None):
+    """cumprod (reductions.rs:1324) = multiply.accumulate, axis=None as in
+    cumsum."""
+    return _scan(multiply, a, axis, dtype, out)
 
 
 def amax(a, axis=None, keepdims=False, initial=None):

@@ -1,4 +1,4 @@
-// accumulate.hip -- ufunc.accumulate for add / maximum / minimum
+// accumulate.hip -- ufunc.accumulate for add / multiply / maximum / minimum
 // (accumulate_axis, ionp-core/src/ufunc.rs:7967 -> accumulate_axis_generic
 // 7830): along one axis, out[0] = x[0] and out[i] = op(out[i-1], x[i]) in
 // the output dtype.  The scan is sequential per fiber (NumPy's float add
@@ -10,8 +10,9 @@
 //                chunks of 64 elements transposed through LDS;
 //   acc_wave   - few long fibers: a wavefront per fiber stages blocks in
 //                LDS and one lane runs the chain over them.
-// Float add runs plain IEEE adds and redoes a fiber that ends in NaN with
-// fadd's x86 NaN rules (they agree until a NaN appears, and a NaN stays).
+// Float add and multiply run plain IEEE arithmetic and redo a fiber that
+// ends in NaN with fadd's / fmul's x86 NaN rules (they agree until a NaN
+// appears, and a NaN stays).
 // An input of another dtype is first cast into scratch memory, as the
 // reference casts before scanning.
 #include "accumulate.h"
@@ -51,35 +52,42 @@ __device__ inline void fiber_offsets(const AccArgs &A, uint32_t f, int64_t *ib, 
     *ob = b;
 }
 
-// one step of the scan in dtype LT (float add: plain IEEE add, see above)
+// one step of the scan in dtype LT (float add / multiply: plain IEEE, see
+// above; bool: add and maximum are or, multiply and minimum and)
 template <int LT, int OP> __device__ inline typename dt_type<LT>::T acc_step(typename dt_type<LT>::T a,
                                                                             typename dt_type<LT>::T b) {
     using T = typename dt_type<LT>::T;
     if constexpr (LT == IONP_BOOL) {
-        return OP == IONP_MINIMUM ? (T)((a != 0) & (b != 0)) : (T)((a != 0) | (b != 0));
+        return (OP == IONP_MINIMUM || OP == IONP_MULTIPLY) ? (T)((a != 0) & (b != 0)) : (T)((a != 0) | (b != 0));
     } else if constexpr (std::is_floating_point<T>::value) {
         if constexpr (OP == IONP_ADD) return a + b;
+        else if constexpr (OP == IONP_MULTIPLY) return a * b;
         else if constexpr (OP == IONP_MAXIMUM) return fmaxn(a, b);
         else return fminn(a, b);
     } else {
         if constexpr (OP == IONP_ADD) return (T)((uint64_t)a + (uint64_t)b);  // two's-complement wrap
+        else if constexpr (OP == IONP_MULTIPLY) return (T)((uint64_t)a * (uint64_t)b);
         else if constexpr (OP == IONP_MAXIMUM) return a > b ? a : b;
         else return a < b ? a : b;
     }
 }
 
+// float add and multiply: the plain IEEE chain needs the x86 NaN redo
+template <typename T, int OP> constexpr bool acc_plain() {
+    return std::is_floating_point<T>::value && (OP == IONP_ADD || OP == IONP_MULTIPLY);
+}
 template <int LT, int OP> __device__ inline bool acc_redo(typename dt_type<LT>::T acc) {
-    if constexpr (std::is_floating_point<typename dt_type<LT>::T>::value && OP == IONP_ADD) return acc != acc;
+    if constexpr (acc_plain<typename dt_type<LT>::T, OP>()) return acc != acc;
     else return false;
 }
 
-// the fiber again with fadd (x86 NaN rules), every output rewritten
-template <typename T>
+// the fiber again with fadd / fmul (x86 NaN rules), every output rewritten
+template <typename T, int OP>
 __device__ inline void acc_redo_fiber(const AccArgs &A, const T *x, T *o) {
     T acc = x[0];
     o[0] = acc;
     for (int64_t i = 1; i < A.len; i++) {
-        acc = fadd(acc, x[i * A.in_st]);
+        acc = OP == IONP_MULTIPLY ? fmul(acc, x[i * A.in_st]) : fadd(acc, x[i * A.in_st]);
         o[i * A.out_st] = acc;
     }
 }
@@ -118,8 +126,8 @@ template <int LT, int OP> __global__ __launch_bounds__(256) void acc_thread(cons
             acc = acc_step<LT, OP>(acc, x[i * A.in_st]);
             o[i * A.out_st] = acc;
         }
-        if constexpr (std::is_floating_point<T>::value && OP == IONP_ADD)
-            if (acc_redo<LT, OP>(acc)) acc_redo_fiber<T>(A, x, o);
+        if constexpr (acc_plain<T, OP>())
+            if (acc_redo<LT, OP>(acc)) acc_redo_fiber<T, OP>(A, x, o);
     }
 }
 
@@ -182,10 +190,10 @@ template <int LT, int OP> __global__ __launch_bounds__(64) void acc_wave(const A
             __builtin_amdgcn_wave_barrier();
         }
         acc = readlane_t(acc, 0);
-        if constexpr (std::is_floating_point<T>::value && OP == IONP_ADD)
+        if constexpr (acc_plain<T, OP>())
             if (acc_redo<LT, OP>(acc)) {
                 __threadfence();  // the wavefront's plain stores land before lane 0 rewrites them
-                if (lane == 0) acc_redo_fiber<T>(A, x, o);
+                if (lane == 0) acc_redo_fiber<T, OP>(A, x, o);
             }
     }
 }
@@ -252,11 +260,11 @@ template <int LT, int OP> __global__ __launch_bounds__(64) void acc_tiled(const 
 #pragma unroll
             for (int r = 0; r < FPW; r++) v[r] = nv[r];
         }
-        if constexpr (std::is_floating_point<T>::value && OP == IONP_ADD) {
+        if constexpr (acc_plain<T, OP>()) {
             const bool redo = lane < nf && acc != acc;
             if (__ballot(redo)) {
                 __threadfence();  // the plain stores land before they are rewritten
-                if (redo) acc_redo_fiber<T>(A, (const T *)A.in + myib, (T *)A.out + myob);
+                if (redo) acc_redo_fiber<T, OP>(A, (const T *)A.in + myib, (T *)A.out + myob);
             }
         }
     }
@@ -273,6 +281,7 @@ template <int LT, int OP> static AccFn acc_pick(int kind) {
 template <int LT> static AccFn acc_fn(int op, int kind) {
     switch (op) {
         case IONP_ADD: return acc_pick<LT, IONP_ADD>(kind);
+        case IONP_MULTIPLY: return acc_pick<LT, IONP_MULTIPLY>(kind);
         case IONP_MAXIMUM: return acc_pick<LT, IONP_MAXIMUM>(kind);
         default: return acc_pick<LT, IONP_MINIMUM>(kind);
     }

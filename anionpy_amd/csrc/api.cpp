@@ -142,13 +142,13 @@ ionp_status ionp_plan_cast(const ionp_array_desc *in, int32_t dtype, ionp_array_
 }
 
 static ionp_status reduce_dtype(int32_t op, int in_dt, int32_t dtype, int *out_dt) {
-    if (op != IONP_ADD && op != IONP_MAXIMUM && op != IONP_MINIMUM)
-        return set_error(IONP_ERR_VALUE, "reduce is only provided for add, maximum and minimum");
+    if (op != IONP_ADD && op != IONP_MULTIPLY && op != IONP_MAXIMUM && op != IONP_MINIMUM)
+        return set_error(IONP_ERR_VALUE, "reduce is only provided for add, multiply, maximum and minimum");
     int dt = dtype < 0 ? reduce_default_dtype(op, in_dt) : dtype;
     if (dt < 0 || dt >= IONP_NTYPES) return set_error(IONP_ERR_TYPE, "invalid dtype code");
     // integer maximum/minimum cast every element to the loop dtype first;
     // the float x86 loop model covers the input's own dtype only
-    if (op != IONP_ADD && dt != in_dt && (is_float_dt(dt) || is_float_dt(in_dt)))
+    if ((op == IONP_MAXIMUM || op == IONP_MINIMUM) && dt != in_dt && (is_float_dt(dt) || is_float_dt(in_dt)))
         return set_error(IONP_ERR_UNSUPPORTED, "float maximum/minimum.reduce with a dtype other than the input's");
     // same_kind casting of the input to the loop dtype (a float can not feed an integer loop)
     if (is_float_dt(in_dt) && !is_float_dt(dt))

@@ -91,7 +91,7 @@ ionp_status binary_types(int op, int a, int b, int *loop, int *out) {
 // reduce_compute_dtype (ufunc.rs:2041-2066): add.reduce widens bool and
 // sub-64-bit integers to the platform long (signedness kept).
 int reduce_default_dtype(int op, int in) {
-    if (op == IONP_ADD) {
+    if (op == IONP_ADD || op == IONP_MULTIPLY) {
         if (in == IONP_BOOL || in == IONP_INT8 || in == IONP_INT16 || in == IONP_INT32) return IONP_INT64;
         if (in == IONP_UINT8 || in == IONP_UINT16 || in == IONP_UINT32) return IONP_UINT64;
     }

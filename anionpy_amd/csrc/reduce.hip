@@ -493,6 +493,108 @@ template <typename T, typename IT> __global__ __launch_bounds__(256) void pw_nar
 template <typename T> static void launch_narrow(const RedArgs &A, int bpc, hipStream_t s);
 
 // ---------------------------------------------------------------------------
+// multiply on floats (sequential_group_ordered, ufunc.rs:6583): no pairwise
+// tree; per output one running product over the group in iterator order
+// (outer reduced index, then the W block), from initial= (default 1).  Plain
+// IEEE multiplies agree with fmul until a NaN appears, and a NaN stays, so
+// an output that ends NaN is recomputed with fmul's x86 NaN rules.
+// ---------------------------------------------------------------------------
+// group element q -> input offset: q = w + wsize * r
+__device__ inline int64_t group_offset(const RedArgs &A, int64_t q) {
+    if (A.nr == 0) return w_offset(A, q);
+    const int64_t r = q / A.wsize, w = q - r * A.wsize;
+    return w_offset(A, w) + r_offset(A, r);
+}
+
+template <typename T> __device__ inline T pseed(const RedArgs &A) {
+    if (!A.has_init) return T(1);
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, A.init_bits);
+    else return __builtin_bit_cast(T, (uint32_t)A.init_bits);
+}
+// group element q (iterator order) -> input offset from the output's base
+__device__ inline int64_t prod_offset(const RedArgs &A, int64_t q) {
+    if (A.nw == 0) return A.nr == 1 ? q * A.r_st[0] : r_offset(A, q);
+    if (A.nr == 0 && A.nw == 1) return q * A.w_st[0];
+    return group_offset(A, q);
+}
+template <typename T, typename IT> __device__ inline T prod_redo(const RedArgs &A, int64_t ib) {
+    constexpr int LT = FDT<T>::v;
+    T acc = pseed<T>(A);
+    for (int64_t q = 0; q < A.group; q++) acc = fmul(acc, ld_t<LT, IT>(A.in, ib + prod_offset(A, q)));
+    return acc;
+}
+// one thread per output, U loads in flight ahead of the multiply chain
+template <typename T, typename IT> __global__ __launch_bounds__(256) void prod_cols(const RedArgs A) {
+    constexpr int LT = FDT<T>::v;
+    constexpr int U = 16;
+    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
+        int64_t ib, ob;
+        out_offsets(A, (uint32_t)o, &ib, &ob);
+        const int64_t n = A.group;
+        T acc = pseed<T>(A);
+        int64_t q = 0;
+        for (; q + U <= n; q += U) {
+            T v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) v[u] = ld_t<LT, IT>(A.in, ib + prod_offset(A, q + u));
+#pragma unroll
+            for (int u = 0; u < U; u++) acc = acc * v[u];
+        }
+        for (; q < n; q++) acc = acc * ld_t<LT, IT>(A.in, ib + prod_offset(A, q));
+        if (acc != acc) acc = prod_redo<T, IT>(A, ib);
+        ((T *)A.out)[ob] = acc;
+    }
+}
+// few long groups: a wavefront per output loads blocks of 64 * B group
+// elements (the next block in flight), and the chain, the same in every
+// lane, reads them lane by lane in iterator order
+template <typename T, typename IT> __global__ __launch_bounds__(256) void prod_wave(const RedArgs A) {
+    constexpr int LT = FDT<T>::v;
+    constexpr int B = 4, BLK = 64 * B;
+    const int lane = threadIdx.x & 63;
+    const int64_t w0 = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+    for (int64_t o = w0; o < A.nout; o += (int64_t)gridDim.x * (blockDim.x >> 6)) {
+        int64_t ib, ob;
+        out_offsets(A, (uint32_t)o, &ib, &ob);
+        const int64_t n = A.group;
+        T cur[B], nxt[B];
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            const int64_t q = k * 64 + lane;
+            cur[k] = q < n ? ld_t<LT, IT>(A.in, ib + prod_offset(A, q)) : T(1);
+            nxt[k] = T(1);
+        }
+        T acc = pseed<T>(A);
+        for (int64_t qb = 0; qb < n; qb += BLK) {
+            const int64_t nb = qb + BLK;
+            if (nb < n) {
+#pragma unroll
+                for (int k = 0; k < B; k++) {
+                    const int64_t q = nb + k * 64 + lane;
+                    nxt[k] = q < n ? ld_t<LT, IT>(A.in, ib + prod_offset(A, q)) : T(1);
+                }
+            }
+            if (nb <= n) {
+#pragma unroll
+                for (int k = 0; k < B; k++)
+#pragma unroll
+                    for (int j = 0; j < 64; j++) acc = acc * readlane_t(cur[k], j);
+            } else {
+                const int cnt = (int)(n - qb);
+                for (int e = 0; e < cnt; e++) {
+                    const int k = e >> 6;
+                    const T v = k == 0 ? cur[0] : k == 1 ? cur[1] : k == 2 ? cur[2] : cur[3];
+                    acc = acc * readlane_t(v, e & 63);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < B; k++) cur[k] = nxt[k];
+        }
+        if (lane == 0) ((T *)A.out)[ob] = acc != acc ? prod_redo<T, IT>(A, ib) : acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // maximum / minimum on floats
 // ---------------------------------------------------------------------------
 // Effect of a run of windows on the running value p (an associative,
@@ -808,23 +910,17 @@ template <typename T, bool MAX> __global__ __launch_bounds__(256) void ext_narro
 }
 
 // ---------------------------------------------------------------------------
-// order-free reductions: integer / bool add, maximum, minimum
+// order-free reductions: integer / bool add, multiply, maximum, minimum
 // ---------------------------------------------------------------------------
 template <typename AT, int OP> __device__ inline AT assoc(AT a, AT b) {
     if constexpr (OP == IONP_ADD) return (AT)((uint64_t)a + (uint64_t)b);  // two's-complement wrap
+    else if constexpr (OP == IONP_MULTIPLY) return (AT)((uint64_t)a * (uint64_t)b);
     else if constexpr (OP == IONP_MAXIMUM) return a > b ? a : b;
     else return a < b ? a : b;
 }
 // a group's result combined with initial= (order-free: once, at the end)
 template <typename AT, int OP> __device__ inline AT with_init(const RedArgs &A, AT acc) {
     return A.has_init ? assoc<AT, OP>(acc, (AT)A.init_bits) : acc;
-}
-
-// group element q -> input offset: q = w + wsize * r
-__device__ inline int64_t group_offset(const RedArgs &A, int64_t q) {
-    if (A.nr == 0) return w_offset(A, q);
-    const int64_t r = q / A.wsize, w = q - r * A.wsize;
-    return w_offset(A, w) + r_offset(A, r);
 }
 
 template <typename AT, int LT> __device__ inline AT load_acc(int dt, const char *base, int64_t off) {
@@ -841,7 +937,7 @@ __global__ __launch_bounds__(256) void int_reduce(const RedArgs A, int64_t seg, 
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
         const int64_t q0 = sg * seg, q1 = min(A.group, q0 + seg);
-        AT acc = OP == IONP_ADD ? AT(0) : (AT)load_acc<AT, LT>(A.in_dt, A.in, ib + group_offset(A, q0));
+        AT acc = OP == IONP_ADD ? AT(0) : OP == IONP_MULTIPLY ? AT(1) : (AT)load_acc<AT, LT>(A.in_dt, A.in, ib + group_offset(A, q0));
         for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x)
             acc = assoc<AT, OP>(acc, (AT)load_acc<ST, LT>(A.in_dt, A.in, ib + group_offset(A, q)));
         red[threadIdx.x] = acc;
@@ -863,7 +959,7 @@ template <typename AT, int LT, int OP> __global__ __launch_bounds__(256) void in
     __shared__ AT red[256];
     for (int64_t o = blockIdx.x; o < A.nout; o += gridDim.x) {
         const AT *p = (const AT *)A.scratch + o * nseg;
-        AT acc = OP == IONP_ADD ? AT(0) : p[0];
+        AT acc = OP == IONP_ADD ? AT(0) : OP == IONP_MULTIPLY ? AT(1) : p[0];
         for (int64_t i = threadIdx.x; i < nseg; i += blockDim.x) acc = assoc<AT, OP>(acc, p[i]);
         red[threadIdx.x] = acc;
         __syncthreads();
@@ -893,7 +989,7 @@ __global__ __launch_bounds__(256) void int_reduce_contig(const RedArgs A, int64_
         out_offsets(A, (uint32_t)o, &ib, &ob);
         const IT *x = (const IT *)A.in + ib;
         const int64_t q0 = sg * seg, q1 = min(A.group, q0 + seg);
-        AT acc = OP == IONP_ADD ? AT(0) : (AT)x[q0];
+        AT acc = OP == IONP_ADD ? AT(0) : OP == IONP_MULTIPLY ? AT(1) : (AT)x[q0];
         // aligned vector body
         int64_t v0 = q0, v1 = q1;
         while (v0 < q1 && ((uintptr_t)(x + v0) % 16)) v0++;
@@ -925,14 +1021,15 @@ template <typename AT, int LT, int OP> __global__ __launch_bounds__(256) void in
     for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
-        AT acc = OP == IONP_ADD ? AT(0) : (AT)load_acc<ST, LT>(A.in_dt, A.in, ib);
+        AT acc = OP == IONP_ADD ? AT(0) : OP == IONP_MULTIPLY ? AT(1) : (AT)load_acc<ST, LT>(A.in_dt, A.in, ib);
         for (int64_t q = 0; q < A.group; q++)
             acc = assoc<AT, OP>(acc, (AT)load_acc<ST, LT>(A.in_dt, A.in, ib + group_offset(A, q)));
         ((ST *)A.out)[ob] = int_result<LT>(with_init<AT, OP>(A, acc));
     }
 }
 
-// zero-size groups: initial= if given, else add's identity 0
+// zero-size groups: initial= if given (multiply's identity 1 is passed the
+// same way), else add's identity 0
 template <typename ST> __global__ void fill_zero(const RedArgs A) {
     for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
         int64_t ib, ob;
@@ -972,6 +1069,28 @@ template <typename T> static void launch_narrow(const RedArgs &A, int bpc, hipSt
     }
 }
 
+template <typename T, typename IT> static void launch_prod_t(const RedArgs &A, int bpc, hipStream_t s) {
+    if (A.nout <= 8192 && A.group >= 256)
+        hipLaunchKernelGGL((prod_wave<T, IT>), dim3(grid_cap((A.nout + 3) / 4, bpc)), dim3(256), 0, s, A);
+    else
+        hipLaunchKernelGGL((prod_cols<T, IT>), dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
+}
+template <typename T> static void launch_prod(const RedArgs &A, int bpc, hipStream_t s) {
+    switch (A.in_dt) {
+        case IONP_BOOL: launch_prod_t<T, bool>(A, bpc, s); break;
+        case IONP_INT8: launch_prod_t<T, int8_t>(A, bpc, s); break;
+        case IONP_UINT8: launch_prod_t<T, uint8_t>(A, bpc, s); break;
+        case IONP_INT16: launch_prod_t<T, int16_t>(A, bpc, s); break;
+        case IONP_UINT16: launch_prod_t<T, uint16_t>(A, bpc, s); break;
+        case IONP_INT32: launch_prod_t<T, int32_t>(A, bpc, s); break;
+        case IONP_UINT32: launch_prod_t<T, uint32_t>(A, bpc, s); break;
+        case IONP_INT64: launch_prod_t<T, int64_t>(A, bpc, s); break;
+        case IONP_UINT64: launch_prod_t<T, uint64_t>(A, bpc, s); break;
+        case IONP_FLOAT32: launch_prod_t<T, float>(A, bpc, s); break;
+        default: launch_prod_t<T, double>(A, bpc, s); break;
+    }
+}
+
 static ionp_status launched() {
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
@@ -993,10 +1112,10 @@ static ionp_status run_int(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
     using ST = typename dt_type<LT>::T;
     const dim3 grid(grid_cap(A.nout * nseg, bpc)), blk(256);
     // raw input values only where that equals casting each to the output
-    // dtype first: integer add (wraps modulo the output width either way) or
-    // no cast; bool outputs (logical or) take int_reduce
+    // dtype first: integer add and multiply (wrap modulo the output width
+    // either way) or no cast; bool outputs (logical or / and) take int_reduce
     if (A.nw == 1 && A.w_st[0] == 1 && A.nr == 0 && A.in_dt != IONP_BOOL && A.out_dt != IONP_BOOL &&
-        A.group < (1ll << 40) && (OP == IONP_ADD || A.in_dt == A.out_dt)) {
+        A.group < (1ll << 40) && (OP == IONP_ADD || OP == IONP_MULTIPLY || A.in_dt == A.out_dt)) {
         switch (A.in_dt) {  // one contiguous run per output: vector loads
             case IONP_INT8: hipLaunchKernelGGL((int_reduce_contig<AT, int8_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
             case IONP_UINT8: hipLaunchKernelGGL((int_reduce_contig<AT, uint8_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
@@ -1179,9 +1298,13 @@ ionp_status run_reduce(const ReducePlan &p, int64_t bufsize, const void *initial
     }
     const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
     if (p.group == 0) {
-        if (p.op != IONP_ADD && !initial)
+        if (p.op != IONP_ADD && p.op != IONP_MULTIPLY && !initial)
             return set_error(IONP_ERR_VALUE, std::string("zero-size array to reduction operation ") +
                                                                op_name(p.op) + " which has no identity");
+        if (p.op == IONP_MULTIPLY && !initial) {  // multiply's identity 1 in the output dtype
+            A.has_init = 1;
+            A.init_bits = p.out_dt == IONP_FLOAT64 ? 0x3ff0000000000000ull : p.out_dt == IONP_FLOAT32 ? 0x3f800000ull : 1ull;
+        }
         const int g = grid_cap((p.nout + 255) / 256, bpc);
         switch (itemsize(p.out_dt)) {
             case 1: hipLaunchKernelGGL(fill_zero<uint8_t>, dim3(g), dim3(256), 0, s, A); break;
@@ -1196,12 +1319,18 @@ ionp_status run_reduce(const ReducePlan &p, int64_t bufsize, const void *initial
             if (bufsize > kPwMaxWin) return set_error(IONP_ERR_UNSUPPORTED, "bufsize above 8192");
             return p.out_dt == IONP_FLOAT64 ? run_float_add<double>(A, cfg, s) : run_float_add<float>(A, cfg, s);
         }
+        if (p.op == IONP_MULTIPLY) {
+            if (p.out_dt == IONP_FLOAT64) launch_prod<double>(A, bpc, s);
+            else launch_prod<float>(A, bpc, s);
+            return launched();
+        }
         if (p.op == IONP_MAXIMUM)
             return p.out_dt == IONP_FLOAT64 ? run_float_ext<double, true>(A, cfg, s) : run_float_ext<float, true>(A, cfg, s);
         return p.out_dt == IONP_FLOAT64 ? run_float_ext<double, false>(A, cfg, s) : run_float_ext<float, false>(A, cfg, s);
     }
     switch (p.op) {
         case IONP_ADD: return run_int_dispatch<IONP_ADD>(A, cfg, s);
+        case IONP_MULTIPLY: return run_int_dispatch<IONP_MULTIPLY>(A, cfg, s);
         case IONP_MAXIMUM: return run_int_dispatch<IONP_MAXIMUM>(A, cfg, s);
         default: return run_int_dispatch<IONP_MINIMUM>(A, cfg, s);
     }

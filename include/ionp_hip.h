@@ -168,12 +168,14 @@ ionp_status ionp_cast(const ionp_array_desc *in, const ionp_array_desc *out, con
                       void *stream);
 
 /* Replaces ionp_core::ufunc::reduce_axis (ufunc.rs:7634) for op in
- * {ADD, MAXIMUM, MINIMUM}: `out` as given by ionp_plan_reduce (its dtype
- * selects the accumulation dtype).  `initial`: NULL, or one element of the
- * output dtype in host memory (reduce_axis's `initial`): the fold starts
- * from it (add: acc = initial; acc += pairwise(window) per window), maximum
- * and minimum feed every element to the loop instead of seeding with the
- * first, and an empty reduction returns it. */
+ * {ADD, MULTIPLY, MAXIMUM, MINIMUM}: `out` as given by ionp_plan_reduce (its
+ * dtype selects the accumulation dtype).  Float multiply is one running
+ * product per output in iterator order (no pairwise tree).  `initial`:
+ * NULL, or one element of the output dtype in host memory (reduce_axis's
+ * `initial`): the fold starts from it (add: acc = initial; acc +=
+ * pairwise(window) per window; multiply: acc = initial), maximum and
+ * minimum feed every element to the loop instead of seeding with the
+ * first, and an empty reduction returns it (else add's 0, multiply's 1). */
 ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes,
                         int32_t keepdims, const void *initial, const ionp_array_desc *out, const ionp_config *cfg,
                         void *stream);
@@ -183,10 +185,11 @@ ionp_status ionp_argext(int32_t ismax, const ionp_array_desc *in, int32_t axis, 
                         const ionp_array_desc *out, const ionp_config *cfg, void *stream);
 
 /* Replaces ionp_core::ufunc::accumulate_axis (ufunc.rs:7967) for op in
- * {ADD, MAXIMUM, MINIMUM}: along `axis` (negative counts from the end),
- * out[0] = x[0], out[i] = op(out[i-1], x[i]) in the output dtype (the
- * reduce dtype rule: small integers and bool add in int64 / uint64; or an
- * explicit dtype).  Float add is the sequential running sum.  The output
+ * {ADD, MULTIPLY, MAXIMUM, MINIMUM}: along `axis` (negative counts from the
+ * end), out[0] = x[0], out[i] = op(out[i-1], x[i]) in the output dtype (the
+ * reduce dtype rule: small integers and bool add and multiply in int64 /
+ * uint64; or an explicit dtype).  Float add and multiply are the sequential
+ * running sum / product.  The output
  * template has the input's shape in its K order. */
 ionp_status ionp_plan_accumulate(int32_t op, const ionp_array_desc *in, int32_t axis, int32_t dtype,
                                  ionp_array_desc *out_template);

@@ -127,7 +127,7 @@ int ora_binary_types(int op, int a, int b, int *loop, int *out)
  * bool accumulates in the platform long; maximum/minimum keep the dtype. */
 int ora_reduce_dtype(int op, int in)
 {
-    if (op == OP_ADD) {
+    if (op == OP_ADD || op == OP_MULTIPLY) {
         if (in == DT_BOOL || in == DT_I8 || in == DT_I16 || in == DT_I32) return DT_I64;
         if (in == DT_U8 || in == DT_U16 || in == DT_U32) return DT_U64;
     }
@@ -781,7 +781,7 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
     int isz = ITEMSIZE[in->dtype], cdt;
     int64_t *woff, okidx[ORA_MAXDIM], nout, o, gsize = 1;
     char *buf = NULL;
-    if (op != OP_ADD && op != OP_MAXIMUM && op != OP_MINIMUM) return ORA_ENOLOOP;
+    if (op != OP_ADD && op != OP_MULTIPLY && op != OP_MAXIMUM && op != OP_MINIMUM) return ORA_ENOLOOP;
     /* the loop runs in the output dtype (the default of ora_reduce_dtype or
        an explicit dtype=); elements are cast to it as they are read */
     cdt = out->dtype;
@@ -794,7 +794,7 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
     }
     nout = 1;
     for (k = 0; k < nk; k++) nout *= in->shape[kept[k]];
-    if (gsize == 0 && op != OP_ADD && nout > 0 && !initial) return ORA_EVALUE; /* no identity */
+    if (gsize == 0 && op != OP_ADD && op != OP_MULTIPLY && nout > 0 && !initial) return ORA_EVALUE; /* no identity */
     woff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(p.wsize > 0 ? p.wsize : 1));
     buf = (char *)malloc((size_t)(bufsize > 0 ? bufsize : 1) * 8);
     if (p.nw) w_offsets(in, &p, woff);
@@ -813,12 +813,22 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
         memset(oidx, 0, sizeof oidx);
         if (gsize == 0) {
             if (initial) memcpy(po, initial, ITEMSIZE[cdt]);
-            else memset(po, 0, ITEMSIZE[cdt]);
+            else if (op == OP_MULTIPLY) { /* multiply's identity 1 */
+                char one[8];
+                const uint8_t b1 = 1;
+                cast_elem(DT_BOOL, (const char *)&b1, cdt, one);
+                memcpy(po, one, ITEMSIZE[cdt]);
+            } else memset(po, 0, ITEMSIZE[cdt]);
         } else if (!is_float(cdt)) {
             /* sequential_group (ufunc.rs:6546) on exact dtypes: order-free */
             uint64_t uacc = 0;
             int64_t sacc = 0, e;
             int first = 1, uns = cdt == DT_BOOL || is_unsigned_int(cdt);
+            if (op == OP_MULTIPLY) { /* identity 1; wrapping products are order-free */
+                uacc = 1;
+                sacc = 1;
+                first = 0;
+            }
             if (initial) { /* initial= seeds the fold (reduce_axis, ufunc.rs:7634 `initial`) */
                 uacc = (uint64_t)load_i(cdt, (const char *)initial);
                 sacc = load_i(cdt, (const char *)initial);
@@ -834,12 +844,14 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
                     if (uns) {
                         uint64_t v = (uint64_t)load_i(cdt, tmp);
                         if (op == OP_ADD) uacc += v;
+                        else if (op == OP_MULTIPLY) uacc *= v;
                         else if (first) uacc = v;
                         else if (op == OP_MAXIMUM) uacc = v > uacc ? v : uacc;
                         else uacc = v < uacc ? v : uacc;
                     } else {
                         int64_t v = load_i(cdt, tmp);
                         if (op == OP_ADD) sacc = (int64_t)((uint64_t)sacc + (uint64_t)v);
+                        else if (op == OP_MULTIPLY) sacc = (int64_t)((uint64_t)sacc * (uint64_t)v);
                         else if (first) sacc = v;
                         else if (op == OP_MAXIMUM) sacc = v > sacc ? v : sacc;
                         else sacc = v < sacc ? v : sacc;
@@ -862,6 +874,29 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
                 default: *(uint64_t *)po = acc; break;
                 }
             }
+        } else if (op == OP_MULTIPLY) {
+            /* sequential_group_ordered (ufunc.rs:6583): no pairwise tree, one
+               running product in iterator order (outer reduced index, then
+               the W block), from initial (default 1) */
+            double acc64 = initial && cdt == DT_F64 ? *(const double *)initial : 1.0;
+            float acc32 = initial && cdt == DT_F32 ? *(const float *)initial : 1.0f;
+            for (r = 0; r < noutr; r++) {
+                int64_t ob = base, e;
+                const char *pb;
+                for (k = 0; k < p.nouter; k++) ob += oidx[k] * in->strides[p.outer_axes[k]];
+                pb = (const char *)in->data + ob * isz;
+                for (e = 0; e < p.wsize; e++) {
+                    const char *pe = pb + (p.nw ? woff[e] : 0);
+                    if (cdt == DT_F64) acc64 = f64_arith(OP_MULTIPLY, acc64, *(const double *)pe);
+                    else acc32 = f32_arith(OP_MULTIPLY, acc32, *(const float *)pe);
+                }
+                for (k = 0; k < p.nouter; k++) {
+                    if (++oidx[k] < in->shape[p.outer_axes[k]]) break;
+                    oidx[k] = 0;
+                }
+            }
+            if (cdt == DT_F64) *(double *)po = acc64;
+            else *(float *)po = acc32;
         } else if (op == OP_ADD) {
             /* acc = initial (default 0); acc = acc + pw(window) per window */
             double acc64 = initial && cdt == DT_F64 ? *(const double *)initial : 0.0;
@@ -1006,7 +1041,7 @@ int ora_accumulate(int op, const ora_desc *in, int axis, const ora_desc *out)
 {
     int n = in->ndim, k, cdt = out->dtype, osz;
     int64_t idx[ORA_MAXDIM], len, nfib = 1, f, i;
-    if (op != OP_ADD && op != OP_MAXIMUM && op != OP_MINIMUM) return ORA_ENOLOOP;
+    if (op != OP_ADD && op != OP_MULTIPLY && op != OP_MAXIMUM && op != OP_MINIMUM) return ORA_ENOLOOP;
     if (axis < 0 || axis >= (n > 0 ? n : 1)) return ORA_EAXIS;
     if (is_float(in->dtype) && !is_float(cdt)) return ORA_ETYPE;
     if (out->ndim != n) return ORA_EVALUE;
@@ -1032,21 +1067,23 @@ int ora_accumulate(int op, const ora_desc *in, int axis, const ora_desc *out)
                 memcpy(acc, v, osz);
             } else if (cdt == DT_F64) {
                 double a = *(double *)acc, b = *(double *)v;
-                a = op == OP_ADD ? f64_arith(OP_ADD, a, b) : op == OP_MAXIMUM ? f64_max(a, b) : f64_min(a, b);
+                a = op == OP_ADD || op == OP_MULTIPLY ? f64_arith(op, a, b) : op == OP_MAXIMUM ? f64_max(a, b) : f64_min(a, b);
                 memcpy(acc, &a, 8);
             } else if (cdt == DT_F32) {
                 float a = *(float *)acc, b = *(float *)v;
-                a = op == OP_ADD ? f32_arith(OP_ADD, a, b) : op == OP_MAXIMUM ? f32_max(a, b) : f32_min(a, b);
+                a = op == OP_ADD || op == OP_MULTIPLY ? f32_arith(op, a, b) : op == OP_MAXIMUM ? f32_max(a, b) : f32_min(a, b);
                 memcpy(acc, &a, 4);
             } else if (cdt == DT_BOOL) {
                 uint8_t a = *(uint8_t *)acc != 0, b = *(uint8_t *)v != 0;
-                a = op == OP_MINIMUM ? (a & b) : (a | b);
+                a = op == OP_MINIMUM || op == OP_MULTIPLY ? (a & b) : (a | b);
                 memcpy(acc, &a, 1);
             } else {
                 int64_t sa = load_i(cdt, acc), sb = load_i(cdt, v);
                 uint64_t r;
                 if (op == OP_ADD) {
                     r = (uint64_t)sa + (uint64_t)sb;
+                } else if (op == OP_MULTIPLY) {
+                    r = (uint64_t)sa * (uint64_t)sb;
                 } else {
                     int gt = is_unsigned_int(cdt) ? (uint64_t)sa > (uint64_t)sb : sa > sb;
                     r = (uint64_t)((op == OP_MAXIMUM) == gt ? sa : sb);

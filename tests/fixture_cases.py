@@ -34,8 +34,13 @@ def big():
 
 def host_input(spec, npz):
     """The operand as a host numpy view (strides preserved)."""
-    if "gen" in spec:
-        x = np.random.default_rng(spec["seed"]).uniform(-1, 1, spec["n"]).astype(spec["dtype"])
+    if "gen" in spec:  # shift + scale * uniform[-1, 1) in float64, cast, reshaped, stepped
+        x = np.random.default_rng(spec["seed"]).uniform(-1, 1, spec["n"])
+        if "scale" in spec:
+            x = spec.get("shift", 0.0) + spec["scale"] * x
+        x = x.astype(spec["dtype"])
+        if "shape" in spec:
+            return x.reshape(spec["shape"])
         return x[:: spec["step"]]
     buf = npz[spec["buf"]]
     isz = buf.itemsize

@@ -46,17 +46,28 @@ DTYPES = [np.bool_, np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32,
 
 
 class Gen:
-    """A large 1-D input regenerated from a seed instead of stored:
-    np.random.default_rng(seed).uniform(-1, 1, n).astype(dtype)[::step]."""
+    """A large input regenerated from a seed instead of stored:
+    (shift + scale * np.random.default_rng(seed).uniform(-1, 1, n)).astype(dtype),
+    then [::step] (1-D) or reshaped to `shape`."""
 
-    def __init__(self, seed, n, dtype, step=1):
+    def __init__(self, seed, n, dtype, step=1, shift=None, scale=None, shape=None):
         self.seed, self.n, self.dtype, self.step = seed, n, np.dtype(dtype), step
+        self.shift, self.scale, self.shape = shift, scale, shape
 
     def array(self):
-        return np.random.default_rng(self.seed).uniform(-1, 1, self.n).astype(self.dtype)[:: self.step]
+        x = np.random.default_rng(self.seed).uniform(-1, 1, self.n)
+        if self.scale is not None:
+            x = (self.shift or 0.0) + self.scale * x
+        x = x.astype(self.dtype)
+        return x.reshape(self.shape) if self.shape is not None else x[:: self.step]
 
     def spec(self):
-        return {"gen": "uniform", "seed": self.seed, "n": self.n, "dtype": self.dtype.name, "step": self.step}
+        d = {"gen": "uniform", "seed": self.seed, "n": self.n, "dtype": self.dtype.name, "step": self.step}
+        if self.scale is not None:
+            d.update(shift=self.shift or 0.0, scale=self.scale)
+        if self.shape is not None:
+            d["shape"] = list(self.shape)
+        return d
 
 
 class Store:
@@ -481,17 +492,18 @@ def accumulate_cases(S):
                   {"axis": ax})
             S.add(f"acc/minimum/{nm}/9000x3/{ax}", "accumulate", "minimum", [A],
                   lambda: np.minimum.accumulate(A, axis=ax), {"axis": ax})
-        B = r5.uniform(-1, 1, (300, 700)).astype(dt)
+        gB = Gen(8000 + len(nm), 300 * 700, dt, shape=(300, 700))
+        B = gB.array()
         for ax in (0, 1):
-            S.add(f"acc/add/{nm}/300x700/{ax}", "accumulate", "add", [B], lambda: np.add.accumulate(B, axis=ax),
+            S.add(f"acc/add/{nm}/300x700/{ax}", "accumulate", "add", [gB], lambda: np.add.accumulate(B, axis=ax),
                   {"axis": ax})
-        T = r5.uniform(-1, 1, (2000, 150)).astype(dt)
+        T = r5.uniform(-1, 1, (600, 150)).astype(dt)
         T[r5.random(T.shape) < 2e-4] = np.nan
         for op in ("add", "maximum"):
-            S.add(f"acc/{op}/{nm}/2000x150/1", "accumulate", op, [T], lambda: getattr(np, op).accumulate(T, axis=1),
+            S.add(f"acc/{op}/{nm}/600x150/1", "accumulate", op, [T], lambda: getattr(np, op).accumulate(T, axis=1),
                   {"axis": 1})
-    W = r5.integers(-2 ** 62, 2 ** 62, (1500, 130))
-    S.add("acc/add/int64/1500x130/1", "accumulate", "add", [W], lambda: np.add.accumulate(W, axis=1), {"axis": 1})
+    W = r5.integers(-2 ** 62, 2 ** 62, (400, 130))
+    S.add("acc/add/int64/400x130/1", "accumulate", "add", [W], lambda: np.add.accumulate(W, axis=1), {"axis": 1})
     I = r5.integers(-2 ** 62, 2 ** 62, (2000, 5))
     S.add("acc/add/int64/wrap", "accumulate", "add", [I], lambda: np.add.accumulate(I, axis=0), {"axis": 0})
     J = r5.integers(-1000, 1000, (40, 50), dtype=np.int32)
@@ -507,6 +519,61 @@ def accumulate_cases(S):
     S.add("cumsum/float64/None", "accumulate", "cumsum", [K.T], lambda: np.cumsum(K.T), {"axis": None})
     S.add("cumsum/float64/1", "accumulate", "cumsum", [K], lambda: np.cumsum(K, axis=1), {"axis": 1})
     S.add("cumsum/int32/None", "accumulate", "cumsum", [J], lambda: np.cumsum(J), {"axis": None})
+
+
+def prod_cases(S):
+    """multiply.reduce / prod (sequential_group_ordered, ufunc.rs:6583: one
+    running product in iterator order, identity 1) and multiply.accumulate /
+    cumprod.  Values near 1 keep products finite.  Own generator (seed 81)."""
+    r6 = np.random.default_rng(81)
+
+    def near1(shape, dt):
+        # NaN and -0.0, but no inf: inf * 0 would make a second NaN payload,
+        # and which of two different NaNs NumPy keeps depends on its SIMD
+        # loop variant (an unpinned detail)
+        x = (1 + r6.standard_normal(shape) * 0.05).astype(dt)
+        m = r6.random(shape)
+        x[m < 0.01] = np.nan
+        x[(m >= 0.01) & (m < 0.02)] = -0.0
+        return x
+
+    for dt in (np.float64, np.float32):
+        nm = np.dtype(dt).name
+        for shp in ((300,), (37, 41), (5, 20, 33)):
+            x = near1(shp, dt)
+            views = [("C", x), ("rev", x[::-1])] + ([("T", x.T), ("step", x[:, ::2])] if len(shp) > 1 else [])
+            for vn, v in views:
+                axes = [None] + list(range(v.ndim)) + ([(0, 2)] if v.ndim == 3 else [])
+                for ax in axes:
+                    S.add(f"prod/{nm}/{shp}/{vn}/{ax}", "reduce", "multiply", [v], lambda: np.multiply.reduce(v, axis=ax),
+                          {"axis": list(ax) if isinstance(ax, tuple) else ax})
+                S.add(f"cumprod/{nm}/{shp}/{vn}", "accumulate", "multiply", [v],
+                      lambda: np.multiply.accumulate(v, axis=v.ndim - 1), {"axis": v.ndim - 1})
+        gb = Gen(8101, 20000, dt, shift=1.0, scale=1e-4)
+        big = gb.array()
+        S.add(f"prod/{nm}/long1d", "reduce", "multiply", [gb], lambda: np.multiply.reduce(big), {"axis": 0})
+        for k, (shp, ax) in enumerate((((9000, 3), 0), ((300, 700), 1), ((2000, 150), 1), ((300, 700), 0))):
+            gA = Gen(8110 + k, int(np.prod(shp)), dt, shift=1.0, scale=1e-3, shape=shp)
+            A = gA.array()
+            S.add(f"prod/{nm}/{shp}/{ax}", "reduce", "multiply", [gA], lambda: np.multiply.reduce(A, axis=ax),
+                  {"axis": ax})
+            S.add(f"cumprod/{nm}/{shp}/{ax}", "accumulate", "multiply", [gA],
+                  lambda: np.multiply.accumulate(A, axis=ax), {"axis": ax})
+        o = np.full(50, 1e30, dtype=dt)
+        S.add(f"prod/{nm}/overflow", "reduce", "multiply", [o], lambda: np.multiply.reduce(o), {"axis": 0})
+        S.add(f"prod/{nm}/initial", "reduce", "multiply", [x], lambda: np.multiply.reduce(x, axis=1, initial=0.5),
+              {"axis": 1, "initial": 0.5})
+    for dt in (np.int64, np.int32, np.int8, np.uint8, np.bool_):
+        nm = np.dtype(dt).name
+        x = specials(dt, 30 * 40, r6).reshape(30, 40)
+        for ax in (None, 0, 1):
+            S.add(f"prod/{nm}/{ax}", "reduce", "multiply", [x], lambda: np.multiply.reduce(x, axis=ax), {"axis": ax})
+        S.add(f"prod/{nm}/T", "reduce", "multiply", [x.T[::2]], lambda: np.multiply.reduce(x.T[::2], axis=1), {"axis": 1})
+        S.add(f"cumprod/{nm}/1", "accumulate", "multiply", [x], lambda: np.multiply.accumulate(x, axis=1), {"axis": 1})
+    E = np.zeros((0, 3))
+    S.add("prod/empty", "reduce", "multiply", [E], lambda: np.multiply.reduce(E, axis=0), {"axis": 0})
+    K = (1 + r6.standard_normal((20, 30)) * 0.01)
+    S.add("cumprod/float64/None", "accumulate", "cumprod", [K.T], lambda: np.cumprod(K.T), {"axis": None})
 
 
 def argext_cases(S, rng):
@@ -584,6 +651,7 @@ def main():
     reduce_cases(S, rng)
     argext_cases(S, rng)
     accumulate_cases(S)
+    prod_cases(S)
     np.savez_compressed(os.path.join(HERE, "fixtures.npz"), **S.arrays)
     with open(os.path.join(HERE, "cases.json"), "w") as f:
         json.dump(S.cases, f)

@@ -37,8 +37,8 @@ def run_device(ap, case, npz, cache):
         return getattr(ap, op).reduce(ins[0], axis=args_axis(args), dtype=args.get("dtype"),
                                       keepdims=bool(args.get("keepdims", False)), initial=args.get("initial"))
     if k == "accumulate":
-        if op == "cumsum":
-            return ap.cumsum(ins[0], axis=args.get("axis"))
+        if op in ("cumsum", "cumprod"):
+            return getattr(ap, op)(ins[0], axis=args.get("axis"))
         return getattr(ap, op).accumulate(ins[0], axis=args["axis"], dtype=args.get("dtype"))
     return getattr(ap, op)(ins[0], axis=args.get("axis"), keepdims=bool(args.get("keepdims", False)))
 

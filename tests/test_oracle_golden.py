@@ -28,11 +28,11 @@ def run_oracle(case, npz):
             r = np.expand_dims(r, tuple(a % x.ndim for a in axes))
         return r
     if k == "accumulate":
-        if op == "cumsum":
+        if op in ("cumsum", "cumprod"):
             x, ax = ins[0], args.get("axis")
             if ax is None:
                 x, ax = np.ravel(x), 0  # C-order ravel (a host copy), then the scan
-            return O.accumulate("add", x, ax)
+            return O.accumulate("add" if op == "cumsum" else "multiply", x, ax)
         return O.accumulate(op, ins[0], args["axis"], dtype=args.get("dtype"))
     r = O.argext(op == "argmax", ins[0], args.get("axis"))
     if args.get("keepdims"):
