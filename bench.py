@@ -11,8 +11,8 @@ with NumPy.  `value` is the step's algorithmic HBM bytes (every operand read
 once, the output written once) divided by the step time, inputs resident in
 HBM.  One JSON line is printed; `roofline` covers the dominant kernel (the
 float64 multiply), timed with HIP events on the stream it runs on, and
-`cpu_baseline` times the scalar C oracle (single thread) on a row sample of
-the same four operations.  `extra` carries the reduction workloads (float64
+`cpu_baseline` times the scalar C oracle on the same four operations, on one
+core and (OpenMP) on every host core.  `extra` carries the reduction workloads (float64
 add.reduce of 2^28 elements, float32 (4096,65536) sums, maximum/argmax).
 """
 import argparse
@@ -291,6 +291,16 @@ def cpu_baseline(min_seconds=10.0):
             del o
         reps += 1
     dt = time.perf_counter() - t0
+    # the same loop on every host core (OpenMP, output rows split per thread)
+    mbytes, mreps, threads = 0, 0, 1
+    t2 = time.perf_counter()
+    while mreps == 0 or time.perf_counter() - t2 < min_seconds / 2:
+        for op, a, b in work:
+            o, threads = O.binary_omp(op, a, b)
+            mbytes += a.nbytes + b.nbytes + o.nbytes
+            del o
+        mreps += 1
+    dmt = time.perf_counter() - t2
     t1 = time.perf_counter()
     for op, a, b in work:
         getattr(np, op)(a, b)
@@ -298,6 +308,8 @@ def cpu_baseline(min_seconds=10.0):
     return {"value": round(nbytes / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
             "sample": f"C oracle (scalar restatement of the reference loop) over the full step workload, "
                       f"{reps} pass(es), {nbytes / 1e9:.1f} GB algorithmic in {dt:.1f} s; host CPU: {cpu_name()}",
+            "openmp": {"value": round(mbytes / dmt / 1e9, 3), "unit": "GB/s", "cores": threads,
+                       "sample": f"{mreps} pass(es) of the same workload, output rows split over {threads} threads"},
             "numpy_same_host_GBps": round(nbytes / reps / dnp / 1e9, 3)}
 
 

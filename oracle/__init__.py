@@ -156,6 +156,20 @@ def binary(op, a, b):
     return out
 
 
+def binary_omp(op, a, b):
+    """binary() on all host cores (OpenMP, rows split per thread): the CPU
+    baseline's multi-thread leg.  Returns (out, threads used)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    _, odt = binary_types(op, a.dtype, b.dtype)
+    shape = broadcast_shape(a.shape, b.shape)
+    out = _empty_strided(shape, korder_strides([a, b], shape), odt)
+    da, db, do = _desc(a), _desc(b), _desc(out)
+    nt = ctypes.c_int()
+    _check(lib().ora_binary_omp(OP[op], ctypes.byref(da), ctypes.byref(db), ctypes.byref(do), ctypes.byref(nt)))
+    return out, nt.value
+
+
 def where(c, x, y):
     c, x, y = np.asarray(c), np.asarray(x), np.asarray(y)
     odt = result_type(x.dtype, y.dtype)

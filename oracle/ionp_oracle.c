@@ -42,6 +42,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define ORA_MAXDIM 16
 
@@ -542,6 +545,52 @@ int ora_binary(int op, const ora_desc *a, const ora_desc *b, const ora_desc *out
         }
     }
     return ORA_OK;
+}
+
+/* ora_binary on every host core (the CPU baseline's multi-thread leg; the
+ * reference itself is single-threaded): the output's first axis is cut
+ * into one block of rows per thread and each block runs the same scalar
+ * loop on sub-descriptors.  Elementwise results do not depend on the
+ * split.  Returns the thread count used in *nthreads. */
+int ora_binary_omp(int op, const ora_desc *a, const ora_desc *b, const ora_desc *out, int *nthreads)
+{
+    int n = out->ndim, ldt, odt, rc = ORA_OK, nt = 1;
+    int64_t sa[ORA_MAXDIM], sb[ORA_MAXDIM], rows;
+    rc = ora_binary_types(op, a->dtype, b->dtype, &ldt, &odt);
+    if (rc) return rc;
+    if (odt != out->dtype) return ORA_ETYPE;
+    if (n == 0) {
+        *nthreads = 1;
+        return ora_binary(op, a, b, out);
+    }
+    bstrides(a, n, out->shape, sa);
+    bstrides(b, n, out->shape, sb);
+    rows = out->shape[0];
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    if (nt > rows) nt = rows > 0 ? (int)rows : 1;
+    *nthreads = nt;
+    {
+        int t, bad = 0;
+#pragma omp parallel for schedule(static) num_threads(nt) reduction(| : bad)
+        for (t = 0; t < nt; t++) {
+            const int64_t r0 = rows * t / nt, r1 = rows * (t + 1) / nt;
+            ora_desc da = *a, db = *b, dout = *out;
+            da.ndim = db.ndim = n;
+            memcpy(da.shape, out->shape, sizeof(int64_t) * n);
+            memcpy(db.shape, out->shape, sizeof(int64_t) * n);
+            memcpy(da.strides, sa, sizeof(int64_t) * n);
+            memcpy(db.strides, sb, sizeof(int64_t) * n);
+            da.shape[0] = db.shape[0] = dout.shape[0] = r1 - r0;
+            da.data = (char *)a->data + r0 * sa[0] * ITEMSIZE[a->dtype];
+            db.data = (char *)b->data + r0 * sb[0] * ITEMSIZE[b->dtype];
+            dout.data = (char *)out->data + r0 * out->strides[0] * ITEMSIZE[out->dtype];
+            if (r1 > r0) bad |= ora_binary(op, &da, &db, &dout);
+        }
+        if (bad) rc = ORA_EVALUE;
+    }
+    return rc;
 }
 
 /* where_select (sort.rs:1869-1914): out = cond ? x : y, three-way

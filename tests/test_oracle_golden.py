@@ -125,3 +125,19 @@ def test_max_nan_rules():
     y[::2] = np.arange(100)
     y[20] = neg
     assert O.reduce("maximum", y[::2]).view(np.uint64) == 0xFFF8000000000456
+
+
+def test_openmp_binary_equals_scalar_loop():
+    """The CPU baseline's multi-thread leg computes the same bytes and layout
+    as the scalar loop (rows split per thread; broadcast and strided views)."""
+    rng = np.random.default_rng(31)
+    X = rng.standard_normal((600, 500))
+    cases = [(X, rng.standard_normal(500)), (X.T[::2], rng.standard_normal((250, 1))),
+             (rng.integers(-2 ** 62, 2 ** 62, (97, 3)), rng.integers(-2 ** 62, 2 ** 62, (97, 1))),
+             (rng.standard_normal(5).astype(np.float32), np.float64(2.0)), (np.float64(1.5), np.float64(2.0))]
+    for a, b in cases:
+        for op in ("add", "multiply", "less"):
+            want = O.binary(op, a, b)
+            got, nt = O.binary_omp(op, a, b)
+            assert nt >= 1 and got.strides == want.strides
+            assert got.tobytes() == want.tobytes()
