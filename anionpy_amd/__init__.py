@@ -275,23 +275,23 @@ class ndarray:
         return bool(self.item())
 
     # ---- reductions as methods (ndarray_attrs.rs mirrors do_reduce_axis) --------------
-    def sum(self, axis=None, dtype=None, keepdims=False, initial=None):
-        return sum(self, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial)
+    def sum(self, axis=None, dtype=None, keepdims=False, initial=None, where=True):
+        return sum(self, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial, where=where)
 
     def cumsum(self, axis=None, dtype=None, out=None):
         return cumsum(self, axis=axis, dtype=dtype, out=out)
 
-    def prod(self, axis=None, dtype=None, keepdims=False, initial=None):
-        return prod(self, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial)
+    def prod(self, axis=None, dtype=None, keepdims=False, initial=None, where=True):
+        return prod(self, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial, where=where)
 
     def cumprod(self, axis=None, dtype=None, out=None):
         return cumprod(self, axis=axis, dtype=dtype, out=out)
 
-    def max(self, axis=None, keepdims=False, initial=None):
-        return amax(self, axis=axis, keepdims=keepdims, initial=initial)
+    def max(self, axis=None, keepdims=False, initial=None, where=True):
+        return amax(self, axis=axis, keepdims=keepdims, initial=initial, where=where)
 
-    def min(self, axis=None, keepdims=False, initial=None):
-        return amin(self, axis=axis, keepdims=keepdims, initial=initial)
+    def min(self, axis=None, keepdims=False, initial=None, where=True):
+        return amin(self, axis=axis, keepdims=keepdims, initial=initial, where=where)
 
     def argmax(self, axis=None, keepdims=False):
         return argmax(self, axis=axis, keepdims=keepdims)
@@ -590,13 +590,14 @@ class Ufunc:
             raise NotImplementedError(f"{self.__name__}.accumulate is not provided by the MI355X backend")
         return _accumulate(self._code, asarray(a), axis, dtype, out)
 
-    def reduce(self, a, axis=0, dtype=None, out=None, keepdims=False, initial=None):
+    def reduce(self, a, axis=0, dtype=None, out=None, keepdims=False, initial=None, where=True):
         """ufunc.reduce (lib.rs:9958) -> reduce_axis (ionp-core/src/ufunc.rs:7634).
         initial=: the fold's starting value (also the result of an empty
-        reduction)."""
+        reduction).  where=: a mask broadcastable to `a`; only its True
+        positions are reduced (maximum/minimum then need initial=)."""
         if not self._reducible:
             raise NotImplementedError(f"{self.__name__}.reduce is not provided by the MI355X backend")
-        return _reduce(self._code, asarray(a), axis, dtype, keepdims, out, initial)
+        return _reduce(self._code, asarray(a), axis, dtype, keepdims, out, initial, where)
 
 
 def _accumulate(code, a, axis, dtype, out):
@@ -627,24 +628,38 @@ def _axes_arg(axis):
     return arr, len(axis)
 
 
-def _reduce(code, a, axis, dtype, keepdims, out=None, initial=None):
+def _reduce(code, a, axis, dtype, keepdims, out=None, initial=None, where=True):
     L = _lib.load()
     axes, naxes = _axes_arg(axis)
     d = a._desc()
     t = _lib.ArrayDesc()
     dt = -1 if dtype is None else _CODE[_dtype(dtype)]
-    _call(L.ionp_plan_reduce, code, ctypes.byref(d), axes, naxes, int(bool(keepdims)), dt, ctypes.byref(t))
+    wd = None
+    if not (where is True or where is None):
+        wm = asarray(where)
+        if wm.dtype != np.bool_:  # reduce_axis casts the mask to bool (ufunc.rs:7682)
+            wm = wm.astype(np.bool_)
+        wd = wm._desc()
+    if wd is None:
+        _call(L.ionp_plan_reduce, code, ctypes.byref(d), axes, naxes, int(bool(keepdims)), dt, ctypes.byref(t))
+    else:
+        _call(L.ionp_plan_reduce_where, code, ctypes.byref(d), ctypes.byref(wd), axes, naxes,
+              int(bool(keepdims)), dt, ctypes.byref(t))
     final = out
-    if out is None or _needs_temp(out, [a]):
+    if out is None or _needs_temp(out, [a] if wd is None else [a, wm]):
         out = empty_strided(tuple(t.shape[: t.ndim]), tuple(t.strides[: t.ndim]),
                             _NP_DTYPES[t.dtype] if final is None else final.dtype)
     init = None
     if initial is not None:  # one element of the output dtype, on the host
         init = np.array(initial).astype(out.dtype).reshape(1)  # unsafe cast, as NumPy does
     do = out._desc()
-    _call(L.ionp_reduce, code, ctypes.byref(d), axes, naxes, int(bool(keepdims)),
-          None if init is None else init.ctypes.data, ctypes.byref(do),
-          _cfg(_lib.CALL_REDUCE, code, (d,), do, axes, naxes, keepdims), _lib.stream_handle())
+    if wd is None:
+        _call(L.ionp_reduce, code, ctypes.byref(d), axes, naxes, int(bool(keepdims)),
+              None if init is None else init.ctypes.data, ctypes.byref(do),
+              _cfg(_lib.CALL_REDUCE, code, (d,), do, axes, naxes, keepdims), _lib.stream_handle())
+    else:
+        _call(L.ionp_reduce_where, code, ctypes.byref(d), ctypes.byref(wd), axes, naxes, int(bool(keepdims)),
+              None if init is None else init.ctypes.data, ctypes.byref(do), None, _lib.stream_handle())
     if final is not None and final is not out:
         _copy_into(out, final)
         return final
@@ -666,15 +681,15 @@ equal = Ufunc("equal", _lib.EQUAL, False)
 not_equal = Ufunc("not_equal", _lib.NOT_EQUAL, False)
 
 
-def sum(a, axis=None, dtype=None, keepdims=False, initial=None):
+def sum(a, axis=None, dtype=None, keepdims=False, initial=None, where=True):
     """sum (ionp-py/src/reductions.rs:1032) = add.reduce with axis=None default."""
-    return add.reduce(a, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial)
+    return add.reduce(a, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial, where=where)
 
 
-def prod(a, axis=None, dtype=None, keepdims=False, initial=None):
+def prod(a, axis=None, dtype=None, keepdims=False, initial=None, where=True):
     """prod (ionp-py/src/reductions.rs:1053) = multiply.reduce with axis=None
     default."""
-    return multiply.reduce(a, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial)
+    return multiply.reduce(a, axis=axis, dtype=dtype, keepdims=keepdims, initial=initial, where=where)
 
 
 def _scan(uf, a, axis, dtype, out):
@@ -697,13 +712,13 @@ def cumprod(a, axis=None, dtype=None, out=None):
     return _scan(multiply, a, axis, dtype, out)
 
 
-def amax(a, axis=None, keepdims=False, initial=None):
+def amax(a, axis=None, keepdims=False, initial=None, where=True):
     """max / amax (reductions.rs:1114) = maximum.reduce, axis=None default."""
-    return maximum.reduce(a, axis=axis, keepdims=keepdims, initial=initial)
+    return maximum.reduce(a, axis=axis, keepdims=keepdims, initial=initial, where=where)
 
 
-def amin(a, axis=None, keepdims=False, initial=None):
-    return minimum.reduce(a, axis=axis, keepdims=keepdims, initial=initial)
+def amin(a, axis=None, keepdims=False, initial=None, where=True):
+    return minimum.reduce(a, axis=axis, keepdims=keepdims, initial=initial, where=where)
 
 
 max = amax  # noqa: A001  (NumPy-named API)

@@ -159,8 +159,11 @@ static ionp_status reduce_dtype(int32_t op, int in_dt, int32_t dtype, int *out_d
     return IONP_OK;
 }
 
-ionp_status ionp_plan_reduce(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes,
-                             int32_t keepdims, int32_t dtype, ionp_array_desc *out_template) {
+// reduce_axis's shape/dtype half; wst: the where= mask's strides broadcast
+// to the input's shape (it joins the iterator's axis order), or null
+static ionp_status plan_reduce_template(int32_t op, const ionp_array_desc *in, const int64_t *wst,
+                                        const int32_t *axes, int32_t naxes, int32_t keepdims, int32_t dtype,
+                                        ionp_array_desc *out_template) {
     ionp_status s;
     if ((s = check_desc(in, "array"))) return s;
     int odt;
@@ -168,10 +171,11 @@ ionp_status ionp_plan_reduce(int32_t op, const ionp_array_desc *in, const int32_
     bool mask[IONP_MAX_DIMS];
     if ((s = normalize_axes(axes, naxes, in->ndim, mask))) return s;
     // output layout: kept axes keep the input's iterator order (array.rs:910)
-    int64_t st[1][IONP_MAX_DIMS];
+    int64_t st[2][IONP_MAX_DIMS];
     std::memcpy(st[0], in->strides, sizeof(int64_t) * in->ndim);
+    if (wst) std::memcpy(st[1], wst, sizeof(int64_t) * in->ndim);
     int perm[IONP_MAX_DIMS];
-    iter_axis_order(in->ndim, in->shape, 1, st, perm);
+    iter_axis_order(in->ndim, in->shape, wst ? 2 : 1, st, perm);
     int64_t kst[IONP_MAX_DIMS] = {0};
     int64_t acc = 1;
     for (int k = 0; k < in->ndim; k++) {
@@ -195,6 +199,40 @@ ionp_status ionp_plan_reduce(int32_t op, const ionp_array_desc *in, const int32_
     }
     fill_template(out_template, odt, nd, shape, ost);
     return IONP_OK;
+}
+
+ionp_status ionp_plan_reduce(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes,
+                             int32_t keepdims, int32_t dtype, ionp_array_desc *out_template) {
+    return plan_reduce_template(op, in, nullptr, axes, naxes, keepdims, dtype, out_template);
+}
+
+// the where= operand: bool, broadcastable to the input's shape
+// (broadcast_strides_to, shape.rs:113); its strides on the input's axes
+static ionp_status where_strides(const ionp_array_desc *in, const ionp_array_desc *where, int64_t *wst) {
+    ionp_status s;
+    if ((s = check_desc(where, "where"))) return s;
+    if (where->dtype != IONP_BOOL) return set_error(IONP_ERR_TYPE, "where= mask must have dtype bool");
+    bool ok = where->ndim <= in->ndim;
+    for (int i = 0; ok && i < where->ndim; i++) {
+        const int64_t d = where->shape[i], t = in->shape[in->ndim - where->ndim + i];
+        if (d != 1 && d != t) ok = false;
+    }
+    if (!ok)
+        return set_error(IONP_ERR_BROADCAST, "operands could not be broadcast together with shapes " +
+                                                 fmt_shape(where->shape, where->ndim) + " " +
+                                                 fmt_shape(in->shape, in->ndim) + " ");
+    broadcast_strides(where, in->ndim, in->shape, wst);
+    return IONP_OK;
+}
+
+ionp_status ionp_plan_reduce_where(int32_t op, const ionp_array_desc *in, const ionp_array_desc *where,
+                                   const int32_t *axes, int32_t naxes, int32_t keepdims, int32_t dtype,
+                                   ionp_array_desc *out_template) {
+    ionp_status s;
+    if ((s = check_desc(in, "array"))) return s;
+    int64_t wst[IONP_MAX_DIMS];
+    if ((s = where_strides(in, where, wst))) return s;
+    return plan_reduce_template(op, in, wst, axes, naxes, keepdims, dtype, out_template);
 }
 
 static ionp_status arg_normalize(const ionp_array_desc *in, int32_t axis, int *ax) {
@@ -292,6 +330,30 @@ ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *ax
     normalize_axes(axes, naxes, in->ndim, mask);
     ReducePlan p;
     if ((s = plan_reduce(op, in, mask, out, &p))) return s;
+    return run_reduce(p, g_bufsize, initial, cfg, as_stream(stream));
+}
+
+ionp_status ionp_reduce_where(int32_t op, const ionp_array_desc *in, const ionp_array_desc *where,
+                              const int32_t *axes, int32_t naxes, int32_t keepdims, const void *initial,
+                              const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
+    if (ionp_status cs = check_cfg(cfg)) return cs;
+    ionp_array_desc t;
+    ionp_status s;
+    if ((s = check_desc(out, "out"))) return s;
+    if ((s = ionp_plan_reduce_where(op, in, where, axes, naxes, keepdims, out->dtype, &t))) return s;
+    if ((s = same_shape(out, t.ndim, t.shape, "reduce"))) return s;
+    // reduce_axis_masked (ufunc.rs:7172): no identity and no initial= is an
+    // error whatever the sizes
+    if ((op == IONP_MAXIMUM || op == IONP_MINIMUM) && !initial)
+        return set_error(IONP_ERR_VALUE, std::string("reduction operation '") + op_name(op) +
+                                             "' does not have an identity, so to use a where mask one has to "
+                                             "specify 'initial'");
+    int64_t wst[IONP_MAX_DIMS];
+    if ((s = where_strides(in, where, wst))) return s;
+    bool mask[IONP_MAX_DIMS];
+    normalize_axes(axes, naxes, in->ndim, mask);
+    ReducePlan p;
+    if ((s = plan_reduce(op, in, mask, out, &p, wst, (const uint8_t *)elem0(where)))) return s;
     return run_reduce(p, g_bufsize, initial, cfg, as_stream(stream));
 }
 

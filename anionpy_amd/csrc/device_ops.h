@@ -194,6 +194,19 @@ __device__ inline int64_t r_offset(const RedArgs &A, int64_t r) {
     if (A.nr == 1) return r * A.r_st[0];
     return dot_idx((uint32_t)r, A.nr, A.r_shape, A.r_div, A.r_st);
 }
+// where= mask offsets (bool bytes) along the same kept / W / outer axes
+__device__ inline int64_t mask_k_offset(const RedArgs &A, uint32_t o) {
+    return dot_idx(o, A.nk, A.k_shape, A.k_div, A.k_m_st);
+}
+__device__ inline int64_t mask_w_offset(const RedArgs &A, int64_t q) {
+    if (A.nw == 1) return q * A.w_m_st[0];
+    return dot_idx((uint32_t)q, A.nw, A.w_shape, A.w_div, A.w_m_st);
+}
+__device__ inline int64_t mask_r_offset(const RedArgs &A, int64_t r) {
+    if (A.nr == 0) return 0;
+    if (A.nr == 1) return r * A.r_m_st[0];
+    return dot_idx((uint32_t)r, A.nr, A.r_shape, A.r_div, A.r_m_st);
+}
 
 template <int LT> __device__ inline typename dt_type<LT>::T ld(int dt, const char *base, int64_t off) {
     switch (dt) {

@@ -272,18 +272,23 @@ ionp_status normalize_axes(const int32_t *axes, int32_t naxes, int ndim, bool *m
 // strides; the innermost run of reduced axes is the wide block W (pairwise
 // per buffer window); reduced axes beyond a kept axis fold sequentially.
 ionp_status plan_reduce(int op, const ionp_array_desc *in, const bool *axmask, const ionp_array_desc *out,
-                        ReducePlan *p) {
+                        ReducePlan *p, const int64_t *mask_st, const uint8_t *mask) {
     memset(p, 0, sizeof *p);
     p->op = op;
     p->in_dt = in->dtype;
     p->out_dt = out->dtype;
     p->in = elem0(in);
     p->out = (char *)elem0(out);
+    p->mask = mask;
     const int nd = in->ndim;
-    int64_t st[1][IONP_MAX_DIMS];
-    for (int k = 0; k < nd; k++) st[0][k] = in->strides[k];
+    int64_t st[2][IONP_MAX_DIMS];
+    for (int k = 0; k < nd; k++) {
+        st[0][k] = in->strides[k];
+        st[1][k] = mask ? mask_st[k] : 0;
+    }
+    const int64_t *mst = st[1];
     int perm[IONP_MAX_DIMS];
-    iter_axis_order(nd, in->shape, 1, st, perm);
+    iter_axis_order(nd, in->shape, mask ? 2 : 1, st, perm);
 
     // output strides of each kept input axis (out may carry keepdims 1s)
     int64_t ost[IONP_MAX_DIMS];
@@ -310,20 +315,24 @@ ionp_status plan_reduce(int op, const ionp_array_desc *in, const bool *axmask, c
             if (n == 0) { p->wsize = 0; }
             if (n <= 1) continue;
             if (in_w) {
-                if (p->nw > 0 && in->strides[ax] == p->w_st[p->nw - 1] * p->w_shape[p->nw - 1]) {
+                if (p->nw > 0 && in->strides[ax] == p->w_st[p->nw - 1] * p->w_shape[p->nw - 1] &&
+                    mst[ax] == p->w_m_st[p->nw - 1] * p->w_shape[p->nw - 1]) {
                     p->w_shape[p->nw - 1] *= n;
                 } else {
                     if (p->nw == kMaxKDims) return set_error(IONP_ERR_UNSUPPORTED, "reduction layout too complex");
                     p->w_shape[p->nw] = n;
+                    p->w_m_st[p->nw] = mst[ax];
                     p->w_st[p->nw++] = in->strides[ax];
                 }
                 if (p->wsize) p->wsize *= n;
             } else {
-                if (p->nr > 0 && in->strides[ax] == p->r_st[p->nr - 1] * p->r_shape[p->nr - 1]) {
+                if (p->nr > 0 && in->strides[ax] == p->r_st[p->nr - 1] * p->r_shape[p->nr - 1] &&
+                    mst[ax] == p->r_m_st[p->nr - 1] * p->r_shape[p->nr - 1]) {
                     p->r_shape[p->nr - 1] *= n;
                 } else {
                     if (p->nr == kMaxKDims) return set_error(IONP_ERR_UNSUPPORTED, "reduction layout too complex");
                     p->r_shape[p->nr] = n;
+                    p->r_m_st[p->nr] = mst[ax];
                     p->r_st[p->nr++] = in->strides[ax];
                 }
                 p->rsize *= n;
@@ -333,12 +342,14 @@ ionp_status plan_reduce(int op, const ionp_array_desc *in, const bool *axmask, c
             if (n <= 1) continue;
             in_w = false;
             if (p->nk > 0 && in->strides[ax] == p->k_in_st[p->nk - 1] * p->k_shape[p->nk - 1] &&
-                ost[ax] == p->k_out_st[p->nk - 1] * p->k_shape[p->nk - 1]) {
+                ost[ax] == p->k_out_st[p->nk - 1] * p->k_shape[p->nk - 1] &&
+                mst[ax] == p->k_m_st[p->nk - 1] * p->k_shape[p->nk - 1]) {
                 p->k_shape[p->nk - 1] *= n;
             } else {
                 if (p->nk == kMaxKDims) return set_error(IONP_ERR_UNSUPPORTED, "reduction layout too complex");
                 p->k_shape[p->nk] = n;
                 p->k_in_st[p->nk] = in->strides[ax];
+                p->k_m_st[p->nk] = mst[ax];
                 p->k_out_st[p->nk++] = ost[ax];
             }
             if (p->nout) p->nout *= n;

@@ -72,11 +72,18 @@ struct ReducePlan {
     int64_t group;  // elements per output = wsize * rsize
     const char *in;
     char *out;
+    // where= mask (bool, broadcast to the input's shape) or null: its
+    // strides along the same coalesced kept / W / outer axes
+    const uint8_t *mask;
+    int64_t k_m_st[kMaxKDims], w_m_st[kMaxKDims], r_m_st[kMaxKDims];
 };
 
 // axes: normalized reduced-axis mask; out: output descriptor as planned.
+// mask_st / mask: the where= operand's strides broadcast to the input's
+// shape and its element-0 address (null: no mask); the mask takes part in
+// the axis order and in coalescing, as NumPy's third reduction operand.
 ionp_status plan_reduce(int op, const ionp_array_desc *in, const bool *axmask, const ionp_array_desc *out,
-                        ReducePlan *p);
+                        ReducePlan *p, const int64_t *mask_st = nullptr, const uint8_t *mask = nullptr);
 
 // normalize_reduce_axes (ufunc.rs:7551-7617)
 ionp_status normalize_axes(const int32_t *axes, int32_t naxes, int ndim, bool *mask);

@@ -24,6 +24,10 @@ struct RedArgs {
     int32_t has_init;    // initial= given: the fold starts from it (and max/min feed every element)
     int32_t pad_;
     uint64_t init_bits;  // its value in the output dtype: float bits, or the integer sign-/zero-extended
+    // where= mask (bool bytes) or null, with its strides along the kept / W /
+    // outer axes (the dividers above decompose indices for it too)
+    const uint8_t *mask;
+    int64_t k_m_st[kMaxKDims], w_m_st[kMaxKDims], r_m_st[kMaxKDims];
 };
 
 // Float add: every window is one pairwise leaf (<= 128 elements) and there

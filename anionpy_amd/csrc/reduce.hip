@@ -505,6 +505,12 @@ __device__ inline int64_t group_offset(const RedArgs &A, int64_t q) {
     const int64_t r = q / A.wsize, w = q - r * A.wsize;
     return w_offset(A, w) + r_offset(A, r);
 }
+// the same element's where= mask offset
+__device__ inline int64_t mask_group_offset(const RedArgs &A, int64_t q) {
+    if (A.nr == 0) return mask_w_offset(A, q);
+    const int64_t r = q / A.wsize, w = q - r * A.wsize;
+    return mask_w_offset(A, w) + mask_r_offset(A, r);
+}
 
 template <typename T> __device__ inline T pseed(const RedArgs &A) {
     if (!A.has_init) return T(1);
@@ -517,51 +523,64 @@ __device__ inline int64_t prod_offset(const RedArgs &A, int64_t q) {
     if (A.nr == 0 && A.nw == 1) return q * A.w_st[0];
     return group_offset(A, q);
 }
-template <typename T, typename IT> __device__ inline T prod_redo(const RedArgs &A, int64_t ib) {
+__device__ inline int64_t prod_moffset(const RedArgs &A, int64_t q) {
+    if (A.nw == 0) return A.nr == 1 ? q * A.r_m_st[0] : mask_r_offset(A, q);
+    if (A.nr == 0 && A.nw == 1) return q * A.w_m_st[0];
+    return mask_group_offset(A, q);
+}
+// group element q as a factor: a where= masked-out element is 1 (x * 1 == x
+// exactly, so skipping it and multiplying by 1 agree)
+template <typename T, typename IT, bool MASK> __device__ inline T prod_factor(const RedArgs &A, int64_t ib, int64_t mb,
+                                                                             int64_t q) {
     constexpr int LT = FDT<T>::v;
+    if constexpr (MASK)
+        if (!A.mask[mb + prod_moffset(A, q)]) return T(1);
+    return ld_t<LT, IT>(A.in, ib + prod_offset(A, q));
+}
+template <typename T, typename IT, bool MASK> __device__ inline T prod_redo(const RedArgs &A, int64_t ib, int64_t mb) {
     T acc = pseed<T>(A);
-    for (int64_t q = 0; q < A.group; q++) acc = fmul(acc, ld_t<LT, IT>(A.in, ib + prod_offset(A, q)));
+    for (int64_t q = 0; q < A.group; q++) acc = fmul(acc, prod_factor<T, IT, MASK>(A, ib, mb, q));
     return acc;
 }
 // one thread per output, U loads in flight ahead of the multiply chain
-template <typename T, typename IT> __global__ __launch_bounds__(256) void prod_cols(const RedArgs A) {
-    constexpr int LT = FDT<T>::v;
+template <typename T, typename IT, bool MASK> __global__ __launch_bounds__(256) void prod_cols(const RedArgs A) {
     constexpr int U = 16;
     for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
+        const int64_t mb = MASK ? mask_k_offset(A, (uint32_t)o) : 0;
         const int64_t n = A.group;
         T acc = pseed<T>(A);
         int64_t q = 0;
         for (; q + U <= n; q += U) {
             T v[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) v[u] = ld_t<LT, IT>(A.in, ib + prod_offset(A, q + u));
+            for (int u = 0; u < U; u++) v[u] = prod_factor<T, IT, MASK>(A, ib, mb, q + u);
 #pragma unroll
             for (int u = 0; u < U; u++) acc = acc * v[u];
         }
-        for (; q < n; q++) acc = acc * ld_t<LT, IT>(A.in, ib + prod_offset(A, q));
-        if (acc != acc) acc = prod_redo<T, IT>(A, ib);
+        for (; q < n; q++) acc = acc * prod_factor<T, IT, MASK>(A, ib, mb, q);
+        if (acc != acc) acc = prod_redo<T, IT, MASK>(A, ib, mb);
         ((T *)A.out)[ob] = acc;
     }
 }
 // few long groups: a wavefront per output loads blocks of 64 * B group
 // elements (the next block in flight), and the chain, the same in every
 // lane, reads them lane by lane in iterator order
-template <typename T, typename IT> __global__ __launch_bounds__(256) void prod_wave(const RedArgs A) {
-    constexpr int LT = FDT<T>::v;
+template <typename T, typename IT, bool MASK> __global__ __launch_bounds__(256) void prod_wave(const RedArgs A) {
     constexpr int B = 4, BLK = 64 * B;
     const int lane = threadIdx.x & 63;
     const int64_t w0 = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
     for (int64_t o = w0; o < A.nout; o += (int64_t)gridDim.x * (blockDim.x >> 6)) {
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
+        const int64_t mb = MASK ? mask_k_offset(A, (uint32_t)o) : 0;
         const int64_t n = A.group;
         T cur[B], nxt[B];
 #pragma unroll
         for (int k = 0; k < B; k++) {
             const int64_t q = k * 64 + lane;
-            cur[k] = q < n ? ld_t<LT, IT>(A.in, ib + prod_offset(A, q)) : T(1);
+            cur[k] = q < n ? prod_factor<T, IT, MASK>(A, ib, mb, q) : T(1);
             nxt[k] = T(1);
         }
         T acc = pseed<T>(A);
@@ -571,7 +590,7 @@ template <typename T, typename IT> __global__ __launch_bounds__(256) void prod_w
 #pragma unroll
                 for (int k = 0; k < B; k++) {
                     const int64_t q = nb + k * 64 + lane;
-                    nxt[k] = q < n ? ld_t<LT, IT>(A.in, ib + prod_offset(A, q)) : T(1);
+                    nxt[k] = q < n ? prod_factor<T, IT, MASK>(A, ib, mb, q) : T(1);
                 }
             }
             if (nb <= n) {
@@ -590,7 +609,7 @@ template <typename T, typename IT> __global__ __launch_bounds__(256) void prod_w
 #pragma unroll
             for (int k = 0; k < B; k++) cur[k] = nxt[k];
         }
-        if (lane == 0) ((T *)A.out)[ob] = acc != acc ? prod_redo<T, IT>(A, ib) : acc;
+        if (lane == 0) ((T *)A.out)[ob] = acc != acc ? prod_redo<T, IT, MASK>(A, ib, mb) : acc;
     }
 }
 
@@ -661,12 +680,77 @@ template <typename T, bool MAX> __device__ inline void cand_merge(ExtCand<T> &a,
     a.vec_nan |= b.vec_nan;
 }
 
+// One call of the x86 maximum/minimum loop on `len` elements: the SIMD
+// loop (V lanes) when `simd`, else the strided loop (8 scalar accumulators);
+// the len % V (or % 8) tail is folded in order after the lanes.
+template <typename T> struct ExtGeom {
+    int64_t len, nv, per_lane;
+    bool simd;
+};
+template <typename T> __device__ inline ExtGeom<T> ext_geom(int64_t len, bool simd) {
+    constexpr int V = sizeof(T) == 8 ? 8 : 16;
+    ExtGeom<T> g;
+    g.len = len;
+    g.simd = simd;
+    g.nv = simd ? len - len % V : (len >= 8 ? len - len % 8 : 0);
+    g.per_lane = simd ? g.nv / V : g.nv / 8;
+    return g;
+}
+// element e of the call enters the candidate with its position in the
+// loop's evaluation order as key
+template <typename T, bool MAX> __device__ inline void ext_feed(ExtCand<T> &c, const ExtGeom<T> &g, int64_t e, T v) {
+    constexpr int V = sizeof(T) == 8 ? 8 : 16;
+    int64_t key;
+    if (e < g.nv) {
+        const int lane = g.simd ? (int)(e % V) : (int)(e & 7);
+        const int rank = g.simd ? (V == 8 ? kLaneRank64[lane] : kLaneRank32[lane]) : lane;
+        key = (int64_t)rank * g.per_lane + e / (g.simd ? V : 8);
+    } else {
+        key = e;
+    }
+    if (v != v) {
+        if (!c.has_nan || key < c.kn) {
+            c.has_nan = 1;
+            c.kn = key;
+            c.nanv = v;
+        }
+        if (g.simd && e < g.nv) c.vec_nan = 1;
+    } else if (!c.has_m || beats<T, MAX>(v, c.m) || (v == c.m && key > c.km)) {
+        c.has_m = 1;
+        c.m = v;
+        c.km = key;
+    }
+}
+template <typename T> __device__ inline void ext_cand_init(ExtCand<T> &c) {
+    c.has_m = c.has_nan = c.vec_nan = 0;
+    c.km = c.kn = 0;
+    c.m = c.nanv = T(0);
+}
+// the call's summary from the merged candidate of all its elements
+template <typename T> __device__ inline ExtSum<T> ext_finish(const ExtCand<T> &b, const ExtGeom<T> &g) {
+    constexpr int V = sizeof(T) == 8 ? 8 : 16;
+    ExtSum<T> S;
+    S.simd = g.simd && g.len >= 1;
+    S.has_m = b.has_m;
+    S.m = b.m;
+    S.flag = g.simd ? (b.km >= (int64_t)(V - 1) * g.per_lane) : 1;
+    S.has_nan = b.has_nan;
+    S.nanv = (g.simd && b.vec_nan) ? canonical_nan(T(0)) : b.nanv;
+    return S;
+}
+// the empty summary (no element): the identity of ext_combine
+template <typename T> __device__ inline ExtSum<T> ext_empty() {
+    ExtSum<T> S;
+    S.m = S.nanv = T(0);
+    S.has_m = S.flag = S.has_nan = S.simd = 0;
+    return S;
+}
+
 // Summary of window g (block-cooperative; `red` is LDS scratch): every
 // element gets its position in the x86 loop's evaluation order, so ties
 // between -0 and +0 and NaN payloads come out as NumPy's.
 template <typename T, bool MAX> __device__ ExtSum<T> ext_window_summary(const RedArgs &A, int64_t g, ExtCand<T> *red) {
     constexpr int LT = FDT<T>::v;
-    constexpr int V = sizeof(T) == 8 ? 8 : 16;
     const int64_t per_out = A.rsize * A.nwin;
     const int64_t o = g / per_out, rem = g - o * per_out;
     const int64_t r = rem / A.nwin, w = rem - r * A.nwin;
@@ -681,35 +765,11 @@ template <typename T, bool MAX> __device__ ExtSum<T> ext_window_summary(const Re
     const bool simd = one_run ? (A.w_st[0] == 1) : true;
     const int64_t skip = (!A.has_init && r == 0 && w == 0) ? 1 : 0;  // first element seeds the fold
     const int64_t len = L - skip;
-    const int64_t nv = simd ? len - len % V : (len >= 8 ? len - len % 8 : 0);
-    const int64_t per_lane = simd ? nv / V : nv / 8;
+    const ExtGeom<T> geo = ext_geom<T>(len, simd);
     ExtCand<T> c;
-    c.has_m = c.has_nan = c.vec_nan = 0;
-    c.km = c.kn = 0;
-    c.m = c.nanv = T(0);
-    for (int64_t e = threadIdx.x; e < len; e += blockDim.x) {
-        const T v = ld<LT>(A.in_dt, A.in, ib + w_offset(A, s0 + skip + e));
-        int64_t key;
-        if (e < nv) {
-            const int lane = simd ? (int)(e % V) : (int)(e & 7);
-            const int rank = simd ? (V == 8 ? kLaneRank64[lane] : kLaneRank32[lane]) : lane;
-            key = (int64_t)rank * per_lane + e / (simd ? V : 8);
-        } else {
-            key = e;
-        }
-        if (v != v) {
-            if (!c.has_nan || key < c.kn) {
-                c.has_nan = 1;
-                c.kn = key;
-                c.nanv = v;
-            }
-            if (simd && e < nv) c.vec_nan = 1;
-        } else if (!c.has_m || beats<T, MAX>(v, c.m) || (v == c.m && key > c.km)) {
-            c.has_m = 1;
-            c.m = v;
-            c.km = key;
-        }
-    }
+    ext_cand_init(c);
+    for (int64_t e = threadIdx.x; e < len; e += blockDim.x)
+        ext_feed<T, MAX>(c, geo, e, ld<LT>(A.in_dt, A.in, ib + w_offset(A, s0 + skip + e)));
     red[threadIdx.x] = c;
     __syncthreads();
     for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
@@ -722,14 +782,7 @@ template <typename T, bool MAX> __device__ ExtSum<T> ext_window_summary(const Re
     }
     const ExtCand<T> b = red[0];
     __syncthreads();
-    ExtSum<T> S;
-    S.simd = simd && len >= 1;
-    S.has_m = b.has_m;
-    S.m = b.m;
-    S.flag = simd ? (b.km >= (int64_t)(V - 1) * per_lane) : 1;
-    S.has_nan = b.has_nan;
-    S.nanv = (simd && b.vec_nan) ? canonical_nan(T(0)) : b.nanv;
-    return S;
+    return ext_finish(b, geo);
 }
 
 template <typename T, bool MAX>
@@ -910,6 +963,198 @@ template <typename T, bool MAX> __global__ __launch_bounds__(256) void ext_narro
 }
 
 // ---------------------------------------------------------------------------
+// where= masked reductions on floats (reduce_axis_masked, ufunc.rs:7172, as
+// NumPy 2.2.6 evaluates it): the fold starts from initial= (add: else 0;
+// maximum/minimum always have one), and inside every bufsize window the
+// reduce loop is called once per maximal run of True mask values:
+//   add              acc = acc + pairwise(run)
+//   maximum/minimum  the window's SIMD (or strided) loop model on the run
+// W empty: acc = op(acc, x) for each selected x in iterator order.
+// Multiply is the product chain with masked-out factors 1 (prod_*).
+// ---------------------------------------------------------------------------
+template <typename T> __device__ inline T init_value(const RedArgs &A, T dflt) {
+    if (!A.has_init) return dflt;
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, A.init_bits);
+    else return __builtin_bit_cast(T, (uint32_t)A.init_bits);
+}
+
+template <typename T, int OP> __global__ __launch_bounds__(256) void mask_narrow(const RedArgs A) {
+    constexpr int LT = FDT<T>::v;
+    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
+        int64_t ib, ob;
+        out_offsets(A, (uint32_t)o, &ib, &ob);
+        const int64_t mb = mask_k_offset(A, (uint32_t)o);
+        T acc = init_value<T>(A, T(0));
+        for (int64_t r = 0; r < A.rsize; r++) {
+            if (!A.mask[mb + mask_r_offset(A, r)]) continue;
+            const T v = ld<LT>(A.in_dt, A.in, ib + r_offset(A, r));
+            if constexpr (OP == IONP_ADD) acc = fadd(acc, v);
+            else if constexpr (OP == IONP_MAXIMUM) acc = fmaxn(acc, v);
+            else acc = fminn(acc, v);
+        }
+        ((T *)A.out)[ob] = acc;
+    }
+}
+
+// exclusive prefix sum of v over the workgroup (blockDim.x a multiple of 64);
+// *total gets the sum.  wtmp: one int per wavefront (LDS).
+__device__ inline int block_excl_scan(int v, int *wtmp, int *total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wtmp[wv] = x;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int k = 0; k < nwv; k++) {
+        const int t = wtmp[k];
+        if (k < wv) off += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + x - v;
+}
+
+// W not empty: a workgroup per output walks its windows in order.  A
+// window's values and mask are staged in LDS; each thread owns the runs
+// that start in its chunk of at most 32 positions, so every run but a
+// thread's last is short, and a run longer than one pairwise leaf (> 128)
+// is always a thread's last.  Short runs are evaluated by their owner;
+// long runs by the whole workgroup (pairwise_block / a keyed candidate
+// reduction).  Add folds the run sums into acc in run order; max/min
+// combine the run summaries in order (ext_combine) and apply the window's
+// summary to acc.
+template <typename T, int OP> __global__ __launch_bounds__(256) void mask_rows(const RedArgs A) {
+    constexpr int LT = FDT<T>::v;
+    constexpr bool ADD = OP == IONP_ADD;
+    constexpr bool MAX = OP == IONP_MAXIMUM;
+    constexpr int NT = 256;
+    __shared__ T vals[kPwMaxWin];
+    __shared__ uint8_t flags[kPwMaxWin];
+    __shared__ int ffalse[NT];  // first masked-out position in each thread's chunk, or -1
+    __shared__ int wtmp[NT / 64];
+    __shared__ int nlong, l_lo[64], l_n[64], l_key[64];
+    __shared__ T runres[ADD ? kPwMaxWin / 2 : 1];  // add: the runs' pairwise sums by ordinal
+    __shared__ T racc[ADD ? 1024 : 1], pval[ADD ? 1024 : 1];
+    __shared__ int leaf_lo[ADD ? 128 : 1], leaf_n[ADD ? 128 : 1], lcnt[1];
+    __shared__ ExtCand<T> red[ADD ? 1 : NT];
+    __shared__ ExtSum<T> tsum[ADD ? 1 : NT];  // max/min: long-run summary, then thread totals
+    const int tid = threadIdx.x;
+    for (int64_t o = blockIdx.x; o < A.nout; o += gridDim.x) {
+        int64_t ib, ob;
+        out_offsets(A, (uint32_t)o, &ib, &ob);
+        const int64_t mb = mask_k_offset(A, (uint32_t)o);
+        T acc = init_value<T>(A, T(0));  // thread 0 keeps the running value
+        for (int64_t r = 0; r < A.rsize; r++) {
+            const int64_t rb = ib + r_offset(A, r), mrb = mb + mask_r_offset(A, r);
+            for (int64_t w = 0; w < A.nwin; w++) {
+                const int64_t s0 = w * A.bufsize;
+                const int L = (int)min(A.bufsize, A.wsize - s0);
+                for (int e = tid; e < L; e += NT) {
+                    const uint8_t f = A.mask[mrb + mask_w_offset(A, s0 + e)] != 0;
+                    flags[e] = f;
+                    if (f) vals[e] = ld<LT>(A.in_dt, A.in, rb + w_offset(A, s0 + e));
+                }
+                if (tid == 0) nlong = 0;
+                __syncthreads();
+                const int C = (L + NT - 1) / NT;
+                const int c0 = min(L, tid * C), c1 = min(L, c0 + C);
+                int cnt = 0, ff = -1;
+                for (int e = c0; e < c1; e++) {
+                    cnt += flags[e] && (e == 0 || !flags[e - 1]);
+                    if (ff < 0 && !flags[e]) ff = e;
+                }
+                ffalse[tid] = ff;
+                int total;
+                int k = block_excl_scan(cnt, wtmp, &total);  // run ordinal of this thread's first run
+                // the max/min loop kind of this window (as for unmasked windows)
+                const bool one_run = (s0 / A.w_shape[0]) == ((s0 + L - 1) / A.w_shape[0]);
+                const bool simd = one_run ? (A.w_st[0] == 1) : true;
+                ExtSum<T> pre = ext_empty<T>();
+                bool haslong = false;
+                for (int e = c0; e < c1; e++) {
+                    if (!(flags[e] && (e == 0 || !flags[e - 1]))) continue;
+                    int f = e + 1;
+                    while (f < c1 && flags[f]) f++;
+                    if (f == c1) {  // runs on past the chunk: the next masked-out position
+                        f = L;
+                        for (int u = tid + 1; u < NT; u++)
+                            if (ffalse[u] >= 0) {
+                                f = ffalse[u];
+                                break;
+                            }
+                    }
+                    const int n = f - e;
+                    if (n > 128) {
+                        const int i = atomicAdd(&nlong, 1);
+                        l_lo[i] = e;
+                        l_n[i] = n;
+                        l_key[i] = ADD ? k : tid;
+                        haslong = true;
+                    } else if constexpr (ADD) {
+                        runres[k] = pw_leaf<T>([&](int i) { return vals[e + i]; }, n);
+                    } else {
+                        const ExtGeom<T> g = ext_geom<T>(n, simd);
+                        ExtCand<T> c;
+                        ext_cand_init(c);
+                        for (int i = 0; i < n; i++) ext_feed<T, MAX>(c, g, i, vals[e + i]);
+                        pre = ext_combine<T, MAX>(pre, ext_finish(c, g));
+                    }
+                    k++;
+                }
+                __syncthreads();
+                const int NL = nlong;
+                for (int i = 0; i < NL; i++) {
+                    const int lo = l_lo[i], n = l_n[i];
+                    if constexpr (ADD) {
+                        const T sres = pairwise_block<T>([&](int j) { return vals[lo + j]; }, n, racc, pval, leaf_lo,
+                                                         leaf_n, lcnt);
+                        if (tid == 0) runres[l_key[i]] = sres;
+                    } else {
+                        const ExtGeom<T> g = ext_geom<T>(n, simd);
+                        ExtCand<T> c;
+                        ext_cand_init(c);
+                        for (int j = tid; j < n; j += NT) ext_feed<T, MAX>(c, g, j, vals[lo + j]);
+                        red[tid] = c;
+                        __syncthreads();
+                        for (int s2 = NT / 2; s2 > 0; s2 >>= 1) {
+                            if (tid < s2) {
+                                ExtCand<T> a = red[tid];
+                                cand_merge<T, MAX>(a, red[tid + s2]);
+                                red[tid] = a;
+                            }
+                            __syncthreads();
+                        }
+                        if (tid == 0) tsum[l_key[i]] = ext_finish(red[0], g);
+                    }
+                    __syncthreads();
+                }
+                if constexpr (ADD) {
+                    if (tid == 0)
+                        for (int i = 0; i < total; i++) acc = fadd(acc, runres[i]);
+                } else {
+                    const ExtSum<T> mine = haslong ? ext_combine<T, MAX>(pre, tsum[tid]) : pre;
+                    __syncthreads();
+                    tsum[tid] = mine;
+                    __syncthreads();
+                    for (int s2 = 1; s2 < NT; s2 <<= 1) {  // ordered: left operand first
+                        if ((tid & (2 * s2 - 1)) == 0) tsum[tid] = ext_combine<T, MAX>(tsum[tid], tsum[tid + s2]);
+                        __syncthreads();
+                    }
+                    if (tid == 0) acc = ext_apply<T, MAX>(tsum[0], acc);
+                }
+                __syncthreads();
+            }
+        }
+        if (tid == 0) ((T *)A.out)[ob] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // order-free reductions: integer / bool add, multiply, maximum, minimum
 // ---------------------------------------------------------------------------
 template <typename AT, int OP> __device__ inline AT assoc(AT a, AT b) {
@@ -917,6 +1162,14 @@ template <typename AT, int OP> __device__ inline AT assoc(AT a, AT b) {
     else if constexpr (OP == IONP_MULTIPLY) return (AT)((uint64_t)a * (uint64_t)b);
     else if constexpr (OP == IONP_MAXIMUM) return a > b ? a : b;
     else return a < b ? a : b;
+}
+// the op's identity on an integer accumulator (the starting value of a
+// where= masked fold, whose elements may all be masked out)
+template <typename AT, int OP> __device__ constexpr AT int_identity() {
+    if constexpr (OP == IONP_ADD) return AT(0);
+    else if constexpr (OP == IONP_MULTIPLY) return AT(1);
+    else if constexpr (OP == IONP_MAXIMUM) return std::numeric_limits<AT>::lowest();
+    else return std::numeric_limits<AT>::max();
 }
 // a group's result combined with initial= (order-free: once, at the end)
 template <typename AT, int OP> __device__ inline AT with_init(const RedArgs &A, AT acc) {
@@ -927,8 +1180,9 @@ template <typename AT, int LT> __device__ inline AT load_acc(int dt, const char 
     return (AT)ld<LT>(dt, base, off);
 }
 
-// AT: int64_t / uint64_t accumulator; LT: dtype the elements are cast to.
-template <typename AT, int LT, int OP>
+// AT: int64_t / uint64_t accumulator; LT: dtype the elements are cast to;
+// MASK: where= masked (elements whose mask byte is 0 are skipped).
+template <typename AT, int LT, int OP, bool MASK = false>
 __global__ __launch_bounds__(256) void int_reduce(const RedArgs A, int64_t seg, int64_t nseg) {
     __shared__ AT red[256];
     using ST = typename dt_type<LT>::T;
@@ -936,10 +1190,15 @@ __global__ __launch_bounds__(256) void int_reduce(const RedArgs A, int64_t seg, 
         const int64_t o = g / nseg, sg = g - o * nseg;
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
+        const int64_t mb = MASK ? mask_k_offset(A, (uint32_t)o) : 0;
         const int64_t q0 = sg * seg, q1 = min(A.group, q0 + seg);
-        AT acc = OP == IONP_ADD ? AT(0) : OP == IONP_MULTIPLY ? AT(1) : (AT)load_acc<AT, LT>(A.in_dt, A.in, ib + group_offset(A, q0));
-        for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x)
+        AT acc = MASK ? int_identity<AT, OP>()
+                 : OP == IONP_ADD ? AT(0) : OP == IONP_MULTIPLY ? AT(1) : (AT)load_acc<AT, LT>(A.in_dt, A.in, ib + group_offset(A, q0));
+        for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+            if constexpr (MASK)
+                if (!A.mask[mb + mask_group_offset(A, q)]) continue;
             acc = assoc<AT, OP>(acc, (AT)load_acc<ST, LT>(A.in_dt, A.in, ib + group_offset(A, q)));
+        }
         red[threadIdx.x] = acc;
         __syncthreads();
         for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -1016,14 +1275,20 @@ __global__ __launch_bounds__(256) void int_reduce_contig(const RedArgs A, int64_
 }
 
 // narrow layouts: one thread per output, elements in any order
-template <typename AT, int LT, int OP> __global__ __launch_bounds__(256) void int_reduce_cols(const RedArgs A) {
+template <typename AT, int LT, int OP, bool MASK = false>
+__global__ __launch_bounds__(256) void int_reduce_cols(const RedArgs A) {
     using ST = typename dt_type<LT>::T;
     for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
-        AT acc = OP == IONP_ADD ? AT(0) : OP == IONP_MULTIPLY ? AT(1) : (AT)load_acc<ST, LT>(A.in_dt, A.in, ib);
-        for (int64_t q = 0; q < A.group; q++)
+        const int64_t mb = MASK ? mask_k_offset(A, (uint32_t)o) : 0;
+        AT acc = MASK ? int_identity<AT, OP>()
+                 : OP == IONP_ADD ? AT(0) : OP == IONP_MULTIPLY ? AT(1) : (AT)load_acc<ST, LT>(A.in_dt, A.in, ib);
+        for (int64_t q = 0; q < A.group; q++) {
+            if constexpr (MASK)
+                if (!A.mask[mb + mask_group_offset(A, q)]) continue;
             acc = assoc<AT, OP>(acc, (AT)load_acc<ST, LT>(A.in_dt, A.in, ib + group_offset(A, q)));
+        }
         ((ST *)A.out)[ob] = int_result<LT>(with_init<AT, OP>(A, acc));
     }
 }
@@ -1070,10 +1335,15 @@ template <typename T> static void launch_narrow(const RedArgs &A, int bpc, hipSt
 }
 
 template <typename T, typename IT> static void launch_prod_t(const RedArgs &A, int bpc, hipStream_t s) {
-    if (A.nout <= 8192 && A.group >= 256)
-        hipLaunchKernelGGL((prod_wave<T, IT>), dim3(grid_cap((A.nout + 3) / 4, bpc)), dim3(256), 0, s, A);
-    else
-        hipLaunchKernelGGL((prod_cols<T, IT>), dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
+    const bool wave = A.nout <= 8192 && A.group >= 256;
+    const dim3 gw(grid_cap((A.nout + 3) / 4, bpc)), gc(grid_cap((A.nout + 255) / 256, bpc)), blk(256);
+    if (A.mask) {
+        if (wave) hipLaunchKernelGGL((prod_wave<T, IT, true>), gw, blk, 0, s, A);
+        else hipLaunchKernelGGL((prod_cols<T, IT, true>), gc, blk, 0, s, A);
+    } else {
+        if (wave) hipLaunchKernelGGL((prod_wave<T, IT, false>), gw, blk, 0, s, A);
+        else hipLaunchKernelGGL((prod_cols<T, IT, false>), gc, blk, 0, s, A);
+    }
 }
 template <typename T> static void launch_prod(const RedArgs &A, int bpc, hipStream_t s) {
     switch (A.in_dt) {
@@ -1096,11 +1366,12 @@ static ionp_status launched() {
     return e == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
 }
 
-template <typename AT, int LT, int OP>
+template <typename AT, int LT, int OP, bool MASK>
 static ionp_status run_int(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
     const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
     if (int_cols_ok(A)) {
-        hipLaunchKernelGGL((int_reduce_cols<AT, LT, OP>), dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
+        hipLaunchKernelGGL((int_reduce_cols<AT, LT, OP, MASK>), dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0,
+                           s, A);
         return launched();
     }
     int64_t seg = 65536;
@@ -1114,7 +1385,7 @@ static ionp_status run_int(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
     // raw input values only where that equals casting each to the output
     // dtype first: integer add and multiply (wrap modulo the output width
     // either way) or no cast; bool outputs (logical or / and) take int_reduce
-    if (A.nw == 1 && A.w_st[0] == 1 && A.nr == 0 && A.in_dt != IONP_BOOL && A.out_dt != IONP_BOOL &&
+    if (!MASK && A.nw == 1 && A.w_st[0] == 1 && A.nr == 0 && A.in_dt != IONP_BOOL && A.out_dt != IONP_BOOL &&
         A.group < (1ll << 40) && (OP == IONP_ADD || OP == IONP_MULTIPLY || A.in_dt == A.out_dt)) {
         switch (A.in_dt) {  // one contiguous run per output: vector loads
             case IONP_INT8: hipLaunchKernelGGL((int_reduce_contig<AT, int8_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
@@ -1127,31 +1398,63 @@ static ionp_status run_int(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
             default: hipLaunchKernelGGL((int_reduce_contig<AT, uint64_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
         }
     } else {
-        hipLaunchKernelGGL((int_reduce<AT, LT, OP>), grid, blk, 0, s, A, seg, nseg);
+        hipLaunchKernelGGL((int_reduce<AT, LT, OP, MASK>), grid, blk, 0, s, A, seg, nseg);
     }
     if (nseg > 1) hipLaunchKernelGGL((int_reduce_fold<AT, LT, OP>), dim3(grid_cap(A.nout, bpc)), dim3(256), 0, s, A, nseg);
     return launched();
 }
 
-template <int LT, int OP> static ionp_status run_int_t(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
+template <int LT, int OP, bool MASK> static ionp_status run_int_t(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
     if constexpr (LT == IONP_UINT64 || LT == IONP_UINT32 || LT == IONP_UINT16 || LT == IONP_UINT8 || LT == IONP_BOOL)
-        return run_int<uint64_t, LT, OP>(A, cfg, s);
+        return run_int<uint64_t, LT, OP, MASK>(A, cfg, s);
     else
-        return run_int<int64_t, LT, OP>(A, cfg, s);
+        return run_int<int64_t, LT, OP, MASK>(A, cfg, s);
 }
 
-template <int OP> static ionp_status run_int_dispatch(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
+template <int OP, bool MASK = false> static ionp_status run_int_dispatch(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
     switch (A.out_dt) {
-        case IONP_BOOL: return run_int_t<IONP_BOOL, OP>(A, cfg, s);
-        case IONP_INT8: return run_int_t<IONP_INT8, OP>(A, cfg, s);
-        case IONP_UINT8: return run_int_t<IONP_UINT8, OP>(A, cfg, s);
-        case IONP_INT16: return run_int_t<IONP_INT16, OP>(A, cfg, s);
-        case IONP_UINT16: return run_int_t<IONP_UINT16, OP>(A, cfg, s);
-        case IONP_INT32: return run_int_t<IONP_INT32, OP>(A, cfg, s);
-        case IONP_UINT32: return run_int_t<IONP_UINT32, OP>(A, cfg, s);
-        case IONP_INT64: return run_int_t<IONP_INT64, OP>(A, cfg, s);
-        case IONP_UINT64: return run_int_t<IONP_UINT64, OP>(A, cfg, s);
+        case IONP_BOOL: return run_int_t<IONP_BOOL, OP, MASK>(A, cfg, s);
+        case IONP_INT8: return run_int_t<IONP_INT8, OP, MASK>(A, cfg, s);
+        case IONP_UINT8: return run_int_t<IONP_UINT8, OP, MASK>(A, cfg, s);
+        case IONP_INT16: return run_int_t<IONP_INT16, OP, MASK>(A, cfg, s);
+        case IONP_UINT16: return run_int_t<IONP_UINT16, OP, MASK>(A, cfg, s);
+        case IONP_INT32: return run_int_t<IONP_INT32, OP, MASK>(A, cfg, s);
+        case IONP_UINT32: return run_int_t<IONP_UINT32, OP, MASK>(A, cfg, s);
+        case IONP_INT64: return run_int_t<IONP_INT64, OP, MASK>(A, cfg, s);
+        case IONP_UINT64: return run_int_t<IONP_UINT64, OP, MASK>(A, cfg, s);
         default: return set_error(IONP_ERR_NO_LOOP, "no integer reduce loop");
+    }
+}
+
+// where= masked reductions (the mask is set in A)
+template <typename T> static ionp_status run_masked_float(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
+    const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
+    if (A.op == IONP_MULTIPLY) {
+        launch_prod<T>(A, bpc, s);
+        return launched();
+    }
+    if (A.nw == 0) {
+        const dim3 g(grid_cap((A.nout + 255) / 256, bpc)), b(256);
+        if (A.op == IONP_ADD) hipLaunchKernelGGL((mask_narrow<T, IONP_ADD>), g, b, 0, s, A);
+        else if (A.op == IONP_MAXIMUM) hipLaunchKernelGGL((mask_narrow<T, IONP_MAXIMUM>), g, b, 0, s, A);
+        else hipLaunchKernelGGL((mask_narrow<T, IONP_MINIMUM>), g, b, 0, s, A);
+        return launched();
+    }
+    if (A.bufsize > kPwMaxWin) return set_error(IONP_ERR_UNSUPPORTED, "bufsize above 8192");
+    const dim3 g(grid_cap(A.nout, bpc)), b(256);
+    if (A.op == IONP_ADD) hipLaunchKernelGGL((mask_rows<T, IONP_ADD>), g, b, 0, s, A);
+    else if (A.op == IONP_MAXIMUM) hipLaunchKernelGGL((mask_rows<T, IONP_MAXIMUM>), g, b, 0, s, A);
+    else hipLaunchKernelGGL((mask_rows<T, IONP_MINIMUM>), g, b, 0, s, A);
+    return launched();
+}
+static ionp_status run_masked(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
+    if (is_float_dt(A.out_dt))
+        return A.out_dt == IONP_FLOAT64 ? run_masked_float<double>(A, cfg, s) : run_masked_float<float>(A, cfg, s);
+    switch (A.op) {
+        case IONP_ADD: return run_int_dispatch<IONP_ADD, true>(A, cfg, s);
+        case IONP_MULTIPLY: return run_int_dispatch<IONP_MULTIPLY, true>(A, cfg, s);
+        case IONP_MAXIMUM: return run_int_dispatch<IONP_MAXIMUM, true>(A, cfg, s);
+        default: return run_int_dispatch<IONP_MINIMUM, true>(A, cfg, s);
     }
 }
 
@@ -1279,21 +1582,25 @@ ionp_status run_reduce(const ReducePlan &p, int64_t bufsize, const void *initial
     A.init_bits = initial ? init_bits_of(p.out_dt, initial) : 0;
     if (p.nout >= (1ll << 31) || p.wsize >= (1ll << 31) * 8 || p.rsize >= (1ll << 31))
         return set_error(IONP_ERR_UNSUPPORTED, "reduction extent too large");
+    A.mask = p.mask;
     for (int d = 0; d < p.nk; d++) {
         A.k_shape[d] = p.k_shape[d];
         A.k_in_st[d] = p.k_in_st[d];
         A.k_out_st[d] = p.k_out_st[d];
+        A.k_m_st[d] = p.k_m_st[d];
         A.k_div[d] = FastDiv((uint32_t)p.k_shape[d]);
     }
     for (int d = 0; d < p.nw; d++) {
         A.w_shape[d] = p.w_shape[d];
         A.w_st[d] = p.w_st[d];
+        A.w_m_st[d] = p.w_m_st[d];
         A.w_div[d] = FastDiv((uint32_t)(p.w_shape[d] < (1ll << 31) ? p.w_shape[d] : 1));
     }
     if (p.nw > 1 && p.wsize >= (1ll << 31)) return set_error(IONP_ERR_UNSUPPORTED, "reduction extent too large");
     for (int d = 0; d < p.nr; d++) {
         A.r_shape[d] = p.r_shape[d];
         A.r_st[d] = p.r_st[d];
+        A.r_m_st[d] = p.r_m_st[d];
         A.r_div[d] = FastDiv((uint32_t)p.r_shape[d]);
     }
     const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
@@ -1314,6 +1621,7 @@ ionp_status run_reduce(const ReducePlan &p, int64_t bufsize, const void *initial
         }
         return launched();
     }
+    if (A.mask) return run_masked(A, cfg, s);
     if (is_float_dt(p.out_dt)) {
         if (p.op == IONP_ADD) {
             if (bufsize > kPwMaxWin) return set_error(IONP_ERR_UNSUPPORTED, "bufsize above 8192");

@@ -147,6 +147,14 @@ ionp_status ionp_plan_cast(const ionp_array_desc *in, int32_t dtype, ionp_array_
 ionp_status ionp_plan_reduce(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes,
                              int32_t keepdims, int32_t dtype, ionp_array_desc *out_template);
 
+/* reduce_axis's shape/dtype half with a where= mask (ufunc.rs:7634 with
+ * `where_mask`): `where` is a bool array broadcastable to the input's
+ * shape (broadcast_strides_to, shape.rs:113); it takes part in the
+ * iterator's axis order, so it can change the output's layout. */
+ionp_status ionp_plan_reduce_where(int32_t op, const ionp_array_desc *in, const ionp_array_desc *where,
+                                   const int32_t *axes, int32_t naxes, int32_t keepdims, int32_t dtype,
+                                   ionp_array_desc *out_template);
+
 /* argext's shape half (ufunc.rs:8131-8206); axis = IONP_AXIS_NONE for the
  * flattened form. */
 ionp_status ionp_plan_argext(int32_t ismax, const ionp_array_desc *in, int32_t axis, int32_t keepdims,
@@ -179,6 +187,17 @@ ionp_status ionp_cast(const ionp_array_desc *in, const ionp_array_desc *out, con
 ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes,
                         int32_t keepdims, const void *initial, const ionp_array_desc *out, const ionp_config *cfg,
                         void *stream);
+
+/* Replaces reduce_axis (ufunc.rs:7634) when `where_mask` is given
+ * (reduce_axis_masked, ufunc.rs:7172), as NumPy 2.2.6 evaluates it: the
+ * fold starts from `initial` (else add's 0, multiply's 1; maximum and
+ * minimum without `initial` are an error whatever the sizes), and inside
+ * every iterator buffer window the reduce loop runs once per maximal run
+ * of True mask values (float add: acc = acc + pairwise(run)).  `out` as
+ * given by ionp_plan_reduce_where. */
+ionp_status ionp_reduce_where(int32_t op, const ionp_array_desc *in, const ionp_array_desc *where,
+                              const int32_t *axes, int32_t naxes, int32_t keepdims, const void *initial,
+                              const ionp_array_desc *out, const ionp_config *cfg, void *stream);
 
 /* Replaces ionp_core::ufunc::argext (ufunc.rs:8131): int64 indices. */
 ionp_status ionp_argext(int32_t ismax, const ionp_array_desc *in, int32_t axis, int32_t keepdims,

@@ -61,6 +61,9 @@ def lib():
                            ("ora_where", [P, P, P, P]),
                            ("ora_cast", [P, P]),
                            ("ora_reduce", [ctypes.c_int, P, ctypes.c_char_p, P, ctypes.c_int64, ctypes.c_void_p]),
+                           ("ora_reduce_where", [ctypes.c_int, P, ctypes.c_char_p, P, P, ctypes.c_int64,
+                                                 ctypes.c_void_p]),
+                           ("ora_binary_omp", [ctypes.c_int, P, P, P, ctypes.POINTER(ctypes.c_int)]),
                            ("ora_argext", [ctypes.c_int, P, ctypes.c_int, P]),
                            ("ora_accumulate", [ctypes.c_int, P, ctypes.c_int, P])]:
             getattr(L, name).argtypes = args
@@ -205,21 +208,33 @@ def _norm_axes(axis, ndim):
     return [] if ndim == 0 else out
 
 
-def reduce(op, x, axis=0, bufsize=BUFSIZE, dtype=None, initial=None):
-    """np.<op>.reduce(x, axis, dtype, initial) for op in add/maximum/minimum."""
+def reduce(op, x, axis=0, bufsize=BUFSIZE, dtype=None, initial=None, where=None):
+    """np.<op>.reduce(x, axis, dtype, initial, where) for op in
+    add/multiply/maximum/minimum.  `where`: a bool mask broadcastable to x
+    (ora_reduce_where), else the unmasked fold (ora_reduce)."""
     x = np.asarray(x)
     axes = _norm_axes(axis, x.ndim)
     mask = bytes(1 if k in axes else 0 for k in range(x.ndim)) or b"\0"
     odt = DTYPES[lib().ora_reduce_dtype(OP[op], _CODE[x.dtype])] if dtype is None else np.dtype(dtype)
     kept = [k for k in range(x.ndim) if k not in axes]
     shape = tuple(x.shape[k] for k in kept)
-    # K-order of the kept axes, following the input's iterator order
-    perm = (ctypes.c_int * MAXDIM)()
-    if x.ndim:
+    wm = None
+    if where is not None:
+        wm = np.broadcast_to(np.asarray(where), x.shape)
+        if wm.dtype != np.bool_:
+            raise OracleError(3)
+    # K-order of the kept axes, following the input's iterator order (with
+    # a mask: the order over input and mask)
+    order = []
+    if x.ndim and wm is None:
+        perm = (ctypes.c_int * MAXDIM)()
         shp = (ctypes.c_int64 * MAXDIM)(*x.shape)
         st = (ctypes.c_int64 * MAXDIM)(*[s // x.itemsize for s in x.strides])
         lib().ora_axis_order(x.ndim, shp, st, perm)
-    order = [perm[i] for i in range(x.ndim) if perm[i] in kept]
+        order = [perm[i] for i in range(x.ndim) if perm[i] in kept]
+    elif x.ndim:
+        full = korder_strides([x, wm], x.shape)
+        order = sorted(kept, key=lambda a: (full[a], -a))
     ostr = [0] * len(kept)
     acc = 1
     for a in order:
@@ -228,8 +243,13 @@ def reduce(op, x, axis=0, bufsize=BUFSIZE, dtype=None, initial=None):
     out = _empty_strided(shape, ostr, odt)
     dx, do = _desc(x), _desc(out)
     init = None if initial is None else np.array(initial, dtype=odt).reshape(1)
-    _check(lib().ora_reduce(OP[op], ctypes.byref(dx), mask, ctypes.byref(do), bufsize,
-                            None if init is None else init.ctypes.data))
+    ip = None if init is None else init.ctypes.data
+    if wm is None:
+        _check(lib().ora_reduce(OP[op], ctypes.byref(dx), mask, ctypes.byref(do), bufsize, ip))
+    else:
+        dm = _desc(wm)
+        _check(lib().ora_reduce_where(OP[op], ctypes.byref(dx), mask, ctypes.byref(dm), ctypes.byref(do),
+                                      bufsize, ip))
     return out[()] if out.ndim == 0 else out
 
 

@@ -775,13 +775,19 @@ typedef struct {
     int64_t w_run_stride;       /* element stride of that run */
 } red_plan;
 
-static void make_red_plan(const ora_desc *in, const uint8_t *axmask, red_plan *p)
+/* mask: NULL, or the where= operand broadcast to in's shape; it takes part
+ * in the iterator's axis order and in coalescing, as NumPy's third
+ * reduction operand does */
+static void make_red_plan_m(const ora_desc *in, const ora_desc *mask, const uint8_t *axmask, red_plan *p)
 {
-    int64_t st[1][ORA_MAXDIM];
+    int64_t st[2][ORA_MAXDIM];
     int perm[ORA_MAXDIM], k, n = in->ndim, inW = 1;
     memset(p, 0, sizeof *p);
-    for (k = 0; k < n; k++) st[0][k] = in->strides[k];
-    axis_order(n, in->shape, 1, st, perm);
+    for (k = 0; k < n; k++) {
+        st[0][k] = in->strides[k];
+        st[1][k] = mask ? mask->strides[k] : 0;
+    }
+    axis_order(n, in->shape, mask ? 2 : 1, st, perm);
     p->wsize = 1;
     for (k = 0; k < n; k++) {
         int a = perm[k];
@@ -795,13 +801,19 @@ static void make_red_plan(const ora_desc *in, const uint8_t *axmask, red_plan *p
     p->w_run_stride = p->nw ? in->strides[p->w_axes[0]] : 1;
     {
         int64_t run = p->nw ? in->shape[p->w_axes[0]] : 1;
+        int64_t mrun_st = p->nw && mask ? mask->strides[p->w_axes[0]] : 0;
         for (k = 1; k < p->nw; k++) {
             int a = p->w_axes[k];
             if (in->strides[a] != p->w_run_stride * run) break;
+            if (mask && mask->strides[a] != mrun_st * run) break;
             run *= in->shape[a];
         }
         p->w_inner_len = run;
     }
+}
+static void make_red_plan(const ora_desc *in, const uint8_t *axmask, red_plan *p)
+{
+    make_red_plan_m(in, NULL, axmask, p);
 }
 
 /* Byte offsets of W's elements (iterator order) relative to a base. */
@@ -1076,6 +1088,239 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
         }
     }
     free(woff);
+    free(buf);
+    return ORA_OK;
+}
+
+/* Byte offsets of W's elements (iterator order) in the mask operand. */
+static void w_mask_offsets(const ora_desc *m, const red_plan *p, int64_t *off)
+{
+    int64_t idx[ORA_MAXDIM], e;
+    int k;
+    memset(idx, 0, sizeof idx);
+    for (e = 0; e < p->wsize; e++) {
+        int64_t o = 0;
+        for (k = 0; k < p->nw; k++) o += idx[k] * m->strides[p->w_axes[k]];
+        off[e] = o;
+        for (k = 0; k < p->nw; k++) {
+            if (++idx[k] < m->shape[p->w_axes[k]]) break;
+            idx[k] = 0;
+        }
+    }
+}
+
+/* Strided (non-SIMD) maximum/minimum inner loop on n elements at byte
+ * offsets off[]: 8 scalar accumulators over blocks of 8 combined in order
+ * and folded into p, then the n % 8 tail in order (the model ora_reduce
+ * uses for windows read in place with a stride other than 1). */
+static double strided_ext_f64(int ismax, double p, const char *pb, const int64_t *off, int64_t n)
+{
+    int64_t nb = n >= 8 ? n - n % 8 : 0, e;
+    if (nb) {
+        double m[8], t;
+        int j;
+        for (j = 0; j < 8; j++) m[j] = *(const double *)(pb + off[j]);
+        for (e = 8; e < nb; e += 8)
+            for (j = 0; j < 8; j++) {
+                double v = *(const double *)(pb + off[e + j]);
+                m[j] = ismax ? f64_max(m[j], v) : f64_min(m[j], v);
+            }
+        t = m[0];
+        for (j = 1; j < 8; j++) t = ismax ? f64_max(t, m[j]) : f64_min(t, m[j]);
+        p = ismax ? f64_max(p, t) : f64_min(p, t);
+    }
+    for (e = nb; e < n; e++) {
+        double v = *(const double *)(pb + off[e]);
+        p = ismax ? f64_max(p, v) : f64_min(p, v);
+    }
+    return p;
+}
+static float strided_ext_f32(int ismax, float p, const char *pb, const int64_t *off, int64_t n)
+{
+    int64_t nb = n >= 8 ? n - n % 8 : 0, e;
+    if (nb) {
+        float m[8], t;
+        int j;
+        for (j = 0; j < 8; j++) m[j] = *(const float *)(pb + off[j]);
+        for (e = 8; e < nb; e += 8)
+            for (j = 0; j < 8; j++) {
+                float v = *(const float *)(pb + off[e + j]);
+                m[j] = ismax ? f32_max(m[j], v) : f32_min(m[j], v);
+            }
+        t = m[0];
+        for (j = 1; j < 8; j++) t = ismax ? f32_max(t, m[j]) : f32_min(t, m[j]);
+        p = ismax ? f32_max(p, t) : f32_min(p, t);
+    }
+    for (e = nb; e < n; e++) {
+        float v = *(const float *)(pb + off[e]);
+        p = ismax ? f32_max(p, v) : f32_min(p, v);
+    }
+    return p;
+}
+
+/* reduce_axis with a where= mask (ufunc.rs:7172-7540 reduce_axis_masked),
+ * as NumPy 2.2.6 evaluates it: the fold starts from initial= (else the
+ * op's identity: add 0, multiply 1; maximum/minimum without initial= are
+ * an error whatever the sizes), the mask takes part in the iterator's axis
+ * order, and the reduce inner loop is called once per maximal run of True
+ * mask values inside each `bufsize` window (NumPy's masked-loop wrapper):
+ *   float add       acc = acc + pw(run)          (per run, not per window)
+ *   float multiply  acc = acc * x, every selected x in order
+ *   float max/min   the window's SIMD (or strided) loop model on the run
+ *   integers        order-free fold of the selected elements
+ * W empty: acc = op(acc, x) for each selected x in iterator order.
+ * `mask` is a bool operand broadcast to in's shape (zero strides allowed).
+ * The reference scans runs over whole W blocks (no bufsize windows); the
+ * oracle follows NumPy, as for the unmasked fold. */
+int ora_reduce_where(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc *mask, const ora_desc *out,
+                     int64_t bufsize, const void *initial)
+{
+    red_plan p;
+    int n = in->ndim, k, nk = 0, kept[ORA_MAXDIM];
+    int isz = ITEMSIZE[in->dtype], cdt = out->dtype, ismax = op == OP_MAXIMUM;
+    int64_t *woff, *moff, okidx[ORA_MAXDIM], nout, o;
+    char *buf;
+    if (op != OP_ADD && op != OP_MULTIPLY && op != OP_MAXIMUM && op != OP_MINIMUM) return ORA_ENOLOOP;
+    if (mask->dtype != DT_BOOL || mask->ndim != n) return ORA_ETYPE;
+    if (is_float(in->dtype) && !is_float(cdt)) return ORA_ETYPE;
+    if (is_float(cdt) && in->dtype != cdt) return ORA_EVALUE; /* float loops read the input as is */
+    if ((op == OP_MAXIMUM || op == OP_MINIMUM) && !initial) return ORA_EVALUE; /* no identity */
+    make_red_plan_m(in, mask, axmask, &p);
+    for (k = 0; k < n; k++)
+        if (!axmask[k]) kept[nk++] = k;
+    nout = 1;
+    for (k = 0; k < nk; k++) nout *= in->shape[kept[k]];
+    if (in->ndim && shape_size(n, in->shape) == 0) p.wsize = 0;
+    woff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(p.wsize > 0 ? p.wsize : 1));
+    moff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(p.wsize > 0 ? p.wsize : 1));
+    buf = (char *)malloc((size_t)(bufsize > 0 ? bufsize : 1) * 8);
+    if (p.nw && p.wsize) {
+        w_offsets(in, &p, woff);
+        w_mask_offsets(mask, &p, moff);
+    }
+    memset(okidx, 0, sizeof okidx);
+    for (o = 0; o < nout; o++) {
+        int64_t base = 0, mbase = 0, oidx[ORA_MAXDIM], noutr = 1, r, ooff = 0;
+        char *po;
+        double a64;
+        float a32;
+        uint64_t ua;
+        int64_t sa;
+        int uns = cdt == DT_BOOL || is_unsigned_int(cdt);
+        for (k = 0; k < nk; k++) ooff += okidx[k] * out->strides[k];
+        po = (char *)out->data + ooff * ITEMSIZE[cdt];
+        for (k = 0; k < nk; k++) {
+            base += okidx[k] * in->strides[kept[k]];
+            mbase += okidx[k] * mask->strides[kept[k]];
+        }
+        for (k = 0; k < p.nouter; k++) noutr *= in->shape[p.outer_axes[k]];
+        if (shape_size(n, in->shape) == 0) noutr = 0;
+        /* seed: initial=, else the identity */
+        a64 = op == OP_MULTIPLY ? 1.0 : 0.0;
+        a32 = op == OP_MULTIPLY ? 1.0f : 0.0f;
+        ua = op == OP_MULTIPLY ? 1 : 0;
+        sa = op == OP_MULTIPLY ? 1 : 0;
+        if (initial) {
+            if (cdt == DT_F64) a64 = *(const double *)initial;
+            else if (cdt == DT_F32) a32 = *(const float *)initial;
+            else {
+                ua = (uint64_t)load_i(cdt, (const char *)initial);
+                sa = load_i(cdt, (const char *)initial);
+            }
+        }
+        memset(oidx, 0, sizeof oidx);
+        for (r = 0; r < noutr; r++) {
+            int64_t ob = base, mb = mbase, s, e;
+            const char *pb;
+            const uint8_t *mp;
+            for (k = 0; k < p.nouter; k++) {
+                ob += oidx[k] * in->strides[p.outer_axes[k]];
+                mb += oidx[k] * mask->strides[p.outer_axes[k]];
+            }
+            pb = (const char *)in->data + ob * isz;
+            mp = (const uint8_t *)mask->data + mb;
+            for (s = 0; s < (p.nw ? p.wsize : 1); s += bufsize) {
+                int64_t len = p.nw ? (p.wsize - s < bufsize ? p.wsize - s : bufsize) : 1, a = 0, b;
+                int contig;
+                if (!p.nw) contig = 1;
+                else if ((s / p.w_inner_len) == ((s + len - 1) / p.w_inner_len)) contig = p.w_run_stride == 1;
+                else contig = 1;
+                while (a < len) {
+                    const int64_t *wo = p.nw ? woff + s + a : NULL;
+                    static const int64_t zero = 0;
+                    if (!mp[p.nw ? moff[s + a] : 0]) { a++; continue; }
+                    b = a + 1;
+                    while (b < len && mp[moff[s + b]]) b++;
+                    if (!wo) wo = &zero;
+                    /* one call of the inner loop on elements [a, b) */
+                    if (cdt == DT_F64 || cdt == DT_F32) {
+                        if (op == OP_ADD && p.nw) {
+                            if (cdt == DT_F64) a64 = f64_arith(OP_ADD, a64, pw_f64(pb, wo, b - a));
+                            else a32 = f32_arith(OP_ADD, a32, pw_f32(pb, wo, b - a));
+                        } else if (op == OP_ADD || op == OP_MULTIPLY) {
+                            for (e = 0; e < b - a; e++) {
+                                if (cdt == DT_F64) a64 = f64_arith(op, a64, *(const double *)(pb + wo[e]));
+                                else a32 = f32_arith(op, a32, *(const float *)(pb + wo[e]));
+                            }
+                        } else if (!p.nw) { /* W empty: the elementwise loop, scalar max/min */
+                            if (cdt == DT_F64) {
+                                double v = *(const double *)pb;
+                                a64 = ismax ? f64_max(a64, v) : f64_min(a64, v);
+                            } else {
+                                float v = *(const float *)pb;
+                                a32 = ismax ? f32_max(a32, v) : f32_min(a32, v);
+                            }
+                        } else if (contig) {
+                            for (e = 0; e < b - a; e++) memcpy(buf + e * isz, pb + wo[e], isz);
+                            if (cdt == DT_F64) a64 = simd_ext_f64(ismax, a64, (const double *)buf, b - a);
+                            else a32 = simd_ext_f32(ismax, a32, (const float *)buf, b - a);
+                        } else {
+                            if (cdt == DT_F64) a64 = strided_ext_f64(ismax, a64, pb, wo, b - a);
+                            else a32 = strided_ext_f32(ismax, a32, pb, wo, b - a);
+                        }
+                    } else {
+                        for (e = 0; e < b - a; e++) {
+                            char tmp[8];
+                            cast_elem(in->dtype, pb + wo[e], cdt, tmp);
+                            if (uns) {
+                                uint64_t v = (uint64_t)load_i(cdt, tmp);
+                                ua = op == OP_ADD ? ua + v : op == OP_MULTIPLY ? ua * v
+                                   : op == OP_MAXIMUM ? (v > ua ? v : ua) : (v < ua ? v : ua);
+                            } else {
+                                int64_t v = load_i(cdt, tmp);
+                                sa = op == OP_ADD ? (int64_t)((uint64_t)sa + (uint64_t)v)
+                                   : op == OP_MULTIPLY ? (int64_t)((uint64_t)sa * (uint64_t)v)
+                                   : op == OP_MAXIMUM ? (v > sa ? v : sa) : (v < sa ? v : sa);
+                            }
+                        }
+                    }
+                    a = b;
+                }
+            }
+            for (k = 0; k < p.nouter; k++) {
+                if (++oidx[k] < in->shape[p.outer_axes[k]]) break;
+                oidx[k] = 0;
+            }
+        }
+        if (cdt == DT_F64) *(double *)po = a64;
+        else if (cdt == DT_F32) *(float *)po = a32;
+        else {
+            uint64_t acc = uns ? ua : (uint64_t)sa;
+            if (cdt == DT_BOOL) acc = acc != 0; /* add: or; multiply: and */
+            switch (ITEMSIZE[cdt]) {
+            case 1: *(uint8_t *)po = (uint8_t)acc; break;
+            case 2: *(uint16_t *)po = (uint16_t)acc; break;
+            case 4: *(uint32_t *)po = (uint32_t)acc; break;
+            default: *(uint64_t *)po = acc; break;
+            }
+        }
+        for (k = nk - 1; k >= 0; k--) {
+            if (++okidx[k] < in->shape[kept[k]]) break;
+            okidx[k] = 0;
+        }
+    }
+    free(woff);
+    free(moff);
     free(buf);
     return ORA_OK;
 }

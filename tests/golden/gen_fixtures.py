@@ -576,6 +576,116 @@ def prod_cases(S):
     S.add("cumprod/float64/None", "accumulate", "cumprod", [K.T], lambda: np.cumprod(K.T), {"axis": None})
 
 
+def where_reduce_cases(S):
+    """reduce with a where= mask (reduce_axis_masked, ufunc.rs:7172, as
+    NumPy 2.2.6 evaluates it): per bufsize window, the reduce loop runs once
+    per maximal run of True mask values; the mask joins the iterator's axis
+    order.  Runs crossing window boundaries, broadcast and F-ordered masks,
+    gapped axes, +-0 ties and NaN for maximum/minimum, integer wrap, empty
+    groups and the no-identity error.  Own generator (seed 82)."""
+    r7 = np.random.default_rng(82)
+
+    def red(op, x, ax, m, init=None, **kw):
+        if init is not None:
+            kw["initial"] = init
+        return getattr(np, op).reduce(x, axis=ax, where=m, **kw)
+
+    def args(ax, init=None, **kw):
+        a = {"axis": list(ax) if isinstance(ax, tuple) else ax, "where": True}
+        if init is not None:
+            a["initial"] = init
+        a.update(kw)
+        return a
+
+    def fvals(shape, dt, op):
+        if op == "multiply":  # near 1, one NaN payload only (see prod_cases)
+            x = (1 + r7.standard_normal(shape) * 0.02).astype(dt)
+            x[r7.random(shape) < 0.002] = np.nan
+            return x
+        x = r7.standard_normal(shape).astype(dt)
+        u = r7.random(shape)
+        x[u < 0.04] = 0.0
+        x[(u >= 0.04) & (u < 0.08)] = -0.0
+        x[(u >= 0.08) & (u < 0.085)] = np.nan
+        return x
+
+    fops = [("add", None), ("multiply", None), ("maximum", 0.25), ("minimum", -0.25)]
+    for dt in (np.float64, np.float32):
+        nm = np.dtype(dt).name
+        for op, init in fops:
+            x = fvals(20000, dt, op)
+            for p in (0.0, 0.3, 0.97, 1.0):
+                m = r7.random(20000) < p
+                S.add(f"wred/{op}/{nm}/1d/{p}", "reduce", op, [x, m], lambda: red(op, x, 0, m, init), args(0, init))
+            m = r7.random(20000) < 0.9
+            S.add(f"wred/{op}/{nm}/1d/rev", "reduce", op, [x[::-1], m], lambda: red(op, x[::-1], 0, m, init),
+                  args(0, init))
+            S.add(f"wred/{op}/{nm}/1d/step", "reduce", op, [x[::2], m[:10000]],
+                  lambda: red(op, x[::2], 0, m[:10000], init), args(0, init))
+            X = fvals((120, 200), dt, op)
+            Xf = np.asfortranarray(X)
+            mc = r7.random((120, 200)) < 0.8
+            mf = np.asfortranarray(r7.random((120, 200)) < 0.8)
+            mrow = r7.random(200) < 0.7
+            for ax in (None, 0, 1):
+                S.add(f"wred/{op}/{nm}/C/{ax}", "reduce", op, [X, mc], lambda: red(op, X, ax, mc, init), args(ax, init))
+                S.add(f"wred/{op}/{nm}/xF-mC/{ax}", "reduce", op, [Xf, mc], lambda: red(op, Xf, ax, mc, init),
+                      args(ax, init))
+                S.add(f"wred/{op}/{nm}/xC-mF/{ax}", "reduce", op, [X, mf], lambda: red(op, X, ax, mf, init),
+                      args(ax, init))
+                S.add(f"wred/{op}/{nm}/row/{ax}", "reduce", op, [X, mrow], lambda: red(op, X, ax, mrow, init),
+                      args(ax, init))
+            S.add(f"wred/{op}/{nm}/step/1", "reduce", op, [X[:, ::2], mc[:, :100]],
+                  lambda: red(op, X[:, ::2], 1, mc[:, :100], init), args(1, init))
+            S.add(f"wred/{op}/{nm}/keep", "reduce", op, [X, mrow], lambda: red(op, X, 1, mrow, init, keepdims=True),
+                  args(1, init, keepdims=True))
+            if op in ("add", "maximum"):  # rows longer than one window
+                W = fvals((6, 9000), dt, op)
+                mw = r7.random((6, 9000)) < 0.95
+                S.add(f"wred/{op}/{nm}/wide/1", "reduce", op, [W, mw], lambda: red(op, W, 1, mw, init), args(1, init))
+                S.add(f"wred/{op}/{nm}/wide/None", "reduce", op, [W, mw], lambda: red(op, W, None, mw, init),
+                      args(None, init))
+            G = fvals((5, 7, 300), dt, op)
+            mg = r7.random((5, 7, 300)) < 0.9
+            S.add(f"wred/{op}/{nm}/gapped", "reduce", op, [G, mg], lambda: red(op, G, (0, 2), mg, init),
+                  args((0, 2), init))
+        # +-0 ties and NaN payloads in short and long runs
+        for op in ("maximum", "minimum"):
+            for k in range(4):
+                z = r7.choice(np.array([0.0, -0.0, 1.0, -1.0, np.nan, 2.0], dtype=dt), 3000)
+                if k == 3:
+                    z = r7.choice(np.array([0.0, -0.0], dtype=dt), 3000)
+                mz = r7.random(3000) < (0.5, 0.9, 0.99, 0.95)[k]
+                for iv in (0.0, -0.0, float("nan")) if k == 3 else (0.5,):
+                    S.add(f"wred/{op}/{nm}/ties/{k}/{iv}", "reduce", op, [z, mz], lambda: red(op, z, 0, mz, iv),
+                          args(0, iv))
+        x = fvals(100, dt, "add")
+        mi = r7.random(100) < 0.5
+        S.add(f"wred/add/{nm}/initial", "reduce", "add", [x, mi], lambda: red("add", x, 0, mi, 3.0), args(0, 3.0))
+        E = np.zeros((0, 4), dtype=dt)
+        me = np.ones(4, dtype=bool)
+        S.add(f"wred/add/{nm}/empty", "reduce", "add", [E, me], lambda: red("add", E, 0, me), args(0))
+        S.add(f"wred/multiply/{nm}/empty", "reduce", "multiply", [E, me], lambda: red("multiply", E, 0, me), args(0))
+        S.add(f"wred/maximum/{nm}/empty", "reduce", "maximum", [E, me], lambda: red("maximum", E, 0, me, 1.5),
+              args(0, 1.5))
+        S.add(f"wred/maximum/{nm}/noinit", "reduce", "maximum", [x, mi], lambda: red("maximum", x, 0, mi),
+              args(0))
+    for dt in (np.int64, np.int32, np.uint8, np.int8, np.bool_):
+        nm = np.dtype(dt).name
+        x = specials(dt, 30 * 40, r7).reshape(30, 40)
+        m = r7.random((30, 40)) < 0.6
+        for op in ("add", "multiply", "maximum", "minimum"):
+            init = 1 if op in ("maximum", "minimum") else None
+            for ax in (None, 0, 1):
+                S.add(f"wred/{op}/{nm}/{ax}", "reduce", op, [x, m], lambda: red(op, x, ax, m, init), args(ax, init))
+            S.add(f"wred/{op}/{nm}/T", "reduce", op, [x.T, m.T], lambda: red(op, x.T, 1, m.T, init), args(1, init))
+        big = specials(dt, 70000, r7)
+        mb = r7.random(70000) < 0.9
+        S.add(f"wred/add/{nm}/big", "reduce", "add", [big, mb], lambda: red("add", big, 0, mb), args(0))
+        S.add(f"wred/add/{nm}/int8dt", "reduce", "add", [x, m], lambda: red("add", x, 1, m, dtype=np.int8),
+              args(1, dtype="int8"))
+
+
 def argext_cases(S, rng):
     for dt in (np.float64, np.float32, np.int64, np.int32, np.uint8, np.bool_):
         nm = np.dtype(dt).name
@@ -652,6 +762,7 @@ def main():
     argext_cases(S, rng)
     accumulate_cases(S)
     prod_cases(S)
+    where_reduce_cases(S)
     np.savez_compressed(os.path.join(HERE, "fixtures.npz"), **S.arrays)
     with open(os.path.join(HERE, "cases.json"), "w") as f:
         json.dump(S.cases, f)

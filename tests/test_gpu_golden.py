@@ -35,7 +35,8 @@ def run_device(ap, case, npz, cache):
         return ins[0].astype(np.dtype(args["dtype"]))
     if k == "reduce":
         return getattr(ap, op).reduce(ins[0], axis=args_axis(args), dtype=args.get("dtype"),
-                                      keepdims=bool(args.get("keepdims", False)), initial=args.get("initial"))
+                                      keepdims=bool(args.get("keepdims", False)), initial=args.get("initial"),
+                                      where=ins[1] if args.get("where") else True)
     if k == "accumulate":
         if op in ("cumsum", "cumprod"):
             return getattr(ap, op)(ins[0], axis=args.get("axis"))

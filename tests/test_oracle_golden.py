@@ -21,7 +21,8 @@ def run_oracle(case, npz):
         x = ins[0]
         if args.get("dtype") and np.dtype(args["dtype"]).kind == "f":
             x = O.cast(x, np.dtype(args["dtype"]))  # NumPy's buffered cast, then the float loop
-        r = O.reduce(op, x, args_axis(args), dtype=args.get("dtype"), initial=args.get("initial"))
+        r = O.reduce(op, x, args_axis(args), dtype=args.get("dtype"), initial=args.get("initial"),
+                     where=ins[1] if args.get("where") else None)
         if args.get("keepdims"):
             ax = args_axis(args)
             axes = range(x.ndim) if ax is None else ((ax,) if isinstance(ax, int) else ax)
