@@ -590,6 +590,15 @@ class Ufunc:
             raise NotImplementedError(f"{self.__name__}.accumulate is not provided by the MI355X backend")
         return _accumulate(self._code, asarray(a), axis, dtype, out)
 
+    def reduceat(self, a, indices, axis=0, dtype=None, out=None):
+        """ufunc.reduceat (ionp-py/src/lib.rs:10127 -> reduceat_binary_axis,
+        ionp-core/src/ufunc.rs:8460): per segment [indices[k], indices[k+1])
+        of `axis` the reduction seeded with its first element; a
+        non-increasing pair gives a[indices[k]]."""
+        if not self._reducible:
+            raise NotImplementedError(f"{self.__name__}.reduceat is not provided by the MI355X backend")
+        return _reduceat(self._code, asarray(a), indices, axis, dtype, out)
+
     def reduce(self, a, axis=0, dtype=None, out=None, keepdims=False, initial=None, where=True):
         """ufunc.reduce (lib.rs:9958) -> reduce_axis (ionp-core/src/ufunc.rs:7634).
         initial=: the fold's starting value (also the result of an empty
@@ -598,6 +607,30 @@ class Ufunc:
         if not self._reducible:
             raise NotImplementedError(f"{self.__name__}.reduce is not provided by the MI355X backend")
         return _reduce(self._code, asarray(a), axis, dtype, keepdims, out, initial, where)
+
+
+def _reduceat(code, a, indices, axis, dtype, out):
+    L = _lib.load()
+    idx = np.ascontiguousarray(np.asarray(indices.get() if isinstance(indices, ndarray) else indices))
+    if idx.dtype.kind not in "iu" and idx.size:
+        raise TypeError("Cannot cast array data from dtype('%s') to dtype('int64') according to the rule 'safe'"
+                        % idx.dtype)
+    idx = np.ascontiguousarray(idx.astype(np.int64).reshape(-1))
+    d = a._desc()
+    t = _lib.ArrayDesc()
+    dt = -1 if dtype is None else _CODE[_dtype(dtype)]
+    _call(L.ionp_plan_reduceat, code, ctypes.byref(d), int(axis), len(idx), dt, ctypes.byref(t))
+    final = out
+    if out is None or _needs_temp(out, [a]):
+        out = empty_strided(tuple(t.shape[: t.ndim]), tuple(t.strides[: t.ndim]),
+                            _NP_DTYPES[t.dtype] if final is None else final.dtype)
+    do = out._desc()
+    _call(L.ionp_reduceat, code, ctypes.byref(d), idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(idx),
+          int(axis), ctypes.byref(do), None, _lib.stream_handle())
+    if final is not None and final is not out:
+        _copy_into(out, final)
+        return final
+    return out
 
 
 def _accumulate(code, a, axis, dtype, out):

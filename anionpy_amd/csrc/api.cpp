@@ -408,6 +408,88 @@ ionp_status ionp_accumulate(int32_t op, const ionp_array_desc *in, int32_t axis,
     return run_accumulate(op, &c, ax, out, cfg, st);
 }
 
+// reduceat's shape/dtype half (ionp-py lib.rs:10127 -> reduceat_binary_axis,
+// ufunc.rs:8460): axis normalised, output dtype by the reduce rule, layout
+// in NumPy's K order over (input without `axis`, indices along `axis`)
+static ionp_status rat_plan(int32_t op, const ionp_array_desc *in, int32_t axis, int64_t nidx, int32_t dtype,
+                            ionp_array_desc *out_template, int *ax_out) {
+    ionp_status s;
+    if ((s = check_desc(in, "array"))) return s;
+    if (in->ndim == 0) return set_error(IONP_ERR_TYPE, "cannot reduceat on a scalar");
+    int ax;
+    if ((s = acc_axis(in, axis, &ax))) return s;
+    if (nidx < 0) return set_error(IONP_ERR_VALUE, "negative number of reduceat indices");
+    int odt;
+    if ((s = reduce_dtype(op, in->dtype, dtype, &odt))) return s;
+    // a float loop reads the input as is (a cast would go through NumPy's
+    // buffers and change the pairwise tree)
+    if (is_float_dt(odt) && odt != in->dtype)
+        return set_error(IONP_ERR_UNSUPPORTED, "float reduceat with a dtype other than the input's");
+    int64_t shape[IONP_MAX_DIMS], st[2][IONP_MAX_DIMS], ost[IONP_MAX_DIMS];
+    for (int k = 0; k < in->ndim; k++) {
+        shape[k] = k == ax ? nidx : in->shape[k];
+        st[0][k] = k == ax ? 0 : in->strides[k];
+        st[1][k] = k == ax ? 1 : 0;
+    }
+    korder_strides(2, st, in->ndim, shape, ost);
+    fill_template(out_template, odt, in->ndim, shape, ost);
+    *ax_out = ax;
+    return IONP_OK;
+}
+
+ionp_status ionp_plan_reduceat(int32_t op, const ionp_array_desc *in, int32_t axis, int64_t nidx, int32_t dtype,
+                               ionp_array_desc *out_template) {
+    int ax;
+    return rat_plan(op, in, axis, nidx, dtype, out_template, &ax);
+}
+
+ionp_status ionp_reduceat(int32_t op, const ionp_array_desc *in, const int64_t *indices, int64_t nidx, int32_t axis,
+                          const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
+    if (ionp_status cs = check_cfg(cfg)) return cs;
+    ionp_status s;
+    ionp_array_desc t;
+    if ((s = check_desc(out, "out"))) return s;
+    if (in->ndim == 0) return set_error(IONP_ERR_TYPE, "cannot reduceat on a scalar");
+    int ax;
+    if ((s = acc_axis(in, axis, &ax))) return s;
+    // indices in call order, before any loop question (validate_reduceat_indices, ufunc.rs:8426)
+    const int64_t n = in->shape[ax];
+    if (nidx > 0 && !indices) return set_error(IONP_ERR_VALUE, "reduceat indices missing");
+    for (int64_t i = 0; i < nidx; i++)
+        if (indices[i] < 0 || indices[i] >= n)
+            return set_error(IONP_ERR_INDEX, "index " + std::to_string(indices[i]) + " out-of-bounds in " +
+                                                 op_name(op) + ".reduceat [0, " + std::to_string(n) + ")");
+    if ((s = rat_plan(op, in, axis, nidx, out->dtype, &t, &ax))) return s;
+    if ((s = same_shape(out, t.ndim, t.shape, "reduceat"))) return s;
+    RatArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.op = op;
+    A.in_dt = in->dtype;
+    A.out_dt = out->dtype;
+    A.len = n;
+    A.in_st = in->strides[ax];
+    A.out_st = out->strides[ax];
+    A.nidx = nidx;
+    A.nfib = 1;
+    // fiber dims innermost first by the input's strides
+    int64_t st[1][IONP_MAX_DIMS];
+    std::memcpy(st[0], in->strides, sizeof(int64_t) * in->ndim);
+    int perm[IONP_MAX_DIMS];
+    iter_axis_order(in->ndim, in->shape, 1, st, perm);
+    for (int k = 0; k < in->ndim; k++) {
+        const int d = perm[k];
+        if (d == ax || in->shape[d] == 1) continue;
+        A.fshape[A.nf] = in->shape[d];
+        A.fin[A.nf] = in->strides[d];
+        A.fout[A.nf] = out->strides[d];
+        A.nf++;
+        A.nfib *= in->shape[d];
+    }
+    A.in = elem0(in);
+    A.out = (char *)elem0(out);
+    return run_reduceat(A, indices, cfg, as_stream(stream));
+}
+
 ionp_status ionp_argext(int32_t ismax, const ionp_array_desc *in, int32_t axis, int32_t keepdims,
                         const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
     if (ionp_status cs = check_cfg(cfg)) return cs;

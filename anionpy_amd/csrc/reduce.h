@@ -72,6 +72,23 @@ struct ArgArgs {
 
 ionp_status run_argext(ArgArgs &a, const ionp_config *cfg, hipStream_t s);
 
+// ufunc.reduceat along one axis: output (fiber f, k) reduces the axis
+// segment that starts at indices[k] (see run_reduceat).
+struct RatArgs {
+    int32_t op, in_dt, out_dt, nf;
+    int64_t len, in_st, out_st;  // the axis: extent and strides (elements)
+    int64_t nfib, nidx;
+    int64_t fshape[IONP_MAX_DIMS];  // the other axes, innermost first
+    FastDiv fdiv[IONP_MAX_DIMS];
+    int64_t fin[IONP_MAX_DIMS], fout[IONP_MAX_DIMS];
+    const int64_t *idx;  // device copy of the indices
+    const char *in;
+    char *out;
+};
+
+// indices: host array of nidx entries, each in [0, len) (checked by the caller)
+ionp_status run_reduceat(RatArgs &a, const int64_t *indices, const ionp_config *cfg, hipStream_t s);
+
 // device scratch owned by the library (grown on demand, stream-ordered use)
 void *scratch_get(size_t bytes, hipStream_t s);
 

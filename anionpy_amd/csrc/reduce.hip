@@ -1894,4 +1894,220 @@ ionp_status run_argext(ArgArgs &A, const ionp_config *cfg, hipStream_t s) {
     return A.ismax ? run_arg_m<true>(A, cfg, s) : run_arg_m<false>(A, cfg, s);
 }
 
+// ---------------------------------------------------------------------------
+// ufunc.reduceat (reduceat_binary_axis, ufunc.rs:8298-8460) as NumPy 2.2.6
+// evaluates it: output (f, k) is x[i_k] when i_{k+1} <= i_k (the last index
+// runs to the axis end), else the reduce loop seeded with x[i_k] and called
+// once on the rest [i_k + 1, i_{k+1}) -- no buffer windows:
+//   float add       x[i_k] + pairwise(rest), the whole rest one tree
+//   float multiply  sequential product (rat_prod)
+//   float max/min   the SIMD loop model on a contiguous rest, the strided
+//                   loop model otherwise
+//   integers        order-free, in the output dtype
+// ---------------------------------------------------------------------------
+__device__ inline void rat_offsets(const RatArgs &A, uint32_t f, int64_t *ib, int64_t *ob) {
+    int64_t a = 0, b = 0;
+    for (int d = 0; d < A.nf; d++) {
+        uint32_t c;
+        if (d + 1 < A.nf) {
+            const uint32_t q = A.fdiv[d].div(f);
+            c = f - q * (uint32_t)A.fshape[d];
+            f = q;
+        } else {
+            c = f;
+        }
+        a += (int64_t)c * A.fin[d];
+        b += (int64_t)c * A.fout[d];
+    }
+    *ib = a;
+    *ob = b;
+}
+__device__ inline int64_t pw_split64(int64_t n) { return (n >> 1) & ~(int64_t)7; }
+
+// pairwise(x[0..n)) with the workgroup for any n: nodes of at most 8192
+// elements go through pairwise_block in tree order, the nodes above them
+// are combined on a stack in LDS (written by thread 0; every thread runs
+// the same walk).  Depth <= 32 for n < 2^44.
+template <typename T> struct PwStack {
+    int64_t lo[32], n[32];
+    T left[32];
+    int state[32];  // 0: left child pending, 1: left value held
+};
+template <typename T, typename Get>
+__device__ T pairwise_any(const Get &get, int64_t n, T *racc, T *val, int *leaf_lo, int *leaf_n, int *cnt,
+                          PwStack<T> &S) {
+    const bool t0 = threadIdx.x == 0;
+    int sp = 0;
+    if (t0) {
+        S.lo[0] = 0;
+        S.n[0] = n;
+        S.state[0] = 0;
+    }
+    __syncthreads();
+    T ret;
+    for (;;) {
+        const int64_t lo = S.lo[sp], m = S.n[sp];
+        if (m <= kPwMaxWin) {
+            ret = pairwise_block<T>([&](int i) { return get(lo + i); }, (int)m, racc, val, leaf_lo, leaf_n, cnt);
+            __syncthreads();  // every thread has read the result before the scratch is reused
+            for (;;) {  // ret: the value of node sp
+                if (sp == 0) return ret;
+                --sp;
+                if (S.state[sp] == 0) {  // left child done: keep it, descend into the right one
+                    const int64_t plo = S.lo[sp], pn = S.n[sp];
+                    __syncthreads();
+                    if (t0) {
+                        const int64_t h = pw_split64(pn);
+                        S.left[sp] = ret;
+                        S.state[sp] = 1;
+                        S.lo[sp + 1] = plo + h;
+                        S.n[sp + 1] = pn - h;
+                        S.state[sp + 1] = 0;
+                    }
+                    __syncthreads();
+                    ++sp;
+                    break;
+                }
+                ret = fadd(S.left[sp], ret);
+            }
+        } else {
+            __syncthreads();
+            if (t0) {
+                S.lo[sp + 1] = lo;
+                S.n[sp + 1] = pw_split64(m);
+                S.state[sp + 1] = 0;
+            }
+            __syncthreads();
+            ++sp;
+        }
+    }
+}
+
+// a workgroup per output (f, k); LT: the output dtype
+template <int LT, int OP> __global__ __launch_bounds__(256) void rat_block(const RatArgs A) {
+    using T = typename dt_type<LT>::T;
+    constexpr bool FLT = std::is_floating_point<T>::value;
+    constexpr bool PW = FLT && OP == IONP_ADD;
+    constexpr bool EXT = FLT && (OP == IONP_MAXIMUM || OP == IONP_MINIMUM);
+    using AT = typename std::conditional<FLT, T,
+                                         typename std::conditional<std::is_signed<T>::value, int64_t, uint64_t>::type>::type;
+    __shared__ T racc[PW ? 1024 : 1], pval[PW ? 1024 : 1];
+    __shared__ int leaf_lo[PW ? 128 : 1], leaf_n[PW ? 128 : 1], lcnt[1];
+    __shared__ PwStack<T> pstack;
+    __shared__ ExtCand<T> red[EXT ? 256 : 1];
+    __shared__ AT ired[(!FLT) ? 256 : 1];
+    const int tid = threadIdx.x;
+    const int64_t nout = A.nfib * A.nidx;
+    for (int64_t g = blockIdx.x; g < nout; g += gridDim.x) {
+        const int64_t f = g / A.nidx, k = g - f * A.nidx;
+        int64_t ib, ob;
+        rat_offsets(A, (uint32_t)f, &ib, &ob);
+        const int64_t a = A.idx[k], b = k + 1 < A.nidx ? A.idx[k + 1] : A.len;
+        const int64_t m = b > a ? b - a - 1 : 0;
+        const T seed = ld<LT>(A.in_dt, A.in, ib + a * A.in_st);
+        auto get = [&](int64_t i) { return ld<LT>(A.in_dt, A.in, ib + (a + 1 + i) * A.in_st); };
+        T res = seed;
+        if (m > 0) {
+            if constexpr (PW) {
+                res = fadd(seed, pairwise_any<T>(get, m, racc, pval, leaf_lo, leaf_n, lcnt, pstack));
+            } else if constexpr (EXT) {
+                constexpr bool MAX = OP == IONP_MAXIMUM;
+                const ExtGeom<T> geo = ext_geom<T>(m, A.in_st == 1);
+                ExtCand<T> c;
+                ext_cand_init(c);
+                for (int64_t e = tid; e < m; e += blockDim.x) ext_feed<T, MAX>(c, geo, e, get(e));
+                red[tid] = c;
+                __syncthreads();
+                for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+                    if (tid < s2) {
+                        ExtCand<T> x = red[tid];
+                        cand_merge<T, MAX>(x, red[tid + s2]);
+                        red[tid] = x;
+                    }
+                    __syncthreads();
+                }
+                res = ext_apply<T, MAX>(ext_finish(red[0], geo), seed);
+                __syncthreads();
+            } else if constexpr (!FLT) {
+                AT acc = int_identity<AT, OP>();
+                for (int64_t e = tid; e < m; e += blockDim.x) acc = assoc<AT, OP>(acc, (AT)get(e));
+                ired[tid] = acc;
+                __syncthreads();
+                for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+                    if (tid < s2) ired[tid] = assoc<AT, OP>(ired[tid], ired[tid + s2]);
+                    __syncthreads();
+                }
+                res = int_result<LT>(assoc<AT, OP>((AT)seed, ired[0]));
+                __syncthreads();
+            }
+        }
+        if (tid == 0) ((T *)A.out)[ob + k * A.out_st] = res;
+    }
+}
+
+// float multiply: a thread per output, the sequential product (plain IEEE
+// multiplies, redone with fmul's x86 NaN rules when the result is NaN)
+template <typename T> __global__ __launch_bounds__(256) void rat_prod(const RatArgs A) {
+    constexpr int LT = FDT<T>::v;
+    const int64_t nout = A.nfib * A.nidx;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nout; g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t f = g / A.nidx, k = g - f * A.nidx;
+        int64_t ib, ob;
+        rat_offsets(A, (uint32_t)f, &ib, &ob);
+        const int64_t a = A.idx[k], b = k + 1 < A.nidx ? A.idx[k + 1] : A.len;
+        const T seed = ld<LT>(A.in_dt, A.in, ib + a * A.in_st);
+        T acc = seed;
+        for (int64_t i = a + 1; i < b; i++) acc = acc * ld<LT>(A.in_dt, A.in, ib + i * A.in_st);
+        if (acc != acc) {
+            acc = seed;
+            for (int64_t i = a + 1; i < b; i++) acc = fmul(acc, ld<LT>(A.in_dt, A.in, ib + i * A.in_st));
+        }
+        ((T *)A.out)[ob + k * A.out_st] = acc;
+    }
+}
+
+template <int LT> static void launch_rat(const RatArgs &A, int bpc, hipStream_t s) {
+    const int64_t nout = A.nfib * A.nidx;
+    const dim3 g(grid_cap(nout, bpc)), b(256);
+    switch (A.op) {
+        case IONP_ADD: hipLaunchKernelGGL((rat_block<LT, IONP_ADD>), g, b, 0, s, A); break;
+        case IONP_MULTIPLY:
+            if constexpr (LT == IONP_FLOAT64 || LT == IONP_FLOAT32)
+                hipLaunchKernelGGL((rat_prod<typename dt_type<LT>::T>), dim3(grid_cap((nout + 255) / 256, bpc)), b, 0, s,
+                                   A);
+            else
+                hipLaunchKernelGGL((rat_block<LT, IONP_MULTIPLY>), g, b, 0, s, A);
+            break;
+        case IONP_MAXIMUM: hipLaunchKernelGGL((rat_block<LT, IONP_MAXIMUM>), g, b, 0, s, A); break;
+        default: hipLaunchKernelGGL((rat_block<LT, IONP_MINIMUM>), g, b, 0, s, A); break;
+    }
+}
+
+ionp_status run_reduceat(RatArgs &A, const int64_t *indices, const ionp_config *cfg, hipStream_t s) {
+    if (A.nfib == 0 || A.nidx == 0) return IONP_OK;
+    if (A.nfib * A.nidx >= (1ll << 40) || A.nfib >= (1ll << 31))
+        return set_error(IONP_ERR_UNSUPPORTED, "reduceat extent too large");
+    for (int d = 0; d < A.nf; d++) A.fdiv[d] = FastDiv((uint32_t)A.fshape[d]);
+    int64_t *didx = (int64_t *)scratch_get((size_t)A.nidx * sizeof(int64_t), s);
+    if (!didx) return set_error(IONP_ERR_DEVICE, "out of device memory for reduceat indices");
+    if (hipMemcpyAsync(didx, indices, (size_t)A.nidx * sizeof(int64_t), hipMemcpyHostToDevice, s) != hipSuccess)
+        return set_error(IONP_ERR_DEVICE, "copying reduceat indices failed");
+    A.idx = didx;
+    const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
+    switch (A.out_dt) {
+        case IONP_BOOL: launch_rat<IONP_BOOL>(A, bpc, s); break;
+        case IONP_INT8: launch_rat<IONP_INT8>(A, bpc, s); break;
+        case IONP_UINT8: launch_rat<IONP_UINT8>(A, bpc, s); break;
+        case IONP_INT16: launch_rat<IONP_INT16>(A, bpc, s); break;
+        case IONP_UINT16: launch_rat<IONP_UINT16>(A, bpc, s); break;
+        case IONP_INT32: launch_rat<IONP_INT32>(A, bpc, s); break;
+        case IONP_UINT32: launch_rat<IONP_UINT32>(A, bpc, s); break;
+        case IONP_INT64: launch_rat<IONP_INT64>(A, bpc, s); break;
+        case IONP_UINT64: launch_rat<IONP_UINT64>(A, bpc, s); break;
+        case IONP_FLOAT32: launch_rat<IONP_FLOAT32>(A, bpc, s); break;
+        default: launch_rat<IONP_FLOAT64>(A, bpc, s); break;
+    }
+    return launched();
+}
+
 }  // namespace ionp

@@ -215,6 +215,21 @@ ionp_status ionp_plan_accumulate(int32_t op, const ionp_array_desc *in, int32_t 
 ionp_status ionp_accumulate(int32_t op, const ionp_array_desc *in, int32_t axis, const ionp_array_desc *out,
                             const ionp_config *cfg, void *stream);
 
+/* Replaces reduceat_binary_axis (ufunc.rs:8460; Python entry
+ * ionp-py/src/lib.rs:10127): along `axis`, output k of every fiber is
+ * x[indices[k]] when indices[k+1] <= indices[k] (the last index runs to
+ * the axis end), else the reduce loop seeded with x[indices[k]] over the
+ * rest of the segment as one call, as NumPy 2.2.6 evaluates it (float
+ * add: x[i] + pairwise(rest), no buffer windows; multiply sequential;
+ * max/min the x86 loop model).  `indices`: nidx host int64 values, each in
+ * [0, n) (else IONP_ERR_INDEX with NumPy's text).  The output template
+ * has the input's shape with `axis` of length nidx, in NumPy's K order.
+ * A float output dtype other than the input's is not provided (21). */
+ionp_status ionp_plan_reduceat(int32_t op, const ionp_array_desc *in, int32_t axis, int64_t nidx, int32_t dtype,
+                               ionp_array_desc *out_template);
+ionp_status ionp_reduceat(int32_t op, const ionp_array_desc *in, const int64_t *indices, int64_t nidx, int32_t axis,
+                          const ionp_array_desc *out, const ionp_config *cfg, void *stream);
+
 /* ---- batching: many small calls, one launch --------------------------- */
 /* No counterpart in the reference (it runs one call at a time). */
 

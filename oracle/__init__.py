@@ -64,6 +64,8 @@ def lib():
                            ("ora_reduce_where", [ctypes.c_int, P, ctypes.c_char_p, P, P, ctypes.c_int64,
                                                  ctypes.c_void_p]),
                            ("ora_binary_omp", [ctypes.c_int, P, P, P, ctypes.POINTER(ctypes.c_int)]),
+                           ("ora_reduceat", [ctypes.c_int, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                             ctypes.c_int64, P]),
                            ("ora_argext", [ctypes.c_int, P, ctypes.c_int, P]),
                            ("ora_accumulate", [ctypes.c_int, P, ctypes.c_int, P])]:
             getattr(L, name).argtypes = args
@@ -251,6 +253,39 @@ def reduce(op, x, axis=0, bufsize=BUFSIZE, dtype=None, initial=None, where=None)
         _check(lib().ora_reduce_where(OP[op], ctypes.byref(dx), mask, ctypes.byref(dm), ctypes.byref(do),
                                       bufsize, ip))
     return out[()] if out.ndim == 0 else out
+
+
+def reduceat(op, x, indices, axis=0, dtype=None):
+    """np.<op>.reduceat(x, indices, axis, dtype) for op in
+    add/multiply/maximum/minimum, with NumPy's output layout."""
+    x = np.asarray(x)
+    if x.ndim == 0:
+        raise OracleError(3)
+    ax = axis + x.ndim if axis < 0 else axis
+    if not 0 <= ax < x.ndim:
+        raise OracleError(6)
+    idx = np.ascontiguousarray(np.asarray(indices, dtype=np.int64).reshape(-1))
+    n = x.shape[ax]
+    for i in idx:  # checked in call order (validate_reduceat_indices, ufunc.rs:8426)
+        if i < 0 or i >= n:
+            raise OracleError(2)
+    odt = DTYPES[lib().ora_reduce_dtype(OP[op], _CODE[x.dtype])] if dtype is None else np.dtype(dtype)
+    shape = list(x.shape)
+    shape[ax] = len(idx)
+    # output layout: NumPy's iterator over (input without `axis`, indices
+    # along `axis`) in K order -- `axis` never compares with the others
+    xs = list(x.strides)
+    xs[ax] = 0
+    iv = np.lib.stride_tricks.as_strided(x, shape=shape, strides=xs)
+    ind = np.zeros(max(len(idx), 1), dtype=np.int64)
+    ist = [0] * x.ndim
+    ist[ax] = 8
+    jv = np.lib.stride_tricks.as_strided(ind, shape=shape, strides=ist)
+    out = _empty_strided(tuple(shape), korder_strides([iv, jv], shape), odt)
+    dx, do = _desc(x), _desc(out)
+    pi = idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    _check(lib().ora_reduceat(OP[op], ctypes.byref(dx), ax, pi, len(idx), ctypes.byref(do)))
+    return out
 
 
 def accumulate(op, x, axis=0, dtype=None):

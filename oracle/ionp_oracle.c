@@ -1325,6 +1325,121 @@ int ora_reduce_where(int op, const ora_desc *in, const uint8_t *axmask, const or
     return ORA_OK;
 }
 
+/* ufunc.reduceat (reduceat_binary_axis, ufunc.rs:8298-8460) as NumPy 2.2.6
+ * evaluates it: along `axis`, output k of every fiber is x[i_k] when
+ * i_{k+1} <= i_k (the last index runs to the end), else the reduce loop
+ * seeded with x[i_k] on the rest of the segment [i_k + 1, i_{k+1}) as one
+ * call (no buffer windows):
+ *   float add       x[i_k] + pw(rest)   (the reference folds sequentially)
+ *   float multiply  sequential product
+ *   float max/min   the SIMD loop model on a contiguous rest (stride 1),
+ *                   the strided loop model otherwise
+ *   integers        order-free fold (in the output dtype)
+ * `out` has the input's shape with axis `axis` of length nidx (any
+ * strides); indices are checked by the caller (0 <= i < n). */
+int ora_reduceat(int op, const ora_desc *in, int axis, const int64_t *idx, int64_t nidx, const ora_desc *out)
+{
+    int n = in->ndim, k, cdt = out->dtype, isz = ITEMSIZE[in->dtype], ismax = op == OP_MAXIMUM;
+    int64_t len, st, ost, nfib = 1, f, i, fidx[ORA_MAXDIM];
+    int64_t *off;
+    char *buf;
+    if (op != OP_ADD && op != OP_MULTIPLY && op != OP_MAXIMUM && op != OP_MINIMUM) return ORA_ENOLOOP;
+    if (axis < 0 || axis >= n || out->ndim != n) return ORA_EAXIS;
+    if (is_float(in->dtype) && !is_float(cdt)) return ORA_ETYPE;
+    if (is_float(cdt) && in->dtype != cdt) return ORA_EVALUE;
+    len = in->shape[axis];
+    st = in->strides[axis];
+    ost = out->strides[axis];
+    for (k = 0; k < n; k++)
+        if (k != axis) nfib *= in->shape[k];
+    for (i = 0; i < nidx; i++)
+        if (idx[i] < 0 || idx[i] >= len) return ORA_EVALUE;
+    if (nidx == 0 || nfib == 0) return ORA_OK;
+    off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(len > 0 ? len : 1));
+    buf = (char *)malloc((size_t)(len > 0 ? len : 1) * 8);
+    for (i = 0; i < len; i++) off[i] = i * st * isz;
+    memset(fidx, 0, sizeof fidx);
+    for (f = 0; f < nfib; f++) {
+        int64_t ib = 0, ob = 0, q;
+        for (k = 0; k < n; k++)
+            if (k != axis) {
+                ib += fidx[k] * in->strides[k];
+                ob += fidx[k] * out->strides[k];
+            }
+        for (q = 0; q < nidx; q++) {
+            const int64_t a = idx[q], b = q + 1 < nidx ? idx[q + 1] : len;
+            const char *pb = (const char *)in->data + ib * isz;
+            char *po = (char *)out->data + (ob + q * ost) * ITEMSIZE[cdt];
+            const int64_t m = b > a ? b - a - 1 : 0; /* elements after the seed */
+            if (cdt == DT_F64 || cdt == DT_F32) {
+                double a64 = cdt == DT_F64 ? *(const double *)(pb + off[a]) : 0.0;
+                float a32 = cdt == DT_F32 ? *(const float *)(pb + off[a]) : 0.0f;
+                if (m > 0) {
+                    const int64_t *ro = off + a + 1;
+                    int64_t e;
+                    if (op == OP_ADD) {
+                        if (cdt == DT_F64) a64 = f64_arith(OP_ADD, a64, pw_f64(pb, ro, m));
+                        else a32 = f32_arith(OP_ADD, a32, pw_f32(pb, ro, m));
+                    } else if (op == OP_MULTIPLY) {
+                        for (e = 0; e < m; e++) {
+                            if (cdt == DT_F64) a64 = f64_arith(OP_MULTIPLY, a64, *(const double *)(pb + ro[e]));
+                            else a32 = f32_arith(OP_MULTIPLY, a32, *(const float *)(pb + ro[e]));
+                        }
+                    } else if (st == 1) {
+                        for (e = 0; e < m; e++) memcpy(buf + e * isz, pb + ro[e], isz);
+                        if (cdt == DT_F64) a64 = simd_ext_f64(ismax, a64, (const double *)buf, m);
+                        else a32 = simd_ext_f32(ismax, a32, (const float *)buf, m);
+                    } else {
+                        if (cdt == DT_F64) a64 = strided_ext_f64(ismax, a64, pb, ro, m);
+                        else a32 = strided_ext_f32(ismax, a32, pb, ro, m);
+                    }
+                }
+                if (cdt == DT_F64) *(double *)po = a64;
+                else *(float *)po = a32;
+            } else {
+                int uns = cdt == DT_BOOL || is_unsigned_int(cdt);
+                char tmp[8];
+                uint64_t ua;
+                int64_t sa, e;
+                cast_elem(in->dtype, pb + off[a], cdt, tmp);
+                ua = (uint64_t)load_i(cdt, tmp);
+                sa = load_i(cdt, tmp);
+                for (e = 0; e < m; e++) {
+                    cast_elem(in->dtype, pb + off[a + 1 + e], cdt, tmp);
+                    if (uns) {
+                        uint64_t v = (uint64_t)load_i(cdt, tmp);
+                        ua = op == OP_ADD ? ua + v : op == OP_MULTIPLY ? ua * v
+                           : op == OP_MAXIMUM ? (v > ua ? v : ua) : (v < ua ? v : ua);
+                    } else {
+                        int64_t v = load_i(cdt, tmp);
+                        sa = op == OP_ADD ? (int64_t)((uint64_t)sa + (uint64_t)v)
+                           : op == OP_MULTIPLY ? (int64_t)((uint64_t)sa * (uint64_t)v)
+                           : op == OP_MAXIMUM ? (v > sa ? v : sa) : (v < sa ? v : sa);
+                    }
+                }
+                {
+                    uint64_t acc = uns ? ua : (uint64_t)sa;
+                    if (cdt == DT_BOOL) acc = acc != 0; /* add: or; multiply: and */
+                    switch (ITEMSIZE[cdt]) {
+                    case 1: *(uint8_t *)po = (uint8_t)acc; break;
+                    case 2: *(uint16_t *)po = (uint16_t)acc; break;
+                    case 4: *(uint32_t *)po = (uint32_t)acc; break;
+                    default: *(uint64_t *)po = acc; break;
+                    }
+                }
+            }
+        }
+        for (k = n - 1; k >= 0; k--) {
+            if (k == axis) continue;
+            if (++fidx[k] < in->shape[k]) break;
+            fidx[k] = 0;
+        }
+    }
+    free(off);
+    free(buf);
+    return ORA_OK;
+}
+
 /* accumulate_axis (ufunc.rs:7967) -> accumulate_axis_generic: along `axis`,
  * out[0] = x[0] and out[i] = op(out[i-1], x[i]), in the output dtype (the
  * reduce-family compute dtype, or an explicit dtype=); elements are cast to

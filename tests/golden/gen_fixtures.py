@@ -686,6 +686,80 @@ def where_reduce_cases(S):
               args(1, dtype="int8"))
 
 
+def reduceat_cases(S):
+    """ufunc.reduceat (reduceat_binary_axis, ufunc.rs:8298-8460, as NumPy
+    2.2.6 evaluates it): per segment the reduce loop seeded with its first
+    element, one call over the rest (float add: x[i] + pairwise(rest), no
+    buffer windows -- segments longer than 8192 and than 2 * 8192 included);
+    non-increasing index pairs pass x[i] through; K-order outputs; index and
+    axis errors.  Float add/multiply inputs carry one NaN payload only (the
+    choice between two differs between NumPy's loop variants).  Own
+    generator (seed 83)."""
+    r8 = np.random.default_rng(83)
+
+    def rat(op, x, idx, ax=0):
+        return getattr(np, op).reduceat(x, idx, axis=ax)
+
+    def args(idx, ax=0):
+        return {"indices": [int(i) for i in idx], "axis": ax}
+
+    def fvals(shape, dt, op):
+        if op == "multiply":
+            x = (1 + r8.standard_normal(shape) * 0.02).astype(dt)
+            x[r8.random(shape) < 0.002] = np.nan
+            return x
+        x = r8.standard_normal(shape).astype(dt)
+        u = r8.random(shape)
+        x[u < 0.04] = 0.0
+        x[(u >= 0.04) & (u < 0.08)] = -0.0
+        x[(u >= 0.08) & (u < 0.083)] = np.nan
+        if op in ("maximum", "minimum"):  # a second NaN payload
+            y = x.view(np.uint64 if np.dtype(dt).itemsize == 8 else np.uint32)
+            y[(u >= 0.08) & (u < 0.0815)] |= 5
+        return x
+
+    ops = ("add", "multiply", "maximum", "minimum")
+    for dt in (np.float64, np.float32):
+        nm = np.dtype(dt).name
+        for op in ops:
+            x = fvals(3000, dt, op)
+            for name, idx in (("sorted", [0, 3, 10, 137, 900, 2999]), ("unsorted", [500, 20, 21, 21, 2000, 7]),
+                              ("one", [0]), ("last", [2999]), ("tail", [1, 2, 2990])):
+                S.add(f"rat/{op}/{nm}/{name}", "reduceat", op, [x], lambda: rat(op, x, idx), args(idx))
+            S.add(f"rat/{op}/{nm}/step", "reduceat", op, [x[::3]], lambda: rat(op, x[::3], [0, 40, 41, 600]),
+                  args([0, 40, 41, 600]))
+            S.add(f"rat/{op}/{nm}/rev", "reduceat", op, [x[::-1]], lambda: rat(op, x[::-1], [0, 1000, 2500]),
+                  args([0, 1000, 2500]))
+            X = fvals((60, 90), dt, op)
+            for ax, idx in ((0, [0, 5, 30, 31, 10]), (1, [3, 50, 89]), (-1, [0, 1]), (-2, [59])):
+                S.add(f"rat/{op}/{nm}/2d/{ax}", "reduceat", op, [X], lambda: rat(op, X, idx, ax), args(idx, ax))
+                Xf = np.asfortranarray(X)
+                S.add(f"rat/{op}/{nm}/2dF/{ax}", "reduceat", op, [Xf], lambda: rat(op, Xf, idx, ax), args(idx, ax))
+            Y = fvals((4, 50, 6), dt, op)
+            S.add(f"rat/{op}/{nm}/3d", "reduceat", op, [Y], lambda: rat(op, Y, [0, 3, 30], 1), args([0, 3, 30], 1))
+            S.add(f"rat/{op}/{nm}/3dT", "reduceat", op, [Y.transpose(2, 0, 1)],
+                  lambda: rat(op, Y.transpose(2, 0, 1), [1, 2, 40], 2), args([1, 2, 40], 2))
+        # segments longer than one and than two pairwise blocks of 8192
+        g = Gen(8301, 60000, dt)
+        big = g.array()
+        for name, idx in (("long", [0, 5, 9000, 27000]), ("whole", [0]), ("mid", [100, 17000, 16999, 33000])):
+            S.add(f"rat/add/{nm}/{name}", "reduceat", "add", [g], lambda: rat("add", big, idx), args(idx))
+            S.add(f"rat/maximum/{nm}/{name}", "reduceat", "maximum", [g], lambda: rat("maximum", big, idx), args(idx))
+    for dt in (np.int64, np.int32, np.uint8, np.int8, np.bool_):
+        nm = np.dtype(dt).name
+        x = specials(dt, 40 * 30, r8).reshape(40, 30)
+        for op in ops:
+            for ax, idx in ((0, [0, 7, 7, 39, 2]), (1, [5, 6, 29]), (1, [0])):
+                S.add(f"rat/{op}/{nm}/{ax}/{len(idx)}", "reduceat", op, [x], lambda: rat(op, x, idx, ax),
+                      args(idx, ax))
+    x = np.arange(10.0)
+    for name, idx, ax in (("oob", [3, 10], 0), ("neg", [-1], 0), ("order", [8, 11, 12], 0), ("axis", [0], 1),
+                          ("empty", [], 0)):
+        S.add(f"rat/err/{name}", "reduceat", "add", [x], lambda: rat("add", x, idx, ax), args(idx, ax))
+    E = np.zeros((0, 3))
+    S.add("rat/add/emptyfib", "reduceat", "add", [E], lambda: rat("add", E, [0, 2], 1), args([0, 2], 1))
+
+
 def argext_cases(S, rng):
     for dt in (np.float64, np.float32, np.int64, np.int32, np.uint8, np.bool_):
         nm = np.dtype(dt).name
@@ -763,6 +837,7 @@ def main():
     accumulate_cases(S)
     prod_cases(S)
     where_reduce_cases(S)
+    reduceat_cases(S)
     np.savez_compressed(os.path.join(HERE, "fixtures.npz"), **S.arrays)
     with open(os.path.join(HERE, "cases.json"), "w") as f:
         json.dump(S.cases, f)

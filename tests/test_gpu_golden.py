@@ -7,7 +7,7 @@ from fixture_cases import args_axis, check_result, host_input, load
 
 pytestmark = pytest.mark.gpu
 
-KINDS = ["binary", "where", "cast", "reduce", "argext", "accumulate"]
+KINDS = ["binary", "where", "cast", "reduce", "argext", "accumulate", "reduceat"]
 
 EXC = {"ValueError": ValueError, "TypeError": TypeError, "IndexError": IndexError,
        "AxisError": ValueError, "UFuncTypeError": TypeError}
@@ -37,6 +37,8 @@ def run_device(ap, case, npz, cache):
         return getattr(ap, op).reduce(ins[0], axis=args_axis(args), dtype=args.get("dtype"),
                                       keepdims=bool(args.get("keepdims", False)), initial=args.get("initial"),
                                       where=ins[1] if args.get("where") else True)
+    if k == "reduceat":
+        return getattr(ap, op).reduceat(ins[0], args["indices"], axis=args["axis"])
     if k == "accumulate":
         if op in ("cumsum", "cumprod"):
             return getattr(ap, op)(ins[0], axis=args.get("axis"))

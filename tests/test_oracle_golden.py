@@ -5,7 +5,7 @@ import pytest
 import oracle as O
 from fixture_cases import args_axis, check_result, host_input, load
 
-KINDS = ["binary", "where", "cast", "reduce", "argext", "accumulate"]
+KINDS = ["binary", "where", "cast", "reduce", "argext", "accumulate", "reduceat"]
 
 
 def run_oracle(case, npz):
@@ -28,6 +28,8 @@ def run_oracle(case, npz):
             axes = range(x.ndim) if ax is None else ((ax,) if isinstance(ax, int) else ax)
             r = np.expand_dims(r, tuple(a % x.ndim for a in axes))
         return r
+    if k == "reduceat":
+        return O.reduceat(op, ins[0], args["indices"], axis=args["axis"])
     if k == "accumulate":
         if op in ("cumsum", "cumprod"):
             x, ax = ins[0], args.get("axis")
