@@ -13,9 +13,13 @@ there is no CPU fallback: without the built library, import fails.
     C.get()                              # HBM -> host numpy (same strides)
 """
 import builtins
+import contextlib
 import ctypes
 import numbers
 import operator
+import sys
+import threading
+import warnings
 
 import numpy as np
 import torch
@@ -27,7 +31,7 @@ __all__ = ["ndarray", "asarray", "empty", "zeros", "Ufunc", "add", "subtract", "
            "true_divide", "maximum", "minimum", "less", "less_equal", "greater", "greater_equal", "equal",
            "not_equal", "where", "sum", "prod", "cumsum", "cumprod", "max", "amax", "min", "amin", "argmax", "argmin", "result_type",
            "AxisError", "UFuncTypeError", "IonpError", "setbufsize", "getbufsize", "set_config_policy",
-           "get_config_policy"]
+           "get_config_policy", "prod", "cumprod", "seterr", "geterr", "seterrcall", "geterrcall", "errstate"]
 
 _NP_DTYPES = [np.bool_, np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64,
               np.uint64, np.float32, np.float64]
@@ -233,6 +237,14 @@ class ndarray:
 
     def astype(self, dtype, order="K"):
         dtype = _dtype(dtype)
+        track = self.dtype.kind == "f" and _fpe_wanted()
+        _fpe_arm(track)
+        r = self._astype(dtype, order)
+        if track:
+            _fpe_check("cast")
+        return r
+
+    def _astype(self, dtype, order):
         t = _lib.ArrayDesc()
         d = self._desc()
         _call(_lib.load().ionp_plan_cast, ctypes.byref(d), _CODE[dtype], ctypes.byref(t))
@@ -541,6 +553,14 @@ class Ufunc:
         return f"<ufunc '{self.__name__}'>"
 
     def __call__(self, a, b, out=None, where=True, dtype=None, order="K"):
+        track = self._code <= _lib.TRUE_DIVIDE and _fpe_wanted()
+        _fpe_arm(track)
+        r = self._call(a, b, out, where, dtype, order)
+        if track:
+            _fpe_check(self.__name__)
+        return r
+
+    def _call(self, a, b, out, where, dtype, order):
         """ufunc.__call__ (lib.rs:9204) -> binary_op (ionp-core/src/ufunc.rs:3628).
 
         out=: any strides; an `out` that overlaps an input other than as the
@@ -697,6 +717,124 @@ def _reduce(code, a, axis, dtype, keepdims, out=None, initial=None, where=True):
         _copy_into(out, final)
         return final
     return out
+
+
+# ---- IEEE exception flags: np.seterr / np.geterr / np.errstate ----------------------
+# (ionp-py/src/fpstate.rs: per-thread modes; the flags come from the device,
+# see ionp_fpe_track / ionp_fpe_fetch)
+_FPE_CATS = ((1, "divide", "divide by zero"), (2, "over", "overflow"), (4, "under", "underflow"),
+             (8, "invalid", "invalid value"))
+_FPE_MODES = ("ignore", "warn", "raise", "call", "print", "log")
+_fpe_local = threading.local()
+_fpe_armed = [None]
+
+
+def _fpe_state():
+    st = getattr(_fpe_local, "modes", None)
+    if st is None:  # NumPy's defaults
+        st = _fpe_local.modes = {"divide": "warn", "over": "warn", "under": "ignore", "invalid": "warn"}
+        _fpe_local.call = None
+    return st
+
+
+def _fpe_wanted():
+    return any(m != "ignore" for m in _fpe_state().values())
+
+
+def _fpe_arm(on):
+    if _fpe_armed[0] != on:
+        _call(_lib.load().ionp_fpe_track, int(on))
+        _fpe_armed[0] = on
+
+
+def _fpe_check(name):
+    fl = ctypes.c_uint32()
+    _call(_lib.load().ionp_fpe_fetch, _lib.stream_handle(), ctypes.byref(fl))
+    flags = fl.value
+    if not flags:
+        return
+    st = _fpe_state()
+    for bit, key, phrase in _FPE_CATS:
+        if not flags & bit:
+            continue
+        mode, msg = st[key], f"{phrase} encountered in {name}"
+        if mode == "warn":
+            warnings.warn(msg, RuntimeWarning, stacklevel=4)
+        elif mode == "raise":
+            raise FloatingPointError(msg)
+        elif mode == "print":
+            print("Warning: " + msg, file=sys.stderr)
+        elif mode == "call":
+            if _fpe_local.call is None:
+                raise NameError(f"call specified for {phrase} (in {name}) but no callable function found")
+            _fpe_local.call(phrase, flags)
+        elif mode == "log":
+            if _fpe_local.call is None:
+                raise NameError(f"log specified for {phrase} (in {name}) but no object with write method found.")
+            _fpe_local.call.write(f"Warning: {msg}\n")
+
+
+def seterr(all=None, divide=None, over=None, under=None, invalid=None):
+    """numpy.seterr (ionp-py/src/fpstate.rs:143): `all` first, then the named
+    categories; returns the previous modes."""
+    st = _fpe_state()
+    old = dict(st)
+    new = dict(st)
+    if all is not None:
+        for k in new:
+            new[k] = all
+    for k, v in (("divide", divide), ("over", over), ("under", under), ("invalid", invalid)):
+        if v is not None:
+            new[k] = v
+    for v in new.values():
+        if v not in _FPE_MODES:
+            raise ValueError(f"invalid error mode {v!r}")
+    st.update(new)
+    return old
+
+
+def geterr():
+    return dict(_fpe_state())
+
+
+def seterrcall(func):
+    """numpy.seterrcall: the callable ('call' mode) or object with a write
+    method ('log' mode); returns the previous one."""
+    _fpe_state()
+    if func is not None and not callable(func) and not callable(getattr(func, "write", None)):
+        raise ValueError("Only callable can be used as callback")
+    old = _fpe_local.call
+    _fpe_local.call = func
+    return old
+
+
+def geterrcall():
+    _fpe_state()
+    return _fpe_local.call
+
+
+_UNSET = object()
+
+
+class errstate(contextlib.ContextDecorator):
+    """numpy.errstate (ionp-py/src/fpstate.rs:288): modes (and call=) for the
+    duration of a with block."""
+
+    def __init__(self, *, call=_UNSET, all=None, divide=None, over=None, under=None, invalid=None):
+        self._call = call
+        self._kw = dict(all=all, divide=divide, over=over, under=under, invalid=invalid)
+
+    def __enter__(self):
+        self._old = seterr(**self._kw)
+        if self._call is not _UNSET:
+            self._oldcall = seterrcall(self._call)
+        return None
+
+    def __exit__(self, *exc):
+        seterr(**self._old)
+        if self._call is not _UNSET:
+            seterrcall(self._oldcall)
+        return False
 
 
 add = Ufunc("add", _lib.ADD, True)

@@ -80,6 +80,8 @@ SIGNATURES = {
     "ionp_reduce_where": [_I, _D, _D, _P(_I), _I, _I, _V, _D, _C, _V],
     "ionp_argext": [_I, _D, _I, _I, _D, _C, _V],
     "ionp_plan_accumulate": [_I, _D, _I, _I, _D],
+    "ionp_fpe_track": [_I],
+    "ionp_fpe_fetch": [_V, _P(ctypes.c_uint32)],
     "ionp_plan_reduceat": [_I, _D, _I, ctypes.c_int64, _I, _D],
     "ionp_reduceat": [_I, _D, _P(ctypes.c_int64), ctypes.c_int64, _I, _D, _C, _V],
     "ionp_accumulate": [_I, _D, _I, _D, _C, _V],

@@ -59,6 +59,34 @@ ionp_status binary_common(int32_t op, const ionp_array_desc *a, const ionp_array
     return broadcast(2, ops, nd, shape);
 }
 
+// IEEE flag tracking (np.errstate): while on, ionp_binary and ionp_cast OR
+// the flags their elements raise into a device word of the current GPU
+std::atomic<int> g_fpe_on{0};
+std::mutex g_fpe_mu;
+uint32_t *g_fpe_word[64] = {};
+
+ionp_status fpe_word(uint32_t **w) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(IONP_ERR_DEVICE, "no current HIP device");
+    std::lock_guard<std::mutex> lk(g_fpe_mu);
+    if (!g_fpe_word[dev]) {
+        void *p = nullptr;
+        if (hipMalloc(&p, 256) != hipSuccess) return set_error(IONP_ERR_DEVICE, "out of device memory for the flag word");
+        if (hipMemset(p, 0, 256) != hipSuccess) {
+            (void)hipFree(p);
+            return set_error(IONP_ERR_DEVICE, "clearing the flag word failed");
+        }
+        g_fpe_word[dev] = (uint32_t *)p;
+    }
+    *w = g_fpe_word[dev];
+    return IONP_OK;
+}
+
+ionp_status attach_fpe(EwArgs &A) {
+    if (!g_fpe_on.load(std::memory_order_relaxed)) return IONP_OK;
+    return fpe_word(&A.fpe);
+}
+
 bool is_mixed_su(int op, int a, int b) {
     return op >= IONP_LESS && ((a == IONP_INT64 && b == IONP_UINT64) || (a == IONP_UINT64 && b == IONP_INT64));
 }
@@ -75,6 +103,25 @@ ionp_status ionp_init(void) {
 }
 
 const char *ionp_last_error(void) { return last_error(); }
+
+ionp_status ionp_fpe_track(int32_t on) {
+    g_fpe_on.store(on ? 1 : 0);
+    return IONP_OK;
+}
+
+ionp_status ionp_fpe_fetch(void *stream, uint32_t *flags) {
+    if (!flags) return set_error(IONP_ERR_VALUE, "flags must not be null");
+    uint32_t *w;
+    ionp_status s;
+    if ((s = fpe_word(&w))) return s;
+    uint32_t v = 0;
+    if (hipStreamSynchronize(as_stream(stream)) != hipSuccess ||
+        hipMemcpy(&v, w, sizeof v, hipMemcpyDeviceToHost) != hipSuccess)
+        return set_error(IONP_ERR_DEVICE, "reading the flag word failed");
+    if (v && hipMemset(w, 0, sizeof v) != hipSuccess) return set_error(IONP_ERR_DEVICE, "clearing the flag word failed");
+    *flags = v;
+    return IONP_OK;
+}
 
 int32_t ionp_version(void) { return (1 << 16) | 0; }
 
@@ -290,6 +337,7 @@ ionp_status ionp_binary(int32_t op, const ionp_array_desc *a, const ionp_array_d
     const ionp_array_desc *ins[2] = {a, b};
     EwArgs A;
     if ((s = plan_elementwise(op, 2, ins, out, loop, &A))) return s;
+    if ((s = attach_fpe(A))) return s;
     A.mixed_su = is_mixed_su(op, a->dtype, b->dtype);
     return run_elementwise(A, cfg, as_stream(stream));
 }
@@ -315,6 +363,7 @@ ionp_status ionp_cast(const ionp_array_desc *in, const ionp_array_desc *out, con
     const ionp_array_desc *ins[1] = {in};
     EwArgs A;
     if ((s = plan_elementwise(OP_CAST, 1, ins, out, out->dtype, &A))) return s;
+    if ((s = attach_fpe(A))) return s;
     return run_elementwise(A, cfg, as_stream(stream));
 }
 

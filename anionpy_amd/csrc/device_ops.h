@@ -150,6 +150,134 @@ __device__ inline void ew_element(const EwArgs &A, const char *p0, const char *p
 }
 
 // ---------------------------------------------------------------------------
+// IEEE exception flags, as NumPy reads them from the x86 status register
+// after a loop (np.errstate categories: divide 1, over 2, under 4, invalid
+// 8).  Derived per element from the operands and the result:
+//   invalid  a NaN result from non-NaN operands (inf - inf, 0 * inf, 0 / 0,
+//            inf / inf)
+//   divide   finite nonzero / 0
+//   over     an infinite result from finite operands
+//   under    tiny after rounding (x86 rule) and inexact; sums are exact
+//            when tiny, so only products and quotients underflow
+// Signalling NaN operands (which also raise invalid on x86) are not
+// modelled.
+// ---------------------------------------------------------------------------
+enum { FPE_DIVIDE = 1, FPE_OVER = 2, FPE_UNDER = 4, FPE_INVALID = 8 };
+template <typename T> struct FpLim;
+template <> struct FpLim<double> {
+    static constexpr int emin = -1022;  // smallest normal 2^emin
+    __device__ static double tiny() { return 2.2250738585072014e-308; }
+};
+template <> struct FpLim<float> {
+    static constexpr int emin = -126;
+    __device__ static float tiny() { return 1.17549435e-38f; }
+};
+
+// m * 2^e with m the correctly rounded mantissa result (m = exact - err):
+// tiny after rounding with unbounded exponent, and not exact on the
+// subnormal grid
+template <typename T> __device__ inline bool fpe_underflow(T m, T err, int e) {
+    int em;
+    (void)frexp(m, &em);
+    if (em + e > FpLim<T>::emin) return false;
+    if (err != T(0)) return true;
+    return ldexp(ldexp(m, e), -e) != m;
+}
+
+template <typename T, int OPK> __device__ inline uint32_t fpe_arith(T a, T b, T r) {
+    const T ar = fabs(r);
+    if (ar > FpLim<T>::tiny() && ar < T(INFINITY)) return 0;  // a normal result raises nothing
+    if (r != r) return (a == a && b == b) ? FPE_INVALID : 0;
+    const bool fin = isfinite(a) && isfinite(b);
+    if (ar == T(INFINITY)) {
+        if (!fin) return 0;
+        return (OPK == IONP_TRUE_DIVIDE && b == T(0)) ? FPE_DIVIDE : FPE_OVER;
+    }
+    if constexpr (OPK == IONP_MULTIPLY || OPK == IONP_TRUE_DIVIDE) {
+        if (!fin || a == T(0) || b == T(0)) return 0;
+        int ea, eb;
+        const T ma = frexp(a, &ea), mb = frexp(b, &eb);  // |m| in [0.5, 1)
+        if constexpr (OPK == IONP_MULTIPLY) {
+            const T m = ma * mb;
+            return fpe_underflow(m, fma(ma, mb, -m), ea + eb) ? FPE_UNDER : 0;
+        } else {
+            const T m = ma / mb;
+            return fpe_underflow(m, fma(-m, mb, ma), ea - eb) ? FPE_UNDER : 0;
+        }
+    }
+    return 0;
+}
+// dynamic-op form (generic kernel): arithmetic ops only
+template <typename T> __device__ inline uint32_t fpe_arith_dyn(int op, T a, T b, T r) {
+    switch (op) {
+        case IONP_ADD: return fpe_arith<T, IONP_ADD>(a, b, r);
+        case IONP_SUBTRACT: return fpe_arith<T, IONP_SUBTRACT>(a, b, r);
+        case IONP_MULTIPLY: return fpe_arith<T, IONP_MULTIPLY>(a, b, r);
+        case IONP_TRUE_DIVIDE: return fpe_arith<T, IONP_TRUE_DIVIDE>(a, b, r);
+        default: return 0;
+    }
+}
+// casts from a float dtype (NumPy's C conversions on x86-64): to an
+// integer, invalid when the value does not fit the hardware conversion the
+// compiler uses (32-bit for int8..int32, 64-bit for uint32 / int64, the
+// two-step 64-bit sequence for uint64); float64 -> float32 overflows or
+// underflows like an arithmetic result
+__device__ inline uint32_t fpe_cast(int in_dt, int out_dt, const char *p) {
+    if (!is_float_dt(in_dt)) return 0;
+    const double x = in_dt == IONP_FLOAT32 ? (double)*(const float *)p : *(const double *)p;
+    switch (out_dt) {
+        case IONP_INT8:
+        case IONP_UINT8:
+        case IONP_INT16:
+        case IONP_UINT16:
+        case IONP_INT32: return (x > -2147483649.0 && x < 2147483648.0) ? 0 : FPE_INVALID;
+        case IONP_UINT32:
+        case IONP_INT64: return (x >= -9223372036854775808.0 && x < 9223372036854775808.0) ? 0 : FPE_INVALID;
+        case IONP_UINT64: return (x >= -9223372036854775808.0 && x < 18446744073709551616.0) ? 0 : FPE_INVALID;
+        case IONP_FLOAT32: {
+            if (in_dt != IONP_FLOAT64 || !isfinite(x)) return 0;
+            const float f = (float)x;
+            if (isinf(f)) return FPE_OVER;
+            if (fabsf(f) > FpLim<float>::tiny() || x == 0.0) return 0;
+            int ex, em;
+            const double mx = frexp(x, &ex);
+            (void)frexpf((float)mx, &em);  // the 24-bit rounding, unbounded exponent
+            return (em + ex <= FpLim<float>::emin && (double)f != x) ? FPE_UNDER : 0;
+        }
+        default: return 0;
+    }
+}
+// OR of the wavefront's flags into *word by one lane (all lanes call it)
+__device__ inline void fpe_commit(uint32_t *word, uint32_t fl) {
+    uint32_t w = 0;
+    if (__ballot(fl & FPE_DIVIDE)) w |= FPE_DIVIDE;
+    if (__ballot(fl & FPE_OVER)) w |= FPE_OVER;
+    if (__ballot(fl & FPE_UNDER)) w |= FPE_UNDER;
+    if (__ballot(fl & FPE_INVALID)) w |= FPE_INVALID;
+    if (w && (threadIdx.x & 63) == 0) atomicOr(word, w);
+}
+
+// One output element of the generic kernel with its IEEE flags.
+template <int LT>
+__device__ inline uint32_t ew_element_fpe(const EwArgs &A, const char *p0, const char *p1, const char *p2, char *po) {
+    using T = typename dt_type<LT>::T;
+    if (A.op == OP_CAST) {
+        store_as<T>(A.out_dt, po, load_as<LT>(A.in_dt[0], p0));
+        return fpe_cast(A.in_dt[0], A.out_dt, p0);
+    }
+    if constexpr (std::is_floating_point<T>::value) {
+        if (A.op <= IONP_TRUE_DIVIDE) {
+            const T a = load_as<LT>(A.in_dt[0], p0), b = load_as<LT>(A.in_dt[1], p1);
+            const T r = arith_op<LT>(A.op, a, b);
+            store_as<T>(A.out_dt, po, r);
+            return fpe_arith_dyn<T>(A.op, a, b, r);
+        }
+    }
+    ew_element<LT>(A, p0, p1, p2, po);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // index helpers
 // ---------------------------------------------------------------------------
 // decompose idx over dims (innermost first) and dot with strides

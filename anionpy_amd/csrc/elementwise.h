@@ -17,6 +17,7 @@ struct RowArgs {
     int32_t pair_ok[2];
     const char *in[2];
     char *out;
+    uint32_t *fpe;  // IEEE flag word (FPE variants only)
 };
 
 int num_cus();

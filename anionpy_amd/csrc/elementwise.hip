@@ -24,10 +24,12 @@ namespace ionp {
 
 // Generic strided kernel: element e of the coalesced space (dim 0
 // innermost) -> per-operand offsets by fast division.  A.total < 2^31.
-template <int LT> __global__ __launch_bounds__(256) void ew_generic(const EwArgs A) {
+// FPE: also OR the IEEE flags of the elements into *A.fpe.
+template <int LT, bool FPE = false> __global__ __launch_bounds__(256) void ew_generic(const EwArgs A) {
     const int i0 = itemsize(A.in_dt[0]), i1 = itemsize(A.in_dt[1]), i2 = itemsize(A.in_dt[2]);
     const int io = itemsize(A.out_dt);
     const uint32_t total = (uint32_t)A.total;
+    uint32_t fl = 0;
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
         uint32_t rem = e;
         int64_t o0 = 0, o1 = 0, o2 = 0, oo = 0;
@@ -45,8 +47,10 @@ template <int LT> __global__ __launch_bounds__(256) void ew_generic(const EwArgs
             o2 += (int64_t)c * A.in_st[2][d];
             oo += (int64_t)c * A.out_st[d];
         }
-        ew_element<LT>(A, A.in[0] + o0 * i0, A.in[1] + o1 * i1, A.in[2] + o2 * i2, A.out + oo * io);
+        if constexpr (FPE) fl |= ew_element_fpe<LT>(A, A.in[0] + o0 * i0, A.in[1] + o1 * i1, A.in[2] + o2 * i2, A.out + oo * io);
+        else ew_element<LT>(A, A.in[0] + o0 * i0, A.in[1] + o1 * i1, A.in[2] + o2 * i2, A.out + oo * io);
     }
+    if constexpr (FPE) fpe_commit(A.fpe, fl);
 }
 
 // ---------------------------------------------------------------------------
@@ -75,9 +79,11 @@ template <typename T, int OPK> __device__ inline auto row_apply(T a, T b) {
 
 // GATHER: inputs whose inner stride is neither 0 nor 1 (e.g. X.T[::2]) are
 // read element by element at that stride; the output stays vectorised.
-template <typename T, int OPK, int VEC, int UNROLL, bool GATHER = false>
+// FPE: also OR the IEEE flags of the elements into *R.fpe.
+template <typename T, int OPK, int VEC, int UNROLL, bool GATHER = false, bool FPE = false>
 __global__ __launch_bounds__(256) void ew_rows(const RowArgs R) {
     using OT = typename std::conditional<(OPK >= IONP_LESS), uint8_t, T>::type;
+    uint32_t fl = 0;
     const T *__restrict__ a = (const T *)R.in[0];
     const T *__restrict__ b = (const T *)R.in[1];
     OT *__restrict__ o = (OT *)R.out;
@@ -147,15 +153,22 @@ __global__ __launch_bounds__(256) void ew_rows(const RowArgs R) {
             if (i + VEC <= R.inner) {
                 VecT<OT, VEC> r;
 #pragma unroll
-                for (int k = 0; k < VEC; k++)
-                    r[k] = row_apply<T, OPK>(sa ? va[u][k] : a0, sb ? vb[u][k] : b0);
+                for (int k = 0; k < VEC; k++) {
+                    const T x = sa ? va[u][k] : a0, y = sb ? vb[u][k] : b0;
+                    r[k] = row_apply<T, OPK>(x, y);
+                    if constexpr (FPE) fl |= fpe_arith<T, OPK>(x, y, r[k]);
+                }
                 __builtin_nontemporal_store(r, (VecT<OT, VEC> *)(po + i));
             } else {
-                for (int64_t j = i; j < R.inner && j < i + VEC; j++)
-                    po[j] = row_apply<T, OPK>(sa ? pa[j * sta] : a0, sb ? pb[j * stb] : b0);
+                for (int64_t j = i; j < R.inner && j < i + VEC; j++) {
+                    const T x = sa ? pa[j * sta] : a0, y = sb ? pb[j * stb] : b0;
+                    po[j] = row_apply<T, OPK>(x, y);
+                    if constexpr (FPE) fl |= fpe_arith<T, OPK>(x, y, po[j]);
+                }
             }
         }
     }
+    if constexpr (FPE) fpe_commit(R.fpe, fl);
 }
 
 // ---------------------------------------------------------------------------
@@ -182,7 +195,22 @@ static int grid_for(int64_t work_items, int block, const ionp_config *cfg) {
 
 typedef void (*GenericFn)(const EwArgs);
 
-static GenericFn generic_kernel(int lt) {
+static GenericFn generic_kernel(int lt, bool fpe) {
+    if (fpe) {
+        switch (lt) {
+            case IONP_BOOL: return ew_generic<IONP_BOOL, true>;
+            case IONP_INT8: return ew_generic<IONP_INT8, true>;
+            case IONP_UINT8: return ew_generic<IONP_UINT8, true>;
+            case IONP_INT16: return ew_generic<IONP_INT16, true>;
+            case IONP_UINT16: return ew_generic<IONP_UINT16, true>;
+            case IONP_INT32: return ew_generic<IONP_INT32, true>;
+            case IONP_UINT32: return ew_generic<IONP_UINT32, true>;
+            case IONP_INT64: return ew_generic<IONP_INT64, true>;
+            case IONP_UINT64: return ew_generic<IONP_UINT64, true>;
+            case IONP_FLOAT32: return ew_generic<IONP_FLOAT32, true>;
+            default: return ew_generic<IONP_FLOAT64, true>;
+        }
+    }
     switch (lt) {
         case IONP_BOOL: return ew_generic<IONP_BOOL>;
         case IONP_INT8: return ew_generic<IONP_INT8>;
@@ -217,7 +245,8 @@ ionp_status launch_generic(EwArgs A, const ionp_config *cfg, hipStream_t s) {
         B.total = inner * n;
         for (int k = 0; k < A.nin; k++) B.in[k] = A.in[k] + o0 * A.in_st[k][last] * itemsize(A.in_dt[k]);
         B.out = A.out + o0 * A.out_st[last] * itemsize(A.out_dt);
-        hipLaunchKernelGGL(generic_kernel(A.loop_dt), dim3(grid_for(B.total, block, cfg)), dim3(block), 0, s, B);
+        hipLaunchKernelGGL(generic_kernel(A.loop_dt, A.fpe != nullptr), dim3(grid_for(B.total, block, cfg)), dim3(block),
+                           0, s, B);
     }
     return hipGetLastError() == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, "kernel launch failed");
 }
@@ -260,6 +289,30 @@ static RowFn row_gather_t(int op) {
         case IONP_TRUE_DIVIDE:
             if constexpr (std::is_floating_point<T>::value) return ew_rows<T, IONP_TRUE_DIVIDE, V, 4, true>;
             else return nullptr;
+        default: return nullptr;
+    }
+}
+
+// IEEE-flag variants: float add / subtract / multiply / true_divide
+template <typename T> static RowFn row_fpe_t(int op, bool gather) {
+    constexpr int V = 16 / sizeof(T) > 4 ? 4 : 16 / sizeof(T);
+    constexpr int VR = 16 / sizeof(T);
+    switch (op) {
+        case IONP_ADD: return gather ? ew_rows<T, IONP_ADD, V, 4, true, true> : ew_rows<T, IONP_ADD, VR, 4, false, true>;
+        case IONP_SUBTRACT:
+            return gather ? ew_rows<T, IONP_SUBTRACT, V, 4, true, true> : ew_rows<T, IONP_SUBTRACT, VR, 4, false, true>;
+        case IONP_MULTIPLY:
+            return gather ? ew_rows<T, IONP_MULTIPLY, V, 4, true, true> : ew_rows<T, IONP_MULTIPLY, VR, 4, false, true>;
+        case IONP_TRUE_DIVIDE:
+            return gather ? ew_rows<T, IONP_TRUE_DIVIDE, V, 4, true, true>
+                          : ew_rows<T, IONP_TRUE_DIVIDE, 4, 4, false, true>;
+        default: return nullptr;
+    }
+}
+static RowFn row_fpe(int dt, int op, bool gather) {
+    switch (dt) {
+        case IONP_FLOAT32: return row_fpe_t<float>(op, gather);
+        case IONP_FLOAT64: return row_fpe_t<double>(op, gather);
         default: return nullptr;
     }
 }
@@ -342,12 +395,22 @@ bool try_rows(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_statu
     // (f64/f32/int64, rows and stride-2 views): 32 workgroups per CU with 2
     // vectors in flight per thread beat 8 per CU with 4 by 10-15%
     int unroll = (cfg && cfg->split > 0) ? cfg->split : 2;
+    // IEEE flags tracked: the float arithmetic ops have flag variants (one
+    // vector width, 4 in flight); other ops raise no flags
+    const bool fpe = A.fpe && is_float_dt(A.loop_dt) && A.op <= IONP_TRUE_DIVIDE;
     RowFn fn;
     if (gather) {
         // only the output is accessed by vector
         if (A.shape[0] % vec || !aligned(A.out, oes, A.out_st, 1)) return false;
         unroll = 4;
-        fn = row_gather(A.loop_dt, A.op);
+        fn = fpe ? row_fpe(A.loop_dt, A.op, true) : row_gather(A.loop_dt, A.op);
+    } else if (fpe) {
+        vec = A.op == IONP_TRUE_DIVIDE ? 4 : 16 / esz;
+        if (A.shape[0] % vec || !aligned(A.in[0], esz, ost0, A.in_st[0][0]) ||
+            !aligned(A.in[1], esz, ost1, A.in_st[1][0]) || !aligned(A.out, oes, A.out_st, 1))
+            return false;
+        unroll = 4;
+        fn = row_fpe(A.loop_dt, A.op, false);
     } else {
         while (vec > 1 && (A.shape[0] % vec || !aligned(A.in[0], esz, ost0, A.in_st[0][0]) ||
                            !aligned(A.in[1], esz, ost1, A.in_st[1][0]) || !aligned(A.out, oes, A.out_st, 1)))
@@ -383,6 +446,7 @@ bool try_rows(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_statu
     R.in[0] = A.in[0];
     R.in[1] = A.in[1];
     R.out = A.out;
+    R.fpe = A.fpe;
     const int64_t ntiles = R.rows * R.tiles_per_row;
     const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 32;
     int64_t grid = (int64_t)bpc * num_cus();
@@ -515,6 +579,7 @@ static TileFn tiled_kernel(int dt, int op) {
 // whatever stride it has).
 bool try_tiled(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_status *st) {
     if (A.nin != 2 || A.op < 0 || A.op >= IONP_NOPS || A.mixed_su || A.ndim < 2) return false;
+    if (A.fpe && is_float_dt(A.loop_dt) && A.op <= IONP_TRUE_DIVIDE) return false;  // flags: rows / generic
     if (A.in_dt[0] != A.loop_dt || A.in_dt[1] != A.loop_dt) return false;
     if (A.out_dt != (A.op >= IONP_LESS ? (int)IONP_BOOL : A.loop_dt) || A.out_st[0] != 1) return false;
     int yk = -1, dstar = -1;

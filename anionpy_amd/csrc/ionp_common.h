@@ -205,6 +205,7 @@ struct EwArgs {
     const char *in[3];            // address of element 0
     char *out;
     int64_t total;
+    uint32_t *fpe;                // IEEE flag word to OR into, or null (not tracked)
 };
 
 }  // namespace ionp

@@ -230,6 +230,22 @@ ionp_status ionp_plan_reduceat(int32_t op, const ionp_array_desc *in, int32_t ax
 ionp_status ionp_reduceat(int32_t op, const ionp_array_desc *in, const int64_t *indices, int64_t nidx, int32_t axis,
                           const ionp_array_desc *out, const ionp_config *cfg, void *stream);
 
+/* ---- IEEE exception flags (np.seterr / np.errstate) ------------------- */
+
+/* The fpe_check_after_call side channel (ionp-py/src/lib.rs:9077-9196,
+ * ionp-core/src/fpe.rs:173-360), computed on the device instead of by a
+ * host rescan: while tracking is on (process-wide), ionp_binary and
+ * ionp_cast OR the IEEE flags their elements raise -- as NumPy 2.2.6 reads
+ * them from the x86 status register after a loop -- into a flag word of
+ * the current GPU.  Bits: 1 divide by zero, 2 overflow, 4 underflow (tiny
+ * after rounding and inexact), 8 invalid.  Float add / subtract / multiply
+ * / true_divide and casts from a float dtype raise flags; comparisons,
+ * maximum / minimum, where and integer loops never do.  Reductions are not
+ * tracked. */
+ionp_status ionp_fpe_track(int32_t on);
+/* Waits for `stream`, returns the current GPU's flags and clears them. */
+ionp_status ionp_fpe_fetch(void *stream, uint32_t *flags);
+
 /* ---- batching: many small calls, one launch --------------------------- */
 /* No counterpart in the reference (it runs one call at a time). */
 

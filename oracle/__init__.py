@@ -64,6 +64,8 @@ def lib():
                            ("ora_reduce_where", [ctypes.c_int, P, ctypes.c_char_p, P, P, ctypes.c_int64,
                                                  ctypes.c_void_p]),
                            ("ora_binary_omp", [ctypes.c_int, P, P, P, ctypes.POINTER(ctypes.c_int)]),
+                           ("ora_binary_fpe", [ctypes.c_int, P, P, P, ctypes.POINTER(ctypes.c_int)]),
+                           ("ora_cast_fpe", [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
                            ("ora_reduceat", [ctypes.c_int, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
                                              ctypes.c_int64, P]),
                            ("ora_argext", [ctypes.c_int, P, ctypes.c_int, P]),
@@ -173,6 +175,28 @@ def binary_omp(op, a, b):
     nt = ctypes.c_int()
     _check(lib().ora_binary_omp(OP[op], ctypes.byref(da), ctypes.byref(db), ctypes.byref(do), ctypes.byref(nt)))
     return out, nt.value
+
+
+def binary_fpe(op, a, b):
+    """IEEE flags (1 divide, 2 over, 4 under, 8 invalid) that np.<op>(a, b)
+    raises, for op in add/subtract/multiply/true_divide."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    _, odt = binary_types(op, a.dtype, b.dtype)
+    shape = broadcast_shape(a.shape, b.shape)
+    out = _empty_strided(shape, korder_strides([a, b], shape), odt)
+    fl = ctypes.c_int()
+    _check(lib().ora_binary_fpe(OP[op], ctypes.byref(_desc(a)), ctypes.byref(_desc(b)), ctypes.byref(_desc(out)),
+                                ctypes.byref(fl)))
+    return fl.value
+
+
+def cast_fpe(x, dtype):
+    """IEEE flags that x.astype(dtype) raises."""
+    x = np.asarray(x)
+    fl = ctypes.c_int()
+    _check(lib().ora_cast_fpe(ctypes.byref(_desc(x)), _CODE[np.dtype(dtype)], ctypes.byref(fl)))
+    return fl.value
 
 
 def where(c, x, y):

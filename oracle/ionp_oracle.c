@@ -42,6 +42,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <fenv.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -634,6 +635,87 @@ int ora_cast(const ora_desc *in, const ora_desc *out)
             idx[i] = 0;
         }
     }
+    return ORA_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* IEEE exception flags (fpe_check_after_call, ionp-py/src/lib.rs:9077;     */
+/* ionp-core/src/fpe.rs:173-360), as NumPy 2.2.6 reads them from the x86    */
+/* status register: 1 divide by zero, 2 overflow, 4 underflow, 8 invalid.  */
+/* ------------------------------------------------------------------------ */
+static int fe_bits(void)
+{
+    int e = fetestexcept(FE_DIVBYZERO | FE_OVERFLOW | FE_UNDERFLOW | FE_INVALID), r = 0;
+    if (e & FE_DIVBYZERO) r |= 1;
+    if (e & FE_OVERFLOW) r |= 2;
+    if (e & FE_UNDERFLOW) r |= 4;
+    if (e & FE_INVALID) r |= 8;
+    return r;
+}
+
+/* ora_binary under the host's own status register: the scalar loop's
+ * arithmetic (plain C float ops on x86-64 SSE) raises exactly the flags
+ * NumPy's loop raises.  Arithmetic ops only (add, subtract, multiply,
+ * true_divide): the oracle's comparisons are ordered C compares, which
+ * signal invalid on NaN where NumPy's do not. */
+int ora_binary_fpe(int op, const ora_desc *a, const ora_desc *b, const ora_desc *out, int *flags)
+{
+    int rc;
+    if (op > OP_TRUE_DIVIDE) return ORA_ENOLOOP;
+    feclearexcept(FE_ALL_EXCEPT);
+    rc = ora_binary(op, a, b, out);
+    *flags = fe_bits();
+    feclearexcept(FE_ALL_EXCEPT);
+    return rc;
+}
+
+/* Flags of casting every element of `in` to dtype `to`: float -> integer
+ * raises invalid when the conversion instruction the compiler uses does
+ * not fit (cvttsd2si to 32 bits for int8..int32, to 64 bits for uint32 and
+ * int64; for uint64 the two-step sequence of cvtt_u64); float64 -> float32
+ * takes its overflow / underflow from the host's own conversion. */
+int ora_cast_fpe(const ora_desc *in, int to, int *flags)
+{
+    int n = in->ndim, i, r = 0;
+    int64_t idx[ORA_MAXDIM], total, e;
+    if (in->dtype != DT_F32 && in->dtype != DT_F64) {
+        *flags = 0;
+        return ORA_OK;
+    }
+    total = shape_size(n, in->shape);
+    memset(idx, 0, sizeof idx);
+    feclearexcept(FE_ALL_EXCEPT);
+    for (e = 0; e < total; e++) {
+        const char *p = elem_ptr(in, idx, in->strides, n);
+        const double x = in->dtype == DT_F32 ? (double)*(const float *)p : *(const double *)p;
+        switch (to) {
+        case DT_I8: case DT_U8: case DT_I16: case DT_U16: case DT_I32:
+            if (!(x > -2147483649.0 && x < 2147483648.0)) r |= 8;
+            break;
+        case DT_U32: case DT_I64:
+            if (!(x >= -9223372036854775808.0 && x < 9223372036854775808.0)) r |= 8;
+            break;
+        case DT_U64:
+            if (!(x >= -9223372036854775808.0 && x < 18446744073709551616.0)) r |= 8;
+            break;
+        case DT_F32:
+            if (in->dtype == DT_F64) {
+                volatile double vx = x;
+                volatile float f = (float)vx;
+                (void)f;
+            }
+            break;
+        default:
+            break;
+        }
+        for (i = n - 1; i >= 0; i--) {
+            if (++idx[i] < in->shape[i]) break;
+            idx[i] = 0;
+        }
+    }
+    r |= fe_bits() & 6; /* over / under from the float64 -> float32 conversions */
+    feclearexcept(FE_ALL_EXCEPT);
+    *flags = r;
     return ORA_OK;
 }
 

@@ -28,4 +28,7 @@ def ap():
         pytest.skip("no HIP device")
     import anionpy_amd
     anionpy_amd._lib.load()
+    # IEEE flag tracking off by default, so the tests exercise the plain
+    # kernels; tests of the flag variants and of np.errstate turn it on
+    anionpy_amd.seterr(all="ignore")
     return anionpy_amd

@@ -760,6 +760,89 @@ def reduceat_cases(S):
     S.add("rat/add/emptyfib", "reduceat", "add", [E], lambda: rat("add", E, [0, 2], 1), args([0, 2], 1))
 
 
+def fpe_cases(S):
+    """IEEE exception flags (np.errstate; ionp-py/src/lib.rs:9077 and
+    ionp-core/src/fpe.rs:173-360 on the reference side), as NumPy 2.2.6
+    raises them: 1 divide, 2 over, 4 under, 8 invalid, OR-ed over one call.
+    Rows of the special-value grid, products and quotients at the underflow
+    boundary (x86 tininess after rounding), casts to every integer width
+    and float64 -> float32, planted flags in large broadcast and strided
+    calls, integer true_divide.  Own generator (seed 84)."""
+    r9 = np.random.default_rng(84)
+
+    def flags_of(fn):
+        got = []
+        with np.errstate(all="call", call=lambda t, f: got.append(f)):
+            fn()
+        r = 0
+        for f in got:
+            r |= f
+        return np.array([r], dtype=np.int64)
+
+    ops = (("add", np.add), ("subtract", np.subtract), ("multiply", np.multiply), ("divide", np.divide))
+    for dt in (np.float64, np.float32):
+        nm = np.dtype(dt).name
+        fi = np.finfo(dt)
+        sp = np.array([0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, np.nan, fi.max, -fi.max, fi.tiny, -fi.tiny,
+                       fi.smallest_subnormal, 2.0, 0.5, 3.0, fi.tiny * 3], dtype=dt)
+        for name, fn in ops:
+            for i in range(len(sp)):
+                a = sp[i:i + 1]
+                S.add(f"fpe/{name}/{nm}/row{i}", "fpe", name, [a, sp], lambda: flags_of(lambda: fn(a, sp)))
+        for k in range(40):  # products and quotients at the underflow boundary, one element each
+            u = dt(r9.uniform(1, 2))
+            v = dt(fi.tiny / u * (1 + r9.uniform(-4, 4) * fi.eps))
+            A, B = np.array([u]), np.array([v])
+            S.add(f"fpe/multiply/{nm}/tiny{k}", "fpe", "multiply", [A, B], lambda: flags_of(lambda: np.multiply(A, B)))
+            C = np.array([dt(fi.tiny * (1 + r9.uniform(-4, 4) * fi.eps) * u)])
+            S.add(f"fpe/divide/{nm}/tiny{k}", "fpe", "divide", [C, A], lambda: flags_of(lambda: np.divide(C, A)))
+        # large calls: rows, gather and generic layouts with one planted flag
+        X = r9.standard_normal((100, 256)).astype(dt)
+        c = r9.uniform(0.5, 2, (100, 1)).astype(dt)
+        S.add(f"fpe/multiply/{nm}/rows/none", "fpe", "multiply", [X, c], lambda: flags_of(lambda: np.multiply(X, c)))
+        Xo = X.copy()
+        Xo[73, 77] = fi.max
+        c2 = c.copy()
+        c2[73, 0] = 4.0
+        S.add(f"fpe/multiply/{nm}/rows/over", "fpe", "multiply", [Xo, c2], lambda: flags_of(lambda: np.multiply(Xo, c2)))
+        Xu = X.copy()
+        Xu[7, 255] = fi.tiny
+        S.add(f"fpe/multiply/{nm}/rows/under", "fpe", "multiply", [Xu, c], lambda: flags_of(lambda: np.multiply(Xu, c)))
+        S.add(f"fpe/multiply/{nm}/gather/over", "fpe", "multiply", [Xo.T[::2], c2.T[:, :128]],
+              lambda: flags_of(lambda: np.multiply(Xo.T[::2], c2.T[:, :128])))
+        Xz = X.copy()
+        Xz[90, 3] = 0.0
+        S.add(f"fpe/divide/{nm}/rows/div", "fpe", "divide", [c, Xz], lambda: flags_of(lambda: np.divide(c, Xz)))
+        S.add(f"fpe/divide/{nm}/generic/inv", "fpe", "divide", [Xz.T, Xz[:, :1].T * 0],
+              lambda: flags_of(lambda: np.divide(Xz.T, Xz[:, :1].T * 0)))
+        Xi = X.copy()
+        Xi[0, 0] = np.inf
+        S.add(f"fpe/subtract/{nm}/rows/inv", "fpe", "subtract", [Xi, Xi[:, :1]],
+              lambda: flags_of(lambda: np.subtract(Xi, Xi[:, :1])))
+        S.add(f"fpe/add/{nm}/rows/over", "fpe", "add", [Xo, Xo], lambda: flags_of(lambda: np.add(Xo, Xo)))
+    # mixed and integer loops
+    I = np.array([1, 0, -3, 0], dtype=np.int64)
+    J = np.array([0, 0, 2, 5], dtype=np.int64)
+    S.add("fpe/divide/int64", "fpe", "divide", [I, J], lambda: flags_of(lambda: np.divide(I, J)))
+    S.add("fpe/divide/int64/nz", "fpe", "divide", [I, J + 1], lambda: flags_of(lambda: np.divide(I, J + 1)))
+    F32 = np.array([3e38, 1.0], dtype=np.float32)
+    F64 = np.array([10.0, 1e-320])
+    S.add("fpe/multiply/f32xf64", "fpe", "multiply", [F32, F64], lambda: flags_of(lambda: np.multiply(F32, F64)))
+    K = np.array([2 ** 62, 3], dtype=np.int64)
+    S.add("fpe/add/int64", "fpe", "add", [K, K], lambda: flags_of(lambda: np.add(K, K)))
+    # casts
+    vals = [300.0, 3e9, -129.0, 1e20, np.nan, 255.5, -1.0, 2.0 ** 64, 2.0 ** 63, -2.0 ** 63, -9.3e18, 3.5e38, 1e-40,
+            1e-46, np.inf, 3e38, 2.0 ** 31, -2.0 ** 31 - 1, -2.0 ** 31, 1.0e-39]
+    for i, v in enumerate(vals):
+        X = np.array([v, 1.0])
+        for t in ("int8", "uint8", "int16", "uint16", "int32", "uint32", "int64", "uint64", "bool", "float32"):
+            S.add(f"fpe/cast/{i}/{t}", "fpe", "cast", [X], lambda: flags_of(lambda: X.astype(t)), {"dtype": t})
+        if np.isfinite(v) and abs(v) < 3.4e38 or not np.isfinite(v):
+            X32 = np.array([v, 1.0], dtype=np.float32)
+            for t in ("int8", "int32", "uint32", "int64", "uint64"):
+                S.add(f"fpe/cast32/{i}/{t}", "fpe", "cast", [X32], lambda: flags_of(lambda: X32.astype(t)), {"dtype": t})
+
+
 def argext_cases(S, rng):
     for dt in (np.float64, np.float32, np.int64, np.int32, np.uint8, np.bool_):
         nm = np.dtype(dt).name
@@ -838,6 +921,7 @@ def main():
     prod_cases(S)
     where_reduce_cases(S)
     reduceat_cases(S)
+    fpe_cases(S)
     np.savez_compressed(os.path.join(HERE, "fixtures.npz"), **S.arrays)
     with open(os.path.join(HERE, "cases.json"), "w") as f:
         json.dump(S.cases, f)

@@ -7,7 +7,7 @@ from fixture_cases import args_axis, check_result, host_input, load
 
 pytestmark = pytest.mark.gpu
 
-KINDS = ["binary", "where", "cast", "reduce", "argext", "accumulate", "reduceat"]
+KINDS = ["binary", "where", "cast", "reduce", "argext", "accumulate", "reduceat", "fpe"]
 
 EXC = {"ValueError": ValueError, "TypeError": TypeError, "IndexError": IndexError,
        "AxisError": ValueError, "UFuncTypeError": TypeError}
@@ -24,6 +24,16 @@ def device_input(ap, spec, npz, cache):
     return ap.ndarray(cache[key], buf.dtype, spec["shape"], spec["strides"], spec["offset"])
 
 
+class _Host:
+    """A host result where the other kinds return device arrays."""
+
+    def __init__(self, a):
+        self.a = a
+
+    def get(self):
+        return self.a
+
+
 def run_device(ap, case, npz, cache):
     ins = [device_input(ap, s, npz, cache) for s in case["inputs"]]
     k, op, args = case["kind"], case["op"], case["args"]
@@ -37,6 +47,17 @@ def run_device(ap, case, npz, cache):
         return getattr(ap, op).reduce(ins[0], axis=args_axis(args), dtype=args.get("dtype"),
                                       keepdims=bool(args.get("keepdims", False)), initial=args.get("initial"),
                                       where=ins[1] if args.get("where") else True)
+    if k == "fpe":
+        got = []
+        with ap.errstate(all="call", call=lambda t, f: got.append(f)):
+            if op == "cast":
+                ins[0].astype(np.dtype(args["dtype"]))
+            else:
+                getattr(ap, op)(ins[0], ins[1])
+        r = 0
+        for f in got:
+            r |= f
+        return _Host(np.array([r], dtype=np.int64))
     if k == "reduceat":
         return getattr(ap, op).reduceat(ins[0], args["indices"], axis=args["axis"])
     if k == "accumulate":
@@ -46,19 +67,26 @@ def run_device(ap, case, npz, cache):
     return getattr(ap, op)(ins[0], axis=args.get("axis"), keepdims=bool(args.get("keepdims", False)))
 
 
-@pytest.mark.parametrize("kind", KINDS)
-def test_golden_on_gpu(ap, kind):
+# every kind with IEEE flag tracking off (the plain kernels), and the kinds
+# whose kernels have flag variants once more with tracking on (same bytes)
+RUNS = [(k, False) for k in KINDS] + [("binary", True), ("cast", True)]
+
+
+@pytest.mark.parametrize("kind,tracked", RUNS, ids=[k + ("-tracked" if t else "") for k, t in RUNS])
+def test_golden_on_gpu(ap, kind, tracked):
     cases, npz = load()
     cache = {}
     bad = []
     n = 0
+    ctx = ap.errstate(all="call", call=lambda t, f: None) if tracked else ap.errstate(all="ignore")
     for c in cases:
         if c["kind"] != kind:
             continue
         n += 1
         exp = c["expected"]
         try:
-            got = run_device(ap, c, npz, cache)
+            with ctx:
+                got = run_device(ap, c, npz, cache)
         except Exception as e:  # noqa: BLE001
             if "error" in exp and isinstance(e, EXC.get(exp["error"], Exception)):
                 if exp["error"] != "AxisError" and not exp["msg"].startswith(str(e)[:20]) and \

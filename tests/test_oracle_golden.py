@@ -5,7 +5,7 @@ import pytest
 import oracle as O
 from fixture_cases import args_axis, check_result, host_input, load
 
-KINDS = ["binary", "where", "cast", "reduce", "argext", "accumulate", "reduceat"]
+KINDS = ["binary", "where", "cast", "reduce", "argext", "accumulate", "reduceat", "fpe"]
 
 
 def run_oracle(case, npz):
@@ -28,6 +28,10 @@ def run_oracle(case, npz):
             axes = range(x.ndim) if ax is None else ((ax,) if isinstance(ax, int) else ax)
             r = np.expand_dims(r, tuple(a % x.ndim for a in axes))
         return r
+    if k == "fpe":
+        if op == "cast":
+            return np.array([O.cast_fpe(ins[0], np.dtype(args["dtype"]))], dtype=np.int64)
+        return np.array([O.binary_fpe("true_divide" if op == "divide" else op, ins[0], ins[1])], dtype=np.int64)
     if k == "reduceat":
         return O.reduceat(op, ins[0], args["indices"], axis=args["axis"])
     if k == "accumulate":
