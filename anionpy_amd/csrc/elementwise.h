@@ -18,6 +18,7 @@ struct RowArgs {
     const char *in[2];
     char *out;
     uint32_t *fpe;  // IEEE flag word (FPE variants only)
+    int32_t nt_loads;  // stream the inputs with nontemporal loads
 };
 
 int num_cus();

@@ -177,7 +177,8 @@ def main():
 
 def timed(torch, fn, reps):
     s = torch.cuda.current_stream()
-    fn()
+    for _ in range(3):
+        fn()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(s)
@@ -193,11 +194,11 @@ def reduction_extras(ap, torch):
     out = {}
     x = ap.asarray(np.random.default_rng(4).uniform(-1, 1, 2 ** 28))
     nb = x.nbytes + 8
-    t = timed(torch, lambda: ap.add.reduce(x), 5)
+    t = timed(torch, lambda: ap.add.reduce(x), 10)
     out["sum_f64_2^28"] = {"us": round(t * 1e6, 1), "GB/s": round(nb / t / 1e9, 1), "frac": round(nb / t / PEAK_HBM, 4)}
-    t = timed(torch, lambda: ap.maximum.reduce(x), 5)
+    t = timed(torch, lambda: ap.maximum.reduce(x), 10)
     out["max_f64_2^28"] = {"us": round(t * 1e6, 1), "GB/s": round(nb / t / 1e9, 1), "frac": round(nb / t / PEAK_HBM, 4)}
-    t = timed(torch, lambda: ap.argmax(x), 5)
+    t = timed(torch, lambda: ap.argmax(x), 10)
     out["argmax_f64_2^28"] = {"us": round(t * 1e6, 1), "GB/s": round(nb / t / 1e9, 1),
                               "frac": round(nb / t / PEAK_HBM, 4)}
     del x
