@@ -34,7 +34,7 @@ __all__ = ["ndarray", "asarray", "empty", "zeros", "Ufunc", "add", "subtract", "
            "get_config_policy", "prod", "cumprod", "seterr", "geterr", "seterrcall", "geterrcall", "errstate"]
 
 _NP_DTYPES = [np.bool_, np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64,
-              np.uint64, np.float32, np.float64]
+              np.uint64, np.float32, np.float64, np.float16]
 _CODE = {np.dtype(d): i for i, d in enumerate(_NP_DTYPES)}
 
 
@@ -237,7 +237,7 @@ class ndarray:
 
     def astype(self, dtype, order="K"):
         dtype = _dtype(dtype)
-        track = self.dtype.kind == "f" and _fpe_wanted()
+        track = (self.dtype.kind == "f" or dtype == np.float16) and _fpe_wanted()  # int -> float16 can overflow
         _fpe_arm(track)
         r = self._astype(dtype, order)
         if track:

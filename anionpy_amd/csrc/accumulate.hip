@@ -270,6 +270,22 @@ template <int LT, int OP> __global__ __launch_bounds__(64) void acc_tiled(const 
     }
 }
 
+// float16 scans: each step rounded to float16 (hstep), a thread per fiber
+__global__ __launch_bounds__(256) void acc_half(const AccArgs A, int op) {
+    for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < A.nfib; f += (int64_t)gridDim.x * blockDim.x) {
+        int64_t ib, ob;
+        fiber_offsets(A, (uint32_t)f, &ib, &ob);
+        const uint16_t *x = (const uint16_t *)A.in + ib;
+        uint16_t *o = (uint16_t *)A.out + ob;
+        uint16_t acc = x[0];
+        o[0] = acc;
+        for (int64_t i = 1; i < A.len; i++) {
+            acc = hstep(op, acc, x[i * A.in_st]);
+            o[i * A.out_st] = acc;
+        }
+    }
+}
+
 typedef void (*AccFn)(const AccArgs);
 
 enum { ACC_THREAD = 0, ACC_WAVE = 1, ACC_TILED = 2 };
@@ -338,6 +354,14 @@ ionp_status run_accumulate(int op, const ionp_array_desc *in, int axis, const io
     // contiguous along the axis: 64 fibers per wavefront through LDS when
     // there are enough of them, else the LDS chain per fiber; otherwise
     // (neighbouring fibers adjacent in memory) a thread per fiber
+    if (out->dtype == IONP_FLOAT16) {
+        const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
+        int64_t grid = (int64_t)bpc * num_cus(), units = (A.nfib + 255) / 256;
+        if (grid > units) grid = units;
+        hipLaunchKernelGGL(acc_half, dim3((unsigned)grid), dim3(256), 0, s, A, op);
+        const hipError_t e = hipGetLastError();
+        return e == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, hipGetErrorString(e));
+    }
     int kind;
     if (A.in_st == 1 && A.nfib >= 256) kind = ACC_TILED;
     else if (A.nfib < 4096 && A.len >= 256) kind = ACC_WAVE;

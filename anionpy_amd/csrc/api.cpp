@@ -87,6 +87,23 @@ ionp_status attach_fpe(EwArgs &A) {
     return fpe_word(&A.fpe);
 }
 
+// `in` cast to dtype `dt` into the library's stage buffer, dense in the
+// input's K order (NumPy's buffered cast ahead of a loop)
+ionp_status cast_copy(const ionp_array_desc *in, int dt, ionp_array_desc *c, hipStream_t st) {
+    ionp_status s;
+    if ((s = ionp_plan_cast(in, dt, c))) return s;
+    int64_t nel = 1;
+    for (int i = 0; i < in->ndim; i++) nel *= in->shape[i];
+    void *tmp = stage_get((size_t)(nel > 0 ? nel : 1) * itemsize(dt), st);
+    if (!tmp) return set_error(IONP_ERR_DEVICE, "out of device memory for a converted input");
+    c->data = tmp;
+    c->offset = 0;
+    const ionp_array_desc *ins[1] = {in};
+    EwArgs E;
+    if ((s = plan_elementwise(OP_CAST, 1, ins, c, dt, &E))) return s;
+    return run_elementwise(E, nullptr, st);
+}
+
 bool is_mixed_su(int op, int a, int b) {
     return op >= IONP_LESS && ((a == IONP_INT64 && b == IONP_UINT64) || (a == IONP_UINT64 && b == IONP_INT64));
 }
@@ -195,10 +212,10 @@ static ionp_status reduce_dtype(int32_t op, int in_dt, int32_t dtype, int *out_d
     if (dt < 0 || dt >= IONP_NTYPES) return set_error(IONP_ERR_TYPE, "invalid dtype code");
     // integer maximum/minimum cast every element to the loop dtype first;
     // the float x86 loop model covers the input's own dtype only
-    if ((op == IONP_MAXIMUM || op == IONP_MINIMUM) && dt != in_dt && (is_float_dt(dt) || is_float_dt(in_dt)))
+    if ((op == IONP_MAXIMUM || op == IONP_MINIMUM) && dt != in_dt && (is_real_float(dt) || is_real_float(in_dt)))
         return set_error(IONP_ERR_UNSUPPORTED, "float maximum/minimum.reduce with a dtype other than the input's");
     // same_kind casting of the input to the loop dtype (a float can not feed an integer loop)
-    if (is_float_dt(in_dt) && !is_float_dt(dt))
+    if (is_real_float(in_dt) && !is_real_float(dt))
         return set_error(IONP_ERR_TYPE, std::string("Cannot cast ufunc '") + op_name(op) + "' input from dtype('" +
                                             dtype_name(in_dt) + "') to dtype('" + dtype_name(dt) +
                                             "') with casting rule 'same_kind'");
@@ -377,6 +394,11 @@ ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *ax
     if ((s = same_shape(out, t.ndim, t.shape, "reduce"))) return s;
     bool mask[IONP_MAX_DIMS];
     normalize_axes(axes, naxes, in->ndim, mask);
+    ionp_array_desc c;
+    if (in->dtype == IONP_FLOAT16 && out->dtype != IONP_FLOAT16) {  // the float32/float64 loop reads a cast copy
+        if ((s = cast_copy(in, out->dtype, &c, as_stream(stream)))) return s;
+        in = &c;
+    }
     ReducePlan p;
     if ((s = plan_reduce(op, in, mask, out, &p))) return s;
     return run_reduce(p, g_bufsize, initial, cfg, as_stream(stream));
@@ -397,6 +419,8 @@ ionp_status ionp_reduce_where(int32_t op, const ionp_array_desc *in, const ionp_
         return set_error(IONP_ERR_VALUE, std::string("reduction operation '") + op_name(op) +
                                              "' does not have an identity, so to use a where mask one has to "
                                              "specify 'initial'");
+    if (in->dtype == IONP_FLOAT16 || out->dtype == IONP_FLOAT16)
+        return set_error(IONP_ERR_UNSUPPORTED, "float16 where= reductions are not provided");
     int64_t wst[IONP_MAX_DIMS];
     if ((s = where_strides(in, where, wst))) return s;
     bool mask[IONP_MAX_DIMS];
@@ -443,17 +467,9 @@ ionp_status ionp_accumulate(int32_t op, const ionp_array_desc *in, int32_t axis,
     acc_axis(in, axis, &ax);
     hipStream_t st = as_stream(stream);
     if (in->dtype == out->dtype) return run_accumulate(op, in, ax, out, cfg, st);
-    // another input dtype: cast into scratch first (dense, K order), then scan
-    const int64_t nel = desc_size(in);
-    void *tmp = scratch_get((size_t)(nel > 0 ? nel : 1) * itemsize(out->dtype), st);
-    if (!tmp) return set_error(IONP_ERR_DEVICE, "out of device memory for accumulate scratch");
-    ionp_array_desc c = t;
-    c.data = tmp;
-    c.offset = 0;
-    const ionp_array_desc *ins[1] = {in};
-    EwArgs E;
-    if ((s = plan_elementwise(OP_CAST, 1, ins, &c, out->dtype, &E))) return s;
-    if ((s = run_elementwise(E, nullptr, st))) return s;
+    // another input dtype: cast first (dense, K order), then scan
+    ionp_array_desc c;
+    if ((s = cast_copy(in, out->dtype, &c, st))) return s;
     return run_accumulate(op, &c, ax, out, cfg, st);
 }
 
@@ -550,6 +566,11 @@ ionp_status ionp_argext(int32_t ismax, const ionp_array_desc *in, int32_t axis, 
     if (out->dtype != IONP_INT64) return set_error(IONP_ERR_TYPE, "argmax/argmin output must be int64");
     int ax;
     arg_normalize(in, axis, &ax);
+    ionp_array_desc c;
+    if (in->dtype == IONP_FLOAT16) {  // float16 -> float32 is exact and keeps the order, NaNs included
+        if ((s = cast_copy(in, IONP_FLOAT32, &c, as_stream(stream)))) return s;
+        in = &c;
+    }
     ArgArgs A;
     std::memset(&A, 0, sizeof A);
     A.in_dt = in->dtype;

@@ -458,6 +458,10 @@ __global__ __launch_bounds__(kBatchThreads) void batch_fold(const BatchCall *__r
 ionp_status plan_call(const ionp_call &c, BatchCall *B, int64_t *bytes) {
     std::memset(B, 0, sizeof *B);
     ionp_status s;
+    const int nops = c.kind == IONP_CALL_WHERE ? 3 : (c.kind == IONP_CALL_BINARY ? 2 : 1);
+    for (int k = 0; k < nops; k++)
+        if (c.in[k].dtype == IONP_FLOAT16) return set_error(IONP_ERR_UNSUPPORTED, "float16 calls are not batched");
+    if (c.out.dtype == IONP_FLOAT16) return set_error(IONP_ERR_UNSUPPORTED, "float16 calls are not batched");
     if (c.kind == IONP_CALL_BINARY || c.kind == IONP_CALL_WHERE || c.kind == IONP_CALL_CAST) {
         ionp_array_desc t;
         int nin, op, loop;

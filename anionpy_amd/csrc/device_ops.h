@@ -278,6 +278,127 @@ __device__ inline uint32_t ew_element_fpe(const EwArgs &A, const char *p0, const
 }
 
 // ---------------------------------------------------------------------------
+// float16 elements (NumPy's HALF loops): a float16 loop computes in float32
+// and rounds the result to float16 (npy_float_to_half); maximum / minimum
+// return an operand's own bits, the first on ties and when it is NaN
+// (npy_half_ge / npy_half_le are false on NaN); comparisons compare the
+// exact float values.  Casts to float16 round the float32 / float64 source
+// directly (integers through float64, exact in float16's range); casts from
+// float16 go through the exact float64 value.
+// ---------------------------------------------------------------------------
+// any dtype as float16 bits (the buffered cast NumPy applies first)
+__device__ inline uint16_t to_half_bits(int dt, const char *p) {
+    uint32_t fl = 0;
+    switch (dt) {
+        case IONP_FLOAT16: return *(const uint16_t *)p;
+        case IONP_FLOAT32: return f32_to_f16_bits(*(const uint32_t *)p, &fl);
+        case IONP_FLOAT64: return f64_to_f16_bits(*(const uint64_t *)p, &fl);
+        default: return d2h(load_as<IONP_FLOAT64>(dt, p), &fl);
+    }
+}
+// one element-wise float16 step acc (op) x: add / multiply in float32
+// rounded back, maximum / minimum keep acc on ties and when it is NaN
+__device__ inline uint16_t hstep(int op, uint16_t acc, uint16_t x) {
+    const float a = h2f(acc), b = h2f(x);
+    uint32_t fl = 0;
+    switch (op) {
+        case IONP_ADD: return f2h(fadd(a, b), &fl);
+        case IONP_MULTIPLY: return f2h(fmul(a, b), &fl);
+        case IONP_MAXIMUM: return (a >= b || a != a) ? acc : x;
+        default: return (a <= b || a != a) ? acc : x;
+    }
+}
+// value of a float16-loop operand (float16, bool or an 8-bit integer): exact
+__device__ inline float hval(int dt, const char *p) {
+    return dt == IONP_FLOAT16 ? h2f(*(const uint16_t *)p) : load_as<IONP_FLOAT32>(dt, p);
+}
+// the same operand as float16 bits (exact)
+__device__ inline uint16_t hbits(int dt, const char *p) { return to_half_bits(dt, p); }
+// any dtype in loop dtype LT (float32 / float64), float16 exactly
+template <int LT> __device__ inline typename dt_type<LT>::T load_hx(int dt, const char *p) {
+    if (dt == IONP_FLOAT16) {
+        const uint16_t h = *(const uint16_t *)p;
+        if constexpr (LT == IONP_FLOAT32) return h2f(h);
+        else return cvt<LT>(h2d(h));
+    }
+    return load_as<LT>(dt, p);
+}
+// element cast with float16 on either side; returns its IEEE flags
+__device__ inline uint32_t cast_half(int in_dt, const char *p, int out_dt, char *po) {
+    uint32_t fl = 0;
+    if (out_dt == IONP_FLOAT16) {
+        uint16_t h;
+        switch (in_dt) {
+            case IONP_FLOAT16: h = *(const uint16_t *)p; break;
+            case IONP_FLOAT32: h = f32_to_f16_bits(*(const uint32_t *)p, &fl); break;
+            case IONP_FLOAT64: h = f64_to_f16_bits(*(const uint64_t *)p, &fl); break;
+            default: h = d2h(load_as<IONP_FLOAT64>(in_dt, p), &fl); break;
+        }
+        *(uint16_t *)po = h;
+        return fl;
+    }
+    if (in_dt != IONP_FLOAT16) {  // neither side float16 (a where operand)
+        store_as<double>(out_dt, po, load_as<IONP_FLOAT64>(in_dt, p));
+        return 0;
+    }
+    const uint16_t h = *(const uint16_t *)p;
+    if (out_dt == IONP_FLOAT32) {
+        *(uint32_t *)po = f16_to_f32_bits(h);
+        return 0;
+    }
+    if (out_dt == IONP_FLOAT64) {
+        *(uint64_t *)po = f16_to_f64_bits(h);
+        return 0;
+    }
+    const double x = h2d(h);
+    store_as<double>(out_dt, po, x);
+    return fpe_cast(IONP_FLOAT64, out_dt, (const char *)&x);
+}
+// one element of an elementwise call that involves float16
+__device__ inline uint32_t ew_half_elem(const EwArgs &A, const char *p0, const char *p1, const char *p2, char *po) {
+    const int op = A.op;
+    if (op == OP_CAST) return cast_half(A.in_dt[0], p0, A.out_dt, po);
+    if (op == OP_WHERE) {
+        const bool c = load_as<IONP_BOOL>(A.in_dt[0], p0) != 0;
+        return cast_half(c ? A.in_dt[1] : A.in_dt[2], c ? p1 : p2, A.out_dt, po);
+    }
+    if (A.loop_dt == IONP_FLOAT16) {
+        const float a = hval(A.in_dt[0], p0), b = hval(A.in_dt[1], p1);
+        if (op >= IONP_LESS) {
+            *(uint8_t *)po = (uint8_t)cmp_op<float>(op, a, b);
+            return 0;
+        }
+        if (op == IONP_MAXIMUM || op == IONP_MINIMUM) {
+            const bool first = (op == IONP_MAXIMUM ? a >= b : a <= b) || a != a;
+            *(uint16_t *)po = first ? hbits(A.in_dt[0], p0) : hbits(A.in_dt[1], p1);
+            return 0;
+        }
+        const float r = arith_op<IONP_FLOAT32>(op, a, b);
+        uint32_t fl = fpe_arith_dyn<float>(op, a, b, r);
+        *(uint16_t *)po = f2h(r, &fl);
+        return fl;
+    }
+    if (A.loop_dt == IONP_FLOAT32) {
+        const float a = load_hx<IONP_FLOAT32>(A.in_dt[0], p0), b = load_hx<IONP_FLOAT32>(A.in_dt[1], p1);
+        if (op >= IONP_LESS) {
+            *(uint8_t *)po = (uint8_t)cmp_op<float>(op, a, b);
+            return 0;
+        }
+        const float r = arith_op<IONP_FLOAT32>(op, a, b);
+        store_as<float>(A.out_dt, po, r);
+        return fpe_arith_dyn<float>(op, a, b, r);
+    }
+    const double a = load_hx<IONP_FLOAT64>(A.in_dt[0], p0), b = load_hx<IONP_FLOAT64>(A.in_dt[1], p1);
+    if (op >= IONP_LESS) {
+        *(uint8_t *)po = (uint8_t)cmp_op<double>(op, a, b);
+        return 0;
+    }
+    const double r = arith_op<IONP_FLOAT64>(op, a, b);
+    store_as<double>(A.out_dt, po, r);
+    return fpe_arith_dyn<double>(op, a, b, r);
+}
+
+// ---------------------------------------------------------------------------
 // index helpers
 // ---------------------------------------------------------------------------
 // decompose idx over dims (innermost first) and dot with strides

@@ -53,6 +53,35 @@ template <int LT, bool FPE = false> __global__ __launch_bounds__(256) void ew_ge
     if constexpr (FPE) fpe_commit(A.fpe, fl);
 }
 
+// Calls with a float16 operand or output: the generic walk with
+// ew_half_elem; FPE: OR the IEEE flags into *A.fpe.
+template <bool FPE> __global__ __launch_bounds__(256) void ew_half(const EwArgs A) {
+    const int i0 = itemsize(A.in_dt[0]), i1 = itemsize(A.in_dt[1]), i2 = itemsize(A.in_dt[2]);
+    const int io = itemsize(A.out_dt);
+    const uint32_t total = (uint32_t)A.total;
+    uint32_t fl = 0;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        uint32_t rem = e;
+        int64_t o0 = 0, o1 = 0, o2 = 0, oo = 0;
+        for (int d = 0; d < A.ndim; d++) {
+            uint32_t c;
+            if (d + 1 < A.ndim) {
+                const uint32_t q = A.div[d].div(rem);
+                c = rem - q * (uint32_t)A.shape[d];
+                rem = q;
+            } else {
+                c = rem;
+            }
+            o0 += (int64_t)c * A.in_st[0][d];
+            o1 += (int64_t)c * A.in_st[1][d];
+            o2 += (int64_t)c * A.in_st[2][d];
+            oo += (int64_t)c * A.out_st[d];
+        }
+        fl |= ew_half_elem(A, A.in[0] + o0 * i0, A.in[1] + o1 * i1, A.in[2] + o2 * i2, A.out + oo * io);
+    }
+    if constexpr (FPE) fpe_commit(A.fpe, fl);
+}
+
 // ---------------------------------------------------------------------------
 // rows x inner fast path
 // ---------------------------------------------------------------------------
@@ -243,6 +272,8 @@ ionp_status launch_generic(EwArgs A, const ionp_config *cfg, hipStream_t s) {
     for (int d = 0; d < last; d++) inner *= A.shape[d];
     if (inner >= (1ll << 31)) return set_error(IONP_ERR_UNSUPPORTED, "inner extent too large");
     for (int d = 0; d < A.ndim; d++) A.div[d] = FastDiv((uint32_t)(A.shape[d] > 0 ? A.shape[d] : 1));
+    bool half = A.out_dt == IONP_FLOAT16;
+    for (int k = 0; k < A.nin; k++) half = half || A.in_dt[k] == IONP_FLOAT16;
     const int64_t outer = A.shape[last];
     int64_t step = ((1ll << 31) - 1) / inner;
     if (step < 1) step = 1;
@@ -253,8 +284,9 @@ ionp_status launch_generic(EwArgs A, const ionp_config *cfg, hipStream_t s) {
         B.total = inner * n;
         for (int k = 0; k < A.nin; k++) B.in[k] = A.in[k] + o0 * A.in_st[k][last] * itemsize(A.in_dt[k]);
         B.out = A.out + o0 * A.out_st[last] * itemsize(A.out_dt);
-        hipLaunchKernelGGL(generic_kernel(A.loop_dt, A.fpe != nullptr), dim3(grid_for(B.total, block, cfg)), dim3(block),
-                           0, s, B);
+        GenericFn fn = generic_kernel(A.loop_dt, A.fpe != nullptr);
+        if (half) fn = A.fpe ? ew_half<true> : ew_half<false>;
+        hipLaunchKernelGGL(fn, dim3(grid_for(B.total, block, cfg)), dim3(block), 0, s, B);
     }
     return hipGetLastError() == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, "kernel launch failed");
 }
@@ -655,6 +687,9 @@ bool try_tiled(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_stat
 
 ionp_status run_elementwise(EwArgs &A, const ionp_config *cfg, hipStream_t s) {
     if (A.total == 0) return IONP_OK;
+    bool half = A.out_dt == IONP_FLOAT16;
+    for (int k = 0; k < A.nin; k++) half = half || A.in_dt[k] == IONP_FLOAT16;
+    if (half) return launch_generic(A, cfg, s);  // float16: the generic walk only
     ionp_status st;
     if (try_tiled(A, cfg, s, &st)) return st;
     if (try_rows(A, cfg, s, &st)) return st;

@@ -32,12 +32,15 @@ enum : int32_t {
 };
 
 __host__ __device__ constexpr int itemsize(int dt) {
-    return dt == IONP_BOOL || dt == IONP_INT8 || dt == IONP_UINT8     ? 1
-           : dt == IONP_INT16 || dt == IONP_UINT16                    ? 2
-           : dt == IONP_INT32 || dt == IONP_UINT32 || dt == IONP_FLOAT32 ? 4
-                                                                         : 8;
+    return dt == IONP_BOOL || dt == IONP_INT8 || dt == IONP_UINT8                    ? 1
+           : dt == IONP_INT16 || dt == IONP_UINT16 || dt == IONP_FLOAT16              ? 2
+           : dt == IONP_INT32 || dt == IONP_UINT32 || dt == IONP_FLOAT32              ? 4
+                                                                                        : 8;
 }
+// float32 / float64: the dtypes the native float kernels compute in
 __host__ __device__ constexpr bool is_float_dt(int dt) { return dt == IONP_FLOAT32 || dt == IONP_FLOAT64; }
+// any floating dtype, float16 included (dtype rules)
+__host__ __device__ constexpr bool is_real_float(int dt) { return is_float_dt(dt) || dt == IONP_FLOAT16; }
 __host__ __device__ constexpr bool is_unsigned_dt(int dt) {
     return dt == IONP_UINT8 || dt == IONP_UINT16 || dt == IONP_UINT32 || dt == IONP_UINT64;
 }
@@ -73,6 +76,133 @@ __host__ __device__ inline float default_nan(float) { return f32_of(0xffc00000u)
 __host__ __device__ inline double canonical_nan(double) { return f64_of(0x7ff8000000000000ull); }
 __host__ __device__ inline float canonical_nan(float) { return f32_of(0x7fc00000u); }
 template <typename T> __host__ __device__ inline bool isnan_(T x) { return x != x; }
+
+// ---------------------------------------------------------------------------
+// float16 (NumPy's npy_half): conversions as NumPy 2.2.6 performs them in
+// software (npy_half_to_float / npy_float_to_half / npy_double_to_half):
+// round to nearest even, NaN payloads kept (top significand bits, never
+// quieted, an all-zero payload becomes 1), overflow (a finite value
+// rounding to inf) and underflow (a result below the smallest normal half
+// that is not exact -- tininess before rounding) reported in *fl as the
+// IEEE flag bits 2 / 4.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint32_t f16_to_f32_bits(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    const uint32_t e = h & 0x7c00u, m = h & 0x03ffu;
+    if (e == 0x7c00u) return sign | 0x7f800000u | (m << 13);  // inf / NaN, payload kept
+    if (e != 0) return sign | (((e >> 10) + 112u) << 23) | (m << 13);
+    if (m == 0) return sign;
+    // subnormal half: m * 2^-24, normalised
+    int sh = 0;
+    uint32_t mm = m;
+    while (!(mm & 0x0400u)) {
+        mm <<= 1;
+        sh++;
+    }
+    return sign | ((uint32_t)(113 - sh) << 23) | ((mm & 0x03ffu) << 13);
+}
+__host__ __device__ inline uint64_t f16_to_f64_bits(uint16_t h) {
+    const uint64_t sign = (uint64_t)(h & 0x8000u) << 48;
+    const uint32_t e = h & 0x7c00u, m = h & 0x03ffu;
+    if (e == 0x7c00u) return sign | 0x7ff0000000000000ull | ((uint64_t)m << 42);
+    const uint32_t fb = f16_to_f32_bits(h);  // finite: exact through float
+    const double d = (double)__builtin_bit_cast(float, fb);
+    return __builtin_bit_cast(uint64_t, d) | (m == 0 && e == 0 ? sign : 0);
+}
+// round the significand `sig` (implicit bit included) right by `drop` bits,
+// nearest even; *inexact set when bits were lost
+__host__ __device__ inline uint64_t rne_shift(uint64_t sig, int drop, bool *inexact) {
+    if (drop <= 0) {
+        *inexact = false;
+        return sig << (-drop);
+    }
+    if (drop >= 64) {
+        *inexact = sig != 0;
+        return 0;
+    }
+    const uint64_t q = sig >> drop, rem = sig & ((1ull << drop) - 1), half = 1ull << (drop - 1);
+    *inexact = rem != 0;
+    return q + ((rem > half || (rem == half && (q & 1))) ? 1 : 0);
+}
+// value = sig * 2^(e2) (sig with its implicit bit, e2 the exponent of the
+// significand's last bit) to half bits
+__host__ __device__ inline uint16_t f16_round(uint32_t sign, uint64_t sig, int e2, int msb, uint32_t *fl) {
+    // msb: index of sig's top bit; the value's binary exponent is e2 + msb
+    const int ex = e2 + msb;
+    if (ex >= 16) {  // >= 2^16: inf
+        *fl |= 2;
+        return (uint16_t)(sign | 0x7c00u);
+    }
+    bool inexact;
+    if (ex < -14) {  // half subnormal range: units of 2^-24
+        const uint64_t q = rne_shift(sig, -24 - e2, &inexact);
+        if (inexact) *fl |= 4;
+        return (uint16_t)(sign | (uint32_t)q);  // q == 0x400 is the smallest normal
+    }
+    const uint64_t q = rne_shift(sig, msb - 10, &inexact);  // 11 significant bits
+    const uint32_t h = (uint32_t)((ex + 15) << 10) + (uint32_t)(q - 0x400u);  // a carry moves into the exponent
+    if (h >= 0x7c00u) {
+        *fl |= 2;
+        return (uint16_t)(sign | 0x7c00u);
+    }
+    return (uint16_t)(sign | h);
+}
+__host__ __device__ inline uint16_t f32_to_f16_bits(uint32_t f, uint32_t *fl) {
+    const uint32_t sign = (f >> 16) & 0x8000u, a = f & 0x7fffffffu;
+    if (a >= 0x7f800000u) {
+        if (a == 0x7f800000u) return (uint16_t)(sign | 0x7c00u);
+        uint32_t r = 0x7c00u + ((a & 0x007fffffu) >> 13);
+        if (r == 0x7c00u) r++;
+        return (uint16_t)(sign | r);
+    }
+    if (a == 0) return (uint16_t)sign;
+    const int e = (int)(a >> 23);
+    uint64_t sig = a & 0x007fffffu;
+    int e2;
+    if (e == 0) {
+        e2 = -149;  // float subnormal
+    } else {
+        sig |= 0x00800000u;
+        e2 = e - 150;
+    }
+    const int msb = 63 - __builtin_clzll(sig);
+    if (e2 + msb < -25) {  // far below half's range: zero, underflowed
+        *fl |= 4;
+        return (uint16_t)sign;
+    }
+    return f16_round(sign, sig, e2, msb, fl);
+}
+__host__ __device__ inline uint16_t f64_to_f16_bits(uint64_t d, uint32_t *fl) {
+    const uint32_t sign = (uint32_t)(d >> 48) & 0x8000u;
+    const uint64_t a = d & 0x7fffffffffffffffull;
+    if (a >= 0x7ff0000000000000ull) {
+        if (a == 0x7ff0000000000000ull) return (uint16_t)(sign | 0x7c00u);
+        uint32_t r = 0x7c00u + (uint32_t)((a & 0x000fffffffffffffull) >> 42);
+        if (r == 0x7c00u) r++;
+        return (uint16_t)(sign | r);
+    }
+    if (a == 0) return (uint16_t)sign;
+    const int e = (int)(a >> 52);
+    uint64_t sig = a & 0x000fffffffffffffull;
+    int e2;
+    if (e == 0) {
+        e2 = -1074;
+    } else {
+        sig |= 0x0010000000000000ull;
+        e2 = e - 1075;
+    }
+    const int msb = 63 - __builtin_clzll(sig);
+    if (e2 + msb < -25) {
+        *fl |= 4;
+        return (uint16_t)sign;
+    }
+    return f16_round(sign, sig, e2, msb, fl);
+}
+__host__ __device__ inline float h2f(uint16_t h) { return __builtin_bit_cast(float, f16_to_f32_bits(h)); }
+__host__ __device__ inline double h2d(uint16_t h) { return __builtin_bit_cast(double, f16_to_f64_bits(h)); }
+__host__ __device__ inline uint16_t f2h(float x, uint32_t *fl) { return f32_to_f16_bits(__builtin_bit_cast(uint32_t, x), fl); }
+__host__ __device__ inline uint16_t d2h(double x, uint32_t *fl) { return f64_to_f16_bits(__builtin_bit_cast(uint64_t, x), fl); }
+__host__ __device__ inline bool h_isnan(uint16_t h) { return (h & 0x7c00u) == 0x7c00u && (h & 0x03ffu); }
 
 // Slow path of an arithmetic op whose hardware result was NaN.
 template <typename T> __host__ __device__ __noinline__ T nan_result(T a, T b) {

@@ -31,7 +31,7 @@ std::string fmt_shape(const int64_t *shape, int n) {
 
 const char *dtype_name(int dt) {
     static const char *names[] = {"bool", "int8", "uint8", "int16", "uint16", "int32",
-                                  "uint32", "int64", "uint64", "float32", "float64"};
+                                  "uint32", "int64", "uint64", "float32", "float64", "float16"};
     return (dt >= 0 && dt < IONP_NTYPES) ? names[dt] : "?";
 }
 const char *op_name(int op) {
@@ -51,6 +51,12 @@ int promote(int a, int b) {
     if (a == b) return a;
     if (a == IONP_BOOL) return b;
     if (b == IONP_BOOL) return a;
+    if (a == IONP_FLOAT16 || b == IONP_FLOAT16) {  // float16 holds 8-bit integers exactly
+        const int o = a == IONP_FLOAT16 ? b : a;
+        if (itemsize(o) == 1) return IONP_FLOAT16;
+        if (o == IONP_FLOAT32 || itemsize(o) == 2) return IONP_FLOAT32;
+        return IONP_FLOAT64;
+    }
     const bool fa = is_float_dt(a), fb = is_float_dt(b);
     if (fa || fb) {
         if (fa && fb) return itemsize(a) >= itemsize(b) ? a : b;
@@ -75,7 +81,7 @@ ionp_status binary_types(int op, int a, int b, int *loop, int *out) {
         return set_error(IONP_ERR_TYPE, "invalid dtype code");
     int rt = promote(a, b);
     if (op == IONP_TRUE_DIVIDE) {
-        if (!is_float_dt(rt)) rt = IONP_FLOAT64;
+        if (!is_real_float(rt)) rt = IONP_FLOAT64;
         *loop = *out = rt;
         return IONP_OK;
     }

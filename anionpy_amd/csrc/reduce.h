@@ -91,5 +91,7 @@ ionp_status run_reduceat(RatArgs &a, const int64_t *indices, const ionp_config *
 
 // device scratch owned by the library (grown on demand, stream-ordered use)
 void *scratch_get(size_t bytes, hipStream_t s);
+// a second such buffer for inputs converted to another dtype first
+void *stage_get(size_t bytes, hipStream_t s);
 
 }  // namespace ionp

@@ -37,6 +37,29 @@ namespace ionp {
 static std::mutex g_scratch_mu;
 static void *g_scratch = nullptr;
 static size_t g_scratch_bytes = 0;
+static void *g_stage = nullptr;
+static size_t g_stage_bytes = 0;
+
+// a second grow-only buffer: inputs converted to another dtype before a
+// kernel that may itself take scratch_get memory
+void *stage_get(size_t bytes, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (bytes <= g_stage_bytes) return g_stage;
+    if (g_stage) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(g_stage);
+        g_stage = nullptr;
+        g_stage_bytes = 0;
+    }
+    size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
+    if (hipMalloc(&g_stage, want) != hipSuccess) {
+        g_stage = nullptr;
+        return nullptr;
+    }
+    g_stage_bytes = want;
+    (void)s;
+    return g_stage;
+}
 
 void *scratch_get(size_t bytes, hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_scratch_mu);
@@ -1545,13 +1568,90 @@ template <typename T, bool MAX> static ionp_status run_float_ext(RedArgs &A, con
     return launched();
 }
 
+// ---------------------------------------------------------------------------
+// float16 outputs (NumPy's HALF loops compute in float32): add -- per
+// bufsize window acc = half(float(acc) + pairwise_f32(window)), the reduce
+// loop storing its float accumulator back to the float16 output after
+// every call; multiply -- per window a float product chain from float(acc),
+// rounded at the window's end; maximum / minimum -- element by element on
+// float16 values (hstep).  W empty: each element is its own call, rounded
+// every step.  A workgroup per output; the serial parts run in thread 0.
+// ---------------------------------------------------------------------------
+template <int OP> __global__ __launch_bounds__(256) void half_reduce(const RedArgs A) {
+    __shared__ float racc[1024], pval[1024];
+    __shared__ int leaf_lo[128], leaf_n[128], lcnt[1];
+    const int tid = threadIdx.x;
+    const int isz = itemsize(A.in_dt);
+    auto xh = [&](int64_t off) { return to_half_bits(A.in_dt, A.in + off * isz); };
+    for (int64_t o = blockIdx.x; o < A.nout; o += gridDim.x) {
+        int64_t ib, ob;
+        out_offsets(A, (uint32_t)o, &ib, &ob);
+        uint16_t acc = A.has_init ? (uint16_t)A.init_bits : (OP == IONP_MULTIPLY ? (uint16_t)0x3c00 : (uint16_t)0);
+        bool first = !A.has_init && (OP == IONP_MAXIMUM || OP == IONP_MINIMUM);  // first element seeds
+        for (int64_t r = 0; r < A.rsize; r++) {
+            const int64_t rb = ib + r_offset(A, r);
+            if (A.nw == 0) {
+                if (tid == 0) {
+                    const uint16_t x = xh(rb);
+                    acc = first ? x : hstep(OP, acc, x);
+                    first = false;
+                }
+                continue;
+            }
+            for (int64_t w = 0; w < A.nwin; w++) {
+                const int64_t s0 = w * A.bufsize;
+                const int L = (int)min(A.bufsize, A.wsize - s0);
+                if constexpr (OP == IONP_ADD) {
+                    const float pw = pairwise_block<float>([&](int i) { return h2f(xh(rb + w_offset(A, s0 + i))); }, L,
+                                                           racc, pval, leaf_lo, leaf_n, lcnt);
+                    if (tid == 0) {
+                        uint32_t fl = 0;
+                        acc = f2h(fadd(h2f(acc), pw), &fl);
+                    }
+                    __syncthreads();
+                } else if (tid == 0) {
+                    if constexpr (OP == IONP_MULTIPLY) {
+                        float f = h2f(acc);
+                        for (int i = 0; i < L; i++) f = fmul(f, h2f(xh(rb + w_offset(A, s0 + i))));
+                        uint32_t fl = 0;
+                        acc = f2h(f, &fl);
+                    } else {
+                        for (int i = 0; i < L; i++) {
+                            const uint16_t x = xh(rb + w_offset(A, s0 + i));
+                            acc = first ? x : hstep(OP, acc, x);
+                            first = false;
+                        }
+                    }
+                }
+            }
+        }
+        if (tid == 0) ((uint16_t *)A.out)[ob] = acc;
+        __syncthreads();
+    }
+}
+
+static ionp_status run_half_reduce(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
+    if (A.mask) return set_error(IONP_ERR_UNSUPPORTED, "float16 where= reductions are not provided");
+    if (A.bufsize > kPwMaxWin) return set_error(IONP_ERR_UNSUPPORTED, "bufsize above 8192");
+    const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
+    const dim3 g(grid_cap(A.nout, bpc)), b(256);
+    switch (A.op) {
+        case IONP_ADD: hipLaunchKernelGGL(half_reduce<IONP_ADD>, g, b, 0, s, A); break;
+        case IONP_MULTIPLY: hipLaunchKernelGGL(half_reduce<IONP_MULTIPLY>, g, b, 0, s, A); break;
+        case IONP_MAXIMUM: hipLaunchKernelGGL(half_reduce<IONP_MAXIMUM>, g, b, 0, s, A); break;
+        default: hipLaunchKernelGGL(half_reduce<IONP_MINIMUM>, g, b, 0, s, A); break;
+    }
+    return launched();
+}
+
 uint64_t init_bits_of(int out_dt, const void *p) {
     switch (out_dt) {
         case IONP_BOOL: return *(const uint8_t *)p != 0;
         case IONP_INT8: return (uint64_t)(int64_t)*(const int8_t *)p;
         case IONP_UINT8: return *(const uint8_t *)p;
         case IONP_INT16: return (uint64_t)(int64_t)*(const int16_t *)p;
-        case IONP_UINT16: return *(const uint16_t *)p;
+        case IONP_UINT16:
+        case IONP_FLOAT16: return *(const uint16_t *)p;
         case IONP_INT32: return (uint64_t)(int64_t)*(const int32_t *)p;
         case IONP_UINT32: return *(const uint32_t *)p;
         case IONP_FLOAT32: return *(const uint32_t *)p;
@@ -1610,7 +1710,10 @@ ionp_status run_reduce(const ReducePlan &p, int64_t bufsize, const void *initial
                                                                op_name(p.op) + " which has no identity");
         if (p.op == IONP_MULTIPLY && !initial) {  // multiply's identity 1 in the output dtype
             A.has_init = 1;
-            A.init_bits = p.out_dt == IONP_FLOAT64 ? 0x3ff0000000000000ull : p.out_dt == IONP_FLOAT32 ? 0x3f800000ull : 1ull;
+            A.init_bits = p.out_dt == IONP_FLOAT64   ? 0x3ff0000000000000ull
+                          : p.out_dt == IONP_FLOAT32 ? 0x3f800000ull
+                          : p.out_dt == IONP_FLOAT16 ? 0x3c00ull
+                                                     : 1ull;
         }
         const int g = grid_cap((p.nout + 255) / 256, bpc);
         switch (itemsize(p.out_dt)) {
@@ -1621,6 +1724,7 @@ ionp_status run_reduce(const ReducePlan &p, int64_t bufsize, const void *initial
         }
         return launched();
     }
+    if (p.out_dt == IONP_FLOAT16) return run_half_reduce(A, cfg, s);
     if (A.mask) return run_masked(A, cfg, s);
     if (is_float_dt(p.out_dt)) {
         if (p.op == IONP_ADD) {
@@ -2066,6 +2170,46 @@ template <typename T> __global__ __launch_bounds__(256) void rat_prod(const RatA
     }
 }
 
+// float16 outputs: x[i] seeds the call, add x[i] + pairwise_f32(rest) and
+// multiply a float product chain, each rounded to float16 once at the end
+// of the call; maximum / minimum element by element (hstep)
+template <int OP> __global__ __launch_bounds__(256) void rat_half(const RatArgs A) {
+    __shared__ float racc[1024], pval[1024];
+    __shared__ int leaf_lo[128], leaf_n[128], lcnt[1];
+    __shared__ PwStack<float> pstack;
+    const int tid = threadIdx.x;
+    const int isz = itemsize(A.in_dt);
+    const int64_t nout = A.nfib * A.nidx;
+    for (int64_t g = blockIdx.x; g < nout; g += gridDim.x) {
+        const int64_t f = g / A.nidx, k = g - f * A.nidx;
+        int64_t ib, ob;
+        rat_offsets(A, (uint32_t)f, &ib, &ob);
+        const int64_t a = A.idx[k], b = k + 1 < A.nidx ? A.idx[k + 1] : A.len;
+        const int64_t m = b > a ? b - a - 1 : 0;
+        auto xh = [&](int64_t i) { return to_half_bits(A.in_dt, A.in + (ib + i * A.in_st) * isz); };
+        uint16_t res = xh(a);
+        if (m > 0) {
+            if constexpr (OP == IONP_ADD) {
+                const float pw = pairwise_any<float>([&](int64_t i) { return h2f(xh(a + 1 + i)); }, m, racc, pval, leaf_lo,
+                                                     leaf_n, lcnt, pstack);
+                uint32_t fl = 0;
+                res = f2h(fadd(h2f(res), pw), &fl);
+            } else if (tid == 0) {
+                if constexpr (OP == IONP_MULTIPLY) {
+                    float p = h2f(res);
+                    for (int64_t i = a + 1; i < b; i++) p = fmul(p, h2f(xh(i)));
+                    uint32_t fl = 0;
+                    res = f2h(p, &fl);
+                } else {
+                    for (int64_t i = a + 1; i < b; i++) res = hstep(OP, res, xh(i));
+                }
+            }
+        }
+        if (tid == 0) ((uint16_t *)A.out)[ob + k * A.out_st] = res;
+        __syncthreads();
+    }
+}
+
 template <int LT> static void launch_rat(const RatArgs &A, int bpc, hipStream_t s) {
     const int64_t nout = A.nfib * A.nidx;
     const dim3 g(grid_cap(nout, bpc)), b(256);
@@ -2094,6 +2238,16 @@ ionp_status run_reduceat(RatArgs &A, const int64_t *indices, const ionp_config *
         return set_error(IONP_ERR_DEVICE, "copying reduceat indices failed");
     A.idx = didx;
     const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
+    if (A.out_dt == IONP_FLOAT16) {
+        const dim3 g(grid_cap(A.nfib * A.nidx, bpc)), b(256);
+        switch (A.op) {
+            case IONP_ADD: hipLaunchKernelGGL(rat_half<IONP_ADD>, g, b, 0, s, A); break;
+            case IONP_MULTIPLY: hipLaunchKernelGGL(rat_half<IONP_MULTIPLY>, g, b, 0, s, A); break;
+            case IONP_MAXIMUM: hipLaunchKernelGGL(rat_half<IONP_MAXIMUM>, g, b, 0, s, A); break;
+            default: hipLaunchKernelGGL(rat_half<IONP_MINIMUM>, g, b, 0, s, A); break;
+        }
+        return launched();
+    }
     switch (A.out_dt) {
         case IONP_BOOL: launch_rat<IONP_BOOL>(A, bpc, s); break;
         case IONP_INT8: launch_rat<IONP_INT8>(A, bpc, s); break;

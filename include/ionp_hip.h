@@ -47,7 +47,8 @@ typedef enum {
     IONP_UINT64 = 8,
     IONP_FLOAT32 = 9,
     IONP_FLOAT64 = 10,
-    IONP_NTYPES = 11
+    IONP_FLOAT16 = 11, /* IEEE binary16 (NumPy float16) */
+    IONP_NTYPES = 12
 } ionp_dtype;
 
 /* binary ufuncs (ionp-core/src/ufunc.rs:43-65 BinaryOp, subset) */

@@ -19,7 +19,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 MAXDIM = 16
 DTYPES = [np.bool_, np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32,
-          np.int64, np.uint64, np.float32, np.float64]
+          np.int64, np.uint64, np.float32, np.float64, np.float16]
 _CODE = {np.dtype(d): i for i, d in enumerate(DTYPES)}
 
 OPS = ["add", "subtract", "multiply", "true_divide", "maximum", "minimum",

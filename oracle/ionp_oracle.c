@@ -51,9 +51,9 @@
 
 enum {
     DT_BOOL = 0, DT_I8, DT_U8, DT_I16, DT_U16, DT_I32, DT_U32, DT_I64, DT_U64,
-    DT_F32, DT_F64, DT_COUNT
+    DT_F32, DT_F64, DT_F16, DT_COUNT
 };
-static const int ITEMSIZE[DT_COUNT] = {1, 1, 1, 2, 2, 4, 4, 8, 8, 4, 8};
+static const int ITEMSIZE[DT_COUNT] = {1, 1, 1, 2, 2, 4, 4, 8, 8, 4, 8, 2};
 
 enum {
     OP_ADD = 0, OP_SUBTRACT, OP_MULTIPLY, OP_TRUE_DIVIDE, OP_MAXIMUM, OP_MINIMUM,
@@ -76,6 +76,7 @@ typedef struct {
 /* dtype classification and promotion (dtype.rs:297-357)                    */
 /* ------------------------------------------------------------------------ */
 static int is_float(int dt) { return dt == DT_F32 || dt == DT_F64; }
+static int is_float_any(int dt) { return is_float(dt) || dt == DT_F16; }
 static int is_signed_int(int dt) { return dt == DT_I8 || dt == DT_I16 || dt == DT_I32 || dt == DT_I64; }
 static int is_unsigned_int(int dt) { return dt == DT_U8 || dt == DT_U16 || dt == DT_U32 || dt == DT_U64; }
 static int signed_of_size(int bytes)
@@ -93,6 +94,12 @@ int ora_result_type(int a, int b)
     if (a == b) return a;
     if (a == DT_BOOL) return b;
     if (b == DT_BOOL) return a;
+    if (a == DT_F16 || b == DT_F16) { /* float16 holds the 8-bit integers exactly */
+        int o = a == DT_F16 ? b : a;
+        if (ITEMSIZE[o] == 1) return DT_F16;
+        if (o == DT_F32 || ITEMSIZE[o] == 2) return DT_F32;
+        return DT_F64;
+    }
     if (is_float(a) && is_float(b)) return (a == DT_F64 || b == DT_F64) ? DT_F64 : DT_F32;
     if (is_float(a) || is_float(b)) {
         int f = is_float(a) ? a : b, i = is_float(a) ? b : a;
@@ -117,7 +124,7 @@ int ora_binary_types(int op, int a, int b, int *loop, int *out)
 {
     int rt = ora_result_type(a, b);
     if (op == OP_TRUE_DIVIDE) {
-        if (!is_float(rt)) rt = DT_F64;
+        if (!is_float_any(rt)) rt = DT_F64;
         *loop = rt; *out = rt; return ORA_OK;
     }
     if (op == OP_SUBTRACT && rt == DT_BOOL) return ORA_ETYPE;
@@ -190,11 +197,104 @@ static int64_t load_i(int dt, const char *p)
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* float16: NumPy's npy_half conversions, restated on exact values: a finite */
+/* value rounds to nearest even on float16's grid (2^-24 below 2^-14, 11    */
+/* significant bits above); |x| >= 65520 overflows to inf; underflow is     */
+/* raised when the value is below 2^-14 and not on the grid (tininess       */
+/* before rounding).  NaN keeps its top 10 payload bits unquieted (an       */
+/* all-zero payload becomes 1).  The flags go to the host status register   */
+/* (feraiseexcept), where ora_*_fpe read them like the hardware's own.      */
+/* ------------------------------------------------------------------------ */
+static uint16_t half_round(double x)
+{
+    double ax = fabs(x), q, r;
+    uint16_t sign = signbit(x) ? 0x8000u : 0;
+    int k;
+    if (ax == 0) return sign;
+    if (isinf(ax)) return (uint16_t)(sign | 0x7c00u); /* exact: no flag */
+    if (ax >= 65520.0) {
+        feraiseexcept(FE_OVERFLOW);
+        return (uint16_t)(sign | 0x7c00u);
+    }
+    if (ax < 6.103515625e-05) { /* 2^-14: the subnormal grid, units of 2^-24 */
+        q = ldexp(ax, 24);
+        r = nearbyint(q);
+        if (r != q) feraiseexcept(FE_UNDERFLOW);
+        return (uint16_t)(sign | (uint16_t)r);
+    }
+    (void)frexp(ax, &k); /* ax in [2^(k-1), 2^k) */
+    q = ldexp(ax, 11 - k); /* in [1024, 2048) */
+    r = nearbyint(q);
+    if (r == 2048.0) { r = 1024.0; k++; }
+    if (k - 1 + 15 >= 31) {
+        feraiseexcept(FE_OVERFLOW);
+        return (uint16_t)(sign | 0x7c00u);
+    }
+    return (uint16_t)(sign | ((uint16_t)(k - 1 + 15) << 10) | (uint16_t)((int)r - 1024));
+}
+static uint16_t f32_to_half(uint32_t fb)
+{
+    uint32_t a = fb & 0x7fffffffu;
+    if (a > 0x7f800000u) { /* NaN */
+        uint16_t p = (uint16_t)((a & 0x007fffffu) >> 13);
+        return (uint16_t)(((fb >> 16) & 0x8000u) | 0x7c00u | (p ? p : 1));
+    }
+    return half_round((double)bits_f32(fb));
+}
+static uint16_t f64_to_half(uint64_t db)
+{
+    uint64_t a = db & 0x7fffffffffffffffull;
+    if (a > 0x7ff0000000000000ull) {
+        uint16_t p = (uint16_t)((a & 0x000fffffffffffffull) >> 42);
+        return (uint16_t)(((db >> 48) & 0x8000u) | 0x7c00u | (p ? p : 1));
+    }
+    return half_round(bits_f64(db));
+}
+/* exact value of a float16 (NaN: its float32 / float64 bits, payload kept) */
+static double half_value(uint16_t h)
+{
+    int e = (h >> 10) & 0x1f, m = h & 0x3ff;
+    double v;
+    if (e == 31) return bits_f64(((uint64_t)(h & 0x8000u) << 48) | 0x7ff0000000000000ull | ((uint64_t)m << 42));
+    v = e == 0 ? ldexp((double)m, -24) : ldexp((double)(1024 + m), e - 25);
+    return (h & 0x8000u) ? -v : v;
+}
+static uint32_t half_f32_bits(uint16_t h)
+{
+    int e = (h >> 10) & 0x1f;
+    if (e == 31) return ((uint32_t)(h & 0x8000u) << 16) | 0x7f800000u | ((uint32_t)(h & 0x3ffu) << 13);
+    return f32_bits((float)half_value(h));
+}
+static float half_f(uint16_t h) { return bits_f32(half_f32_bits(h)); }
+
 /* buffer.rs:81-210 cast matrix, with NumPy's x86 results for NaN payloads
  * and out-of-range float->int. */
 static void cast_elem(int sdt, const char *s, int ddt, char *d)
 {
     if (sdt == ddt) { memcpy(d, s, ITEMSIZE[sdt]); return; }
+    if (ddt == DT_F16) { /* npy_float_to_half / npy_double_to_half; integers through float64 */
+        uint16_t h;
+        if (sdt == DT_F32) h = f32_to_half(*(const uint32_t *)s);
+        else if (sdt == DT_F64) h = f64_to_half(*(const uint64_t *)s);
+        else {
+            char tmp[8];
+            cast_elem(sdt, s, DT_F64, tmp);
+            h = half_round(*(const double *)tmp);
+        }
+        memcpy(d, &h, 2);
+        return;
+    }
+    if (sdt == DT_F16) { /* through the exact float64 value */
+        uint16_t h;
+        double x;
+        memcpy(&h, s, 2);
+        if (ddt == DT_F32) { uint32_t b = half_f32_bits(h); memcpy(d, &b, 4); return; }
+        x = half_value(h);
+        if (ddt == DT_BOOL) { *(uint8_t *)d = (uint8_t)(x != 0.0 || x != x); return; }
+        cast_elem(DT_F64, (const char *)&x, ddt, d);
+        return;
+    }
     if (ddt == DT_BOOL) {
         int nz;
         if (sdt == DT_F64) nz = *(const double *)s != 0.0;
@@ -353,6 +453,21 @@ static uint64_t wrap_arith(int op, uint64_t a, uint64_t b)
  * Writes the output element (dtype `odt`). */
 static void binary_elem(int op, int ldt, const char *pa, const char *pb, int odt, char *po)
 {
+    if (ldt == DT_F16) { /* HALF loops: float32 arithmetic rounded back; max/min on the halves */
+        uint16_t ha, hb, hr;
+        float a, b;
+        memcpy(&ha, pa, 2);
+        memcpy(&hb, pb, 2);
+        a = half_f(ha);
+        b = half_f(hb);
+        if (op >= OP_LESS) { *(uint8_t *)po = (uint8_t)cmp_d(op, a, b); return; }
+        if (op == OP_MAXIMUM) hr = (a != a || (b == b && a >= b)) ? ha : hb;
+        else if (op == OP_MINIMUM) hr = (a != a || (b == b && a <= b)) ? ha : hb;
+        else hr = f32_to_half(f32_bits(f32_arith(op, a, b)));
+        memcpy(po, &hr, 2);
+        (void)odt;
+        return;
+    }
     if (op >= OP_LESS) {
         int r;
         if (ldt == DT_F64) r = cmp_d(op, *(const double *)pa, *(const double *)pb);
@@ -678,7 +793,7 @@ int ora_cast_fpe(const ora_desc *in, int to, int *flags)
 {
     int n = in->ndim, i, r = 0;
     int64_t idx[ORA_MAXDIM], total, e;
-    if (in->dtype != DT_F32 && in->dtype != DT_F64) {
+    if (!is_float_any(in->dtype) && to != DT_F16) {
         *flags = 0;
         return ORA_OK;
     }
@@ -687,7 +802,14 @@ int ora_cast_fpe(const ora_desc *in, int to, int *flags)
     feclearexcept(FE_ALL_EXCEPT);
     for (e = 0; e < total; e++) {
         const char *p = elem_ptr(in, idx, in->strides, n);
-        const double x = in->dtype == DT_F32 ? (double)*(const float *)p : *(const double *)p;
+        double x;
+        if (to == DT_F16) { /* over / underflow raised by the conversion itself */
+            char tmp[2];
+            cast_elem(in->dtype, p, DT_F16, tmp);
+            goto next;
+        }
+        x = in->dtype == DT_F32 ? (double)*(const float *)p
+          : in->dtype == DT_F16 ? half_value(*(const uint16_t *)p) : *(const double *)p;
         switch (to) {
         case DT_I8: case DT_U8: case DT_I16: case DT_U16: case DT_I32:
             if (!(x > -2147483649.0 && x < 2147483648.0)) r |= 8;
@@ -708,12 +830,13 @@ int ora_cast_fpe(const ora_desc *in, int to, int *flags)
         default:
             break;
         }
+    next:
         for (i = n - 1; i >= 0; i--) {
             if (++idx[i] < in->shape[i]) break;
             idx[i] = 0;
         }
     }
-    r |= fe_bits() & 6; /* over / under from the float64 -> float32 conversions */
+    r |= fe_bits() & 6; /* over / under from the float64 -> float32 / float16 conversions */
     feclearexcept(FE_ALL_EXCEPT);
     *flags = r;
     return ORA_OK;
@@ -915,6 +1038,97 @@ static void w_offsets(const ora_desc *in, const red_plan *p, int64_t *off)
     }
 }
 
+/* one float16 step acc (op) v of the HALF loops (binary_elem's float16 loop) */
+static uint16_t half_step(int op, uint16_t acc, uint16_t v)
+{
+    uint16_t r;
+    binary_elem(op, DT_F16, (const char *)&acc, (const char *)&v, DT_F16, (char *)&r);
+    return r;
+}
+
+/* reduce_axis with a float16 output, as NumPy's HALF loops evaluate it:
+ * the reduce loop keeps a float32 accumulator for one call (one bufsize
+ * window) and stores it back to the float16 output: add -- acc =
+ * half(float(acc) + pw_f32(window)); multiply -- a float32 product chain
+ * from float(acc), rounded once per window; maximum / minimum -- element by
+ * element on the halves.  W empty: every element is its own call.  Inputs
+ * of another dtype are cast to float16 first (NumPy's buffered cast). */
+static int ora_reduce_half(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc *out, int64_t bufsize,
+                           const void *initial)
+{
+    red_plan p;
+    int n = in->ndim, k, nk = 0, kept[ORA_MAXDIM], isz = ITEMSIZE[in->dtype];
+    int64_t *woff, *foff, okidx[ORA_MAXDIM], nout = 1, o, gsize = 1, e;
+    float *fbuf;
+    uint16_t *hbuf;
+    make_red_plan(in, axmask, &p);
+    for (k = 0; k < n; k++) {
+        if (!axmask[k]) kept[nk++] = k;
+        else gsize *= in->shape[k];
+    }
+    for (k = 0; k < nk; k++) nout *= in->shape[kept[k]];
+    if (gsize == 0 && (op == OP_MAXIMUM || op == OP_MINIMUM) && nout > 0 && !initial) return ORA_EVALUE;
+    woff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(p.wsize > 0 ? p.wsize : 1));
+    foff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(bufsize > 0 ? bufsize : 1));
+    fbuf = (float *)malloc(sizeof(float) * (size_t)(bufsize > 0 ? bufsize : 1));
+    hbuf = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)(bufsize > 0 ? bufsize : 1));
+    for (e = 0; e < bufsize; e++) foff[e] = e * 4;
+    if (p.nw) w_offsets(in, &p, woff);
+    memset(okidx, 0, sizeof okidx);
+    for (o = 0; o < nout; o++) {
+        int64_t base = 0, ooff = 0, oidx[ORA_MAXDIM], noutr = 1, r;
+        uint16_t acc = op == OP_MULTIPLY ? 0x3c00u : 0;
+        int first = !initial && (op == OP_MAXIMUM || op == OP_MINIMUM);
+        if (initial) memcpy(&acc, initial, 2);
+        for (k = 0; k < nk; k++) {
+            base += okidx[k] * in->strides[kept[k]];
+            ooff += okidx[k] * out->strides[k];
+        }
+        for (k = 0; k < p.nouter; k++) noutr *= in->shape[p.outer_axes[k]];
+        if (gsize == 0) noutr = 0;
+        memset(oidx, 0, sizeof oidx);
+        for (r = 0; r < noutr; r++) {
+            int64_t ob = base, s0;
+            const char *pb;
+            for (k = 0; k < p.nouter; k++) ob += oidx[k] * in->strides[p.outer_axes[k]];
+            pb = (const char *)in->data + ob * isz;
+            for (s0 = 0; s0 < (p.nw ? p.wsize : 1); s0 += bufsize) {
+                int64_t len = p.nw ? (p.wsize - s0 < bufsize ? p.wsize - s0 : bufsize) : 1;
+                for (e = 0; e < len; e++) {
+                    cast_elem(in->dtype, pb + (p.nw ? woff[s0 + e] : 0), DT_F16, (char *)&hbuf[e]);
+                    fbuf[e] = half_f(hbuf[e]);
+                }
+                if (!p.nw || op == OP_MAXIMUM || op == OP_MINIMUM) {
+                    for (e = 0; e < len; e++) {
+                        acc = first ? hbuf[e] : half_step(op, acc, hbuf[e]);
+                        first = 0;
+                    }
+                } else if (op == OP_ADD) {
+                    acc = f32_to_half(f32_bits(f32_arith(OP_ADD, half_f(acc), pw_f32((const char *)fbuf, foff, len))));
+                } else {
+                    float f = half_f(acc);
+                    for (e = 0; e < len; e++) f = f32_arith(OP_MULTIPLY, f, fbuf[e]);
+                    acc = f32_to_half(f32_bits(f));
+                }
+            }
+            for (k = 0; k < p.nouter; k++) {
+                if (++oidx[k] < in->shape[p.outer_axes[k]]) break;
+                oidx[k] = 0;
+            }
+        }
+        memcpy((char *)out->data + ooff * 2, &acc, 2);
+        for (k = nk - 1; k >= 0; k--) {
+            if (++okidx[k] < in->shape[kept[k]]) break;
+            okidx[k] = 0;
+        }
+    }
+    free(woff);
+    free(foff);
+    free(fbuf);
+    free(hbuf);
+    return ORA_OK;
+}
+
 /* reduce op codes for ora_reduce: OP_ADD, OP_MAXIMUM, OP_MINIMUM */
 int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc *out, int64_t bufsize,
                const void *initial)
@@ -928,6 +1142,8 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
     /* the loop runs in the output dtype (the default of ora_reduce_dtype or
        an explicit dtype=); elements are cast to it as they are read */
     cdt = out->dtype;
+    if (cdt == DT_F16) return ora_reduce_half(op, in, axmask, out, bufsize, initial);
+    if (in->dtype == DT_F16) return is_float(cdt) ? ORA_EVALUE : ORA_ETYPE; /* float loops: cast it first */
     if (is_float(in->dtype) && !is_float(cdt)) return ORA_ETYPE;
     if (is_float(cdt) && in->dtype != cdt) return ORA_EVALUE; /* float loops read the input as is: cast it first */
     make_red_plan(in, axmask, &p);
@@ -1264,6 +1480,7 @@ int ora_reduce_where(int op, const ora_desc *in, const uint8_t *axmask, const or
     char *buf;
     if (op != OP_ADD && op != OP_MULTIPLY && op != OP_MAXIMUM && op != OP_MINIMUM) return ORA_ENOLOOP;
     if (mask->dtype != DT_BOOL || mask->ndim != n) return ORA_ETYPE;
+    if (in->dtype == DT_F16 || cdt == DT_F16) return ORA_ENOLOOP; /* not provided for float16 */
     if (is_float(in->dtype) && !is_float(cdt)) return ORA_ETYPE;
     if (is_float(cdt) && in->dtype != cdt) return ORA_EVALUE; /* float loops read the input as is */
     if ((op == OP_MAXIMUM || op == OP_MINIMUM) && !initial) return ORA_EVALUE; /* no identity */
@@ -1427,7 +1644,7 @@ int ora_reduceat(int op, const ora_desc *in, int axis, const int64_t *idx, int64
     char *buf;
     if (op != OP_ADD && op != OP_MULTIPLY && op != OP_MAXIMUM && op != OP_MINIMUM) return ORA_ENOLOOP;
     if (axis < 0 || axis >= n || out->ndim != n) return ORA_EAXIS;
-    if (is_float(in->dtype) && !is_float(cdt)) return ORA_ETYPE;
+    if (is_float_any(in->dtype) && !is_float_any(cdt)) return ORA_ETYPE;
     if (is_float(cdt) && in->dtype != cdt) return ORA_EVALUE;
     len = in->shape[axis];
     st = in->strides[axis];
@@ -1453,7 +1670,35 @@ int ora_reduceat(int op, const ora_desc *in, int axis, const int64_t *idx, int64
             const char *pb = (const char *)in->data + ib * isz;
             char *po = (char *)out->data + (ob + q * ost) * ITEMSIZE[cdt];
             const int64_t m = b > a ? b - a - 1 : 0; /* elements after the seed */
-            if (cdt == DT_F64 || cdt == DT_F32) {
+            if (cdt == DT_F16) { /* the HALF loop: one call seeded with x[a], float32 inside */
+                uint16_t acc, v;
+                int64_t e;
+                cast_elem(in->dtype, pb + off[a], DT_F16, (char *)&acc);
+                if (m > 0 && op == OP_ADD) {
+                    float *fb = (float *)buf;
+                    int64_t *fo = (int64_t *)malloc(sizeof(int64_t) * (size_t)m);
+                    for (e = 0; e < m; e++) {
+                        cast_elem(in->dtype, pb + off[a + 1 + e], DT_F16, (char *)&v);
+                        fb[e] = half_f(v);
+                        fo[e] = e * 4;
+                    }
+                    acc = f32_to_half(f32_bits(f32_arith(OP_ADD, half_f(acc), pw_f32(buf, fo, m))));
+                    free(fo);
+                } else if (m > 0 && op == OP_MULTIPLY) {
+                    float p = half_f(acc);
+                    for (e = 0; e < m; e++) {
+                        cast_elem(in->dtype, pb + off[a + 1 + e], DT_F16, (char *)&v);
+                        p = f32_arith(OP_MULTIPLY, p, half_f(v));
+                    }
+                    acc = f32_to_half(f32_bits(p));
+                } else {
+                    for (e = 0; e < m; e++) {
+                        cast_elem(in->dtype, pb + off[a + 1 + e], DT_F16, (char *)&v);
+                        acc = half_step(op, acc, v);
+                    }
+                }
+                memcpy(po, &acc, 2);
+            } else if (cdt == DT_F64 || cdt == DT_F32) {
                 double a64 = cdt == DT_F64 ? *(const double *)(pb + off[a]) : 0.0;
                 float a32 = cdt == DT_F32 ? *(const float *)(pb + off[a]) : 0.0f;
                 if (m > 0) {
@@ -1534,7 +1779,7 @@ int ora_accumulate(int op, const ora_desc *in, int axis, const ora_desc *out)
     int64_t idx[ORA_MAXDIM], len, nfib = 1, f, i;
     if (op != OP_ADD && op != OP_MULTIPLY && op != OP_MAXIMUM && op != OP_MINIMUM) return ORA_ENOLOOP;
     if (axis < 0 || axis >= (n > 0 ? n : 1)) return ORA_EAXIS;
-    if (is_float(in->dtype) && !is_float(cdt)) return ORA_ETYPE;
+    if (is_float_any(in->dtype) && !is_float_any(cdt)) return ORA_ETYPE;
     if (out->ndim != n) return ORA_EVALUE;
     for (k = 0; k < n; k++)
         if (out->shape[k] != in->shape[k]) return ORA_EVALUE;
@@ -1556,6 +1801,12 @@ int ora_accumulate(int op, const ora_desc *in, int axis, const ora_desc *out)
             cast_elem(in->dtype, pi, cdt, v);
             if (i == 0) {
                 memcpy(acc, v, osz);
+            } else if (cdt == DT_F16) {
+                uint16_t a, b;
+                memcpy(&a, acc, 2);
+                memcpy(&b, v, 2);
+                a = half_step(op, a, b);
+                memcpy(acc, &a, 2);
             } else if (cdt == DT_F64) {
                 double a = *(double *)acc, b = *(double *)v;
                 a = op == OP_ADD || op == OP_MULTIPLY ? f64_arith(op, a, b) : op == OP_MAXIMUM ? f64_max(a, b) : f64_min(a, b);
@@ -1638,8 +1889,9 @@ int ora_argext(int ismax, const ora_desc *in, int axis, const ora_desc *out)
                 for (k = n - 1; k >= 0; k--) { if (++ridx[k] < in->shape[k]) break; ridx[k] = 0; }
             } else off = base + i * in->strides[axis];
             pe = (const char *)in->data + off * isz;
-            if (is_float(dt)) {
-                double v = dt == DT_F64 ? *(const double *)pe : (double)*(const float *)pe;
+            if (is_float_any(dt)) {
+                double v = dt == DT_F64 ? *(const double *)pe
+                         : dt == DT_F32 ? (double)*(const float *)pe : half_value(*(const uint16_t *)pe);
                 int vn = v != v;
                 if (i == 0) { bd = v; bnan = vn; best = 0; if (vn) break; continue; }
                 if (vn) { best = i; break; }

@@ -843,6 +843,148 @@ def fpe_cases(S):
                 S.add(f"fpe/cast32/{i}/{t}", "fpe", "cast", [X32], lambda: flags_of(lambda: X32.astype(t)), {"dtype": t})
 
 
+def half_cases(S):
+    """float16 (NumPy's HALF loops and npy_half conversions): arithmetic in
+    float32 rounded back, maximum / minimum on the halves (first operand on
+    ties), promotion with every dtype, casts both ways at the rounding and
+    overflow / underflow boundaries, where, reductions per bufsize window
+    (float32 accumulator stored back each window), accumulate, reduceat,
+    argmax, IEEE flags.  Add / multiply inputs never pair two NaNs (which
+    payload NumPy keeps then depends on the loop variant).  Own generator
+    (seed 85)."""
+    r10 = np.random.default_rng(85)
+    h = np.float16
+
+    def hvals(shape):
+        x = (r10.standard_normal(shape) * r10.choice([1e-6, 1e-3, 1.0, 100.0, 3e4], shape)).astype(h)
+        u = r10.random(shape)
+        x[u < 0.03] = 0.0
+        x[(u >= 0.03) & (u < 0.06)] = -0.0
+        x[(u >= 0.06) & (u < 0.07)] = np.inf
+        x[(u >= 0.07) & (u < 0.08)] = -np.inf
+        x[(u >= 0.08) & (u < 0.09)] = np.nan
+        v = x.view(np.uint16)
+        m = (u >= 0.09) & (u < 0.1)
+        v[m] = r10.integers(1, 1024, int(m.sum()))  # subnormals
+        return x
+
+    def partner(t, shape):
+        if t == np.bool_:
+            return r10.random(shape) < 0.5
+        if np.dtype(t).kind in "iu":
+            info = np.iinfo(t)
+            return r10.integers(builtins_max(info.min, -70000), builtins_min(info.max, 70000), shape).astype(t)
+        if t == h:
+            return hvals(shape)
+        return (r10.standard_normal(shape) * 100).astype(t)
+
+    def flags_of(fn):
+        got = []
+        with np.errstate(all="call", call=lambda t, f: got.append(f)):
+            fn()
+        r = 0
+        for f in got:
+            r |= f
+        return np.array([r], dtype=np.int64)
+
+    ops = ("add", "subtract", "multiply", "true_divide", "maximum", "minimum", "less", "equal", "greater_equal")
+    for t in (h, np.int8, np.uint8, np.bool_, np.int16, np.float32, np.float64, np.int64):
+        nm = np.dtype(t).name
+        a = hvals(400)
+        b = partner(t, 400)
+        for op in ops:
+            if op == "subtract" and t == np.bool_:
+                continue
+            aa, bb = a, b
+            if op in ("add", "multiply") and bb.dtype.kind == "f":
+                bb = b.copy()
+                bb[np.isnan(a.astype(np.float64)) & np.isnan(b.astype(np.float64))] = 1
+            fn = getattr(np, op)
+            S.add(f"half/{op}/{nm}", "binary", op, [aa, bb], lambda: fn(aa, bb))
+            S.add(f"half/{op}/{nm}/rev", "binary", op, [bb, aa], lambda: fn(bb, aa))
+    X = hvals((30, 40))
+    col = hvals((30, 1))
+    S.add("half/multiply/bcast", "binary", "multiply", [X, col], lambda: np.multiply(X, col))
+    S.add("half/add/T", "binary", "add", [X.T, X[:1].T], lambda: np.add(X.T, X[:1].T))
+    S.add("half/maximum/step", "binary", "maximum", [X[:, ::3], X[:, 1::3][:, :14]],
+          lambda: np.maximum(X[:, ::3], X[:, 1::3][:, :14]))
+    # casts
+    x = hvals(600)
+    for t in ("bool", "int8", "uint8", "int16", "uint16", "int32", "uint32", "int64", "uint64", "float32", "float64"):
+        S.add(f"half/cast/to/{t}", "cast", "cast", [x], lambda: x.astype(t), {"dtype": t})
+    edge = np.array([1 + 2 ** -11 + 2 ** -30, 1 + 2 ** -11, 1 + 3 * 2 ** -11, 2 ** -25, 2 ** -25 * 1.0000001, 65519.99,
+                     65520.0, 2 ** -24 * 1.5, 2 ** -24 * 2.5, 6.1e-5, 6.103515625e-05 * (1 - 2 ** -12), -0.0, 1e300,
+                     -1e-300, np.inf, np.nan], dtype=np.float64)
+    S.add("half/cast/from/float64/edge", "cast", "cast", [edge], lambda: edge.astype(h), {"dtype": "float16"})
+    for t in (np.float32, np.float64):
+        y = (r10.standard_normal(600) * r10.choice([1e-9, 1e-6, 1e-4, 1.0, 100.0, 6e4, 7e4], 600)).astype(t)
+        y[::50] = np.nan
+        if t == np.float32:
+            yv = y.view(np.uint32)
+            yv[1::97] = 0x7F800001 + r10.integers(0, 1 << 22, len(yv[1::97]), dtype=np.uint32)  # NaN payloads, signalling too
+        S.add(f"half/cast/from/{np.dtype(t).name}", "cast", "cast", [y], lambda: y.astype(h), {"dtype": "float16"})
+    for t in (np.bool_, np.int8, np.uint8, np.int16, np.int32, np.uint64):
+        y = partner(t, 600)
+        S.add(f"half/cast/from/{np.dtype(t).name}", "cast", "cast", [y], lambda: y.astype(h), {"dtype": "float16"})
+    # where
+    c = r10.random(300) < 0.5
+    xh, yi, yf = hvals(300), partner(np.int8, 300), partner(np.float32, 300)
+    S.add("half/where/f16,i8", "where", "where", [c, xh, yi], lambda: np.where(c, xh, yi))
+    S.add("half/where/f16,f32", "where", "where", [c, xh, yf], lambda: np.where(c, xh, yf))
+    # reductions
+    for shape in ((100,), (20000,), (3, 9000), (300, 7)):
+        g = (r10.standard_normal(shape) * 3).astype(h)
+        gm = (1 + r10.standard_normal(shape) * 0.001).astype(h)
+        for op in ("add", "multiply", "maximum", "minimum"):
+            xx = gm if op == "multiply" else g
+            for ax in [None] + list(range(len(shape))):
+                S.add(f"half/reduce/{op}/{shape}/{ax}", "reduce", op, [xx], lambda: getattr(np, op).reduce(xx, axis=ax),
+                      {"axis": ax})
+            S.add(f"half/reduce/{op}/{shape}/init", "reduce", op, [xx],
+                  lambda: getattr(np, op).reduce(xx, axis=0, initial=0.5), {"axis": 0, "initial": 0.5})
+            S.add(f"half/accumulate/{op}/{shape}", "accumulate", op, [xx],
+                  lambda: getattr(np, op).accumulate(xx, axis=len(shape) - 1), {"axis": len(shape) - 1})
+            idx = sorted(set(r10.integers(0, shape[-1], 5).tolist()))
+            S.add(f"half/reduceat/{op}/{shape}", "reduceat", op, [xx],
+                  lambda: getattr(np, op).reduceat(xx, idx, axis=len(shape) - 1),
+                  {"indices": [int(i) for i in idx], "axis": len(shape) - 1})
+        S.add(f"half/argmax/{shape}", "argext", "argmax", [g], lambda: np.argmax(g, axis=len(shape) - 1),
+              {"axis": len(shape) - 1})
+    g = hvals(700)
+    S.add("half/reduce/maximum/specials", "reduce", "maximum", [g], lambda: np.maximum.reduce(g), {"axis": 0})
+    S.add("half/argmin/specials", "argext", "argmin", [g], lambda: np.argmin(g), {"axis": None})
+    i8 = partner(np.int8, 5000)
+    S.add("half/reduce/add/int8-dtype", "reduce", "add", [i8], lambda: np.add.reduce(i8, dtype=h),
+          {"axis": 0, "dtype": "float16"})
+    f32 = (r10.standard_normal(5000) * 3).astype(np.float32)
+    S.add("half/reduce/add/f32-dtype", "reduce", "add", [f32], lambda: np.add.reduce(f32, dtype=h),
+          {"axis": 0, "dtype": "float16"})
+    g16 = (r10.standard_normal((40, 300)) * 3).astype(h)
+    S.add("half/reduce/add/f16-dtype-f32", "reduce", "add", [g16], lambda: np.add.reduce(g16, axis=1, dtype=np.float32),
+          {"axis": 1, "dtype": "float32"})
+    S.add("half/cumsum/None", "accumulate", "cumsum", [g16], lambda: np.cumsum(g16), {"axis": None})
+    # IEEE flags
+    sp = np.array([0, -0.0, 1, -1, np.inf, -np.inf, np.nan, 65504, -65504, 6.1e-5, 2 ** -24, 2.0, 0.5, 3.0, 300, 1e-3],
+                  dtype=h)
+    for name in ("add", "subtract", "multiply", "divide"):
+        fn = getattr(np, name)
+        for i in range(len(sp)):
+            a1 = sp[i:i + 1]
+            S.add(f"half/fpe/{name}/row{i}", "fpe", name, [a1, sp], lambda: flags_of(lambda: fn(a1, sp)))
+    for i, v in enumerate([70000.0, 65519.9, 65520.0, 1e-8, 2 ** -25, 2 ** -24 * 1.5, 3e-5, 1.0, -1e5]):
+        for t in ("float32", "float64"):
+            Y = np.array([v, 1.0], dtype=t)
+            S.add(f"half/fpe/cast/{t}/{i}", "fpe", "cast", [Y], lambda: flags_of(lambda: Y.astype(h)),
+                  {"dtype": "float16"})
+    Yi = np.array([70000, 5], dtype=np.int32)
+    S.add("half/fpe/cast/int32", "fpe", "cast", [Yi], lambda: flags_of(lambda: Yi.astype(h)), {"dtype": "float16"})
+    for t in ("int8", "int32", "uint32", "int64", "uint64"):
+        S.add(f"half/fpe/cast/to/{t}", "fpe", "cast", [sp], lambda: flags_of(lambda: sp.astype(t)), {"dtype": t})
+
+
+builtins_max, builtins_min = max, min
+
+
 def argext_cases(S, rng):
     for dt in (np.float64, np.float32, np.int64, np.int32, np.uint8, np.bool_):
         nm = np.dtype(dt).name
@@ -922,6 +1064,7 @@ def main():
     where_reduce_cases(S)
     reduceat_cases(S)
     fpe_cases(S)
+    half_cases(S)
     np.savez_compressed(os.path.join(HERE, "fixtures.npz"), **S.arrays)
     with open(os.path.join(HERE, "cases.json"), "w") as f:
         json.dump(S.cases, f)
