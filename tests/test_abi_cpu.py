@@ -117,3 +117,80 @@ def test_plan_reduce_layout_and_dtype(lib):
         assert ap._NP_DTYPES[t.dtype] == want.dtype.type
         st = [t.strides[i] for i in range(t.ndim) if t.shape[i] > 1]
         assert st == [s // want.itemsize for s, n in zip(want.strides, want.shape) if n > 1]
+
+
+def test_plan_reduce_where_layout_follows_mask(lib):
+    """The where= mask joins the iterator's axis order: an F-ordered mask on a
+    C-ordered input changes NumPy's output layout."""
+    from anionpy_amd import _lib
+    import anionpy_amd as ap
+    t = _lib.ArrayDesc()
+    x = np.zeros((4, 5, 6))
+    for m in (np.ones((4, 5, 6), bool), np.asfortranarray(np.ones((4, 5, 6), bool)), np.ones(6, bool)):
+        for axis in ((1,), (0, 2), (2,)):
+            axes = (ctypes.c_int32 * len(axis))(*axis)
+            dx, dm = _desc(x), _desc(m)
+            assert lib.ionp_plan_reduce_where(0, ctypes.byref(dx), ctypes.byref(dm), axes, len(axis), 0, -1,
+                                              ctypes.byref(t)) == 0
+            want = np.add.reduce(x, axis=axis, where=m)
+            assert tuple(t.shape[: t.ndim]) == want.shape
+            assert ap._NP_DTYPES[t.dtype] == want.dtype.type
+            st = [t.strides[i] for i in range(t.ndim) if t.shape[i] > 1]
+            assert st == [s // 8 for s, n in zip(want.strides, want.shape) if n > 1]
+    bad = _desc(np.ones((2, 6), bool))
+    assert lib.ionp_plan_reduce_where(0, ctypes.byref(_desc(np.zeros(6))), ctypes.byref(bad), None, -1, 0, -1,
+                                      ctypes.byref(t)) == 1
+
+
+def test_reduce_where_identity_error_before_any_device_work(lib):
+    from anionpy_amd import _lib
+    t = _lib.ArrayDesc()
+    x = _desc(np.zeros((3, 4)))
+    m = _desc(np.ones((3, 4), bool))
+    axes = (ctypes.c_int32 * 1)(1)
+    assert lib.ionp_plan_reduce_where(4, ctypes.byref(x), ctypes.byref(m), axes, 1, 0, -1, ctypes.byref(t)) == 0
+    assert lib.ionp_reduce_where(4, ctypes.byref(x), ctypes.byref(m), axes, 1, 0, None, ctypes.byref(t), None,
+                                 None) == 5
+    assert lib.ionp_last_error().decode() == ("reduction operation 'maximum' does not have an identity, so to use "
+                                              "a where mask one has to specify 'initial'")
+
+
+def test_plan_reduceat_layout_and_index_errors(lib):
+    from anionpy_amd import _lib
+    import anionpy_amd as ap
+    t = _lib.ArrayDesc()
+    base = np.zeros((4, 5, 6), np.int8)
+    for a, axis in ((base, 1), (np.asfortranarray(base), 2), (base.transpose(2, 0, 1), 0), (base[:, ::2], 1)):
+        idx = [0, 2, 1]
+        assert lib.ionp_plan_reduceat(0, ctypes.byref(_desc(a)), axis, len(idx), -1, ctypes.byref(t)) == 0
+        want = np.add.reduceat(a, idx, axis=axis)
+        assert tuple(t.shape[: t.ndim]) == want.shape
+        assert ap._NP_DTYPES[t.dtype] == want.dtype.type
+        st = [t.strides[i] for i in range(t.ndim) if t.shape[i] > 1]
+        assert st == [s // want.itemsize for s, n in zip(want.strides, want.shape) if n > 1]
+    x = _desc(np.zeros(5))
+    idx = (ctypes.c_int64 * 2)(8, 7)
+    assert lib.ionp_reduceat(0, ctypes.byref(x), idx, 2, 0, ctypes.byref(x), None, None) == 3
+    assert lib.ionp_last_error().decode() == "index 8 out-of-bounds in add.reduceat [0, 5)"
+    scalar = _desc(np.zeros(()))
+    assert lib.ionp_plan_reduceat(0, ctypes.byref(scalar), 0, 1, -1, ctypes.byref(t)) == 4
+    assert lib.ionp_last_error().decode() == "cannot reduceat on a scalar"
+
+
+def test_float16_dtype_rules(lib):
+    """float16 promotion, true_divide's loop and the reduce dtype, against NumPy."""
+    from anionpy_amd import _lib
+    import anionpy_amd as ap
+    h = ap._CODE[np.dtype(np.float16)]
+    loop, out = ctypes.c_int32(), ctypes.c_int32()
+    for other in ap._NP_DTYPES:
+        o = ap._CODE[np.dtype(other)]
+        assert lib.ionp_binary_types(3, h, o, ctypes.byref(loop), ctypes.byref(out)) == 0
+        want = np.divide(np.ones(1, np.float16), np.ones(1, other)).dtype
+        assert ap._NP_DTYPES[out.value] == want.type
+    t = _lib.ArrayDesc()
+    for op, dt in ((0, np.float16), (2, np.float16), (4, np.float16)):
+        a = np.zeros((3, 4), dt)
+        axes = (ctypes.c_int32 * 1)(1)
+        assert lib.ionp_plan_reduce(op, ctypes.byref(_desc(a)), axes, 1, 0, -1, ctypes.byref(t)) == 0
+        assert ap._NP_DTYPES[t.dtype] == np.float16
