@@ -22,6 +22,9 @@ struct RowArgs {
 };
 
 int num_cus();
+// MI355X's Infinity Cache (last level, shared by all XCDs): inputs larger
+// than this are streamed with nontemporal loads
+constexpr int64_t kInfinityCacheBytes = 256ll << 20;
 ionp_status run_elementwise(EwArgs &A, const ionp_config *cfg, hipStream_t s);
 ionp_status launch_generic(EwArgs A, const ionp_config *cfg, hipStream_t s);
 

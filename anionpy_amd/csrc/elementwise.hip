@@ -401,9 +401,6 @@ static RowFn row_kernel(int dt, int op, int vec, int unroll) {
     }
 }
 
-// MI355X's Infinity Cache (last level, shared by all XCDs)
-constexpr int64_t kInfinityCacheBytes = 256ll << 20;
-
 // Try the rows fast path for a binary op; returns false if the layout or
 // dtypes do not qualify (the caller then uses the generic kernel).
 bool try_rows(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_status *st) {

@@ -83,7 +83,7 @@ void *scratch_get(size_t bytes, hipStream_t s) {
 // One workgroup per window g = (o, r, w), windows enumerated output-major.
 template <typename T>
 // tail_nwin > 0: only the last (partial) window of each output, written to
-// partial index o * tail_nwin + tail_nwin - 1 (the full windows go to pw_fast).
+// partial index o * tail_nwin + tail_nwin - 1 (the full windows go to pw_stream).
 __global__ __launch_bounds__(kPwThreads) void pw_windows(const RedArgs A, int64_t nwindows, int direct, int64_t tail_nwin) {
     __shared__ T xs[kPwMaxWin];
     __shared__ T racc[kPwMaxWin / 8];
@@ -122,15 +122,15 @@ __global__ __launch_bounds__(kPwThreads) void pw_windows(const RedArgs A, int64_
 
 // ---------------------------------------------------------------------------
 // Full 8192-element windows of contiguous rows: NumPy's tree on a balanced
-// 64-leaf shape, evaluated from registers, with the ordered fold of window
-// partials running inside the same launch.
+// 64-leaf shape, evaluated from registers (pw_stream), with the ordered fold
+// of window partials running inside the same launch (pw_fold_ring).
 // ---------------------------------------------------------------------------
 // Leaf l of a window holds elements [128 l, 128 l + 128); its accumulator j
 // sums elements 8k + j (k = 0..15) in order, and the window value is the
 // balanced tree over the 64 leaf values (ufunc.rs:6416-6448 with n = 8192:
 // every split is exact, every leaf has 128 elements).
 //
-// Producer workgroups take windows round-robin (window g = producer + k *
+// Producer wavefronts take windows round-robin (window g = producer + k *
 // producers, o = g / nfull, w = g % nfull), so windows finish roughly in
 // order.  Each publishes partial[o][w] with one atomic store; a partial slot
 // starts as a signalling-NaN sentinel that no IEEE add can produce (every
@@ -138,13 +138,13 @@ __global__ __launch_bounds__(kPwThreads) void pw_windows(const RedArgs A, int64_
 // final value and no flag, fence or counter is needed.  The fold
 // acc = 0; acc = acc + partial[w] (w order) is done
 //   - for few outputs: by a folder workgroup per output (blocks 0..nout-1)
-//     that polls the partial slots 256 at a time while the producers stream,
-//     so the inherently serial fold (32768 adds for 2^28 elements) overlaps
-//     the HBM traffic instead of following it;
+//     that polls the partial slots while the producers stream, so the
+//     inherently serial fold (32768 adds for 2^28 elements) overlaps the
+//     HBM traffic instead of following it;
 //   - otherwise: by pw_fold after the launch.
 // Producers never wait; a folder waits only for slots producers always fill
 // and gives up (writing NaN) after a bounded number of polls.
-struct PwFastState {
+struct PwStreamState {
     void *partials;    // nout * nwin of T, pre-filled with the sentinel
     int64_t nfull, nwin;
     int32_t folders;   // folder workgroups (0 or nout)
@@ -184,115 +184,8 @@ template <typename T> __device__ inline T xor_combine(T v, int m, int lane) {
     return (lane & m) ? o + v : v + o;
 }
 
-// Folder (one wave): acc = acc + partial[w] for w = 0 .. nwin-1 as the
-// slots fill.  Slots are read 1024 at a time (16 per lane, slot cur + 64 q +
-// lane in register q) and the next two blocks are requested before the
-// current one is folded, so the load latency of a saturated HBM overlaps
-// the dependent add chain.  The chain itself runs in lane 0 from LDS.  A
-// NaN result is redone with x86 NaN rules.
-template <typename T> __device__ void pw_folder(const RedArgs &A, const PwFastState &F, int64_t o, T *stage) {
-    constexpr int Q = 16, BLK = 64 * Q;
-    const int lane = threadIdx.x & 63;
-    const T *part = (const T *)F.partials + o * F.nwin;
-    T acc = fseed<T>(A);
-    int64_t cur = 0;
-    uint32_t idle = 0;
-    bool failed = false;
-    uint64_t b0[Q], b1[Q], b2[Q];
-    auto poll = [&](uint64_t *b, int64_t base) {
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int64_t idx = base + 64 * q + lane;
-            b[q] = idx < F.nwin ? ld_bits(part + idx) : Sentinel<T>::bits;
-        }
-    };
-    poll(b0, cur);
-    poll(b1, cur + BLK);
-    poll(b2, cur + 2 * BLK);
-    while (cur < F.nwin) {
-        int n = 0;  // filled prefix of block b0 (stops at the first sentinel)
-        bool open = true;
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int64_t idx0 = cur + 64 * q;
-            const uint64_t ok = __ballot(b0[q] != Sentinel<T>::bits || idx0 + lane >= F.nwin);
-            stage[64 * q + lane] = from_bits<T>(b0[q]);
-            if (open) {
-                const int m = ~ok == 0 ? 64 : __builtin_ctzll(~ok);
-                const int valid = (int)min<int64_t>(64, F.nwin - idx0 > 0 ? F.nwin - idx0 : 0);
-                const int take = m < valid ? m : valid;
-                n += take;
-                open = take == 64;
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        if (lane == 0) {
-            // double-buffered: the next 16 operands are read while the
-            // current 16 are added
-            int i = 0;
-            T t[16], u[16];
-            if (n >= 16) {
-#pragma unroll
-                for (int k = 0; k < 16; k++) t[k] = stage[k];
-                for (i = 16; i + 16 <= n; i += 16) {
-#pragma unroll
-                    for (int k = 0; k < 16; k++) u[k] = stage[i + k];
-#pragma unroll
-                    for (int k = 0; k < 16; k++) acc = acc + t[k];
-#pragma unroll
-                    for (int k = 0; k < 16; k++) t[k] = u[k];
-                }
-#pragma unroll
-                for (int k = 0; k < 16; k++) acc = acc + t[k];
-            }
-            for (; i < n; i++) acc = acc + stage[i];
-        }
-        __builtin_amdgcn_wave_barrier();
-        cur += n;
-        if (n == BLK) {
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                b0[q] = b1[q];
-                b1[q] = b2[q];
-            }
-            poll(b2, cur + 2 * BLK);
-            idle = 0;
-        } else {
-            if (n == 0) {
-                // wait on the first unfilled slot alone (one load per poll,
-                // not three blocks), then re-read the blocks
-                while (ld_bits(part + cur) == Sentinel<T>::bits) {
-                    if (++idle > (1u << 26)) {  // producers always fill every slot; safety bound only
-                        failed = true;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(2);
-                }
-                if (failed) break;
-            } else {
-                idle = 0;
-            }
-            // re-read from the first unfilled slot
-            poll(b0, cur);
-            poll(b1, cur + BLK);
-            poll(b2, cur + 2 * BLK);
-        }
-    }
-    if (lane == 0) {
-        if (failed) {
-            acc = canonical_nan(T(0));
-        } else if (acc != acc) {
-            acc = fseed<T>(A);
-            for (int64_t i = 0; i < F.nwin; i++) acc = fadd(acc, from_bits<T>(ld_bits(part + i)));
-        }
-        int64_t ib, ob;
-        out_offsets(A, (uint32_t)o, &ib, &ob);
-        ((T *)A.out)[ob] = acc;
-    }
-}
-
 // One wavefront evaluates a whole 8192-element window (NumPy's tree with 64
-// leaves of 128; pw_fast uses it to redo a window whose plain-IEEE value
+// leaves of 128; pw_stream uses it to redo a window whose plain-IEEE value
 // came out NaN, with EXACT set).  Every
 // lane owns CH = 16 / sizeof(T) consecutive accumulators of one leaf and
 // reads them with one 16-byte load per step k (k = 0..15, element
@@ -350,93 +243,284 @@ template <typename T, bool EXACT> __device__ inline T pw_window_wave(const T *x,
     return t;  // the same in every lane
 }
 
-// Producers: a workgroup of 64 * 8 / CH lanes evaluates one window at a time
-// (every lane owns CH accumulators of one leaf and reads them with one
-// 16-byte load per step k), the 64 leaf values meet in LDS and wavefront 0
-// combines them by the balanced tree over lanes.  (Measured on MI355X: one
-// wavefront per window, or one workgroup per CU, are both slower.)
-template <typename T> constexpr int pw_fast_threads() { return sizeof(T) == 8 ? 256 : 128; }  // one window
-
-template <typename T, bool PREFETCH>
-__global__ __launch_bounds__(512) void pw_fast(const RedArgs A, const PwFastState F, int64_t nwindows) {
-    constexpr int CH = 16 / sizeof(T);
-    constexpr int LPL = 8 / CH;
-    constexpr int TPW = 64 * LPL;  // threads per window
-    using V = T __attribute__((ext_vector_type(CH)));
-    __shared__ T stage[1024];  // folder staging
-    __shared__ T leafv[4][64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if ((int64_t)blockIdx.x < F.folders) {
-        if (wave == 0) {
-            // the serial fold must keep pace with the producers that share
-            // its CU: its wavefront issues first
-            __builtin_amdgcn_s_setprio(3);
-            pw_folder<T>(A, F, blockIdx.x, stage);
+// Ring folder: the fold acc = acc + partial[w] of one output split over two
+// wavefronts of one workgroup.  The poller (wave 1) reads the partial slots
+// 1024 at a time, three blocks ahead, and copies each filled prefix into an
+// LDS ring of kRing entries, then publishes the count (ctl[0]).  The chain
+// (wave 0, every lane computing the same sums from broadcast LDS reads,
+// issued a chunk ahead) only adds: its dependent add chain, about 11 cycles an add on MI355X, sets the
+// pace (32768 windows in ~150 us), well under the producers' HBM time, where
+// one wavefront that also polls and stages runs at twice that.  It returns
+// its position (ctl[1]) so the poller never overwrites unread entries.
+// Ordering: each wavefront's LDS operations execute in issue order, so the
+// ring writes land before the count that publishes them and the chain's
+// reads follow the count it read; the wavefront-scope fences keep the
+// compiler from reordering across the counter accesses.  A poller that gives
+// up (bounded wait on a slot producers always fill) sets ctl[2] and the
+// output becomes NaN.  A NaN result is redone with x86 NaN rules.
+constexpr int kRing = 4096;
+template <typename T>
+__device__ void pw_fold_ring(const RedArgs &A, const PwStreamState &F, int64_t o, T *ring, uint32_t *ctl, int wave) {
+    constexpr int Q = 16, BLK = 64 * Q;
+    const int lane = threadIdx.x & 63;
+    const uint32_t nwin = (uint32_t)F.nwin;
+    const T *part = (const T *)F.partials + o * F.nwin;
+    auto lds_load = [](uint32_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
+    auto lds_store = [](uint32_t *p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
+    if (wave == 1) {
+        __builtin_amdgcn_s_setprio(2);
+        uint32_t cur = 0, idle = 0;
+        bool failed = false;
+        uint64_t b0[Q], b1[Q], b2[Q];
+        auto poll = [&](uint64_t *b, int64_t base) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const int64_t idx = base + 64 * q + lane;
+                b[q] = idx < (int64_t)nwin ? ld_bits(part + idx) : Sentinel<T>::bits;
+            }
+        };
+        poll(b0, cur);
+        poll(b1, (int64_t)cur + BLK);
+        poll(b2, (int64_t)cur + 2 * BLK);
+        while (cur < nwin) {
+            int n = 0;  // filled prefix of block b0 (stops at the first sentinel)
+            bool open = true;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const int64_t idx0 = (int64_t)cur + 64 * q;
+                const uint64_t ok = __ballot(b0[q] != Sentinel<T>::bits || idx0 + lane >= nwin);
+                if (open) {
+                    const int m = ~ok == 0 ? 64 : __builtin_ctzll(~ok);
+                    const int valid = (int)min<int64_t>(64, (int64_t)nwin - idx0 > 0 ? (int64_t)nwin - idx0 : 0);
+                    const int take = m < valid ? m : valid;
+                    n += take;
+                    open = take == 64;
+                }
+            }
+            if (n > 0) {
+                // room in the ring for entries cur .. cur + n - 1
+                while (cur + n - lds_load(&ctl[1]) > (uint32_t)kRing) __builtin_amdgcn_s_sleep(1);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+                for (int q = 0; q < Q; q++)
+                    if (64 * q + lane < n) ring[(cur + 64 * q + lane) & (kRing - 1)] = from_bits<T>(b0[q]);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                if (lane == 0) lds_store(&ctl[0], cur + n);
+                cur += n;
+            }
+            if (n == BLK) {
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    b0[q] = b1[q];
+                    b1[q] = b2[q];
+                }
+                poll(b2, (int64_t)cur + 2 * BLK);
+                idle = 0;
+            } else if (cur < nwin) {
+                if (n == 0) {
+                    // wait on the first unfilled slot alone, then re-read the blocks
+                    while (ld_bits(part + cur) == Sentinel<T>::bits) {
+                        if (++idle > (1u << 26)) {  // producers always fill every slot; safety bound only
+                            failed = true;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                    if (failed) break;
+                } else {
+                    idle = 0;
+                }
+                poll(b0, cur);
+                poll(b1, (int64_t)cur + BLK);
+                poll(b2, (int64_t)cur + 2 * BLK);
+            }
         }
+        if (failed && lane == 0) lds_store(&ctl[2], 1u);
         return;
     }
-    // WPB windows side by side per workgroup (slot `sub`), taken round-robin
-    // in groups of WPB consecutive windows
-    const int wpb = blockDim.x / TPW, sub = threadIdx.x / TPW, tin = threadIdx.x % TPW;
-    const int lf = tin / LPL, jq = (lane % LPL) * CH;
-    const int64_t nprod = (int64_t)gridDim.x - F.folders;
+    // wave 0: the chain
+    __builtin_amdgcn_s_setprio(3);
+    T acc = fseed<T>(A);
+    uint32_t i = 0;
+    bool failed = false;
+    while (i < nwin) {
+        const uint32_t avail = __builtin_amdgcn_readfirstlane(lds_load(&ctl[0]));  // the same in every lane
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (avail == i) {
+            if (lds_load(&ctl[2])) {
+                failed = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(0);
+            continue;
+        }
+        for (; i < avail && (i & 15); i++) acc = acc + ring[i & (kRing - 1)];
+        // 16-entry chunks (16-aligned: no wrap inside one), two per step in
+        // alternating registers; the next chunk's LDS reads are issued before
+        // the current chunk's adds and the scheduling barriers keep them there
+        using V2 = T __attribute__((ext_vector_type(16 / sizeof(T))));
+        constexpr int NV = 16 * sizeof(T) / 16;  // 16-byte reads per chunk
+        auto rd = [&](V2 *c, uint32_t at) {
+            const V2 *r = (const V2 *)(ring + (at & (kRing - 1)));
+#pragma unroll
+            for (int k = 0; k < NV; k++) c[k] = r[k];
+        };
+        auto addc = [&](const V2 *c) {
+#pragma unroll
+            for (int k = 0; k < NV; k++)
+#pragma unroll
+                for (int e = 0; e < 16 / (int)sizeof(T); e++) acc = acc + c[k][e];
+        };
+        if (avail - i >= 16) {
+            V2 ca[NV], cb[NV];
+            rd(ca, i);
+            while (avail - i >= 32) {  // chunks i (in ca) and i + 16
+                rd(cb, i + 16);
+                __builtin_amdgcn_sched_barrier(0);
+                addc(ca);
+                __builtin_amdgcn_sched_barrier(0);
+                rd(ca, i + 32);  // entries past avail are read but not used
+                __builtin_amdgcn_sched_barrier(0);
+                addc(cb);
+                __builtin_amdgcn_sched_barrier(0);
+                i += 32;
+                if ((i & 255) < 32) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    lds_store(&ctl[1], i);
+                }
+            }
+            if (avail - i >= 16) {
+                addc(ca);
+                i += 16;
+            }
+        }
+        for (; i < avail; i++) acc = acc + ring[i & (kRing - 1)];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        lds_store(&ctl[1], i);
+    }
+    if (lane == 0) {
+        if (failed) {
+            acc = canonical_nan(T(0));
+        } else if (acc != acc) {
+            acc = fseed<T>(A);
+            for (uint32_t w = 0; w < nwin; w++) acc = fadd(acc, from_bits<T>(ld_bits(part + w)));
+        }
+        int64_t ib, ob;
+        out_offsets(A, (uint32_t)o, &ib, &ob);
+        ((T *)A.out)[ob] = acc;
+    }
+}
+
+// Streaming producers: every wavefront owns whole windows (window g = its
+// global wave index + k * producer waves) and reads each one as batches of
+// 16 KB, one fully coalesced 1 KB load per instruction (lane L takes bytes
+// 16 L .. 16 L + 15 of chunk k), which HBM serves faster than loading the
+// leaf layout directly (measured on MI355X: 6.6-7.0 against 5.7-6.2 TB/s for
+// reads alone).  The batch is moved through the wave's own LDS region into
+// the leaf layout (lane: accumulators jq .. jq+CH-1 of leaf lf, 16 steps of
+// one 16-byte read), so every add keeps NumPy's order.  A batch holds LPB whole
+// leaves, a complete subtree of the window's balanced tree, so its value is
+// combined across lanes at once and the window's NB batch values finish the
+// tree in registers: no workgroup barrier.  The next batch's loads are issued
+// before the current batch is combined.  LDS rows are padded by 8 elements
+// per leaf, so the leaf-layout reads hit distinct banks.  NT: nontemporal
+// loads (inputs larger than the Infinity Cache).
+constexpr int kPsWaves = 4;  // wavefronts per workgroup
+template <typename T> constexpr int ps_leaves() { return 128 / sizeof(T); }  // leaves per batch: 16 f64, 32 f32
+
+template <typename T, bool NT>
+__global__ __launch_bounds__(64 * kPsWaves) __attribute__((amdgpu_waves_per_eu(2))) void pw_stream(const RedArgs A, const PwStreamState F, int64_t nwindows) {
+    constexpr int CH = 16 / sizeof(T);  // elements per 16-byte load
+    constexpr int LPL = 8 / CH;         // lanes per leaf
+    constexpr int LPB = ps_leaves<T>(); // leaves per batch (16 loads of 1 KB)
+    constexpr int NB = 64 / LPB;        // batches per window (4 f64, 2 f32)
+    constexpr int LS = 128 + 8;         // LDS leaf stride (elements)
+    using V = T __attribute__((ext_vector_type(CH)));
+    __shared__ T lds[kPsWaves][LPB * LS];
+    __shared__ uint32_t ctl[4];
+    static_assert(kPsWaves * LPB * LS >= kRing, "the folder's ring reuses the batch regions");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if ((int64_t)blockIdx.x < F.folders) {
+        if (threadIdx.x < 4) ctl[threadIdx.x] = 0;
+        __syncthreads();
+        if (wave < 2) pw_fold_ring<T>(A, F, blockIdx.x, &lds[0][0], ctl, wave);
+        return;
+    }
+    T *L = lds[wave];
+    const int lf = lane / LPL, jq = (lane % LPL) * CH;
+    const int64_t pwaves = ((int64_t)gridDim.x - F.folders) * kPsWaves;
     auto window_base = [&](int64_t g) {
         const int64_t o = g / F.nfull, w = g - o * F.nfull;
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
         return (const T *)A.in + ib + w * 8192;
     };
-    V v[16], nv[16];
-    int64_t gb = ((int64_t)blockIdx.x - F.folders) * wpb;  // first window of this round
-    if (PREFETCH && gb + sub < nwindows) {
-        const T *x = window_base(gb + sub) + (int64_t)lf * 128 + jq;
+    auto load = [&](V *r, const T *p) {
 #pragma unroll
-        for (int k = 0; k < 16; k++) nv[k] = *(const V *)(x + 8 * k);
-    }
-    for (; gb < nwindows; gb += nprod * wpb) {
-        const int64_t g = gb + sub;
-        const bool live = g < nwindows;
-        const int64_t o = live ? g / F.nfull : 0, w = live ? g - o * F.nfull : 0;
-        const T *xw = live ? window_base(g) : (const T *)A.in;
-        if constexpr (PREFETCH) {
-            // this window was loaded one round ago; request the next one
-#pragma unroll
-            for (int k = 0; k < 16; k++) v[k] = nv[k];
-            if (g + nprod * wpb < nwindows) {
-                const T *xn = window_base(g + nprod * wpb) + (int64_t)lf * 128 + jq;
-#pragma unroll
-                for (int k = 0; k < 16; k++) nv[k] = *(const V *)(xn + 8 * k);
-            }
-        } else if (live) {
-            const T *x = xw + (int64_t)lf * 128 + jq;
-#pragma unroll
-            for (int k = 0; k < 16; k++) v[k] = *(const V *)(x + 8 * k);
+        for (int k = 0; k < 16; k++) {
+            const V *q = (const V *)(p + (k * 64 + lane) * CH);
+            r[k] = NT ? __builtin_nontemporal_load(q) : *q;
         }
-        if (live) {
-            // accumulators jq .. jq+CH-1 of leaf lf, each over k in order
-            V r = v[0];
+    };
+    // one batch: registers (coalesced layout) -> LDS -> leaf layout -> subtree value
+    auto batch = [&](const V *r) {
 #pragma unroll
-            for (int k = 1; k < 16; k++) r = r + v[k];
-            T a;
-            if constexpr (CH == 2) {
-                a = r[0] + r[1];              // r0+r1 (lane holds accumulators 2m, 2m+1)
-                a = xor_combine(a, 1, lane);  // (r0+r1)+(r2+r3)
-                a = xor_combine(a, 2, lane);  // leaf value
-            } else {
-                a = (r[0] + r[1]) + (r[2] + r[3]);
-                a = xor_combine(a, 1, lane);  // leaf value
-            }
-            if (lane % LPL == 0) leafv[sub][lf] = a;
+        for (int k = 0; k < 16; k++) {
+            const int e = (k * 64 + lane) * CH;  // element of the batch; CH elements never straddle a leaf
+            *(V *)(L + e + (e >> 7) * 8) = r[k];
         }
-        __syncthreads();
-        if (live && tin < 64) {  // the slot's first wavefront: the tree over its 64 leaves
-            T t = leafv[sub][lane];
+        // the wave's DS operations execute in order; the fences keep the
+        // compiler from moving the reads above the other lanes' writes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        V v[16];
 #pragma unroll
-            for (int m = 1; m < 64; m <<= 1) t = xor_combine(t, m, lane);
-            if (t != t) t = pw_window_wave<T, true>(xw, lane);  // NaN: x86 NaN rules, exactly
-            if (lane == 0) st_value((T *)F.partials + o * F.nwin + w, t);
+        for (int k = 0; k < 16; k++) v[k] = *(const V *)(L + lf * LS + 8 * k + jq);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        V r0 = v[0];
+#pragma unroll
+        for (int k = 1; k < 16; k++) r0 = r0 + v[k];
+        T a;
+        if constexpr (CH == 2) {
+            a = r0[0] + r0[1];
+            a = xor_combine(a, 1, lane);
+            a = xor_combine(a, 2, lane);  // leaf value
+        } else {
+            a = (r0[0] + r0[1]) + (r0[2] + r0[3]);
+            a = xor_combine(a, 1, lane);  // leaf value
         }
-        __syncthreads();
+#pragma unroll
+        for (int m = LPL; m < 64; m <<= 1) a = xor_combine(a, m, lane);  // the batch's subtree
+        return a;
+    };
+    int64_t g = ((int64_t)blockIdx.x - F.folders) * kPsWaves + wave;
+    if (g >= nwindows) return;
+    V ra[16], rb[16];
+    load(ra, window_base(g));
+    for (; g < nwindows; g += pwaves) {
+        const T *xw = window_base(g);
+        // the next window's first batch (the last window re-reads its own:
+        // an unconditional load keeps the wait counts static across windows)
+        const T *xn = g + pwaves < nwindows ? window_base(g + pwaves) : xw;
+        T sb[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            // batch b sits in ra (b even) or rb (b odd); request the next one into the other
+            V *cur = (b & 1) ? rb : ra, *nxt = (b & 1) ? ra : rb;
+            load(nxt, b + 1 < NB ? xw + (b + 1) * LPB * 128 : xn);
+            sb[b] = batch(cur);
+        }
+        T t;
+        if constexpr (NB == 4) t = (sb[0] + sb[1]) + (sb[2] + sb[3]);
+        else t = sb[0] + sb[1];
+        if (t != t) t = pw_window_wave<T, true>(xw, lane);  // NaN: x86 NaN rules, exactly
+        if (lane == 0) {
+            const int64_t o = g / F.nfull, w = g - o * F.nfull;
+            st_value((T *)F.partials + o * F.nwin + w, t);
+        }
     }
 }
 
@@ -1481,9 +1565,17 @@ static ionp_status run_masked(RedArgs &A, const ionp_config *cfg, hipStream_t s)
     }
 }
 
-template <typename T> static ionp_status run_pw_fast(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
-    const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
-    PwFastState F;
+// Launch of pw_stream (one workgroup per CU: the dynamic LDS request below
+// puts a workgroup above half of a CU's 160 KB).  So a folder has its CU to
+// itself (sharing one with producers, its chain fell behind them: 2^28
+// float64 elements in 358 against 337 us, measured on MI355X), and every
+// producer is resident at once, which the folders' waits rely on.  Producers
+// on 7/8 of the CUs stream faster than on all of them (float32
+// (4096,65536) rows: 165 against 170 us); the other CUs host the folders.
+// The launch shape is fixed by residency, so ionp_config does not apply.
+constexpr size_t kPsPadLds = 16u << 10;
+template <typename T> static ionp_status run_pw_stream(RedArgs &A, hipStream_t s) {
+    PwStreamState F;
     F.nfull = A.wsize / 8192;
     const int64_t has_tail = A.wsize % 8192 != 0;
     F.nwin = F.nfull + has_tail;
@@ -1496,22 +1588,24 @@ template <typename T> static ionp_status run_pw_fast(RedArgs &A, const ionp_conf
     hipLaunchKernelGGL(fill_sentinel<T>, dim3((unsigned)std::min<int64_t>(1024, (nw_total + 255) / 256)), dim3(256), 0, s,
                        part, nw_total);
     if (has_tail)  // the last, partial window of each output (generic tree)
-        hipLaunchKernelGGL(pw_windows<T>, dim3(grid_cap(A.nout, bpc)), dim3(kPwThreads), 0, s, A, A.nout, 0, F.nwin);
+        hipLaunchKernelGGL(pw_windows<T>, dim3(grid_cap(A.nout, 8)), dim3(kPwThreads), 0, s, A, A.nout, 0, F.nwin);
     const int64_t nwindows = A.nout * F.nfull;
-    // a folder shares its CU with producers: with one producer workgroup per
-    // CU the serial fold keeps pace with HBM (measured on MI355X)
-    const int pbpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : (F.folders ? 1 : 2);
-    // with one producer per CU (folders) the next window's loads are issued
-    // before the current one is combined; with several per CU they are not
-    // (measured on MI355X both ways); split 1 / 2 forces either
-    const bool prefetch = (cfg && cfg->split > 0) ? cfg->split == 2 : F.folders > 0;
-    // vec: windows side by side per workgroup (1, 2 or 4; 512 threads at most)
-    int wpb = (cfg && cfg->vec > 0) ? cfg->vec : 1;
-    while (wpb > 1 && pw_fast_threads<T>() * wpb > 512) wpb >>= 1;
-    const dim3 grid(F.folders + grid_cap((nwindows + wpb - 1) / wpb, pbpc)), blk(pw_fast_threads<T>() * wpb);
-    if (prefetch) hipLaunchKernelGGL((pw_fast<T, true>), grid, blk, 0, s, A, F, nwindows);
-    else hipLaunchKernelGGL((pw_fast<T, false>), grid, blk, 0, s, A, F, nwindows);
-    if (!F.folders) hipLaunchKernelGGL(pw_fold<T>, dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
+    static int resident = 0;  // workgroups per CU (1 with the pad)
+    if (!resident) {
+        int b = 0;
+        resident = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pw_stream<T, true>, 64 * kPsWaves, kPsPadLds) == hipSuccess && b > 0
+                       ? b
+                       : 1;
+    }
+    const int64_t cus = num_cus();
+    int64_t prod = std::min<int64_t>((nwindows + kPsWaves - 1) / kPsWaves, resident * (cus * 7 / 8));
+    prod = std::max<int64_t>(1, std::min<int64_t>(prod, resident * cus - F.folders));
+    const dim3 grid((unsigned)(F.folders + prod)), blk(64 * kPsWaves);
+    if ((double)A.nout * (double)A.wsize * sizeof(T) > (double)kInfinityCacheBytes)  // streamed once: nontemporal
+        hipLaunchKernelGGL((pw_stream<T, true>), grid, blk, kPsPadLds, s, A, F, nwindows);
+    else
+        hipLaunchKernelGGL((pw_stream<T, false>), grid, blk, kPsPadLds, s, A, F, nwindows);
+    if (!F.folders) hipLaunchKernelGGL(pw_fold<T>, dim3(grid_cap((A.nout + 255) / 256, 8)), dim3(256), 0, s, A);
     return launched();
 }
 
@@ -1527,7 +1621,7 @@ template <typename T> static ionp_status run_float_add(RedArgs &A, const ionp_co
                          A.in_dt == A.out_dt && ((uintptr_t)A.in % sizeof(T)) == 0 &&
                          A.wsize / 8192 * A.nout < (1ll << 31);
     if (rows_ok)
-        return run_pw_fast<T>(A, cfg, s);
+        return run_pw_stream<T>(A, s);
     if (pw_small_ok(A)) {
         hipLaunchKernelGGL(pw_small<T>, dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
         return launched();
