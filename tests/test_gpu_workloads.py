@@ -75,3 +75,23 @@ def test_cfg3_reductions(ap, digests):
     s = ap.sum(I)
     assert s.dtype == np.int64 and int(s.get()) == d["i32_sum"]
     torch.cuda.empty_cache()
+
+
+def test_single_launch_folds_repeat(ap):
+    """Reductions that fold per-workgroup partials (maximum/minimum over
+    contiguous windows, argmax/argmin over 65536-element segments, float
+    row sums without folder workgroups) reuse the library's scratch memory;
+    interleaved repeats must each match NumPy."""
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal(3 * 2 ** 20 + 5)
+    R = rng.standard_normal((300, 2 * 8192 + 3)).astype(np.float32)
+    xd, Rd = ap.asarray(x), ap.asarray(R)
+    exp = {"max": x.max(), "min": x.min(), "argmax": int(x.argmax()), "argmin": int(x.argmin()),
+           "rows": np.add.reduce(R, axis=1), "rowmax": R.max(axis=1)}
+    for _ in range(6):
+        assert ap.maximum.reduce(xd).get() == exp["max"]
+        assert int(ap.argmax(xd).get()) == exp["argmax"]
+        assert np.array_equal(ap.add.reduce(Rd, axis=1).get().view(np.uint32), exp["rows"].view(np.uint32))
+        assert ap.minimum.reduce(xd).get() == exp["min"]
+        assert int(ap.argmin(xd).get()) == exp["argmin"]
+        assert np.array_equal(ap.maximum.reduce(Rd, axis=1).get(), exp["rowmax"])
