@@ -257,14 +257,21 @@ template <typename T, bool EXACT> __device__ inline T pw_window_wave(const T *x,
 // reads follow the count it read; the wavefront-scope fences keep the
 // compiler from reordering across the counter accesses.  A poller that gives
 // up (bounded wait on a slot producers always fill) sets ctl[2] and the
-// output becomes NaN.  A NaN result is redone with x86 NaN rules.
+// output becomes NaN.
+// The slots live in a buffer that holds the sentinel between calls: a third
+// wavefront writes it back into every slot the chain has consumed, so no
+// fill precedes the next call.  NaN: the chain checks acc at checkpoints
+// (every 256 entries); the stretch that turned it NaN is redone from the
+// ring with x86 NaN rules (fadd), and that NaN is final (an x86 add keeps a
+// NaN first operand), so the partials are never needed again.
 constexpr int kRing = 4096;
 template <typename T>
 __device__ void pw_fold_ring(const RedArgs &A, const PwStreamState &F, int64_t o, T *ring, uint32_t *ctl, int wave) {
     constexpr int Q = 16, BLK = 64 * Q;
     const int lane = threadIdx.x & 63;
     const uint32_t nwin = (uint32_t)F.nwin;
-    const T *part = (const T *)F.partials + o * F.nwin;
+    T *slots = (T *)F.partials + o * F.nwin;
+    const T *part = slots;
     auto lds_load = [](uint32_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
     auto lds_store = [](uint32_t *p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
     if (wave == 1) {
@@ -335,18 +342,62 @@ __device__ void pw_fold_ring(const RedArgs &A, const PwStreamState &F, int64_t o
                 poll(b2, (int64_t)cur + 2 * BLK);
             }
         }
-        if (failed && lane == 0) lds_store(&ctl[2], 1u);
+        if (failed) {
+            // leave every slot of this output holding the sentinel again
+            for (uint32_t i = cur + lane; i < nwin; i += 64) st_value(slots + i, from_bits<T>(Sentinel<T>::bits));
+            if (lane == 0) lds_store(&ctl[2], 1u);
+        }
+        return;
+    }
+    if (wave == 2) {
+        // the resetter: the sentinel back into every slot the chain has
+        // consumed (ctl[1]), until the chain has finished (ctl[3]); its
+        // stores are kept out of the poller, whose loads would wait on them
+        uint32_t done = 0;
+        for (;;) {
+            const uint32_t fin = lds_load(&ctl[3]);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint32_t c = __builtin_amdgcn_readfirstlane(lds_load(&ctl[1]));
+            for (uint32_t j = done + lane; j < c; j += 64) st_value(slots + j, from_bits<T>(Sentinel<T>::bits));
+            const bool moved = c > done;
+            done = c > done ? c : done;
+            if (done >= nwin || fin) break;  // fin was read before c: c is final
+            if (!moved) __builtin_amdgcn_s_sleep(2);
+        }
         return;
     }
     // wave 0: the chain
     __builtin_amdgcn_s_setprio(3);
-    T acc = fseed<T>(A);
-    uint32_t i = 0;
-    bool failed = false;
+    T acc = fseed<T>(A), ck_acc = acc, nan_val = acc;
+    uint32_t i = 0, ck = 0;  // checkpoint: acc was ck_acc before entry ck
+    bool failed = false, nan = false;  // nan: the result is nan_val
+    // Plain adds; at a checkpoint, a NaN that arose since the last one is
+    // redone from the ring with x86 NaN rules (fadd) and kept as the result
+    // (an x86 add keeps a NaN first operand).  Entries from ck on are still
+    // in the ring: the poller overwrites only below ctl[1] + kRing, and ctl[1]
+    // moves only here.
+    auto checkpoint = [&]() {
+        if (!nan && acc != acc) {
+            T r = ck_acc;
+            for (uint32_t j = ck; j < i; j++) {
+                r = fadd(r, ring[j & (kRing - 1)]);
+                if (r != r) break;
+            }
+            nan_val = r;
+            nan = true;
+        }
+        ck_acc = acc;
+        ck = i;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        lds_store(&ctl[1], i);
+    };
     while (i < nwin) {
-        const uint32_t avail = __builtin_amdgcn_readfirstlane(lds_load(&ctl[0]));  // the same in every lane
+        // whole 16-entry chunks only, so i stays 16-aligned; the last few
+        // entries once every slot is in
+        const uint32_t pub = __builtin_amdgcn_readfirstlane(lds_load(&ctl[0]));  // the same in every lane
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (avail == i) {
+        const uint32_t avail = pub == nwin ? pub : pub & ~15u;
+        if (avail <= i) {
             if (lds_load(&ctl[2])) {
                 failed = true;
                 break;
@@ -354,7 +405,6 @@ __device__ void pw_fold_ring(const RedArgs &A, const PwStreamState &F, int64_t o
             __builtin_amdgcn_s_sleep(0);
             continue;
         }
-        for (; i < avail && (i & 15); i++) acc = acc + ring[i & (kRing - 1)];
         // 16-entry chunks (16-aligned: no wrap inside one), two per step in
         // alternating registers; the next chunk's LDS reads are issued before
         // the current chunk's adds and the scheduling barriers keep them there
@@ -384,10 +434,7 @@ __device__ void pw_fold_ring(const RedArgs &A, const PwStreamState &F, int64_t o
                 addc(cb);
                 __builtin_amdgcn_sched_barrier(0);
                 i += 32;
-                if ((i & 255) < 32) {
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    lds_store(&ctl[1], i);
-                }
+                if ((i & 255) < 32) checkpoint();
             }
             if (avail - i >= 16) {
                 addc(ca);
@@ -395,16 +442,13 @@ __device__ void pw_fold_ring(const RedArgs &A, const PwStreamState &F, int64_t o
             }
         }
         for (; i < avail; i++) acc = acc + ring[i & (kRing - 1)];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        lds_store(&ctl[1], i);
+        checkpoint();
     }
+    if (nan) acc = nan_val;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    lds_store(&ctl[3], 1u);  // finished: ctl[1] is final
     if (lane == 0) {
-        if (failed) {
-            acc = canonical_nan(T(0));
-        } else if (acc != acc) {
-            acc = fseed<T>(A);
-            for (uint32_t w = 0; w < nwin; w++) acc = fadd(acc, from_bits<T>(ld_bits(part + w)));
-        }
+        if (failed) acc = canonical_nan(T(0));
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
         ((T *)A.out)[ob] = acc;
@@ -443,7 +487,7 @@ __global__ __launch_bounds__(64 * kPsWaves) __attribute__((amdgpu_waves_per_eu(2
     if ((int64_t)blockIdx.x < F.folders) {
         if (threadIdx.x < 4) ctl[threadIdx.x] = 0;
         __syncthreads();
-        if (wave < 2) pw_fold_ring<T>(A, F, blockIdx.x, &lds[0][0], ctl, wave);
+        if (wave < 3) pw_fold_ring<T>(A, F, blockIdx.x, &lds[0][0], ctl, wave);
         return;
     }
     T *L = lds[wave];
@@ -535,6 +579,43 @@ __global__ void zero_u32(uint32_t *p, int64_t n) {
 
 // Ordered fold per output: acc = 0; acc = acc + partial[i] (ufunc.rs:7137
 // zero seed; the outer-axis sequential fold of pairwise_group 6504-6531).
+// Partial slots of the in-launch folds: a grow-only buffer that holds the
+// sentinel between calls (the folders write it back into every slot they
+// consume), refilled only when it grows or its element size changes.
+static void *g_slots = nullptr;
+static size_t g_slots_bytes = 0;
+static int g_slots_isz = 0;  // element size of the sentinel it holds (0: none)
+template <typename T> static T *slots_get(int64_t n, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    const size_t bytes = (size_t)n * sizeof(T);
+    if (bytes > g_slots_bytes) {
+        if (g_slots) {
+            (void)hipDeviceSynchronize();  // earlier launches may still use it
+            (void)hipFree(g_slots);
+            g_slots = nullptr;
+            g_slots_bytes = 0;
+        }
+        const size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
+        if (hipMalloc(&g_slots, want) != hipSuccess) {
+            g_slots = nullptr;
+            return nullptr;
+        }
+        g_slots_bytes = want;
+        g_slots_isz = 0;
+    }
+    if (g_slots_isz != (int)sizeof(T)) {
+        const int64_t cnt = (int64_t)(g_slots_bytes / sizeof(T));
+        hipLaunchKernelGGL(fill_sentinel<T>, dim3((unsigned)std::min<int64_t>(1024, (cnt + 255) / 256)), dim3(256), 0, s,
+                           (T *)g_slots, cnt);
+        g_slots_isz = (int)sizeof(T);
+    }
+    return (T *)g_slots;
+}
+static void slots_dirty() {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    g_slots_isz = 0;
+}
+
 template <typename T> __global__ __launch_bounds__(256) void pw_fold(const RedArgs A) {
     const int64_t per_out = A.rsize * A.nwin;
     for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < A.nout; o += (int64_t)gridDim.x * blockDim.x) {
@@ -1581,12 +1662,12 @@ template <typename T> static ionp_status run_pw_stream(RedArgs &A, hipStream_t s
     F.nwin = F.nfull + has_tail;
     F.folders = (A.nout <= 64 && F.nwin > 8) ? (int32_t)A.nout : 0;
     const int64_t nw_total = A.nout * F.nwin;
-    T *part = (T *)scratch_get((size_t)nw_total * sizeof(T), s);
+    // folders poll the slots for the sentinel; pw_fold runs after the launch
+    // and needs none
+    T *part = F.folders ? slots_get<T>(nw_total, s) : (T *)scratch_get((size_t)nw_total * sizeof(T), s);
     if (!part) return set_error(IONP_ERR_DEVICE, "out of device memory for reduction scratch");
     F.partials = part;
     A.scratch = part;
-    hipLaunchKernelGGL(fill_sentinel<T>, dim3((unsigned)std::min<int64_t>(1024, (nw_total + 255) / 256)), dim3(256), 0, s,
-                       part, nw_total);
     if (has_tail)  // the last, partial window of each output (generic tree)
         hipLaunchKernelGGL(pw_windows<T>, dim3(grid_cap(A.nout, 8)), dim3(kPwThreads), 0, s, A, A.nout, 0, F.nwin);
     const int64_t nwindows = A.nout * F.nfull;
@@ -1606,7 +1687,9 @@ template <typename T> static ionp_status run_pw_stream(RedArgs &A, hipStream_t s
     else
         hipLaunchKernelGGL((pw_stream<T, false>), grid, blk, kPsPadLds, s, A, F, nwindows);
     if (!F.folders) hipLaunchKernelGGL(pw_fold<T>, dim3(grid_cap((A.nout + 255) / 256, 8)), dim3(256), 0, s, A);
-    return launched();
+    const ionp_status st = launched();
+    if (st != IONP_OK && F.folders) slots_dirty();  // slots may hold values: refill before the next use
+    return st;
 }
 
 template <typename T> static ionp_status run_float_add(RedArgs &A, const ionp_config *cfg, hipStream_t s) {

@@ -34,6 +34,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extra", action="store_true")
+    p.add_argument("--reductions", action="store_true",
+                   help="only run the reduction workloads and print their timings (the workload of the "
+                        "reduction kernel profiles)")
     p.add_argument("--roofline-pmc", action="store_true",
                    help="only run the dominant kernel and a plain copy of known size, 5 times each "
                         "(the workload of the rocprofv3 FETCH_SIZE / WRITE_SIZE passes)")
@@ -73,6 +76,13 @@ def main():
     args = parse()
     if args.roofline_pmc:
         roofline_pmc_workload()
+        return
+    if args.reductions:
+        import torch
+        import anionpy_amd as ap
+        torch.cuda.set_device(0)
+        ap.seterr(all="ignore")
+        print(json.dumps(reduction_extras(ap, torch)), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this backend runs on one GPU; multi-GPU sharding is described "

@@ -95,3 +95,29 @@ def test_single_launch_folds_repeat(ap):
         assert ap.minimum.reduce(xd).get() == exp["min"]
         assert int(ap.argmin(xd).get()) == exp["argmin"]
         assert np.array_equal(ap.maximum.reduce(Rd, axis=1).get(), exp["rowmax"])
+
+
+def test_folder_sum_special_values_and_repeat(ap):
+    """One-output float sums long enough for the in-launch folder (more than
+    8 windows, with a partial last window): NaN payloads, infinities of both
+    signs and overflow must give NumPy's bits, and every call must leave the
+    partial slots ready for the next one (repeats, alternating float32 and
+    float64)."""
+    rng = np.random.default_rng(13)
+    base = rng.standard_normal(2 ** 20 + 77)
+    nan1 = np.array([0x7FF8000000000123], dtype=np.uint64).view(np.float64)[0]
+    cases = []
+    x = base.copy()
+    cases.append(x)
+    x = base.copy(); x[5000] = nan1; cases.append(x)
+    x = base.copy(); x[900001] = nan1; x[1000000] = np.inf; cases.append(x)
+    x = base.copy(); x[10] = np.inf; x[700000] = -np.inf; cases.append(x)
+    x = base.copy(); x[123456] = np.inf; cases.append(x)
+    x = base.copy(); x[:3] = 1.7e308; cases.append(x)
+    for _ in range(2):
+        for h in cases:
+            for dt in (np.float64, np.float32):
+                hd = h.astype(dt)
+                exp = np.atleast_1d(np.add.reduce(hd)).view(np.uint64 if dt == np.float64 else np.uint32)
+                got = np.atleast_1d(ap.add.reduce(ap.asarray(hd)).get()).view(exp.dtype)
+                assert got[0] == exp[0], (dt, hex(int(got[0])), hex(int(exp[0])))
