@@ -1013,6 +1013,17 @@ __global__ __launch_bounds__(256) void ext_fast(const RedArgs A, int64_t wpb, in
         T m = T(0), tv = T(0);
         uint32_t has_m = 0, nan_early = 0, vnan_last = 0, any_simd = 0;
         int64_t tk = INT64_MAX;  // first NaN in the last window's tail
+        // windows [w0, wf) are full and 16-byte aligned: each is scanned
+        // while the next one's loads are in flight
+        const T *xb = (const T *)A.in + ib;
+        const int64_t wf = (A.bufsize == 8192 && ((uintptr_t)xb % 16) == 0) ? min(w1, A.wsize / 8192) : w0;
+        VT v[NV], pf[NV];
+        auto load = [&](VT *r, int64_t w) {
+#pragma unroll
+            for (int k = 0; k < NV; k++)
+                r[k] = __builtin_nontemporal_load((const VT *)(xb + w * 8192 + ((int64_t)k * 256 + threadIdx.x) * VEC));
+        };
+        if (w0 < wf) load(v, w0);
         for (int64_t w = w0; w < w1; w++) {
             const bool last = w == w1 - 1;
             const int64_t s0 = w * A.bufsize;
@@ -1036,15 +1047,35 @@ __global__ __launch_bounds__(256) void ext_fast(const RedArgs A, int64_t wpb, in
                     has_m = 1;
                 }
             };
-            if (L == 8192 && ((uintptr_t)x % 16) == 0) {
-                VT v[NV];
+            if (w < wf) {
+                if (w + 1 < wf) load(pf, w + 1);
+                // Usual case, no NaN among this thread's elements: only the
+                // best value matters, and the hardware max/min give it (equal
+                // non-zero values have equal bits; a zero best is redone
+                // exactly below).  Otherwise, or when the thread holds the
+                // seed element, every element is visited.
+                bool isn = false;
+                T mx = v[0][0];
 #pragma unroll
                 for (int k = 0; k < NV; k++)
-                    v[k] = __builtin_nontemporal_load((const VT *)(x + ((int64_t)k * 256 + threadIdx.x) * VEC));
 #pragma unroll
-                for (int k = 0; k < NV; k++)
+                    for (int c = 0; c < VEC; c++) {
+                        isn |= v[k][c] != v[k][c];
+                        mx = MAX ? __builtin_fmax(mx, v[k][c]) : __builtin_fmin(mx, v[k][c]);
+                    }
+                if (!isn && !(skip && threadIdx.x == 0)) {
+                    if (!has_m || beats<T, MAX>(mx, m)) {
+                        m = mx;
+                        has_m = 1;
+                    }
+                } else {
 #pragma unroll
-                    for (int c = 0; c < VEC; c++) visit(v[k][c], ((int64_t)k * 256 + threadIdx.x) * VEC + c);
+                    for (int k = 0; k < NV; k++)
+#pragma unroll
+                        for (int c = 0; c < VEC; c++) visit(v[k][c], ((int64_t)k * 256 + threadIdx.x) * VEC + c);
+                }
+#pragma unroll
+                for (int k = 0; k < NV; k++) v[k] = pf[k];
             } else {
                 for (int64_t e = threadIdx.x; e < L; e += blockDim.x) visit(x[e], e);
             }
@@ -1730,8 +1761,14 @@ template <typename T, bool MAX> static ionp_status run_float_ext(RedArgs &A, con
     A.scratch = scratch_get((size_t)nwindows * sizeof(ExtSum<T>), s);
     if (!A.scratch) return set_error(IONP_ERR_DEVICE, "out of device memory for reduction scratch");
     if (A.nw == 1 && A.w_st[0] == 1 && A.nr == 0 && A.in_dt == A.out_dt) {
-        // enough workgroups to fill the chip, each folding a run of windows
-        const int64_t want = (int64_t)num_cus() * 8;
+        // one round of resident workgroups, each folding a run of windows
+        // (fewer, longer runs: fewer summaries for ext_fold)
+        static int resident = 0;
+        if (!resident) {
+            int b = 0;
+            resident = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, ext_fast<T, MAX>, 256, 0) == hipSuccess && b > 0 ? b : 2;
+        }
+        const int64_t want = (int64_t)num_cus() * std::min(resident, bpc);
         int64_t wpb = (nwindows + want - 1) / want;
         if (wpb < 1) wpb = 1;
         if (wpb > A.nwin) wpb = A.nwin;
@@ -2136,7 +2173,10 @@ template <typename T, bool MAX> static ionp_status run_arg_t(ArgArgs &A, const i
         hipLaunchKernelGGL((arg_cols<T, MAX>), dim3(grid_cap((A.nout + 255) / 256, bpc)), dim3(256), 0, s, A);
         return launched();
     }
-    A.seg = 65536;
+    // segments of at least 65536 indices, and no more of them than one
+    // round of workgroups (fewer partials for arg_fold)
+    const int64_t slots = (int64_t)num_cus() * bpc;
+    A.seg = std::max<int64_t>(65536, ((A.len + slots - 1) / slots + 4095) / 4096 * 4096);
     A.nseg = (A.len + A.seg - 1) / A.seg;
     if (A.nseg > 1) {
         A.scratch = scratch_get((size_t)(A.nout * A.nseg) * sizeof(ArgBest<T, MAX>), s);
