@@ -1472,7 +1472,20 @@ __global__ __launch_bounds__(256) void int_reduce_contig(const RedArgs A, int64_
         int64_t v0 = q0, v1 = q1;
         while (v0 < q1 && ((uintptr_t)(x + v0) % 16)) v0++;
         v1 = v0 + (q1 - v0) / VEC * VEC;
-        for (int64_t q = v0 + (int64_t)threadIdx.x * VEC; q < v1; q += (int64_t)blockDim.x * VEC) {
+        // four vectors per thread in flight, then the rest one at a time
+        constexpr int U = 4;
+        const int64_t step = (int64_t)blockDim.x * VEC;
+        int64_t q = v0 + (int64_t)threadIdx.x * VEC;
+        for (; q + (U - 1) * step < v1; q += U * step) {
+            V v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) v[u] = __builtin_nontemporal_load((const V *)(x + q + u * step));
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int k = 0; k < VEC; k++) acc = assoc<AT, OP>(acc, (AT)v[u][k]);
+        }
+        for (; q < v1; q += step) {
             const V v = __builtin_nontemporal_load((const V *)(x + q));
 #pragma unroll
             for (int k = 0; k < VEC; k++) acc = assoc<AT, OP>(acc, (AT)v[k]);
@@ -1593,8 +1606,11 @@ static ionp_status run_int(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
                            s, A);
         return launched();
     }
-    int64_t seg = 65536;
-    int64_t nseg = (A.group + seg - 1) / seg;
+    // segments of at least 65536 elements, and no more of them than one
+    // round of workgroups (fewer partials for int_reduce_fold)
+    const int64_t slots = (int64_t)num_cus() * bpc;
+    const int64_t seg = std::max<int64_t>(65536, ((A.group + slots - 1) / slots + 4095) / 4096 * 4096);
+    const int64_t nseg = (A.group + seg - 1) / seg;
     if (nseg > 1) {
         A.scratch = scratch_get((size_t)(A.nout * nseg) * sizeof(AT), s);
         if (!A.scratch) return set_error(IONP_ERR_DEVICE, "out of device memory for reduction scratch");
