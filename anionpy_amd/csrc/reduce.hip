@@ -546,9 +546,11 @@ __global__ __launch_bounds__(64 * kPsWaves) __attribute__((amdgpu_waves_per_eu(2
     load(ra, window_base(g));
     for (; g < nwindows; g += pwaves) {
         const T *xw = window_base(g);
-        // the next window's first batch (the last window re-reads its own:
-        // an unconditional load keeps the wait counts static across windows)
-        const T *xn = g + pwaves < nwindows ? window_base(g + pwaves) : xw;
+        // the next window's first batch; after its last window every
+        // wavefront reads the same first window, which HBM then serves about
+        // once (an unconditional load keeps the wait counts static across
+        // windows)
+        const T *xn = g + pwaves < nwindows ? window_base(g + pwaves) : window_base(0);
         T sb[NB];
 #pragma unroll
         for (int b = 0; b < NB; b++) {
