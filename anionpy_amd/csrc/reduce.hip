@@ -168,15 +168,6 @@ template <typename T> __device__ inline T from_bits(uint64_t u) {
     if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, u);
     else return __builtin_bit_cast(T, (uint32_t)u);
 }
-template <typename T> __device__ inline T lane_value(T v, int i) {
-    if constexpr (sizeof(T) == 8) {
-        const uint64_t u = __builtin_bit_cast(uint64_t, v);
-        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, i), hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), i);
-        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
-    } else {
-        return __builtin_bit_cast(T, (uint32_t)__builtin_amdgcn_readlane(__builtin_bit_cast(uint32_t, v), i));
-    }
-}
 // pairwise combine across lanes at distance m: the lower lane's value is
 // the left operand, so both lanes compute left + right
 template <typename T> __device__ inline T xor_combine(T v, int m, int lane) {
@@ -575,9 +566,6 @@ template <typename T> __global__ void fill_sentinel(T *p, int64_t n) {
         p[i] = from_bits<T>(Sentinel<T>::bits);
 }
 
-__global__ void zero_u32(uint32_t *p, int64_t n) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
-}
 
 // Ordered fold per output: acc = 0; acc = acc + partial[i] (ufunc.rs:7137
 // zero seed; the outer-axis sequential fold of pairwise_group 6504-6531).
@@ -1423,7 +1411,7 @@ __global__ __launch_bounds__(256) void int_reduce(const RedArgs A, int64_t seg, 
         red[threadIdx.x] = acc;
         __syncthreads();
         for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) red[threadIdx.x] = assoc<AT, OP>(red[threadIdx.x], red[threadIdx.x + s]);
+            if ((int)threadIdx.x < s) red[threadIdx.x] = assoc<AT, OP>(red[threadIdx.x], red[threadIdx.x + s]);
             __syncthreads();
         }
         if (threadIdx.x == 0) {
@@ -2063,7 +2051,7 @@ template <typename T, bool MAX> __global__ __launch_bounds__(256) void arg_rows(
         red[threadIdx.x] = b;
         __syncthreads();
         for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) {
+            if ((int)threadIdx.x < s) {
                 ArgBest<T, MAX> a = red[threadIdx.x];
                 arg_merge(a, red[threadIdx.x + s]);
                 red[threadIdx.x] = a;
