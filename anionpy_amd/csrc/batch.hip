@@ -31,12 +31,14 @@ enum : int32_t {
     BK_FADD_SMALL = 5,   // float add.reduce, windows of <= 128: one thread per output
     BK_IADD_SMALL = 6,   // integer / bool add.reduce: one thread per output
     BK_FADD_NARROW_WAVE = 7,  // float add.reduce, W empty, long chains: one wavefront per output
+    BK_IADD_WAVE = 8,    // integer / bool add.reduce, groups of 65 .. kIntWaveMax: one wavefront per output
 };
 
 constexpr int kBatchThreads = 256;
 constexpr int64_t kEwTile = kBatchThreads * 64;  // elements per elementwise tile
 constexpr int64_t kIntSeg = 8192;                // elements per integer-reduce tile
 constexpr int64_t kNarrowOutputs = kBatchThreads;  // outputs per narrow tile
+constexpr int64_t kIntWaveMax = 16384;           // longest group a wavefront reduces alone
 
 struct BatchCall {
     int32_t kind;
@@ -273,13 +275,14 @@ template <typename T> __device__ void fadd_narrow(const BatchCall &C, int64_t t)
     ((T *)A.out)[ob] = narrow_sum<T, 8>(A, ib);  // few registers: the persistent kernel runs every tile kind
 }
 
-template <typename T> __device__ void fadd_narrow_wave(const BatchCall &C, int64_t t) {
+// lds: PwScratch::racc, 256 elements per wavefront
+template <typename T> __device__ void fadd_narrow_wave(const BatchCall &C, int64_t t, double *lds) {
     const RedArgs &A = C.red;
     const int64_t o = t * (kBatchThreads / 64) + (threadIdx.x >> 6);
     if (o >= A.nout) return;
     int64_t ib, ob;
     out_offsets(A, (uint32_t)o, &ib, &ob);
-    const T acc = narrow_wave_sum<T>(A, ib);
+    const T acc = narrow_wave_sum<T, 2>(A, ib, (T *)lds + (threadIdx.x >> 6) * 256);  // few registers: every tile kind shares the kernel
     if ((threadIdx.x & 63) == 0) ((T *)A.out)[ob] = acc;
 }
 
@@ -312,9 +315,46 @@ template <int LT> __device__ void iadd_small(const BatchCall &C, int64_t t) {
     if (o >= A.nout) return;
     int64_t ib, ob;
     out_offsets(A, (uint32_t)o, &ib, &ob);
+    // order-free: eight loads in flight, then their adds
+    constexpr int U = 8;
     uint64_t acc = 0;
-    for (int64_t q = 0; q < A.group; q++) acc += (uint64_t)ld<LT>(A.in_dt, A.in, ib + group_off(A, q));
+    int64_t q = 0;
+    for (; q + U <= A.group; q += U) {
+        uint64_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = (uint64_t)ld<LT>(A.in_dt, A.in, ib + group_off(A, q + u));
+#pragma unroll
+        for (int u = 0; u < U; u++) acc += v[u];
+    }
+    for (; q < A.group; q++) acc += (uint64_t)ld<LT>(A.in_dt, A.in, ib + group_off(A, q));
     ((ST *)A.out)[ob] = int_result<LT>(acc + (A.has_init ? A.init_bits : 0));
+}
+
+// one output per wavefront (kBatchThreads / 64 per tile): lanes take
+// group elements lane, lane + 64, ... (four loads in flight), then a
+// wavefront sum; the add wraps modulo 2^64, so any order is exact
+template <int LT> __device__ void iadd_wave(const BatchCall &C, int64_t t) {
+    using ST = typename dt_type<LT>::T;
+    const RedArgs &A = C.red;
+    const int64_t o = t * (kBatchThreads / 64) + (threadIdx.x >> 6);
+    if (o >= A.nout) return;
+    const int lane = threadIdx.x & 63;
+    int64_t ib, ob;
+    out_offsets(A, (uint32_t)o, &ib, &ob);
+    constexpr int U = 4;
+    uint64_t acc = 0;
+    int64_t q = lane;
+    for (; q + (U - 1) * 64 < A.group; q += U * 64) {
+        uint64_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = (uint64_t)ld<LT>(A.in_dt, A.in, ib + group_off(A, q + u * 64));
+#pragma unroll
+        for (int u = 0; u < U; u++) acc += v[u];
+    }
+    for (; q < A.group; q += 64) acc += (uint64_t)ld<LT>(A.in_dt, A.in, ib + group_off(A, q));
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) acc += __shfl_xor(acc, m, 64);
+    if (lane == 0) ((ST *)A.out)[ob] = int_result<LT>(acc + (A.has_init ? A.init_bits : 0));
 }
 
 template <int LT> __device__ void iadd_tile(const BatchCall &C, char *partials, int64_t g, uint64_t *red) {
@@ -423,14 +463,17 @@ __global__ __launch_bounds__(kBatchThreads) void batch_persistent(
                 else fadd_narrow<float>(C, lt);
                 break;
             case BK_FADD_NARROW_WAVE:
-                if (C.red.out_dt == IONP_FLOAT64) fadd_narrow_wave<double>(C, lt);
-                else fadd_narrow_wave<float>(C, lt);
+                if (C.red.out_dt == IONP_FLOAT64) fadd_narrow_wave<double>(C, lt, S.racc);
+                else fadd_narrow_wave<float>(C, lt, S.racc);
                 break;
             case BK_IADD:
                 int_dispatch(C.red.out_dt, [&](auto lt_) { iadd_tile<decltype(lt_)::value>(C, partials, lt, ired); });
                 break;
             case BK_IADD_SMALL:
                 int_dispatch(C.red.out_dt, [&](auto lt_) { iadd_small<decltype(lt_)::value>(C, lt); });
+                break;
+            case BK_IADD_WAVE:
+                int_dispatch(C.red.out_dt, [&](auto lt_) { iadd_wave<decltype(lt_)::value>(C, lt); });
                 break;
             default: fill_zero_tile(C, lt); break;
         }
@@ -566,6 +609,10 @@ ionp_status plan_call(const ionp_call &c, BatchCall *B, int64_t *bytes) {
     } else if (int_cols_ok(A)) {
         B->kind = BK_IADD_SMALL;
         B->tiles = (p.nout + kNarrowOutputs - 1) / kNarrowOutputs;
+    } else if (p.group <= kIntWaveMax) {
+        // a workgroup per output would idle most lanes on short groups
+        B->kind = BK_IADD_WAVE;
+        B->tiles = (p.nout + kBatchThreads / 64 - 1) / (kBatchThreads / 64);
     } else {
         B->kind = BK_IADD;
         B->nseg = (p.group + kIntSeg - 1) / kIntSeg;
@@ -610,6 +657,7 @@ ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, i
         serial[i] = B.kind == BK_FADD_NARROW || B.kind == BK_FADD_NARROW_WAVE ? B.red.rsize
                     : B.kind == BK_FADD_SMALL ? B.red.rsize * B.red.wsize
                     : B.kind == BK_IADD_SMALL ? B.red.group
+                    : B.kind == BK_IADD_WAVE  ? B.red.group / 64
                                               : 0;
         order[i] = i;
     }

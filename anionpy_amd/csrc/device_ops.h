@@ -605,17 +605,28 @@ template <typename T> __device__ inline T readlane_t(T v, int lane) {
     }
 }
 
-// Narrow float add of one output by a whole wavefront: the 64 lanes load
-// blocks of 64 * B consecutive reduced elements (the next block in flight
-// while the current one is added), and the add chain, the same in every
-// lane, reads the block lane by lane in element order.  Plain IEEE adds with
-// the NaN redo of narrow_chain.  Result in every lane.
-template <typename T, typename IT> __device__ inline T narrow_wave_chain(const RedArgs &A, int64_t ib) {
+// One wavefront runs one output's chain acc = seed; acc = acc + x[r] (r in
+// iterator order).  Blocks of 256 elements are loaded one per lane and slot
+// (the next block's loads in flight meanwhile), staged in the wavefront's
+// own LDS region `lb` (256 elements) in element order, and added from
+// broadcast 16-byte LDS reads: the chain then runs at the dependent-add
+// latency (readlane operands, one per element, ran about 3 times slower on
+// MI355X).  Plain IEEE adds agree with fadd until a NaN arises, and a NaN
+// stays, so an output that ends NaN is redone with fadd's x86 NaN rules.
+// R: 16-byte LDS reads per chunk (registers: 2 R vectors)
+template <typename T, typename IT, int R = 8> __device__ inline T narrow_wave_chain(const RedArgs &A, int64_t ib, T *lb) {
     constexpr int LT = FDT<T>::v;
     constexpr int B = 4, BLK = 64 * B;
+    constexpr int CH = 16 / sizeof(T);
+    using V = T __attribute__((ext_vector_type(CH)));
     const int lane = threadIdx.x & 63;
     const int64_t n = A.rsize;
     auto off = [&](int64_t r) { return A.nr == 1 ? r * A.r_st[0] : r_offset(A, r); };
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
     T cur[B], nxt[B];
 #pragma unroll
     for (int k = 0; k < B; k++) {
@@ -632,19 +643,35 @@ template <typename T, typename IT> __device__ inline T narrow_wave_chain(const R
                 nxt[k] = r < n ? ld_t<LT, IT>(A.in, ib + off(r)) : T(0);
             }
         }
+#pragma unroll
+        for (int k = 0; k < B; k++) lb[k * 64 + lane] = cur[k];
+        wave_sync();
         if (nb <= n) {
+            // chunks of R reads, the next chunk's reads issued before the
+            // current chunk's adds (kept there by the scheduling barriers),
+            // so LDS latency stays off the chain
+            constexpr int CK = R * CH, NCK = BLK / CK;
+            V ch[2][R];
 #pragma unroll
-            for (int k = 0; k < B; k++)
+            for (int j = 0; j < R; j++) ch[0][j] = *(const V *)(lb + j * CH);
 #pragma unroll
-                for (int j = 0; j < 64; j++) acc = acc + readlane_t(cur[k], j);
+            for (int c = 0; c < NCK; c++) {
+                if (c + 1 < NCK) {
+#pragma unroll
+                    for (int j = 0; j < R; j++) ch[(c + 1) & 1][j] = *(const V *)(lb + (c + 1) * CK + j * CH);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int j = 0; j < R; j++)
+#pragma unroll
+                    for (int e = 0; e < CH; e++) acc = acc + ch[c & 1][j][e];
+                __builtin_amdgcn_sched_barrier(0);
+            }
         } else {
             const int cnt = (int)(n - rb);
-            for (int e = 0; e < cnt; e++) {
-                const int k = e >> 6;
-                const T v = k == 0 ? cur[0] : k == 1 ? cur[1] : k == 2 ? cur[2] : cur[3];
-                acc = acc + readlane_t(v, e & 63);
-            }
+            for (int e = 0; e < cnt; e++) acc = acc + lb[e];
         }
+        wave_sync();  // every lane has read the block before the next one is stored
 #pragma unroll
         for (int k = 0; k < B; k++) cur[k] = nxt[k];
     }
@@ -654,8 +681,9 @@ template <typename T, typename IT> __device__ inline T narrow_wave_chain(const R
     }
     return acc;
 }
-template <typename T> __device__ inline T narrow_wave_sum(const RedArgs &A, int64_t ib) {
-    return in_type_dispatch(A.in_dt, [&](auto it) { return narrow_wave_chain<T, decltype(it)>(A, ib); });
+// lb: the wavefront's LDS region of 256 elements, 16-byte aligned
+template <typename T, int R = 8> __device__ inline T narrow_wave_sum(const RedArgs &A, int64_t ib, T *lb) {
+    return in_type_dispatch(A.in_dt, [&](auto it) { return narrow_wave_chain<T, decltype(it), R>(A, ib, lb); });
 }
 
 // ---------------------------------------------------------------------------

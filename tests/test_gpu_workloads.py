@@ -121,3 +121,21 @@ def test_folder_sum_special_values_and_repeat(ap):
                 exp = np.atleast_1d(np.add.reduce(hd)).view(np.uint64 if dt == np.float64 else np.uint32)
                 got = np.atleast_1d(ap.add.reduce(ap.asarray(hd)).get()).view(exp.dtype)
                 assert got[0] == exp[0], (dt, hex(int(got[0])), hex(int(exp[0])))
+
+
+def test_narrow_wave_chains(ap):
+    """Few outputs with long sequential chains (a wavefront per output, the
+    chain read back from LDS): block tails, a NaN payload, infinities, both
+    float widths, against NumPy's bits."""
+    rng = np.random.default_rng(14)
+    nan1 = np.array([0x7FF8000000000321], dtype=np.uint64).view(np.float64)[0]
+    for shape in ((300, 5), (256, 3), (257, 2), (70001, 3), (4099, 7)):
+        for dt in (np.float64, np.float32):
+            h = rng.standard_normal(shape).astype(dt)
+            if shape[0] > 1000:
+                h[1000, 0] = nan1
+                h[shape[0] // 2, 1] = np.inf
+                h[shape[0] - 1, 1] = -np.inf
+            exp = np.add.reduce(h, axis=0)
+            got = ap.add.reduce(ap.asarray(h), axis=0).get()
+            assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), (shape, dt)
