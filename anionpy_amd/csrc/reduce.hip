@@ -748,6 +748,18 @@ template <typename T, typename IT, bool MASK> __global__ __launch_bounds__(256) 
 // lane, reads them lane by lane in iterator order
 template <typename T, typename IT, bool MASK> __global__ __launch_bounds__(256) void prod_wave(const RedArgs A) {
     constexpr int B = 4, BLK = 64 * B;
+    constexpr int CH = 16 / sizeof(T), R = 8, CK = R * CH, NCK = BLK / CK;
+    using V = T __attribute__((ext_vector_type(CH)));
+    // each block of factors is staged in the wavefront's LDS region and the
+    // chain reads it back with broadcast 16-byte reads, a chunk ahead (the
+    // same scheme as narrow_wave_chain)
+    __shared__ __align__(16) T lbs[4][BLK];
+    T *lb = lbs[threadIdx.x >> 6];
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
     const int lane = threadIdx.x & 63;
     const int64_t w0 = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
     for (int64_t o = w0; o < A.nout; o += (int64_t)gridDim.x * (blockDim.x >> 6)) {
@@ -772,19 +784,31 @@ template <typename T, typename IT, bool MASK> __global__ __launch_bounds__(256) 
                     nxt[k] = q < n ? prod_factor<T, IT, MASK>(A, ib, mb, q) : T(1);
                 }
             }
+#pragma unroll
+            for (int k = 0; k < B; k++) lb[k * 64 + lane] = cur[k];
+            wave_sync();
             if (nb <= n) {
+                V ch[2][R];
 #pragma unroll
-                for (int k = 0; k < B; k++)
+                for (int j = 0; j < R; j++) ch[0][j] = *(const V *)(lb + j * CH);
 #pragma unroll
-                    for (int j = 0; j < 64; j++) acc = acc * readlane_t(cur[k], j);
+                for (int c = 0; c < NCK; c++) {
+                    if (c + 1 < NCK) {
+#pragma unroll
+                        for (int j = 0; j < R; j++) ch[(c + 1) & 1][j] = *(const V *)(lb + (c + 1) * CK + j * CH);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int j = 0; j < R; j++)
+#pragma unroll
+                        for (int e = 0; e < CH; e++) acc = acc * ch[c & 1][j][e];
+                    __builtin_amdgcn_sched_barrier(0);
+                }
             } else {
                 const int cnt = (int)(n - qb);
-                for (int e = 0; e < cnt; e++) {
-                    const int k = e >> 6;
-                    const T v = k == 0 ? cur[0] : k == 1 ? cur[1] : k == 2 ? cur[2] : cur[3];
-                    acc = acc * readlane_t(v, e & 63);
-                }
+                for (int e = 0; e < cnt; e++) acc = acc * lb[e];
             }
+            wave_sync();  // every lane has read the block before the next one is stored
 #pragma unroll
             for (int k = 0; k < B; k++) cur[k] = nxt[k];
         }

@@ -139,3 +139,22 @@ def test_narrow_wave_chains(ap):
             exp = np.add.reduce(h, axis=0)
             got = ap.add.reduce(ap.asarray(h), axis=0).get()
             assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), (shape, dt)
+
+
+def test_product_chains(ap):
+    """multiply.reduce chains long enough for a wavefront per output (the
+    factors read back from LDS): 1-D and axis-0 layouts, block tails, a NaN,
+    a where= mask, against NumPy's bits."""
+    rng = np.random.default_rng(15)
+    for shape, axis in (((70001,), 0), ((5000, 3), 0), ((3, 4099), 1), ((257, 2), 0)):
+        for dt in (np.float64, np.float32):
+            h = (1 + 1e-3 * rng.standard_normal(shape)).astype(dt)
+            if shape[0] == 5000:
+                h[2500, 1] = np.nan
+            exp = np.multiply.reduce(h, axis=axis)
+            got = ap.multiply.reduce(ap.asarray(h), axis=axis).get()
+            assert np.array_equal(np.atleast_1d(got).view(np.uint8), np.atleast_1d(exp).view(np.uint8)), (shape, dt)
+            m = rng.random(shape) < 0.7
+            exp = np.multiply.reduce(h, axis=axis, where=m)
+            got = ap.multiply.reduce(ap.asarray(h), axis=axis, where=ap.asarray(m)).get()
+            assert np.array_equal(np.atleast_1d(got).view(np.uint8), np.atleast_1d(exp).view(np.uint8)), (shape, dt, "where")
