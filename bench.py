@@ -13,7 +13,8 @@ HBM.  One JSON line is printed; `roofline` covers the dominant kernel (the
 float64 multiply), timed with HIP events on the stream it runs on, and
 `cpu_baseline` times the scalar C oracle on the same four operations, on one
 core and (OpenMP) on every host core.  `extra` carries the reduction workloads (float64
-add.reduce of 2^28 elements, float32 (4096,65536) sums, maximum/argmax).
+add.reduce of 2^28 elements, float32 (4096,65536) sums, maximum/argmax, also
+with NaN at rate 1e-6).
 """
 import argparse
 import json
@@ -211,6 +212,16 @@ def reduction_extras(ap, torch):
     t = timed(torch, lambda: ap.argmax(x), 10)
     out["argmax_f64_2^28"] = {"us": round(t * 1e6, 1), "GB/s": round(nb / t / 1e9, 1),
                               "frac": round(nb / t / PEAK_HBM, 4)}
+    del x
+    # the same array with NaN at rate 1e-6 (rng(5).random(n) < 1e-6): the
+    # maximum / argmax workload as specified, NaN propagation included
+    h = np.random.default_rng(4).uniform(-1, 1, 2 ** 28)
+    h[np.random.default_rng(5).random(2 ** 28) < 1e-6] = np.nan
+    x = ap.asarray(h)
+    del h
+    for key, fn in (("max_f64_2^28_nan1e-6", lambda: ap.maximum.reduce(x)), ("argmax_f64_2^28_nan1e-6", lambda: ap.argmax(x))):
+        t = timed(torch, fn, 10)
+        out[key] = {"us": round(t * 1e6, 1), "GB/s": round(nb / t / 1e9, 1), "frac": round(nb / t / PEAK_HBM, 4)}
     del x
     F = ap.asarray(np.random.default_rng(6).uniform(-1, 1, (4096, 65536)).astype(np.float32))
     for ax, nbo in ((0, 65536 * 4), (1, 4096 * 4)):
