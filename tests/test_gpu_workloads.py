@@ -158,3 +158,23 @@ def test_product_chains(ap):
             exp = np.multiply.reduce(h, axis=axis, where=m)
             got = ap.multiply.reduce(ap.asarray(h), axis=axis, where=ap.asarray(m)).get()
             assert np.array_equal(np.atleast_1d(got).view(np.uint8), np.atleast_1d(exp).view(np.uint8)), (shape, dt, "where")
+
+
+def test_streaming_sum_misaligned_views(ap):
+    """The streaming sum reads 16-byte vectors: views whose rows start off a
+    16-byte boundary (an element offset, odd row lengths) must still give
+    NumPy's bits, with and without folder workgroups."""
+    rng = np.random.default_rng(16)
+    for dt in (np.float64, np.float32):
+        base = rng.standard_normal(2 ** 20 + 7).astype(dt)
+        d = ap.asarray(base)
+        for k in (1, 2, 3):
+            exp = np.add.reduce(base[k:])
+            got = ap.add.reduce(d[k:]).get()
+            assert np.atleast_1d(got).view(np.uint8).tobytes() == np.atleast_1d(exp).view(np.uint8).tobytes(), (dt, k)
+        R = rng.standard_normal((70, 3 * 8192 + 5)).astype(dt)
+        Rd = ap.asarray(R)
+        for sl in (np.s_[:, :], np.s_[:, 1:], np.s_[1:, 3:]):
+            exp = np.add.reduce(R[sl], axis=1)
+            got = ap.add.reduce(Rd[sl], axis=1).get()
+            assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), (dt, sl)
