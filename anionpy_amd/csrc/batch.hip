@@ -282,7 +282,7 @@ template <typename T> __device__ void fadd_narrow_wave(const BatchCall &C, int64
     if (o >= A.nout) return;
     int64_t ib, ob;
     out_offsets(A, (uint32_t)o, &ib, &ob);
-    const T acc = narrow_wave_sum<T, 2>(A, ib, (T *)lds + (threadIdx.x >> 6) * 256);  // few registers: every tile kind shares the kernel
+    const T acc = narrow_wave_sum<T, 4>(A, ib, (T *)lds + (threadIdx.x >> 6) * 256);  // few registers: every tile kind shares the kernel
     if ((threadIdx.x & 63) == 0) ((T *)A.out)[ob] = acc;
 }
 
