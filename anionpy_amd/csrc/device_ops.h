@@ -606,17 +606,19 @@ template <typename T> __device__ inline T readlane_t(T v, int lane) {
 }
 
 // One wavefront runs one output's chain acc = seed; acc = acc + x[r] (r in
-// iterator order).  Blocks of 256 elements are loaded one per lane and slot
+// iterator order).  Blocks of 64 B elements are loaded one per lane and slot
 // (the next block's loads in flight meanwhile), staged in the wavefront's
-// own LDS region `lb` (256 elements) in element order, and added from
+// own LDS region `lb` in element order, and added from
 // broadcast 16-byte LDS reads: the chain then runs at the dependent-add
 // latency (readlane operands, one per element, ran about 3 times slower on
 // MI355X).  Plain IEEE adds agree with fadd until a NaN arises, and a NaN
 // stays, so an output that ends NaN is redone with fadd's x86 NaN rules.
-// R: 16-byte LDS reads per chunk (registers: 2 R vectors)
-template <typename T, typename IT, int R = 8> __device__ inline T narrow_wave_chain(const RedArgs &A, int64_t ib, T *lb) {
+// R: 16-byte LDS reads per chunk (registers: 2 R vectors); B: loads per lane
+// per block (a block is 64 B elements, `lb` holds one block)
+template <typename T, typename IT, int R = 8, int B = 4>
+__device__ inline T narrow_wave_chain(const RedArgs &A, int64_t ib, T *lb) {
     constexpr int LT = FDT<T>::v;
-    constexpr int B = 4, BLK = 64 * B;
+    constexpr int BLK = 64 * B;
     constexpr int CH = 16 / sizeof(T);
     using V = T __attribute__((ext_vector_type(CH)));
     const int lane = threadIdx.x & 63;
@@ -681,9 +683,9 @@ template <typename T, typename IT, int R = 8> __device__ inline T narrow_wave_ch
     }
     return acc;
 }
-// lb: the wavefront's LDS region of 256 elements, 16-byte aligned
-template <typename T, int R = 8> __device__ inline T narrow_wave_sum(const RedArgs &A, int64_t ib, T *lb) {
-    return in_type_dispatch(A.in_dt, [&](auto it) { return narrow_wave_chain<T, decltype(it), R>(A, ib, lb); });
+// lb: the wavefront's LDS region of 64 B elements, 16-byte aligned
+template <typename T, int R = 8, int B = 4> __device__ inline T narrow_wave_sum(const RedArgs &A, int64_t ib, T *lb) {
+    return in_type_dispatch(A.in_dt, [&](auto it) { return narrow_wave_chain<T, decltype(it), R, B>(A, ib, lb); });
 }
 
 // ---------------------------------------------------------------------------

@@ -659,12 +659,15 @@ template <typename T, typename IT> __global__ __launch_bounds__(256) void pw_nar
 
 // the same with a wavefront per output (narrow_wave_ok)
 template <typename T, typename IT> __global__ __launch_bounds__(256) void pw_narrow_wave(const RedArgs A) {
-    __shared__ __align__(16) T lb[4][256];  // a block of the chain per wavefront
+    // a block of 512 elements of the chain per wavefront: about 3 us of adds,
+    // enough to cover the next block's HBM loads
+    constexpr int B = 8;
+    __shared__ __align__(16) T lb[4][64 * B];
     const int64_t w0 = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
     for (int64_t o = w0; o < A.nout; o += (int64_t)gridDim.x * (blockDim.x >> 6)) {
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
-        const T acc = narrow_wave_chain<T, IT>(A, ib, lb[threadIdx.x >> 6]);
+        const T acc = narrow_wave_chain<T, IT, 8, B>(A, ib, lb[threadIdx.x >> 6]);
         if ((threadIdx.x & 63) == 0) ((T *)A.out)[ob] = acc;
     }
 }
