@@ -15,8 +15,10 @@ there is no CPU fallback: without the built library, import fails.
 import builtins
 import contextlib
 import ctypes
+import math
 import numbers
 import operator
+import re
 import sys
 import threading
 import warnings
@@ -38,8 +40,10 @@ _NP_DTYPES = [np.bool_, np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uin
 _CODE = {np.dtype(d): i for i, d in enumerate(_NP_DTYPES)}
 
 
-class AxisError(ValueError, IndexError):
-    """numpy.exceptions.AxisError counterpart (error.rs AxisError)."""
+class AxisError(np.exceptions.AxisError):
+    """The reference's AxisError (ionp-py/src/errors.rs, raised by
+    lib.rs:76-80 axis_error): a subclass of numpy.exceptions.AxisError (so
+    also of ValueError and IndexError), built as AxisError(axis, ndim)."""
 
 
 class UFuncTypeError(TypeError):
@@ -59,7 +63,8 @@ def _raise(e):
     if code == 6:
         raise UFuncTypeError(msg) from None
     if code == 7:
-        raise AxisError(msg) from None
+        m = re.fullmatch(r"axis (-?\d+) is out of bounds for array of dimension (\d+)", msg)
+        raise (AxisError(int(m.group(1)), int(m.group(2))) if m else AxisError(msg)) from None
     if code == 21:
         raise NotImplementedError(msg) from None
     raise RuntimeError(msg) from None
@@ -237,8 +242,8 @@ class ndarray:
 
     def astype(self, dtype, order="K"):
         dtype = _dtype(dtype)
+        _fpe_arm(_fpe_wanted())
         track = (self.dtype.kind == "f" or dtype == np.float16) and _fpe_wanted()  # int -> float16 can overflow
-        _fpe_arm(track)
         r = self._astype(dtype, order)
         if track:
             _fpe_check("cast")
@@ -264,7 +269,7 @@ class ndarray:
         isz = self.itemsize
         nbytes = (hi - lo + 1) * isz
         start = (self.offset + lo) * isz
-        torch.cuda.current_stream().synchronize()
+        _call(_lib.load().ionp_check, _lib.stream_handle())  # waits for the stream
         host = self._buf[start:start + nbytes].cpu().numpy()
         base = host.view(self.dtype)
         return np.lib.stride_tricks.as_strided(base[-lo:] if lo else base, shape=self.shape,
@@ -461,40 +466,154 @@ def result_type(a, b):
 
 
 # ---- operands --------------------------------------------------------------------------
-def _is_weak(x):
-    return isinstance(x, (bool, int, float)) and not isinstance(x, np.generic)
+# NEP 50 weak Python scalars, as ionp-py marshals them (lib.rs:8238-8272
+# extract_binary_pair -> scalar_against 8104, weak_target_dtype 8022): a bare
+# Python bool / int / float takes the other operand's dtype within its kind;
+# numpy scalars and arrays carry a dtype and are strong.
+_I64, _U64, _F64, _BOOL = (np.dtype(t) for t in (np.int64, np.uint64, np.float64, np.bool_))
 
 
-def _weak_dtype(x, other):
-    """NEP 50: a Python scalar takes the other operand's dtype within its kind."""
-    od = other.dtype
+def _scalar_kind(x):
+    """'b' / 'i' / 'f' for a weak Python bool / int / float, else None."""
+    if isinstance(x, (np.generic, np.ndarray, ndarray)):
+        return None
     if isinstance(x, bool):
-        return od
+        return "b"
     if isinstance(x, int):
-        if od.kind in "iu":
-            info = np.iinfo(od)
-            if not info.min <= x <= info.max:
-                raise OverflowError(f"Python integer {x} out of bounds for {od}")
-            return od
-        if od.kind == "b":
-            return np.dtype(np.int64)
-        return od
-    # float
-    return od if od.kind == "f" else np.dtype(np.float64)
+        return "i"
+    if isinstance(x, float):
+        return "f"
+    return None
 
 
-def _operands(a, b):
-    if _is_weak(a) and _is_weak(b):
-        a, b = asarray(a), asarray(b)
-    elif _is_weak(a):
-        b = asarray(b)
-        a = asarray(np.array(a, dtype=_weak_dtype(a, b)))
-    elif _is_weak(b):
-        a = asarray(a)
-        b = asarray(np.array(b, dtype=_weak_dtype(b, a)))
+def _is_weak(x):
+    return _scalar_kind(x) is not None
+
+
+def _weak_target(kind, od, code):
+    """The dtype a weak scalar takes against an operand of dtype `od`: a
+    bool takes od; an int takes od unless od is bool (then int64); a float
+    takes od if od is a float (else float64).  true_divide's loops are float,
+    so an integer or bool target becomes float64 (lib.rs:8125)."""
+    if kind == "b":
+        t = od
+    elif kind == "i":
+        t = _I64 if od.kind == "b" else od
     else:
-        a, b = asarray(a), asarray(b)
-    return a, b
+        t = od if od.kind == "f" else _F64
+    if code == _lib.TRUE_DIVIDE and t.kind in "biu":
+        t = _F64
+    return t
+
+
+def _int_fits(v, t):
+    info = np.iinfo(t)
+    return info.min <= v <= info.max
+
+
+def _weak_int_overflow(v, t):
+    """numpy's OverflowError for a Python int outside dtype t: beyond the C
+    long range (unless t is uint64 and v fits it) the conversion itself
+    fails, else the value is out of t's bounds."""
+    if not -2 ** 63 <= v < 2 ** 63 and not (t == _U64 and 0 <= v < 2 ** 64):
+        return OverflowError("Python int too large to convert to C long")
+    return OverflowError(f"Python integer {v} out of bounds for {t}")
+
+
+def _weak_float(v, t):
+    """A Python float as a 0-d host array of float dtype t; the conversion's
+    IEEE flags (overflow, underflow) are reported under 'cast' by the
+    np.errstate modes of this package."""
+    with np.errstate(all="ignore"):
+        r = np.array(v, dtype=t)
+    if t != _F64 and _fpe_wanted():
+        rv = float(r)
+        fl = 0
+        if math.isfinite(v) and math.isinf(rv):
+            fl |= 2
+        if v != 0 and math.isfinite(v) and rv != v and abs(rv) < np.finfo(t).tiny:
+            fl |= 4
+        if fl:
+            _fpe_report(fl, "cast")
+    return r
+
+
+def _weak_array(x, kind, t, relaxed=False):
+    """The weak scalar x materialised as a 0-d host array of dtype t; for an
+    integer t and an int outside it: comparisons (relaxed) widen to int64 /
+    uint64 and compare by value (lib.rs:8146), None beyond 64 bits; other
+    ops raise numpy's OverflowError."""
+    if kind == "f" or (t.kind == "f" and kind != "b"):
+        return _weak_float(float(x), t) if t.kind == "f" else np.array(x, dtype=t)
+    if kind == "b" or t.kind == "b":
+        return np.array(bool(x) if t.kind == "b" else int(x), dtype=t)
+    v = int(x)
+    if _int_fits(v, t):
+        return np.array(v, dtype=t)
+    if relaxed:
+        t2 = _I64 if v < 0 else _U64
+        return np.array(v, dtype=t2) if _int_fits(v, t2) else None
+    raise _weak_int_overflow(v, t)
+
+
+def _default_weak(x, kind):
+    """Both operands weak: each takes its kind's default dtype."""
+    if kind == "b":
+        return np.array(x, dtype=_BOOL)
+    if kind == "f":
+        return np.array(x, dtype=_F64)
+    v = int(x)
+    if _int_fits(v, _I64):
+        return np.array(v, dtype=_I64)
+    if _int_fits(v, _U64):
+        return np.array(v, dtype=_U64)
+    raise OverflowError("Python int too large to convert to C long")
+
+
+_CMP_CONST = {  # op(array value, huge Python int v): known from the sign of v
+    "less": lambda v: v > 0, "less_equal": lambda v: v > 0, "greater": lambda v: v < 0,
+    "greater_equal": lambda v: v < 0, "equal": lambda v: False, "not_equal": lambda v: True}
+_CMP_SWAP = {"less": "greater", "less_equal": "greater_equal", "greater": "less",
+             "greater_equal": "less_equal", "equal": "equal", "not_equal": "not_equal"}
+
+
+def _operands(a, b, code=None, loop=None):
+    """Device operands of a binary call.  loop: the dtype= loop, into which
+    weak scalars are converted directly.  Returns (a, b, const): const is
+    the bool result of a comparison against a Python int beyond 64 bits."""
+    ka, kb = _scalar_kind(a), _scalar_kind(b)
+    relaxed = code is not None and code >= _lib.LESS
+    if ka and kb:
+        if loop is not None:
+            return asarray(_weak_array(a, ka, loop)), asarray(_weak_array(b, kb, loop)), None
+        return asarray(_default_weak(a, ka)), asarray(_default_weak(b, kb)), None
+    if ka or kb:
+        arr = asarray(b if ka else a)
+        x, k = (a, ka) if ka else (b, kb)
+        if loop is not None:
+            w = _weak_array(x, k, loop)
+        else:
+            w = _weak_array(x, k, _weak_target(k, arr.dtype, code), relaxed)
+        if w is None:  # a comparison with a Python int beyond 64 bits
+            name = _CODE_NAMES[code] if ka is None else _CMP_SWAP[_CODE_NAMES[code]]
+            return arr, None, bool(_CMP_CONST[name](int(x)))
+        w = asarray(w)
+        return (w, arr, None) if ka else (arr, w, None)
+    return asarray(a), asarray(b), None
+
+
+def _resolve_stand_in(x, requested):
+    """The dtype dtype= resolution checks for operand x: a weak int casts to
+    any numeric loop but bool (it is converted straight into the loop's
+    dtype), a weak float counts as float64, a weak bool as bool."""
+    k = _scalar_kind(x)
+    if k is None:
+        return asarray(x).dtype
+    if k == "b":
+        return _BOOL
+    if k == "f":
+        return _F64
+    return requested if requested.kind != "b" else _I64
 
 
 def _finish(out):
@@ -552,31 +671,52 @@ class Ufunc:
     def __repr__(self):
         return f"<ufunc '{self.__name__}'>"
 
-    def __call__(self, a, b, out=None, where=True, dtype=None, order="K"):
-        track = self._code <= _lib.TRUE_DIVIDE and _fpe_wanted()
-        _fpe_arm(track)
-        r = self._call(a, b, out, where, dtype, order)
-        if track:
-            _fpe_check(self.__name__)
-        return r
-
-    def _call(self, a, b, out, where, dtype, order):
+    def __call__(self, a, b, out=None, where=True, dtype=None, order="K", casting="same_kind"):
         """ufunc.__call__ (lib.rs:9204) -> binary_op (ionp-core/src/ufunc.rs:3628).
 
         out=: any strides; an `out` that overlaps an input other than as the
         same view gets the result through a temporary (write_out,
         ufunc.rs:10112).  where=: elements where the mask is False keep
-        out's previous values.  dtype=: the loop dtype (inputs cast first).
-        order=: K (NumPy's default), C, F or A for a freshly allocated out."""
+        out's previous values.  dtype=: the loop whose output is dtype
+        (ionp_resolve_dtype, lib.rs:9568-9804); the inputs are cast into it
+        first (weak Python scalars straight from their value), the casts
+        checked under `casting`.  order=: K (NumPy's default), C, F or A for
+        a freshly allocated out.  IEEE flags of the input casts and of the
+        loop are reported together under the ufunc's name."""
+        wanted = _fpe_wanted()
+        _fpe_arm(wanted)
+        track = wanted and self._code <= _lib.TRUE_DIVIDE
+        r = self._call(a, b, out, where, dtype, order, casting)
+        if track:
+            _fpe_check(self.__name__)
+        return r
+
+    def _call(self, a, b, out, where, dtype, order, casting):
         if isinstance(out, tuple):
             if len(out) != 1:
                 raise ValueError("The 'out' tuple must have exactly one entry per ufunc output")
             out = out[0]
-        a, b = _operands(a, b)
-        if dtype is not None:
-            dt = _dtype(dtype)
-            a, b = a.astype(dt), b.astype(dt)
+        if casting not in _lib.CASTING:
+            raise ValueError("casting must be one of 'no', 'equiv', 'safe', 'same_kind', or 'unsafe'")
         L = _lib.load()
+        if not _is_weak(a):
+            a = asarray(a)
+        if not _is_weak(b):
+            b = asarray(b)
+        loop = None
+        if dtype is not None:
+            req = _dtype(dtype)
+            lp = ctypes.c_int32()
+            _call(L.ionp_resolve_dtype, self._code, _CODE[_resolve_stand_in(a, req)],
+                  _CODE[_resolve_stand_in(b, req)], _CODE[req], _lib.CASTING[casting], ctypes.byref(lp))
+            if self._code < _lib.LESS:  # comparisons run their promotion's loop
+                loop = np.dtype(_NP_DTYPES[lp.value])
+        a, b, const = _operands(a, b, self._code, loop)
+        if loop is not None:  # the inputs are cast into the loop first (a buffered cast in NumPy)
+            a = a if a.dtype == loop else a._astype(loop, "K")
+            b = b if b.dtype == loop else b._astype(loop, "K")
+        if const is not None:
+            return self._constant(a, const, out, where, order)
         da, db = a._desc(), b._desc()
         t = _lib.ArrayDesc()
         _call(L.ionp_plan_binary, self._code, ctypes.byref(da), ctypes.byref(db), ctypes.byref(t))
@@ -593,14 +733,31 @@ class Ufunc:
         _call(L.ionp_binary, self._code, ctypes.byref(da), ctypes.byref(db), ctypes.byref(do),
               _cfg(_lib.CALL_BINARY, self._code, (da, db), do), _lib.stream_handle())
         if masked:
-            mask = asarray(where)
-            if mask.dtype != np.bool_:
-                raise TypeError("where= must be a boolean array")
-            dm, dr, dout = mask._desc(), target._desc(), out._desc()
-            _call(L.ionp_where, ctypes.byref(dm), ctypes.byref(dr), ctypes.byref(dout), ctypes.byref(dout), None,
-                  _lib.stream_handle())
+            self._masked_into(target, out, where)
         elif target is not out:
             _copy_into(target, out)
+        return out
+
+    def _masked_into(self, target, out, where):
+        mask = asarray(where)
+        if mask.dtype != np.bool_:
+            raise TypeError("where= must be a boolean array")
+        dm, dr, dout = mask._desc(), target._desc(), out._desc()
+        _call(_lib.load().ionp_where, ctypes.byref(dm), ctypes.byref(dr), ctypes.byref(dout), ctypes.byref(dout),
+              None, _lib.stream_handle())
+
+    def _constant(self, arr, value, out, where, order):
+        """A comparison whose result is known without a loop (a Python int
+        beyond 64 bits against an integer array): `value` everywhere."""
+        shape = arr.shape
+        res = empty_strided(shape, _layout(shape, _c_strides(shape), order, [arr]), np.bool_)
+        res._buf.fill_(int(value))
+        if out is None:
+            return res
+        if not (where is True):
+            self._masked_into(res, out, where)
+        else:
+            _copy_into(res, out)
         return out
 
     def accumulate(self, a, axis=0, dtype=None, out=None):
@@ -726,7 +883,6 @@ _FPE_CATS = ((1, "divide", "divide by zero"), (2, "over", "overflow"), (4, "unde
              (8, "invalid", "invalid value"))
 _FPE_MODES = ("ignore", "warn", "raise", "call", "print", "log")
 _fpe_local = threading.local()
-_fpe_armed = [None]
 
 
 def _fpe_state():
@@ -742,17 +898,23 @@ def _fpe_wanted():
 
 
 def _fpe_arm(on):
-    if _fpe_armed[0] != on:
+    """The library's flag tracking of this thread follows the modes (it is
+    per thread in the library too); switching waits for the device."""
+    on = bool(on)
+    if getattr(_fpe_local, "armed", None) != on:
         _call(_lib.load().ionp_fpe_track, int(on))
-        _fpe_armed[0] = on
+        _fpe_local.armed = on
 
 
 def _fpe_check(name):
     fl = ctypes.c_uint32()
     _call(_lib.load().ionp_fpe_fetch, _lib.stream_handle(), ctypes.byref(fl))
-    flags = fl.value
-    if not flags:
-        return
+    if fl.value:
+        _fpe_report(fl.value, name)
+
+
+def _fpe_report(flags, name):
+    """Warn / raise / call / print / log for each raised flag, by the modes."""
     st = _fpe_state()
     for bit, key, phrase in _FPE_CATS:
         if not flags & bit:
@@ -790,6 +952,8 @@ def seterr(all=None, divide=None, over=None, under=None, invalid=None):
         if v not in _FPE_MODES:
             raise ValueError(f"invalid error mode {v!r}")
     st.update(new)
+    if getattr(_fpe_local, "armed", None) is not None:
+        _fpe_arm(_fpe_wanted())  # tracking follows the new modes
     return old
 
 
@@ -836,6 +1000,9 @@ class errstate(contextlib.ContextDecorator):
             seterrcall(self._oldcall)
         return False
 
+
+_CODE_NAMES = {_lib.LESS: "less", _lib.LESS_EQUAL: "less_equal", _lib.GREATER: "greater",
+               _lib.GREATER_EQUAL: "greater_equal", _lib.EQUAL: "equal", _lib.NOT_EQUAL: "not_equal"}
 
 add = Ufunc("add", _lib.ADD, True)
 subtract = Ufunc("subtract", _lib.SUBTRACT, False)
@@ -920,16 +1087,10 @@ def argmin(a, axis=None, keepdims=False):
 
 
 def where(condition, x, y):
-    """where (reductions.rs:3316) -> where_select (ionp-core/src/sort.rs:1869)."""
+    """where (reductions.rs:3316) -> where_select (ionp-core/src/sort.rs:1869);
+    a weak Python scalar takes the other operand's dtype within its kind."""
     c = asarray(condition)
-    if _is_weak(x) and not _is_weak(y):
-        y = asarray(y)
-        x = asarray(np.array(x, dtype=_weak_dtype(x, y)))
-    elif _is_weak(y) and not _is_weak(x):
-        x = asarray(x)
-        y = asarray(np.array(y, dtype=_weak_dtype(y, x)))
-    else:
-        x, y = asarray(x), asarray(y)
+    x, y, _ = _operands(x, y)
     L = _lib.load()
     dc, dx, dy = c._desc(), x._desc(), y._desc()
     t = _lib.ArrayDesc()

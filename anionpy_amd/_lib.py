@@ -20,6 +20,8 @@ BOOL, INT8, UINT8, INT16, UINT16, INT32, UINT32, INT64, UINT64, FLOAT32, FLOAT64
 ADD, SUBTRACT, MULTIPLY, TRUE_DIVIDE, MAXIMUM, MINIMUM, LESS, LESS_EQUAL, GREATER, GREATER_EQUAL, EQUAL, NOT_EQUAL = range(12)
 CALL_BINARY, CALL_WHERE, CALL_CAST, CALL_REDUCE = range(4)
 BATCH_PERSISTENT, BATCH_GRAPH, BATCH_PER_CALL = range(3)
+CAST_NO, CAST_EQUIV, CAST_SAFE, CAST_SAME_KIND, CAST_UNSAFE = range(5)
+CASTING = {"no": CAST_NO, "equiv": CAST_EQUIV, "safe": CAST_SAFE, "same_kind": CAST_SAME_KIND, "unsafe": CAST_UNSAFE}
 
 STATUS_NAMES = {0: "OK", 1: "BROADCAST", 2: "RESHAPE", 3: "INDEX", 4: "TYPE", 5: "VALUE",
                 6: "NO_LOOP", 7: "AXIS", 20: "DEVICE", 21: "UNSUPPORTED"}
@@ -67,6 +69,10 @@ SIGNATURES = {
     "ionp_get_bufsize": [],
     "ionp_result_type": [_I, _I, _P(_I)],
     "ionp_binary_types": [_I, _I, _I, _P(_I), _P(_I)],
+    "ionp_can_cast": [_I, _I, _I, _P(_I)],
+    "ionp_resolve_dtype": [_I, _I, _I, _I, _I, _P(_I)],
+    "ionp_check": [_V],
+    "ionp_debug_fold_stall": [_L, ctypes.c_uint32],
     "ionp_plan_binary": [_I, _D, _D, _D],
     "ionp_plan_where": [_D, _D, _D, _D],
     "ionp_plan_cast": [_D, _I, _D],

@@ -37,8 +37,20 @@ def _hipcc():
     return "hipcc"
 
 
-def _compile(src):
+def _stale(src, obj):
+    """True when obj is missing or older than its source or any header."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    deps.append(os.path.join(os.path.dirname(HERE), "include", "ionp_hip.h"))
+    return any(os.path.getmtime(d) > t for d in deps if os.path.exists(d))
+
+
+def _compile(src, incremental=False):
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    if incremental and not _stale(src, obj):
+        return obj, ""
     lang = ["-x", "hip"] if src.endswith(".cpp") else []
     cmd = [_hipcc()] + lang + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -49,11 +61,13 @@ def _compile(src):
     return obj, err
 
 
-def build(verbose=False):
+def build(verbose=False, incremental=False):
+    """Compile every source (incremental: only those older than their
+    object or a header) and link the library."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        results = list(ex.map(_compile, srcs))
+        results = list(ex.map(lambda src: _compile(src, incremental), srcs))
     if verbose:
         for obj, err in results:
             if err:
@@ -69,4 +83,4 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, incremental="--incremental" in sys.argv[1:]))

@@ -5,6 +5,7 @@
 // errors carry NumPy's message text (error.rs:64-90).
 #include <atomic>
 #include <mutex>
+#include <vector>
 
 #include "elementwise.h"
 #include "planner.h"
@@ -59,32 +60,71 @@ ionp_status binary_common(int32_t op, const ionp_array_desc *a, const ionp_array
     return broadcast(2, ops, nd, shape);
 }
 
-// IEEE flag tracking (np.errstate): while on, ionp_binary and ionp_cast OR
-// the flags their elements raise into a device word of the current GPU
-std::atomic<int> g_fpe_on{0};
-std::mutex g_fpe_mu;
-uint32_t *g_fpe_word[64] = {};
+// IEEE flag tracking (np.errstate), per thread (fpstate.rs:103 keeps the
+// reference's FPE state thread-local): while a thread has it on, its
+// ionp_binary and ionp_cast calls OR the flags their elements raise into
+// the thread's own word.  The words of one GPU are one array of kFpeSlots;
+// a thread takes a slot index on first use and returns it when it ends.
+constexpr int kFpeSlots = 1024;
+struct FpePool {
+    std::mutex mu;
+    uint32_t *words[64] = {};  // per device
+    std::vector<int> free_slots;
+    int next = 0;
+};
+FpePool &fpe_pool() {
+    static FpePool *p = new FpePool();  // never destroyed: threads may end after static destructors ran
+    return *p;
+}
+struct FpeThread {
+    int on = 0;
+    int slot = -1;
+    ~FpeThread() {
+        if (slot < 0) return;
+        FpePool &P = fpe_pool();
+        std::lock_guard<std::mutex> lk(P.mu);
+        P.free_slots.push_back(slot);
+    }
+};
+thread_local FpeThread t_fpe;
 
+// this thread's flag word on the current device
 ionp_status fpe_word(uint32_t **w) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(IONP_ERR_DEVICE, "no current HIP device");
-    std::lock_guard<std::mutex> lk(g_fpe_mu);
-    if (!g_fpe_word[dev]) {
-        void *p = nullptr;
-        if (hipMalloc(&p, 256) != hipSuccess) return set_error(IONP_ERR_DEVICE, "out of device memory for the flag word");
-        if (hipMemset(p, 0, 256) != hipSuccess) {
-            (void)hipFree(p);
-            return set_error(IONP_ERR_DEVICE, "clearing the flag word failed");
+    FpePool &P = fpe_pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    if (t_fpe.slot < 0) {
+        if (!P.free_slots.empty()) {
+            t_fpe.slot = P.free_slots.back();
+            P.free_slots.pop_back();
+        } else if (P.next < kFpeSlots) {
+            t_fpe.slot = P.next++;
+        } else {
+            return set_error(IONP_ERR_UNSUPPORTED, "more than 1024 threads track IEEE flags at once");
         }
-        g_fpe_word[dev] = (uint32_t *)p;
     }
-    *w = g_fpe_word[dev];
+    if (!P.words[dev]) {
+        void *p = nullptr;
+        if (hipMalloc(&p, kFpeSlots * sizeof(uint32_t)) != hipSuccess)
+            return set_error(IONP_ERR_DEVICE, "out of device memory for the flag words");
+        if (hipMemset(p, 0, kFpeSlots * sizeof(uint32_t)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+            (void)hipFree(p);
+            return set_error(IONP_ERR_DEVICE, "clearing the flag words failed");
+        }
+        P.words[dev] = (uint32_t *)p;
+    }
+    *w = P.words[dev] + t_fpe.slot;
     return IONP_OK;
 }
 
-ionp_status attach_fpe(EwArgs &A) {
-    if (!g_fpe_on.load(std::memory_order_relaxed)) return IONP_OK;
-    return fpe_word(&A.fpe);
+// only loops that can raise a flag get the word (float arithmetic, casts
+// from a float or to float16): the others run the plain kernels
+ionp_status attach_fpe(EwArgs &A, int op, int loop, int in_dt, int out_dt) {
+    if (!t_fpe.on) return IONP_OK;
+    const bool can = op == OP_CAST ? is_real_float(in_dt) || out_dt == IONP_FLOAT16
+                                   : op <= IONP_TRUE_DIVIDE && is_real_float(loop);
+    return can ? fpe_word(&A.fpe) : IONP_OK;
 }
 
 // `in` cast to dtype `dt` into the library's stage buffer, dense in the
@@ -110,6 +150,14 @@ bool is_mixed_su(int op, int a, int b) {
 
 }  // namespace
 
+namespace ionp {
+int fpe_thread_swap(int on) {
+    const int old = t_fpe.on;
+    t_fpe.on = on;
+    return old;
+}
+}  // namespace ionp
+
 extern "C" {
 
 ionp_status ionp_init(void) {
@@ -122,7 +170,49 @@ ionp_status ionp_init(void) {
 const char *ionp_last_error(void) { return last_error(); }
 
 ionp_status ionp_fpe_track(int32_t on) {
-    g_fpe_on.store(on ? 1 : 0);
+    // every launch that may still OR into the word finishes, then the word
+    // is cleared: flags raised under the old state never reach a later fetch
+    uint32_t *w;
+    ionp_status s;
+    if ((s = fpe_word(&w))) return s;
+    if (hipDeviceSynchronize() != hipSuccess || hipMemset(w, 0, sizeof *w) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess)
+        return set_error(IONP_ERR_DEVICE, "clearing the flag word failed");
+    t_fpe.on = on ? 1 : 0;
+    return IONP_OK;
+}
+
+static ionp_status fold_failed() {
+    return set_error(IONP_ERR_DEVICE,
+                     "an add.reduce fold gave up waiting for window partials (a producer workgroup was not "
+                     "scheduled); that result is NaN");
+}
+
+ionp_status ionp_check(void *stream) {
+    if (hipStreamSynchronize(as_stream(stream)) != hipSuccess)
+        return set_error(IONP_ERR_DEVICE, "hipStreamSynchronize failed");
+    return fold_error_take() ? fold_failed() : IONP_OK;
+}
+
+ionp_status ionp_debug_fold_stall(int64_t window, uint32_t polls) {
+    fold_debug_stall(window, polls);
+    return IONP_OK;
+}
+
+ionp_status ionp_can_cast(int32_t from, int32_t to, int32_t casting, int32_t *ok) {
+    if (from < 0 || from >= IONP_NTYPES || to < 0 || to >= IONP_NTYPES) return set_error(IONP_ERR_TYPE, "invalid dtype code");
+    if (casting < IONP_CAST_NO || casting > IONP_CAST_UNSAFE) return set_error(IONP_ERR_VALUE, "invalid casting rule");
+    if (!ok) return set_error(IONP_ERR_VALUE, "ok must not be null");
+    *ok = can_cast(from, to, casting) ? 1 : 0;
+    return IONP_OK;
+}
+
+ionp_status ionp_resolve_dtype(int32_t op, int32_t a, int32_t b, int32_t requested, int32_t casting, int32_t *loop) {
+    if (!loop) return set_error(IONP_ERR_VALUE, "loop must not be null");
+    int l;
+    ionp_status s = resolve_dtype(op, a, b, requested, casting, &l);
+    if (s) return s;
+    *loop = l;
     return IONP_OK;
 }
 
@@ -354,7 +444,7 @@ ionp_status ionp_binary(int32_t op, const ionp_array_desc *a, const ionp_array_d
     const ionp_array_desc *ins[2] = {a, b};
     EwArgs A;
     if ((s = plan_elementwise(op, 2, ins, out, loop, &A))) return s;
-    if ((s = attach_fpe(A))) return s;
+    if ((s = attach_fpe(A, op, loop, a->dtype, out->dtype))) return s;
     A.mixed_su = is_mixed_su(op, a->dtype, b->dtype);
     return run_elementwise(A, cfg, as_stream(stream));
 }
@@ -380,13 +470,14 @@ ionp_status ionp_cast(const ionp_array_desc *in, const ionp_array_desc *out, con
     const ionp_array_desc *ins[1] = {in};
     EwArgs A;
     if ((s = plan_elementwise(OP_CAST, 1, ins, out, out->dtype, &A))) return s;
-    if ((s = attach_fpe(A))) return s;
+    if ((s = attach_fpe(A, OP_CAST, out->dtype, in->dtype, out->dtype))) return s;
     return run_elementwise(A, cfg, as_stream(stream));
 }
 
 ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes, int32_t keepdims,
                         const void *initial, const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
     if (ionp_status cs = check_cfg(cfg)) return cs;
+    if (fold_error_take()) return fold_failed();
     ionp_array_desc t;
     ionp_status s;
     if ((s = check_desc(out, "out"))) return s;

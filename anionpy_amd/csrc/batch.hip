@@ -65,6 +65,10 @@ struct ionp_batch {
     hipGraphExec_t graph;
     bool graph_ready;
     size_t scratch_need;
+    // GRAPH mode's own scratch (partials, converted inputs, fold slots): the
+    // captured kernels point into it, so no later call can free or refill it
+    char *d_graph_scratch;
+    ionp::ScratchArena graph_arena;
 };
 
 namespace ionp {
@@ -631,6 +635,68 @@ static ionp_status run_one(const ionp_call &c, hipStream_t s, const ionp_config 
     }
 }
 
+// GRAPH mode, built on first use: one eager pass of the single-call launch
+// sequence records the largest scratch request of each kind; the batch
+// allocates its own regions of those sizes; the same sequence is then
+// captured with those regions installed, so every captured kernel points
+// into memory the batch owns and nothing allocates during the capture.
+static ionp_status graph_build(ionp_batch *bt, hipStream_t s) {
+    ScratchUse use;
+    {
+        ScratchRecord rec(&use);
+        for (const ionp_call &c : bt->calls) {
+            ionp_status st0 = run_one(c, s);
+            if (st0) return st0;
+        }
+    }
+    auto up = [](size_t b) { return (std::max<size_t>(b, 256) + 255) / 256 * 256; };
+    const size_t bs = up(use.scratch), bg = up(use.stage), bl = up(use.slots);
+    if (hipStreamSynchronize(s) != hipSuccess || hipMalloc(&bt->d_graph_scratch, bs + bg + bl) != hipSuccess) {
+        bt->d_graph_scratch = nullptr;
+        return set_error(IONP_ERR_DEVICE, "out of device memory for the graph's scratch");
+    }
+    ScratchArena &A = bt->graph_arena;
+    A = ScratchArena();
+    A.scratch = bt->d_graph_scratch;
+    A.stage = bt->d_graph_scratch + bs;
+    A.slots = bt->d_graph_scratch + bs + bg;
+    A.size.scratch = bs;
+    A.size.stage = bg;
+    A.size.slots = bl;
+    (void)num_cus();
+    // capture on a private stream (the legacy default stream can not be
+    // captured); the graph is then launched on the caller's stream
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+        return set_error(IONP_ERR_DEVICE, "hipStreamCreate failed");
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        return set_error(IONP_ERR_DEVICE, "hipStreamBeginCapture failed");
+    }
+    ionp_status st = IONP_OK;
+    {
+        ScratchScope scope(&A);
+        for (const ionp_call &c : bt->calls)
+            if (!st) st = run_one(c, cs);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (!st && A.short_) st = set_error(IONP_ERR_DEVICE, "graph capture asked for more scratch than the eager pass");
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return set_error(IONP_ERR_DEVICE, "hipStreamEndCapture failed");
+    if (hipGraphInstantiate(&bt->graph, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return set_error(IONP_ERR_DEVICE, "hipGraphInstantiate failed");
+    }
+    (void)hipGraphDestroy(g);
+    bt->graph_ready = true;
+    return IONP_OK;
+}
+
 }  // namespace ionp
 
 using namespace ionp;
@@ -683,6 +749,7 @@ ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, i
     bt->calls.assign(calls, calls + n);
     bt->graph_ready = false;
     bt->scratch_need = scratch_need;
+    bt->d_graph_scratch = nullptr;
     hipStream_t s = (hipStream_t)stream;
     bt->nfold = (int32_t)fold_list.size();
     if (begin[n] >= (1ll << 31)) {
@@ -716,6 +783,7 @@ ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, i
 ionp_status ionp_batch_run(ionp_batch *bt, int32_t mode, void *stream) {
     if (!bt) return set_error(IONP_ERR_VALUE, "null batch");
     hipStream_t s = (hipStream_t)stream;
+    FpeOff plain;  // batches run the plain kernels whatever the thread's IEEE flag tracking
     if (mode == IONP_BATCH_PER_CALL) {
         for (const ionp_call &c : bt->calls) {
             ionp_status st = run_one(c, s);
@@ -725,39 +793,8 @@ ionp_status ionp_batch_run(ionp_batch *bt, int32_t mode, void *stream) {
     }
     if (mode == IONP_BATCH_GRAPH) {
         if (!bt->graph_ready) {
-            // one eager pass first: it sizes the library scratch, so nothing
-            // allocates inside the capture
-            for (const ionp_call &c : bt->calls) {
-                ionp_status st0 = run_one(c, s);
-                if (st0) return st0;
-            }
-            (void)num_cus();
-            // capture on a private stream (the legacy default stream can not be
-            // captured); the graph is then launched on the caller's stream
-            hipStream_t cs;
-            if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
-                return set_error(IONP_ERR_DEVICE, "hipStreamCreate failed");
-            if (hipStreamSynchronize(s) != hipSuccess || hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-                (void)hipStreamDestroy(cs);
-                return set_error(IONP_ERR_DEVICE, "hipStreamBeginCapture failed");
-            }
-            ionp_status st = IONP_OK;
-            for (const ionp_call &c : bt->calls)
-                if (!st) st = run_one(c, cs);
-            hipGraph_t g = nullptr;
-            const hipError_t e = hipStreamEndCapture(cs, &g);
-            (void)hipStreamDestroy(cs);
-            if (st) {
-                if (g) (void)hipGraphDestroy(g);
-                return st;
-            }
-            if (e != hipSuccess) return set_error(IONP_ERR_DEVICE, "hipStreamEndCapture failed");
-            if (hipGraphInstantiate(&bt->graph, g, nullptr, nullptr, 0) != hipSuccess) {
-                (void)hipGraphDestroy(g);
-                return set_error(IONP_ERR_DEVICE, "hipGraphInstantiate failed");
-            }
-            (void)hipGraphDestroy(g);
-            bt->graph_ready = true;
+            ionp_status st0 = graph_build(bt, s);
+            if (st0) return st0;
         }
         return hipGraphLaunch(bt->graph, s) == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, "hipGraphLaunch failed");
     }
@@ -796,6 +833,7 @@ ionp_status ionp_autotune(const ionp_call *calls, int32_t n, const ionp_config *
     if (n < 0 || ncfg <= 0 || reps <= 0 || (n > 0 && (!calls || !cfgs || !times_us)))
         return set_error(IONP_ERR_VALUE, "bad autotune arguments");
     hipStream_t s = (hipStream_t)stream;
+    FpeOff plain;  // timings of the plain kernels
     const int nev = ncfg * reps;
     std::vector<hipEvent_t> ev(2 * (size_t)nev);
     for (auto &e : ev)
@@ -837,6 +875,7 @@ void ionp_batch_destroy(ionp_batch *bt) {
     if (bt->d_counters) (void)hipFree(bt->d_counters);
     if (bt->d_fold_list) (void)hipFree(bt->d_fold_list);
     if (bt->d_partials) (void)hipFree(bt->d_partials);
+    if (bt->d_graph_scratch) (void)hipFree(bt->d_graph_scratch);
     delete bt;
 }
 

@@ -27,5 +27,13 @@ int num_cus();
 constexpr int64_t kInfinityCacheBytes = 256ll << 20;
 ionp_status run_elementwise(EwArgs &A, const ionp_config *cfg, hipStream_t s);
 ionp_status launch_generic(EwArgs A, const ionp_config *cfg, hipStream_t s);
+// IEEE flag tracking of the calling thread: sets it on/off, returns the old state
+int fpe_thread_swap(int on);
+// FpeOff: the plain kernels for the lifetime of the guard (batches, autotuning)
+struct FpeOff {
+    int old;
+    FpeOff() : old(fpe_thread_swap(0)) {}
+    ~FpeOff() { fpe_thread_swap(old); }
+};
 
 }  // namespace ionp

@@ -94,6 +94,70 @@ ionp_status binary_types(int op, int a, int b, int *loop, int *out) {
     return IONP_OK;
 }
 
+// dtype::can_cast (ionp-core/src/dtype.rs) over the numeric subset.
+// safe: bool to anything; an integer to a wider one of its signedness, to a
+// signed one wider than it, or to the float whose mantissa holds it (8 bits
+// float16, 16 bits float32, else float64); a float to a wider float.
+// same_kind: safe, or any cast that keeps or raises the kind (bool < unsigned
+// < signed < float).  equiv / no: the same dtype (all of these are native
+// byte order).  unsafe: anything.
+bool can_cast(int from, int to, int casting) {
+    if (from == to || casting == IONP_CAST_UNSAFE) return true;
+    if (casting == IONP_CAST_NO || casting == IONP_CAST_EQUIV) return false;
+    auto kind = [](int d) { return d == IONP_BOOL ? 0 : is_real_float(d) ? 3 : is_signed_dt(d) ? 2 : 1; };
+    const int kf = kind(from), kt = kind(to);
+    bool safe;
+    if (kf == 0) safe = true;
+    else if (kf == 3) safe = kt == 3 && itemsize(to) >= itemsize(from);
+    else if (kt == 3) safe = itemsize(from) <= 1 ? true : itemsize(from) == 2 ? to != IONP_FLOAT16 : to == IONP_FLOAT64;
+    else if (kt == 0) safe = false;
+    else if (kf == kt) safe = itemsize(to) >= itemsize(from);
+    else safe = kf == 1 && kt == 2 && itemsize(to) > itemsize(from);  // unsigned into a wider signed
+    if (safe || casting == IONP_CAST_SAFE) return safe;
+    return kt >= kf;  // same_kind
+}
+
+// The loop Ufunc.__call__ runs for `dtype=requested` (ionp-py/src/lib.rs:
+// 9568-9804 with resolve_binary_output_dtype, ionp-core/src/ufunc.rs:1031):
+// among the loops whose output is `requested`, the first (NumPy's table
+// order) that both operands reach by safe casting, else the one whose inputs
+// are `requested` itself; the operands must then reach its inputs under
+// `casting`.  Arithmetic, maximum and minimum have one loop per dtype
+// (T, T -> T; subtract none for bool, true_divide only the floats), so the
+// loop is `requested`; comparisons (T, T -> bool) run the loop their
+// promotion picks, and no comparison loop has another output.
+ionp_status resolve_dtype(int op, int a, int b, int requested, int casting, int *loop) {
+    if (op < 0 || op >= IONP_NOPS) return set_error(IONP_ERR_VALUE, "invalid ufunc op code");
+    if (a < 0 || a >= IONP_NTYPES || b < 0 || b >= IONP_NTYPES || requested < 0 || requested >= IONP_NTYPES)
+        return set_error(IONP_ERR_TYPE, "invalid dtype code");
+    if (casting < IONP_CAST_NO || casting > IONP_CAST_UNSAFE) return set_error(IONP_ERR_VALUE, "invalid casting rule");
+    const std::string no_loop =
+        std::string("No loop matching the specified signature and casting was found for ufunc ") + op_name(op);
+    if (op >= IONP_LESS) {
+        if (requested != IONP_BOOL) return set_error(IONP_ERR_TYPE, no_loop);
+        int l, o;
+        ionp_status s = binary_types(op, a, b, &l, &o);
+        if (s) return s;
+        *loop = l;
+        return IONP_OK;
+    }
+    if (op == IONP_SUBTRACT && requested == IONP_BOOL)
+        return set_error(IONP_ERR_TYPE,
+                         "numpy boolean subtract, the `-` operator, is not supported, use the bitwise_xor, "
+                         "the `^` operator, or the logical_xor function instead.");
+    if (op == IONP_TRUE_DIVIDE && !is_real_float(requested)) return set_error(IONP_ERR_TYPE, no_loop);
+    static const char *rules[] = {"no", "equiv", "safe", "same_kind", "unsafe"};
+    const int ins[2] = {a, b};
+    for (int i = 0; i < 2; i++)
+        if (!can_cast(ins[i], requested, casting))
+            return set_error(IONP_ERR_NO_LOOP, std::string("Cannot cast ufunc '") + op_name(op) + "' input " +
+                                                   std::to_string(i) + " from dtype('" + dtype_name(ins[i]) +
+                                                   "') to dtype('" + dtype_name(requested) + "') with casting rule '" +
+                                                   rules[casting] + "'");
+    *loop = requested;
+    return IONP_OK;
+}
+
 // reduce_compute_dtype (ufunc.rs:2041-2066): add.reduce widens bool and
 // sub-64-bit integers to the platform long (signedness kept).
 int reduce_default_dtype(int op, int in) {

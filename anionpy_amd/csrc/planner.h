@@ -25,6 +25,10 @@ const char *op_name(int op);
 int promote(int a, int b);
 ionp_status binary_types(int op, int a, int b, int *loop, int *out);
 int reduce_default_dtype(int op, int in);
+// dtype::can_cast under an ionp_casting rule
+bool can_cast(int from, int to, int casting);
+// Ufunc.__call__'s dtype= loop selection (lib.rs:9568-9804, ufunc.rs:1031)
+ionp_status resolve_dtype(int op, int a, int b, int requested, int casting, int *loop);
 
 // ---- descriptors
 ionp_status check_desc(const ionp_array_desc *d, const char *what);

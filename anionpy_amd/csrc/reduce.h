@@ -94,4 +94,44 @@ void *scratch_get(size_t bytes, hipStream_t s);
 // a second such buffer for inputs converted to another dtype first
 void *stage_get(size_t bytes, hipStream_t s);
 
+// Bytes of each scratch kind: reduction partials (scratch_get), converted
+// inputs (stage_get) and the in-launch folders' partial slots.
+struct ScratchUse {
+    size_t scratch = 0, stage = 0, slots = 0;
+};
+
+// Scratch regions a caller owns (a batch's captured graph, the
+// no-allocation entry point).  While installed on a thread (ScratchScope),
+// the getters above return these regions instead of the library's buffers:
+// nothing is allocated or freed, the fold slots are refilled with their
+// sentinel by a kernel ahead of every use, and a request larger than its
+// region returns null and sets `short_` (`need` records the largest
+// request of each kind).
+struct ScratchArena {
+    char *scratch = nullptr, *stage = nullptr, *slots = nullptr;
+    ScratchUse size;
+    ScratchUse need;
+    bool short_ = false;
+};
+struct ScratchScope {
+    explicit ScratchScope(ScratchArena *a);
+    ~ScratchScope();
+    ScratchArena *old;
+};
+// True (once) when an in-launch fold of a one-output add.reduce gave up
+// waiting for a window partial since the last call; the output it wrote is
+// NaN.  Reads host-visible memory: no synchronisation.
+bool fold_error_take();
+// Test hook: the folders treat window `window` (-1: none) as never filled
+// and give up after `polls` polls of one unfilled slot (default 2^26).
+void fold_debug_stall(int64_t window, uint32_t polls);
+
+// Records the largest request of each kind made on this thread while alive
+// (the library's buffers still serve them).
+struct ScratchRecord {
+    explicit ScratchRecord(ScratchUse *u);
+    ~ScratchRecord();
+    ScratchUse *old;
+};
+
 }  // namespace ionp

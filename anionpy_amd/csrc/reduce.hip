@@ -24,6 +24,7 @@
 // Integer reductions and argmax/argmin are order-free (associative).
 #include "reduce.h"
 
+#include <atomic>
 #include <mutex>
 
 #include "elementwise.h"
@@ -40,9 +41,33 @@ static size_t g_scratch_bytes = 0;
 static void *g_stage = nullptr;
 static size_t g_stage_bytes = 0;
 
+// Caller-owned scratch regions installed on this thread (ScratchScope), and
+// the recorder of the largest requests (ScratchRecord).
+static thread_local ScratchArena *t_arena = nullptr;
+static thread_local ScratchUse *t_record = nullptr;
+
+ScratchScope::ScratchScope(ScratchArena *a) : old(t_arena) { t_arena = a; }
+ScratchScope::~ScratchScope() { t_arena = old; }
+ScratchRecord::ScratchRecord(ScratchUse *u) : old(t_record) { t_record = u; }
+ScratchRecord::~ScratchRecord() { t_record = old; }
+
+// a request against an installed arena: its region, or null (and the
+// arena marked short) when the region is smaller
+static void *arena_take(char *base, size_t have, size_t bytes, size_t ScratchUse::*need) {
+    ScratchArena &A = *t_arena;
+    if (bytes > A.need.*need) A.need.*need = bytes;
+    if (bytes > have || !base) {
+        A.short_ = true;
+        return nullptr;
+    }
+    return base;
+}
+
 // a second grow-only buffer: inputs converted to another dtype before a
 // kernel that may itself take scratch_get memory
 void *stage_get(size_t bytes, hipStream_t s) {
+    if (t_record && bytes > t_record->stage) t_record->stage = bytes;
+    if (t_arena) return arena_take(t_arena->stage, t_arena->size.stage, bytes, &ScratchUse::stage);
     std::lock_guard<std::mutex> lk(g_scratch_mu);
     if (bytes <= g_stage_bytes) return g_stage;
     if (g_stage) {
@@ -62,6 +87,8 @@ void *stage_get(size_t bytes, hipStream_t s) {
 }
 
 void *scratch_get(size_t bytes, hipStream_t s) {
+    if (t_record && bytes > t_record->scratch) t_record->scratch = bytes;
+    if (t_arena) return arena_take(t_arena->scratch, t_arena->size.scratch, bytes, &ScratchUse::scratch);
     std::lock_guard<std::mutex> lk(g_scratch_mu);
     if (bytes <= g_scratch_bytes) return g_scratch;
     if (g_scratch) {
@@ -143,11 +170,19 @@ __global__ __launch_bounds__(kPwThreads) void pw_windows(const RedArgs A, int64_
 //     HBM traffic instead of following it;
 //   - otherwise: by pw_fold after the launch.
 // Producers never wait; a folder waits only for slots producers always fill
-// and gives up (writing NaN) after a bounded number of polls.
+// and gives up after a bounded number of polls of one slot (a producer that
+// can not be scheduled, e.g. CUs held by another process): the output is
+// then NaN, `dirty` asks the next folded call to refill every slot first
+// (late producers may still write some), and `err` -- host-visible memory
+// -- lets the host report the failure without a synchronisation.
 struct PwStreamState {
     void *partials;    // nout * nwin of T, pre-filled with the sentinel
     int64_t nfull, nwin;
     int32_t folders;   // folder workgroups (0 or nout)
+    uint32_t poll_limit;  // polls of one unfilled slot before the folder gives up
+    int64_t stall;     // test hook: a window index the folder never sees filled (-1: none)
+    uint32_t *dirty;   // device word, or null (slots refilled on every use anyway)
+    uint32_t *err;     // fine-grained host word, or null
 };
 
 template <typename T> struct Sentinel;
@@ -274,7 +309,7 @@ __device__ void pw_fold_ring(const RedArgs &A, const PwStreamState &F, int64_t o
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 const int64_t idx = base + 64 * q + lane;
-                b[q] = idx < (int64_t)nwin ? ld_bits(part + idx) : Sentinel<T>::bits;
+                b[q] = idx < (int64_t)nwin && idx != F.stall ? ld_bits(part + idx) : Sentinel<T>::bits;
             }
         };
         poll(b0, cur);
@@ -317,8 +352,8 @@ __device__ void pw_fold_ring(const RedArgs &A, const PwStreamState &F, int64_t o
             } else if (cur < nwin) {
                 if (n == 0) {
                     // wait on the first unfilled slot alone, then re-read the blocks
-                    while (ld_bits(part + cur) == Sentinel<T>::bits) {
-                        if (++idle > (1u << 26)) {  // producers always fill every slot; safety bound only
+                    while ((int64_t)cur == F.stall || ld_bits(part + cur) == Sentinel<T>::bits) {
+                        if (++idle > F.poll_limit) {  // producers always fill every slot; safety bound only
                             failed = true;
                             break;
                         }
@@ -334,9 +369,15 @@ __device__ void pw_fold_ring(const RedArgs &A, const PwStreamState &F, int64_t o
             }
         }
         if (failed) {
-            // leave every slot of this output holding the sentinel again
+            // leave every slot of this output holding the sentinel again (a
+            // producer still running may overwrite one: `dirty` has the next
+            // folded call refill them all first)
             for (uint32_t i = cur + lane; i < nwin; i += 64) st_value(slots + i, from_bits<T>(Sentinel<T>::bits));
-            if (lane == 0) lds_store(&ctl[2], 1u);
+            if (lane == 0) {
+                lds_store(&ctl[2], 1u);
+                if (F.dirty) __hip_atomic_store(F.dirty, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (F.err) __hip_atomic_store(F.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
         }
         return;
     }
@@ -566,6 +607,16 @@ template <typename T> __global__ void fill_sentinel(T *p, int64_t n) {
         p[i] = from_bits<T>(Sentinel<T>::bits);
 }
 
+// One workgroup: after a folder gave up (dirty set), the sentinel into all
+// n slots, then dirty cleared; otherwise nothing.  Runs in stream order
+// ahead of the next folded call, so every late producer has finished.
+template <typename T> __global__ __launch_bounds__(256) void slots_refill(T *p, int64_t n, uint32_t *dirty) {
+    if (__hip_atomic_load(dirty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = from_bits<T>(Sentinel<T>::bits);
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(dirty, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 
 // Ordered fold per output: acc = 0; acc = acc + partial[i] (ufunc.rs:7137
 // zero seed; the outer-axis sequential fold of pairwise_group 6504-6531).
@@ -576,8 +627,19 @@ static void *g_slots = nullptr;
 static size_t g_slots_bytes = 0;
 static int g_slots_isz = 0;  // element size of the sentinel it holds (0: none)
 template <typename T> static T *slots_get(int64_t n, hipStream_t s) {
-    std::lock_guard<std::mutex> lk(g_scratch_mu);
     const size_t bytes = (size_t)n * sizeof(T);
+    if (t_record && bytes > t_record->slots) t_record->slots = bytes;
+    if (t_arena) {
+        // a caller's region keeps no state between uses: its n slots are
+        // filled with the sentinel ahead of every use (a kernel, so a
+        // captured graph refills them on every replay)
+        T *p = (T *)arena_take(t_arena->slots, t_arena->size.slots, bytes, &ScratchUse::slots);
+        if (p && n > 0)
+            hipLaunchKernelGGL(fill_sentinel<T>, dim3((unsigned)std::min<int64_t>(1024, (n + 255) / 256)), dim3(256), 0, s,
+                               p, n);
+        return p;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
     if (bytes > g_slots_bytes) {
         if (g_slots) {
             (void)hipDeviceSynchronize();  // earlier launches may still use it
@@ -602,6 +664,7 @@ template <typename T> static T *slots_get(int64_t n, hipStream_t s) {
     return (T *)g_slots;
 }
 static void slots_dirty() {
+    if (t_arena) return;  // refilled on every use
     std::lock_guard<std::mutex> lk(g_scratch_mu);
     g_slots_isz = 0;
 }
@@ -1720,17 +1783,77 @@ static ionp_status run_masked(RedArgs &A, const ionp_config *cfg, hipStream_t s)
 // (4096,65536) rows: 165 against 170 us); the other CUs host the folders.
 // The launch shape is fixed by residency, so ionp_config does not apply.
 constexpr size_t kPsPadLds = 16u << 10;
+// The folders' give-up record of each device (see PwStreamState): `dirty`
+// in device memory, `err` in fine-grained host memory.  Allocated with the
+// library's fold slots; an installed ScratchArena refills its slots on
+// every use and reports through `err` only if it already exists.
+struct FoldStatus {
+    uint32_t *dirty = nullptr, *err = nullptr;
+};
+static FoldStatus g_fold[64];
+static std::atomic<int64_t> g_fold_stall{-1};
+static std::atomic<uint32_t> g_fold_polls{1u << 26};
+
+static FoldStatus fold_status(bool alloc) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return FoldStatus();
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    FoldStatus &F = g_fold[dev];
+    if (alloc && !F.dirty) {
+        void *d = nullptr, *h = nullptr;
+        if (hipMalloc(&d, 256) != hipSuccess) return FoldStatus();
+        if (hipHostMalloc(&h, 64, hipHostMallocCoherent) != hipSuccess || hipMemset(d, 0, 256) != hipSuccess ||
+            hipDeviceSynchronize() != hipSuccess) {
+            (void)hipFree(d);
+            if (h) (void)hipHostFree(h);
+            return FoldStatus();
+        }
+        *(volatile uint32_t *)h = 0;
+        F.dirty = (uint32_t *)d;
+        F.err = (uint32_t *)h;
+    }
+    return F;
+}
+
+bool fold_error_take() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return false;
+    uint32_t *e = g_fold[dev].err;
+    return e && __atomic_exchange_n(e, 0u, __ATOMIC_ACQ_REL) != 0;
+}
+
+void fold_debug_stall(int64_t window, uint32_t polls) {
+    g_fold_stall = window;
+    g_fold_polls = polls;
+}
+
 template <typename T> static ionp_status run_pw_stream(RedArgs &A, hipStream_t s) {
     PwStreamState F;
     F.nfull = A.wsize / 8192;
     const int64_t has_tail = A.wsize % 8192 != 0;
     F.nwin = F.nfull + has_tail;
     F.folders = (A.nout <= 64 && F.nwin > 8) ? (int32_t)A.nout : 0;
+    F.poll_limit = g_fold_polls.load();
+    F.stall = g_fold_stall.load();
+    F.dirty = nullptr;
+    F.err = nullptr;
     const int64_t nw_total = A.nout * F.nwin;
     // folders poll the slots for the sentinel; pw_fold runs after the launch
     // and needs none
     T *part = F.folders ? slots_get<T>(nw_total, s) : (T *)scratch_get((size_t)nw_total * sizeof(T), s);
     if (!part) return set_error(IONP_ERR_DEVICE, "out of device memory for reduction scratch");
+    if (F.folders) {
+        const FoldStatus fs = fold_status(!t_arena);
+        F.err = fs.err;
+        if (!t_arena) {
+            if (!fs.dirty) return set_error(IONP_ERR_DEVICE, "out of memory for the fold status words");
+            F.dirty = fs.dirty;
+            // a folder that gave up in an earlier call may have left late
+            // values in the slots: refill them all first (a no-op otherwise)
+            hipLaunchKernelGGL(slots_refill<T>, dim3(1), dim3(256), 0, s, part, (int64_t)(g_slots_bytes / sizeof(T)),
+                               F.dirty);
+        }
+    }
     F.partials = part;
     A.scratch = part;
     if (has_tail)  // the last, partial window of each output (generic tree)

@@ -128,6 +128,29 @@ ionp_status ionp_result_type(int32_t a, int32_t b, int32_t *out);
 /* binary_out_dtype (ionp-core/src/ufunc.rs:1101): loop and output dtype. */
 ionp_status ionp_binary_types(int32_t op, int32_t a, int32_t b, int32_t *loop, int32_t *out);
 
+/* casting rules (NumPy's casting= keyword) */
+typedef enum {
+    IONP_CAST_NO = 0,
+    IONP_CAST_EQUIV = 1,
+    IONP_CAST_SAFE = 2,
+    IONP_CAST_SAME_KIND = 3,
+    IONP_CAST_UNSAFE = 4
+} ionp_casting;
+
+/* dtype::can_cast (ionp-core/src/dtype.rs): *ok = 1 when `from` casts to
+ * `to` under `casting`. */
+ionp_status ionp_can_cast(int32_t from, int32_t to, int32_t casting, int32_t *ok);
+
+/* Ufunc.__call__'s dtype= resolution (ionp-py/src/lib.rs:9568-9804 with
+ * resolve_binary_output_dtype, ionp-core/src/ufunc.rs:1031): the loop dtype
+ * the inputs are cast to before the call.  Errors as NumPy's: TypeError
+ * "No loop matching the specified signature and casting was found for
+ * ufunc <name>" (IONP_ERR_TYPE), UFuncTypeError "Cannot cast ufunc '<name>'
+ * input <i> from dtype('<a>') to dtype('<b>') with casting rule '<rule>'"
+ * (IONP_ERR_NO_LOOP). */
+ionp_status ionp_resolve_dtype(int32_t op, int32_t a, int32_t b, int32_t requested, int32_t casting,
+                               int32_t *loop);
+
 /* ---- planning (host only: fills an output template) ------------------ */
 
 /* binary_op's shape/dtype/layout half (ufunc.rs:3628-3695, shape.rs:54,
@@ -235,17 +258,38 @@ ionp_status ionp_reduceat(int32_t op, const ionp_array_desc *in, const int64_t *
 
 /* The fpe_check_after_call side channel (ionp-py/src/lib.rs:9077-9196,
  * ionp-core/src/fpe.rs:173-360), computed on the device instead of by a
- * host rescan: while tracking is on (process-wide), ionp_binary and
- * ionp_cast OR the IEEE flags their elements raise -- as NumPy 2.2.6 reads
- * them from the x86 status register after a loop -- into a flag word of
- * the current GPU.  Bits: 1 divide by zero, 2 overflow, 4 underflow (tiny
- * after rounding and inexact), 8 invalid.  Float add / subtract / multiply
- * / true_divide and casts from a float dtype raise flags; comparisons,
- * maximum / minimum, where and integer loops never do.  Reductions are not
- * tracked. */
+ * host rescan.  Tracking is per calling thread, as the reference keeps its
+ * FPE state per thread (ionp-py/src/fpstate.rs:103): while it is on for a
+ * thread, that thread's ionp_binary and ionp_cast calls OR the IEEE flags
+ * their elements raise -- as NumPy 2.2.6 reads them from the x86 status
+ * register after a loop -- into the thread's own flag word on the current
+ * GPU.  Bits: 1 divide by zero, 2 overflow, 4 underflow (tiny after
+ * rounding and inexact), 8 invalid.  Float add / subtract / multiply /
+ * true_divide and casts from a float dtype raise flags; comparisons,
+ * maximum / minimum, where and integer loops never do.  Reductions,
+ * batches (ionp_batch_run) and ionp_autotune are never tracked.
+ * Switching tracking on or off waits for the device and clears the
+ * thread's word, so no flag outlives the state that raised it. */
 ionp_status ionp_fpe_track(int32_t on);
-/* Waits for `stream`, returns the current GPU's flags and clears them. */
+/* Waits for `stream`, returns the calling thread's flags on the current
+ * GPU and clears them. */
 ionp_status ionp_fpe_fetch(void *stream, uint32_t *flags);
+
+/* ---- device failures reported without a synchronisation -------------- */
+
+/* Waits for `stream`, then reports a failure a launch recorded on the
+ * device: the in-launch fold of a one-output add.reduce (over 64K
+ * elements) waits a bounded time for window partials, and if a producer
+ * workgroup is never scheduled (its CUs held by another process) it gives
+ * up, writes NaN as that output and records it here (IONP_ERR_DEVICE).
+ * ionp_reduce also returns such a recorded failure at its next call. */
+ionp_status ionp_check(void *stream);
+
+/* Test hook (no counterpart in the reference): the folds of later
+ * add.reduce calls treat window `window` (-1: none) as never filled and
+ * give up after `polls` polls of one unfilled slot (default 2^26), so the
+ * failure path above runs while every kernel still exits normally. */
+ionp_status ionp_debug_fold_stall(int64_t window, uint32_t polls);
 
 /* ---- batching: many small calls, one launch --------------------------- */
 /* No counterpart in the reference (it runs one call at a time). */

@@ -194,3 +194,53 @@ def test_float16_dtype_rules(lib):
         axes = (ctypes.c_int32 * 1)(1)
         assert lib.ionp_plan_reduce(op, ctypes.byref(_desc(a)), axes, 1, 0, -1, ctypes.byref(t)) == 0
         assert ap._NP_DTYPES[t.dtype] == np.float16
+
+
+def test_can_cast_table_matches_numpy(lib):
+    """ionp_can_cast against numpy.can_cast over every dtype pair and rule
+    (the rules are numpy's casting table; it has not changed in NumPy 2)."""
+    import anionpy_amd as ap
+    from anionpy_amd import _lib
+    ok = ctypes.c_int32()
+    for a in ap._NP_DTYPES:
+        for b in ap._NP_DTYPES:
+            for rule, code in _lib.CASTING.items():
+                assert lib.ionp_can_cast(ap._CODE[np.dtype(a)], ap._CODE[np.dtype(b)], code, ctypes.byref(ok)) == 0
+                assert bool(ok.value) == np.can_cast(a, b, rule), (a, b, rule)
+
+
+@pytest.mark.parametrize("op", ["add", "subtract", "multiply", "true_divide", "maximum", "less", "equal"])
+def test_resolve_dtype_matches_numpy_loops(lib, op):
+    """ionp_resolve_dtype against NumPy 2.2.6's ufunc resolution for
+    dtype= on every (a, b, requested) triple of same_kind / safe / unsafe
+    casting: the loop dtype (the result's dtype, or the inputs' for a
+    comparison), or the error class and message."""
+    import warnings
+    import anionpy_amd as ap
+    from anionpy_amd import _lib
+    if np.__version__ != "2.2.6":
+        pytest.skip("expectations are NumPy 2.2.6's")
+    code = getattr(_lib, op.upper())
+    f = getattr(np, op)
+    loop = ctypes.c_int32()
+    dts = [np.dtype(d) for d in ap._NP_DTYPES]
+    for a in dts:
+        for b in dts[::3] + [a]:
+            for req in dts:
+                for rule in ("same_kind", "safe", "unsafe"):
+                    rc = lib.ionp_resolve_dtype(code, ap._CODE[a], ap._CODE[b], ap._CODE[req], _lib.CASTING[rule],
+                                                ctypes.byref(loop))
+                    try:
+                        with warnings.catch_warnings():
+                            warnings.simplefilter("ignore")
+                            r = f(np.zeros(1, a), np.ones(1, b), dtype=req, casting=rule)
+                    except TypeError as e:
+                        assert rc != 0, (op, a, b, req, rule, str(e))
+                        msg = lib.ionp_last_error().decode()
+                        assert msg == str(e), (op, a, b, req, rule)
+                        assert (rc == 6) == (type(e).__name__ == "UFuncTypeError" or "Cannot cast" in str(e))
+                        continue
+                    assert rc == 0, (op, a, b, req, rule, lib.ionp_last_error().decode())
+                    assert r.dtype == req
+                    if code < _lib.LESS:
+                        assert np.dtype(ap._NP_DTYPES[loop.value]) == req

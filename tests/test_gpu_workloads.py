@@ -1,11 +1,14 @@
 """Full-size workloads (inputs regenerated from their seeds) compared
-bit-for-bit with NumPy 2.2.6 digests recorded in tests/golden/big.json."""
+bit-for-bit with NumPy 2.2.6 digests recorded in tests/golden/big.json, and
+reductions at the folder / chain / window boundaries compared with the C
+oracle (itself pinned to NumPy 2.2.6 by the golden vectors)."""
 import hashlib
 
 import numpy as np
 import pytest
 import torch
 
+import oracle as O
 from fixture_cases import big
 
 pytestmark = pytest.mark.gpu
@@ -86,8 +89,9 @@ def test_single_launch_folds_repeat(ap):
     x = rng.standard_normal(3 * 2 ** 20 + 5)
     R = rng.standard_normal((300, 2 * 8192 + 3)).astype(np.float32)
     xd, Rd = ap.asarray(x), ap.asarray(R)
-    exp = {"max": x.max(), "min": x.min(), "argmax": int(x.argmax()), "argmin": int(x.argmin()),
-           "rows": np.add.reduce(R, axis=1), "rowmax": R.max(axis=1)}
+    exp = {"max": O.reduce("maximum", x, None), "min": O.reduce("minimum", x, None),
+           "argmax": int(O.argmax(x)), "argmin": int(O.argmin(x)),
+           "rows": np.asarray(O.reduce("add", R, 1)), "rowmax": np.asarray(O.reduce("maximum", R, 1))}
     for _ in range(6):
         assert ap.maximum.reduce(xd).get() == exp["max"]
         assert int(ap.argmax(xd).get()) == exp["argmax"]
@@ -118,7 +122,7 @@ def test_folder_sum_special_values_and_repeat(ap):
         for h in cases:
             for dt in (np.float64, np.float32):
                 hd = h.astype(dt)
-                exp = np.atleast_1d(np.add.reduce(hd)).view(np.uint64 if dt == np.float64 else np.uint32)
+                exp = np.atleast_1d(O.reduce("add", hd, None)).view(np.uint64 if dt == np.float64 else np.uint32)
                 got = np.atleast_1d(ap.add.reduce(ap.asarray(hd)).get()).view(exp.dtype)
                 assert got[0] == exp[0], (dt, hex(int(got[0])), hex(int(exp[0])))
 
@@ -136,7 +140,7 @@ def test_narrow_wave_chains(ap):
                 h[1000, 0] = nan1
                 h[shape[0] // 2, 1] = np.inf
                 h[shape[0] - 1, 1] = -np.inf
-            exp = np.add.reduce(h, axis=0)
+            exp = np.asarray(O.reduce("add", h, 0))
             got = ap.add.reduce(ap.asarray(h), axis=0).get()
             assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), (shape, dt)
 
@@ -151,11 +155,11 @@ def test_product_chains(ap):
             h = (1 + 1e-3 * rng.standard_normal(shape)).astype(dt)
             if shape[0] == 5000:
                 h[2500, 1] = np.nan
-            exp = np.multiply.reduce(h, axis=axis)
+            exp = np.asarray(O.reduce("multiply", h, axis))
             got = ap.multiply.reduce(ap.asarray(h), axis=axis).get()
             assert np.array_equal(np.atleast_1d(got).view(np.uint8), np.atleast_1d(exp).view(np.uint8)), (shape, dt)
             m = rng.random(shape) < 0.7
-            exp = np.multiply.reduce(h, axis=axis, where=m)
+            exp = np.asarray(O.reduce("multiply", h, axis, where=m))
             got = ap.multiply.reduce(ap.asarray(h), axis=axis, where=ap.asarray(m)).get()
             assert np.array_equal(np.atleast_1d(got).view(np.uint8), np.atleast_1d(exp).view(np.uint8)), (shape, dt, "where")
 
@@ -169,12 +173,12 @@ def test_streaming_sum_misaligned_views(ap):
         base = rng.standard_normal(2 ** 20 + 7).astype(dt)
         d = ap.asarray(base)
         for k in (1, 2, 3):
-            exp = np.add.reduce(base[k:])
+            exp = O.reduce("add", base[k:], None)
             got = ap.add.reduce(d[k:]).get()
             assert np.atleast_1d(got).view(np.uint8).tobytes() == np.atleast_1d(exp).view(np.uint8).tobytes(), (dt, k)
         R = rng.standard_normal((70, 3 * 8192 + 5)).astype(dt)
         Rd = ap.asarray(R)
         for sl in (np.s_[:, :], np.s_[:, 1:], np.s_[1:, 3:]):
-            exp = np.add.reduce(R[sl], axis=1)
+            exp = np.asarray(O.reduce("add", R[sl], 1))
             got = ap.add.reduce(Rd[sl], axis=1).get()
             assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), (dt, sl)
