@@ -82,6 +82,7 @@ SIGNATURES = {
     "ionp_where": [_D, _D, _D, _D, _C, _V],
     "ionp_cast": [_D, _D, _C, _V],
     "ionp_reduce": [_I, _D, _P(_I), _I, _I, _V, _D, _C, _V],
+    "ionp_reduce_noalloc": [_I, _D, _P(_I), _I, _I, _V, _D, _C, _V],
     "ionp_plan_reduce_where": [_I, _D, _D, _P(_I), _I, _I, _I, _D],
     "ionp_reduce_where": [_I, _D, _D, _P(_I), _I, _I, _V, _D, _C, _V],
     "ionp_argext": [_I, _D, _I, _I, _D, _C, _V],

@@ -148,6 +148,33 @@ bool is_mixed_su(int op, int a, int b) {
     return op >= IONP_LESS && ((a == IONP_INT64 && b == IONP_UINT64) || (a == IONP_UINT64 && b == IONP_INT64));
 }
 
+// Scratch of ionp_reduce_noalloc: one fixed array per GPU, in the code
+// object, so no call allocates.  Regions: reduction partials, converted
+// (float16) inputs, the in-launch folders' slots.
+constexpr size_t kNoAllocPartials = 4u << 20, kNoAllocStage = 2u << 20, kNoAllocSlots = 2u << 20;
+constexpr size_t kNoAllocBytes = kNoAllocPartials + kNoAllocStage + kNoAllocSlots;
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+// The array's address on each device (hipGetSymbolAddress with that
+// device current; the host-side name `scratch` is not a device address).
+std::mutex g_noalloc_mu;
+char *g_noalloc_ptr[64] = {};
+
+ionp_status noalloc_base(char **base) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(IONP_ERR_DEVICE, "no current HIP device");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(IONP_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        g_noalloc_ptr[dev] = (char *)p;
+    }
+    *base = g_noalloc_ptr[dev];
+    return IONP_OK;
+}
+
 }  // namespace
 
 namespace ionp {
@@ -164,7 +191,8 @@ ionp_status ionp_init(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_error(IONP_ERR_DEVICE, "no HIP device available");
     (void)num_cus();
-    return IONP_OK;
+    char *base;
+    return noalloc_base(&base);  // the no-allocation scratch of the current device
 }
 
 const char *ionp_last_error(void) { return last_error(); }
@@ -493,6 +521,35 @@ ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *ax
     ReducePlan p;
     if ((s = plan_reduce(op, in, mask, out, &p))) return s;
     return run_reduce(p, g_bufsize, initial, cfg, as_stream(stream));
+}
+
+ionp_status ionp_reduce_noalloc(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes,
+                                int32_t keepdims, const void *initial, const ionp_array_desc *out,
+                                const ionp_config *cfg, void *stream) {
+    char *base;
+    ionp_status s;
+    if ((s = noalloc_base(&base))) return s;
+    ScratchArena A;
+    A.scratch = base;
+    A.stage = base + kNoAllocPartials;
+    A.slots = base + kNoAllocPartials + kNoAllocStage;
+    A.size.scratch = kNoAllocPartials;
+    A.size.stage = kNoAllocStage;
+    A.size.slots = kNoAllocSlots;
+    {
+        ScratchScope scope(&A);
+        s = ionp_reduce(op, in, axes, naxes, keepdims, initial, out, cfg, stream);
+    }
+    if (A.short_) {
+        auto mib = [](size_t b) { return std::to_string((b + (1u << 20) - 1) >> 20) + " MiB"; };
+        std::string what;
+        if (A.need.scratch > A.size.scratch) what = "partials " + mib(A.need.scratch) + " (maximum " + mib(A.size.scratch) + ")";
+        else if (A.need.stage > A.size.stage) what = "a float16 input converted to " + mib(A.need.stage) + " (maximum " + mib(A.size.stage) + ")";
+        else what = "fold slots " + mib(A.need.slots) + " (maximum " + mib(A.size.slots) + ")";
+        return set_error(IONP_ERR_VALUE, "ionp_reduce_noalloc: this reduction needs " + what +
+                                             " of its fixed scratch; use ionp_reduce");
+    }
+    return s;
 }
 
 ionp_status ionp_reduce_where(int32_t op, const ionp_array_desc *in, const ionp_array_desc *where,

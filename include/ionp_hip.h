@@ -212,6 +212,25 @@ ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *ax
                         int32_t keepdims, const void *initial, const ionp_array_desc *out, const ionp_config *cfg,
                         void *stream);
 
+/* No counterpart in the reference: ionp_reduce for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, results and
+ * errors as ionp_reduce, but every scratch buffer (reduction partials, a
+ * float16 input converted for a float32 / float64 loop, the in-launch
+ * folds' slots) comes from one fixed 8 MiB array per GPU that the library
+ * holds until the process ends (its address is looked up by ionp_init, or
+ * by the first call on a device), so no call allocates or frees anything.
+ * Limits: 4 MiB of partials (2^19 float64 window partials), 2 MiB of
+ * converted input (2^19 float16 elements into float32), 2 MiB of fold
+ * slots (2^18 float64 windows: one-output sums of up to 2^31 elements); a
+ * reduction needing more is rejected with IONP_ERR_VALUE before its output
+ * is written.  All calls on
+ * one GPU share the array: calls that overlap in time on one GPU (other
+ * streams or threads) race. */
+ionp_status ionp_reduce_noalloc(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes,
+                                int32_t keepdims, const void *initial, const ionp_array_desc *out,
+                                const ionp_config *cfg, void *stream);
+
 /* Replaces reduce_axis (ufunc.rs:7634) when `where_mask` is given
  * (reduce_axis_masked, ufunc.rs:7172), as NumPy 2.2.6 evaluates it: the
  * fold starts from `initial` (else add's 0, multiply's 1; maximum and
