@@ -50,6 +50,30 @@ def test_uneven_batch_modes_agree_with_oracle(ap):
     assert not bad, f"{len(bad)} calls differ from the oracle, first {bad[:5]}"
 
 
+def test_uneven_batch_full_size_matches_oracle(ap):
+    """The bench's batch at its stated size (10,000 calls from seed 5, sizes
+    log-uniform in 16..2^20 elements): PERSISTENT and GRAPH outputs, every
+    byte, against the C oracle on host copies -- including the long narrow
+    chains and multi-window wide sums inside the persistent kernel."""
+    from anionpy_amd.batch import Batch
+    from anionpy_amd.layouts import uneven_batch
+    specs = uneven_batch(seed=5, n=10000)
+    with Batch(specs) as bt:
+        for mode in ("persistent", "graph"):
+            for s in specs:
+                s.out._buf.fill_(0xAB)
+            bt.run(mode)
+            torch.cuda.synchronize()
+            bad = []
+            for i, s in enumerate(specs):
+                want = np.asarray(expected(s))  # 0-d outputs stay 0-d (ascontiguousarray would make them 1-d)
+                got = s.out.get()
+                if got.shape != want.shape or np.ascontiguousarray(got).tobytes() != np.ascontiguousarray(want).tobytes():
+                    bad.append(i)
+            print(f"{mode}: {len(specs) - len(bad)}/{len(specs)} calls equal to the oracle")
+            assert not bad, f"{mode}: {len(bad)} calls differ from the oracle, first {bad[:5]}"
+
+
 def test_batch_rejects_bad_calls(ap):
     from anionpy_amd.batch import Batch, reduce_call
     x = ap.asarray(np.zeros((4, 5)))
