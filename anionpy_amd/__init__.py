@@ -512,10 +512,12 @@ def _int_fits(v, t):
 
 
 def _weak_int_overflow(v, t):
-    """numpy's OverflowError for a Python int outside dtype t: beyond the C
-    long range (unless t is uint64 and v fits it) the conversion itself
-    fails, else the value is out of t's bounds."""
-    if not -2 ** 63 <= v < 2 ** 63 and not (t == _U64 and 0 <= v < 2 ** 64):
+    """numpy's OverflowError for a Python int outside dtype t: the value is
+    first converted to a C long (for uint32 / uint64 targets: a C long or a
+    C unsigned long), which fails beyond that range; else it is out of t's
+    bounds."""
+    wide_unsigned = t.kind == "u" and t.itemsize >= 4
+    if not (-2 ** 63 <= v < 2 ** 63 or (wide_unsigned and 0 <= v < 2 ** 64)):
         return OverflowError("Python int too large to convert to C long")
     return OverflowError(f"Python integer {v} out of bounds for {t}")
 
@@ -538,11 +540,13 @@ def _weak_float(v, t):
     return r
 
 
-def _weak_array(x, kind, t, relaxed=False):
-    """The weak scalar x materialised as a 0-d host array of dtype t; for an
-    integer t and an int outside it: comparisons (relaxed) widen to int64 /
-    uint64 and compare by value (lib.rs:8146), None beyond 64 bits; other
-    ops raise numpy's OverflowError."""
+def _weak_array(x, kind, t, mode="strict"):
+    """The weak scalar x materialised as a 0-d host array of dtype t.  An
+    int outside an integer t: "strict" (arithmetic) raises numpy's
+    OverflowError; "relaxed" (comparisons) widens to int64 / uint64 and
+    compares by value (lib.rs:8146), None beyond 64 bits; "wrap" (where, in
+    NumPy 2.2.6: the C cast of the concatenation family, lib.rs:8192) wraps
+    to t's width and reports the cast's overflow."""
     if kind == "f" or (t.kind == "f" and kind != "b"):
         return _weak_float(float(x), t) if t.kind == "f" else np.array(x, dtype=t)
     if kind == "b" or t.kind == "b":
@@ -550,14 +554,19 @@ def _weak_array(x, kind, t, relaxed=False):
     v = int(x)
     if _int_fits(v, t):
         return np.array(v, dtype=t)
-    if relaxed:
+    if mode == "relaxed":
         t2 = _I64 if v < 0 else _U64
         return np.array(v, dtype=t2) if _int_fits(v, t2) else None
+    if mode == "wrap" and -2 ** 63 <= v < 2 ** 64:
+        if _fpe_wanted():
+            _fpe_report(2, "cast")
+        return np.array(v % (1 << (8 * t.itemsize)), dtype=np.dtype(f"u{t.itemsize}")).view(t)
     raise _weak_int_overflow(v, t)
 
 
-def _default_weak(x, kind):
-    """Both operands weak: each takes its kind's default dtype."""
+def _default_weak(x, kind, relaxed=False):
+    """Both operands weak: each takes its kind's default dtype (an int
+    beyond int64 fits uint64 only in a comparison)."""
     if kind == "b":
         return np.array(x, dtype=_BOOL)
     if kind == "f":
@@ -565,7 +574,7 @@ def _default_weak(x, kind):
     v = int(x)
     if _int_fits(v, _I64):
         return np.array(v, dtype=_I64)
-    if _int_fits(v, _U64):
+    if relaxed and _int_fits(v, _U64):
         return np.array(v, dtype=_U64)
     raise OverflowError("Python int too large to convert to C long")
 
@@ -577,29 +586,40 @@ _CMP_SWAP = {"less": "greater", "less_equal": "greater_equal", "greater": "less"
              "greater_equal": "less_equal", "equal": "equal", "not_equal": "not_equal"}
 
 
-def _operands(a, b, code=None, loop=None):
-    """Device operands of a binary call.  loop: the dtype= loop, into which
-    weak scalars are converted directly.  Returns (a, b, const): const is
-    the bool result of a comparison against a Python int beyond 64 bits."""
+def _weak_operands(a, b, code=None, loop=None, wrap=False):
+    """Host half of a binary call's operands: weak Python scalars become
+    0-d host arrays (numpy's OverflowError for an int they cannot hold);
+    arrays pass through.  loop: the dtype= loop, into which weak scalars
+    are converted directly.  Returns (a, b, const): const is the bool
+    result of a comparison against a Python int beyond 64 bits (a and b
+    are then the array and None).  wrap: where()'s wrapping int cast."""
     ka, kb = _scalar_kind(a), _scalar_kind(b)
-    relaxed = code is not None and code >= _lib.LESS
+    cmp = code is not None and code >= _lib.LESS and loop is None
     if ka and kb:
         if loop is not None:
-            return asarray(_weak_array(a, ka, loop)), asarray(_weak_array(b, kb, loop)), None
-        return asarray(_default_weak(a, ka)), asarray(_default_weak(b, kb)), None
+            return _weak_array(a, ka, loop), _weak_array(b, kb, loop), None
+        return _default_weak(a, ka, cmp), _default_weak(b, kb, cmp), None
+    # comparisons against an integer array compare a Python int by value
+    relaxed = cmp and (ka or kb) and np.dtype((b if ka else a).dtype).kind in "iu"
     if ka or kb:
-        arr = asarray(b if ka else a)
+        arr = b if ka else a
         x, k = (a, ka) if ka else (b, kb)
         if loop is not None:
             w = _weak_array(x, k, loop)
         else:
-            w = _weak_array(x, k, _weak_target(k, arr.dtype, code), relaxed)
+            mode = "wrap" if wrap else "relaxed" if relaxed else "strict"
+            w = _weak_array(x, k, _weak_target(k, np.dtype(arr.dtype), code), mode)
         if w is None:  # a comparison with a Python int beyond 64 bits
             name = _CODE_NAMES[code] if ka is None else _CMP_SWAP[_CODE_NAMES[code]]
             return arr, None, bool(_CMP_CONST[name](int(x)))
-        w = asarray(w)
         return (w, arr, None) if ka else (arr, w, None)
-    return asarray(a), asarray(b), None
+    return a, b, None
+
+
+def _operands(a, b, code=None, loop=None, wrap=False):
+    """Device operands of a binary call (see _weak_operands)."""
+    a, b, const = _weak_operands(a, b, code, loop, wrap)
+    return asarray(a), (None if b is None else asarray(b)), const
 
 
 def _resolve_stand_in(x, requested):
@@ -608,12 +628,27 @@ def _resolve_stand_in(x, requested):
     dtype), a weak float counts as float64, a weak bool as bool."""
     k = _scalar_kind(x)
     if k is None:
-        return asarray(x).dtype
+        return np.dtype(x.dtype)
     if k == "b":
         return _BOOL
     if k == "f":
         return _F64
     return requested if requested.kind != "b" else _I64
+
+
+def _resolve_loop(code, a, b, dtype, casting):
+    """The dtype= loop of a binary call (ionp_resolve_dtype: host only), or
+    None when the call runs its promotion's loop (no dtype=, or a
+    comparison, whose only loops output bool)."""
+    if casting not in _lib.CASTING:
+        raise ValueError("casting must be one of 'no', 'equiv', 'safe', 'same_kind', or 'unsafe'")
+    if dtype is None:
+        return None
+    req = _dtype(dtype)
+    lp = ctypes.c_int32()
+    _call(_lib.load().ionp_resolve_dtype, code, _CODE[_resolve_stand_in(a, req)], _CODE[_resolve_stand_in(b, req)],
+          _CODE[req], _lib.CASTING[casting], ctypes.byref(lp))
+    return np.dtype(_NP_DTYPES[lp.value]) if code < _lib.LESS else None
 
 
 def _finish(out):
@@ -696,21 +731,12 @@ class Ufunc:
             if len(out) != 1:
                 raise ValueError("The 'out' tuple must have exactly one entry per ufunc output")
             out = out[0]
-        if casting not in _lib.CASTING:
-            raise ValueError("casting must be one of 'no', 'equiv', 'safe', 'same_kind', or 'unsafe'")
         L = _lib.load()
         if not _is_weak(a):
             a = asarray(a)
         if not _is_weak(b):
             b = asarray(b)
-        loop = None
-        if dtype is not None:
-            req = _dtype(dtype)
-            lp = ctypes.c_int32()
-            _call(L.ionp_resolve_dtype, self._code, _CODE[_resolve_stand_in(a, req)],
-                  _CODE[_resolve_stand_in(b, req)], _CODE[req], _lib.CASTING[casting], ctypes.byref(lp))
-            if self._code < _lib.LESS:  # comparisons run their promotion's loop
-                loop = np.dtype(_NP_DTYPES[lp.value])
+        loop = _resolve_loop(self._code, a, b, dtype, casting)
         a, b, const = _operands(a, b, self._code, loop)
         if loop is not None:  # the inputs are cast into the loop first (a buffered cast in NumPy)
             a = a if a.dtype == loop else a._astype(loop, "K")
@@ -1088,9 +1114,10 @@ def argmin(a, axis=None, keepdims=False):
 
 def where(condition, x, y):
     """where (reductions.rs:3316) -> where_select (ionp-core/src/sort.rs:1869);
-    a weak Python scalar takes the other operand's dtype within its kind."""
+    a weak Python scalar takes the other operand's dtype within its kind (an
+    int outside an integer dtype wraps, as NumPy 2.2.6's where casts it)."""
     c = asarray(condition)
-    x, y, _ = _operands(x, y)
+    x, y, _ = _operands(x, y, wrap=True)
     L = _lib.load()
     dc, dx, dy = c._desc(), x._desc(), y._desc()
     t = _lib.ArrayDesc()

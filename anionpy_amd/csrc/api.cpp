@@ -144,6 +144,17 @@ ionp_status cast_copy(const ionp_array_desc *in, int dt, ionp_array_desc *c, hip
     return run_elementwise(E, nullptr, st);
 }
 
+// A float32 / float64 loop that reads its input through NumPy's buffered
+// cast: float16 input (add / multiply convert as they read otherwise), and
+// maximum / minimum with another dtype (the x86 loop models hold for the
+// loop's own dtype, read from the contiguous cast buffer).  Those loops run
+// on a cast copy in the library's stage buffer, dense in the input's K
+// order (the iterator's axis order is kept).
+bool needs_cast_copy(int op, int in_dt, int out_dt) {
+    if (!is_float_dt(out_dt) || in_dt == out_dt) return false;
+    return in_dt == IONP_FLOAT16 || op == IONP_MAXIMUM || op == IONP_MINIMUM;
+}
+
 bool is_mixed_su(int op, int a, int b) {
     return op >= IONP_LESS && ((a == IONP_INT64 && b == IONP_UINT64) || (a == IONP_UINT64 && b == IONP_INT64));
 }
@@ -328,15 +339,9 @@ static ionp_status reduce_dtype(int32_t op, int in_dt, int32_t dtype, int *out_d
         return set_error(IONP_ERR_VALUE, "reduce is only provided for add, multiply, maximum and minimum");
     int dt = dtype < 0 ? reduce_default_dtype(op, in_dt) : dtype;
     if (dt < 0 || dt >= IONP_NTYPES) return set_error(IONP_ERR_TYPE, "invalid dtype code");
-    // integer maximum/minimum cast every element to the loop dtype first;
-    // the float x86 loop model covers the input's own dtype only
-    if ((op == IONP_MAXIMUM || op == IONP_MINIMUM) && dt != in_dt && (is_real_float(dt) || is_real_float(in_dt)))
-        return set_error(IONP_ERR_UNSUPPORTED, "float maximum/minimum.reduce with a dtype other than the input's");
-    // same_kind casting of the input to the loop dtype (a float can not feed an integer loop)
-    if (is_real_float(in_dt) && !is_real_float(dt))
-        return set_error(IONP_ERR_TYPE, std::string("Cannot cast ufunc '") + op_name(op) + "' input from dtype('" +
-                                            dtype_name(in_dt) + "') to dtype('" + dtype_name(dt) +
-                                            "') with casting rule 'same_kind'");
+    // the input is cast to the loop dtype unsafely (NumPy's reduce-family
+    // default): a float input feeds an integer / bool loop element by
+    // element with the C cast
     *out_dt = dt;
     return IONP_OK;
 }
@@ -514,7 +519,7 @@ ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *ax
     bool mask[IONP_MAX_DIMS];
     normalize_axes(axes, naxes, in->ndim, mask);
     ionp_array_desc c;
-    if (in->dtype == IONP_FLOAT16 && out->dtype != IONP_FLOAT16) {  // the float32/float64 loop reads a cast copy
+    if (needs_cast_copy(op, in->dtype, out->dtype)) {
         if ((s = cast_copy(in, out->dtype, &c, as_stream(stream)))) return s;
         in = &c;
     }
@@ -567,12 +572,15 @@ ionp_status ionp_reduce_where(int32_t op, const ionp_array_desc *in, const ionp_
         return set_error(IONP_ERR_VALUE, std::string("reduction operation '") + op_name(op) +
                                              "' does not have an identity, so to use a where mask one has to "
                                              "specify 'initial'");
-    if (in->dtype == IONP_FLOAT16 || out->dtype == IONP_FLOAT16)
-        return set_error(IONP_ERR_UNSUPPORTED, "float16 where= reductions are not provided");
     int64_t wst[IONP_MAX_DIMS];
     if ((s = where_strides(in, where, wst))) return s;
     bool mask[IONP_MAX_DIMS];
     normalize_axes(axes, naxes, in->ndim, mask);
+    ionp_array_desc c;
+    if (needs_cast_copy(op, in->dtype, out->dtype)) {  // a K-order copy keeps the iterator's axis order
+        if ((s = cast_copy(in, out->dtype, &c, as_stream(stream)))) return s;
+        in = &c;
+    }
     ReducePlan p;
     if ((s = plan_reduce(op, in, mask, out, &p, wst, (const uint8_t *)elem0(where)))) return s;
     return run_reduce(p, g_bufsize, initial, cfg, as_stream(stream));

@@ -32,6 +32,7 @@ enum : int32_t {
     BK_IADD_SMALL = 6,   // integer / bool add.reduce: one thread per output
     BK_FADD_NARROW_WAVE = 7,  // float add.reduce, W empty, long chains: one wavefront per output
     BK_IADD_WAVE = 8,    // integer / bool add.reduce, groups of 65 .. kIntWaveMax: one wavefront per output
+    BK_SIDE = 9,         // float16 call: its own single-call launches, ahead of the persistent kernel
 };
 
 constexpr int kBatchThreads = 256;
@@ -69,6 +70,7 @@ struct ionp_batch {
     // captured kernels point into it, so no later call can free or refill it
     char *d_graph_scratch;
     ionp::ScratchArena graph_arena;
+    std::vector<int32_t> side;  // float16 calls (original order): single-call launches ahead of the persistent kernel
 };
 
 namespace ionp {
@@ -502,13 +504,24 @@ __global__ __launch_bounds__(kBatchThreads) void batch_fold(const BatchCall *__r
     }
 }
 
+ionp_status plan_call_tiles(const ionp_call &c, BatchCall *B, int64_t *bytes);
+
 ionp_status plan_call(const ionp_call &c, BatchCall *B, int64_t *bytes) {
     std::memset(B, 0, sizeof *B);
-    ionp_status s;
     const int nops = c.kind == IONP_CALL_WHERE ? 3 : (c.kind == IONP_CALL_BINARY ? 2 : 1);
-    for (int k = 0; k < nops; k++)
-        if (c.in[k].dtype == IONP_FLOAT16) return set_error(IONP_ERR_UNSUPPORTED, "float16 calls are not batched");
-    if (c.out.dtype == IONP_FLOAT16) return set_error(IONP_ERR_UNSUPPORTED, "float16 calls are not batched");
+    bool half = c.out.dtype == IONP_FLOAT16;
+    for (int k = 0; k < nops; k++) half = half || c.in[k].dtype == IONP_FLOAT16;
+    ionp_status st = plan_call_tiles(c, B, bytes);
+    if (st == IONP_OK && half) {  // validated and planned; the tile code has no float16 loops
+        B->kind = BK_SIDE;
+        B->tiles = 0;
+        B->direct = 0;
+    }
+    return st;
+}
+
+ionp_status plan_call_tiles(const ionp_call &c, BatchCall *B, int64_t *bytes) {
+    ionp_status s;
     if (c.kind == IONP_CALL_BINARY || c.kind == IONP_CALL_WHERE || c.kind == IONP_CALL_CAST) {
         ionp_array_desc t;
         int nin, op, loop;
@@ -708,7 +721,7 @@ ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, i
     *out = nullptr;
     std::vector<BatchCall> planned(n > 0 ? n : 1), table(n > 0 ? n : 1);
     std::vector<int64_t> begin(n + 1, 0), serial(n > 0 ? n : 1, 0);
-    std::vector<int32_t> fold_list, order(n);
+    std::vector<int32_t> fold_list, order(n), side;
     int64_t bytes = 0, part = 0;
     size_t scratch_need = 0;
     for (int32_t i = 0; i < n; i++) {
@@ -726,6 +739,7 @@ ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, i
                     : B.kind == BK_IADD_WAVE  ? B.red.group / 64
                                               : 0;
         order[i] = i;
+        if (B.kind == BK_SIDE) side.push_back(i);
     }
     // tiles are handed out in table order: long serial tiles first, so they do
     // not finish the launch alone (the order of calls never changes a result)
@@ -747,6 +761,7 @@ ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, i
     bt->bytes = bytes;
     bt->part_bytes = part;
     bt->calls.assign(calls, calls + n);
+    bt->side = side;
     bt->graph_ready = false;
     bt->scratch_need = scratch_need;
     bt->d_graph_scratch = nullptr;
@@ -799,6 +814,10 @@ ionp_status ionp_batch_run(ionp_batch *bt, int32_t mode, void *stream) {
         return hipGraphLaunch(bt->graph, s) == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, "hipGraphLaunch failed");
     }
     if (mode != IONP_BATCH_PERSISTENT) return set_error(IONP_ERR_VALUE, "unknown batch mode");
+    for (int32_t i : bt->side) {  // float16 calls: their own launches, in stream order with the kernel below
+        ionp_status st = run_one(bt->calls[i], s);
+        if (st) return st;
+    }
     if (bt->n == 0 || bt->ntiles == 0) return IONP_OK;
     hipLaunchKernelGGL(batch_zero, dim3(1), dim3(64), 0, s, bt->d_counters, (int64_t)64);
     // persistent grid: as many workgroups as fit on the chip at once (the

@@ -469,6 +469,7 @@ template <int LT> __device__ inline typename dt_type<LT>::T ld(int dt, const cha
         case IONP_INT64: return cvt<LT>(((const int64_t *)base)[off]);
         case IONP_UINT64: return cvt<LT>(((const uint64_t *)base)[off]);
         case IONP_FLOAT32: return cvt<LT>(((const float *)base)[off]);
+        case IONP_FLOAT16: return cvt<LT>(h2f(((const uint16_t *)base)[off]));  // exact in float32
         default: return cvt<LT>(((const double *)base)[off]);
     }
 }

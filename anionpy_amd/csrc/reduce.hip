@@ -1702,6 +1702,7 @@ static ionp_status run_int(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
     // dtype first: integer add and multiply (wrap modulo the output width
     // either way) or no cast; bool outputs (logical or / and) take int_reduce
     if (!MASK && A.nw == 1 && A.w_st[0] == 1 && A.nr == 0 && A.in_dt != IONP_BOOL && A.out_dt != IONP_BOOL &&
+        !is_real_float(A.in_dt) &&  // float input: each element goes through the C cast (ld)
         A.group < (1ll << 40) && (OP == IONP_ADD || OP == IONP_MULTIPLY || A.in_dt == A.out_dt)) {
         switch (A.in_dt) {  // one contiguous run per output: vector loads
             case IONP_INT8: hipLaunchKernelGGL((int_reduce_contig<AT, int8_t, ST, OP>), grid, blk, 0, s, A, seg, nseg); break;
@@ -1948,21 +1949,29 @@ template <typename T, bool MAX> static ionp_status run_float_ext(RedArgs &A, con
 // float16 values (hstep).  W empty: each element is its own call, rounded
 // every step.  A workgroup per output; the serial parts run in thread 0.
 // ---------------------------------------------------------------------------
-template <int OP> __global__ __launch_bounds__(256) void half_reduce(const RedArgs A) {
+// MASK (where=): NumPy calls the reduce loop once per maximal run of True
+// mask values inside each window (W empty: per selected element); each call
+// keeps its float32 accumulator and stores it back to float16 at its end,
+// as without a mask (reduce_axis_masked_f16_widen, ufunc.rs:7414, widens
+// the same way).  Thread 0 finds the runs of a window in order.
+template <int OP, bool MASK> __global__ __launch_bounds__(256) void half_reduce(const RedArgs A) {
     __shared__ float racc[1024], pval[1024];
     __shared__ int leaf_lo[128], leaf_n[128], lcnt[1];
+    __shared__ int run_lo, run_hi;
     const int tid = threadIdx.x;
     const int isz = itemsize(A.in_dt);
     auto xh = [&](int64_t off) { return to_half_bits(A.in_dt, A.in + off * isz); };
     for (int64_t o = blockIdx.x; o < A.nout; o += gridDim.x) {
         int64_t ib, ob;
         out_offsets(A, (uint32_t)o, &ib, &ob);
+        const int64_t mb = MASK ? mask_k_offset(A, (uint32_t)o) : 0;
         uint16_t acc = A.has_init ? (uint16_t)A.init_bits : (OP == IONP_MULTIPLY ? (uint16_t)0x3c00 : (uint16_t)0);
         bool first = !A.has_init && (OP == IONP_MAXIMUM || OP == IONP_MINIMUM);  // first element seeds
         for (int64_t r = 0; r < A.rsize; r++) {
             const int64_t rb = ib + r_offset(A, r);
+            const int64_t rmb = MASK ? mb + mask_r_offset(A, r) : 0;
             if (A.nw == 0) {
-                if (tid == 0) {
+                if (tid == 0 && (!MASK || A.mask[rmb])) {
                     const uint16_t x = xh(rb);
                     acc = first ? x : hstep(OP, acc, x);
                     first = false;
@@ -1972,27 +1981,51 @@ template <int OP> __global__ __launch_bounds__(256) void half_reduce(const RedAr
             for (int64_t w = 0; w < A.nwin; w++) {
                 const int64_t s0 = w * A.bufsize;
                 const int L = (int)min(A.bufsize, A.wsize - s0);
-                if constexpr (OP == IONP_ADD) {
-                    const float pw = pairwise_block<float>([&](int i) { return h2f(xh(rb + w_offset(A, s0 + i))); }, L,
-                                                           racc, pval, leaf_lo, leaf_n, lcnt);
-                    if (tid == 0) {
-                        uint32_t fl = 0;
-                        acc = f2h(fadd(h2f(acc), pw), &fl);
+                // the calls of this window: [lo, hi) runs (the whole window unmasked)
+                int lo = 0;
+                for (;;) {
+                    int hi = L;
+                    if constexpr (MASK) {
+                        auto sel = [&](int i) { return A.mask[rmb + mask_w_offset(A, s0 + i)] != 0; };
+                        if (tid == 0) {
+                            int a = lo;
+                            while (a < L && !sel(a)) a++;
+                            int b = a;
+                            while (b < L && sel(b)) b++;
+                            run_lo = a;
+                            run_hi = b;
+                        }
+                        __syncthreads();
+                        lo = run_lo;
+                        hi = run_hi;
+                        __syncthreads();  // read by every thread before thread 0 writes the next run
+                        if (lo >= L) break;
                     }
-                    __syncthreads();
-                } else if (tid == 0) {
-                    if constexpr (OP == IONP_MULTIPLY) {
-                        float f = h2f(acc);
-                        for (int i = 0; i < L; i++) f = fmul(f, h2f(xh(rb + w_offset(A, s0 + i))));
-                        uint32_t fl = 0;
-                        acc = f2h(f, &fl);
-                    } else {
-                        for (int i = 0; i < L; i++) {
-                            const uint16_t x = xh(rb + w_offset(A, s0 + i));
-                            acc = first ? x : hstep(OP, acc, x);
-                            first = false;
+                    const int base = lo;
+                    if constexpr (OP == IONP_ADD) {
+                        const float pw = pairwise_block<float>([&](int i) { return h2f(xh(rb + w_offset(A, s0 + base + i))); },
+                                                               hi - lo, racc, pval, leaf_lo, leaf_n, lcnt);
+                        if (tid == 0) {
+                            uint32_t fl = 0;
+                            acc = f2h(fadd(h2f(acc), pw), &fl);
+                        }
+                        __syncthreads();
+                    } else if (tid == 0) {
+                        if constexpr (OP == IONP_MULTIPLY) {
+                            float f = h2f(acc);
+                            for (int i = lo; i < hi; i++) f = fmul(f, h2f(xh(rb + w_offset(A, s0 + i))));
+                            uint32_t fl = 0;
+                            acc = f2h(f, &fl);
+                        } else {
+                            for (int i = lo; i < hi; i++) {
+                                const uint16_t x = xh(rb + w_offset(A, s0 + i));
+                                acc = first ? x : hstep(OP, acc, x);
+                                first = false;
+                            }
                         }
                     }
+                    if (!MASK) break;
+                    lo = hi;
                 }
             }
         }
@@ -2001,17 +2034,21 @@ template <int OP> __global__ __launch_bounds__(256) void half_reduce(const RedAr
     }
 }
 
+template <bool MASK> static void launch_half(const RedArgs &A, dim3 g, dim3 b, hipStream_t s) {
+    switch (A.op) {
+        case IONP_ADD: hipLaunchKernelGGL((half_reduce<IONP_ADD, MASK>), g, b, 0, s, A); break;
+        case IONP_MULTIPLY: hipLaunchKernelGGL((half_reduce<IONP_MULTIPLY, MASK>), g, b, 0, s, A); break;
+        case IONP_MAXIMUM: hipLaunchKernelGGL((half_reduce<IONP_MAXIMUM, MASK>), g, b, 0, s, A); break;
+        default: hipLaunchKernelGGL((half_reduce<IONP_MINIMUM, MASK>), g, b, 0, s, A); break;
+    }
+}
+
 static ionp_status run_half_reduce(RedArgs &A, const ionp_config *cfg, hipStream_t s) {
-    if (A.mask) return set_error(IONP_ERR_UNSUPPORTED, "float16 where= reductions are not provided");
     if (A.bufsize > kPwMaxWin) return set_error(IONP_ERR_UNSUPPORTED, "bufsize above 8192");
     const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
     const dim3 g(grid_cap(A.nout, bpc)), b(256);
-    switch (A.op) {
-        case IONP_ADD: hipLaunchKernelGGL(half_reduce<IONP_ADD>, g, b, 0, s, A); break;
-        case IONP_MULTIPLY: hipLaunchKernelGGL(half_reduce<IONP_MULTIPLY>, g, b, 0, s, A); break;
-        case IONP_MAXIMUM: hipLaunchKernelGGL(half_reduce<IONP_MAXIMUM>, g, b, 0, s, A); break;
-        default: hipLaunchKernelGGL(half_reduce<IONP_MINIMUM>, g, b, 0, s, A); break;
-    }
+    if (A.mask) launch_half<true>(A, g, b, s);
+    else launch_half<false>(A, g, b, s);
     return launched();
 }
 

@@ -339,7 +339,10 @@ typedef enum {
 } ionp_batch_mode;
 
 /* Validate and plan `n` calls, pack their descriptors and tile table into
- * device memory.  The calls' device buffers must outlive the batch. */
+ * device memory.  The calls' device buffers must outlive the batch.  Calls
+ * are independent: no call's output may overlap another call's operands.
+ * In PERSISTENT mode float16 calls run as their own single-call launches,
+ * in stream order ahead of the persistent kernel. */
 ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, ionp_batch **out);
 ionp_status ionp_batch_run(ionp_batch *batch, int32_t mode, void *stream);
 /* number of tiles / bytes moved (algorithmic) by one run */

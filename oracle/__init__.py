@@ -242,6 +242,11 @@ def reduce(op, x, axis=0, bufsize=BUFSIZE, dtype=None, initial=None, where=None)
     axes = _norm_axes(axis, x.ndim)
     mask = bytes(1 if k in axes else 0 for k in range(x.ndim)) or b"\0"
     odt = DTYPES[lib().ora_reduce_dtype(OP[op], _CODE[x.dtype])] if dtype is None else np.dtype(dtype)
+    if odt in (np.float32, np.float64) and x.dtype != odt and (x.dtype == np.float16 or op in ("maximum", "minimum")):
+        # NumPy's buffered cast ahead of a float32 / float64 loop (ora_reduce
+        # reads float inputs as they are): the loop sees the cast values, the
+        # iterator's axis order unchanged
+        x = x.astype(odt, order="K")
     kept = [k for k in range(x.ndim) if k not in axes]
     shape = tuple(x.shape[k] for k in kept)
     wm = None

@@ -1143,8 +1143,10 @@ int ora_reduce(int op, const ora_desc *in, const uint8_t *axmask, const ora_desc
        an explicit dtype=); elements are cast to it as they are read */
     cdt = out->dtype;
     if (cdt == DT_F16) return ora_reduce_half(op, in, axmask, out, bufsize, initial);
-    if (in->dtype == DT_F16) return is_float(cdt) ? ORA_EVALUE : ORA_ETYPE; /* float loops: cast it first */
-    if (is_float(in->dtype) && !is_float(cdt)) return ORA_ETYPE;
+    /* a float loop reads float input as is: another float dtype is cast first
+       (the caller's buffered cast); an integer / bool loop casts each element
+       (reductions cast unsafely, as NumPy's ufunc.reduce does by default) */
+    if (in->dtype == DT_F16 && is_float(cdt)) return ORA_EVALUE;
     if (is_float(cdt) && in->dtype != cdt) return ORA_EVALUE; /* float loops read the input as is: cast it first */
     make_red_plan(in, axmask, &p);
     for (k = 0; k < n; k++) {
@@ -1467,6 +1469,10 @@ static float strided_ext_f32(int ismax, float p, const char *pb, const int64_t *
  *   float max/min   the window's SIMD (or strided) loop model on the run
  *   integers        order-free fold of the selected elements
  * W empty: acc = op(acc, x) for each selected x in iterator order.
+ * float16 output: the HALF loop per call -- add acc = half(float(acc) +
+ * pw_f32(run)), multiply a float32 chain from float(acc) rounded at the
+ * run's end, maximum / minimum element by element on the halves
+ * (reduce_axis_masked_f16_widen, ufunc.rs:7414, widens the same way).
  * `mask` is a bool operand broadcast to in's shape (zero strides allowed).
  * The reference scans runs over whole W blocks (no bufsize windows); the
  * oracle follows NumPy, as for the unmasked fold. */
@@ -1476,12 +1482,11 @@ int ora_reduce_where(int op, const ora_desc *in, const uint8_t *axmask, const or
     red_plan p;
     int n = in->ndim, k, nk = 0, kept[ORA_MAXDIM];
     int isz = ITEMSIZE[in->dtype], cdt = out->dtype, ismax = op == OP_MAXIMUM;
-    int64_t *woff, *moff, okidx[ORA_MAXDIM], nout, o;
+    int64_t *woff, *moff, *foff, okidx[ORA_MAXDIM], nout, o;
     char *buf;
     if (op != OP_ADD && op != OP_MULTIPLY && op != OP_MAXIMUM && op != OP_MINIMUM) return ORA_ENOLOOP;
     if (mask->dtype != DT_BOOL || mask->ndim != n) return ORA_ETYPE;
-    if (in->dtype == DT_F16 || cdt == DT_F16) return ORA_ENOLOOP; /* not provided for float16 */
-    if (is_float(in->dtype) && !is_float(cdt)) return ORA_ETYPE;
+    if (in->dtype == DT_F16 && is_float(cdt)) return ORA_EVALUE; /* a float32 / float64 loop: cast it first */
     if (is_float(cdt) && in->dtype != cdt) return ORA_EVALUE; /* float loops read the input as is */
     if ((op == OP_MAXIMUM || op == OP_MINIMUM) && !initial) return ORA_EVALUE; /* no identity */
     make_red_plan_m(in, mask, axmask, &p);
@@ -1493,6 +1498,8 @@ int ora_reduce_where(int op, const ora_desc *in, const uint8_t *axmask, const or
     woff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(p.wsize > 0 ? p.wsize : 1));
     moff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(p.wsize > 0 ? p.wsize : 1));
     buf = (char *)malloc((size_t)(bufsize > 0 ? bufsize : 1) * 8);
+    foff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(bufsize > 0 ? bufsize : 1));
+    for (o = 0; o < bufsize; o++) foff[o] = o * 4;
     if (p.nw && p.wsize) {
         w_offsets(in, &p, woff);
         w_mask_offsets(mask, &p, moff);
@@ -1505,6 +1512,7 @@ int ora_reduce_where(int op, const ora_desc *in, const uint8_t *axmask, const or
         float a32;
         uint64_t ua;
         int64_t sa;
+        uint16_t ah = op == OP_MULTIPLY ? 0x3c00u : 0; /* float16 output: the accumulator's bits */
         int uns = cdt == DT_BOOL || is_unsigned_int(cdt);
         for (k = 0; k < nk; k++) ooff += okidx[k] * out->strides[k];
         po = (char *)out->data + ooff * ITEMSIZE[cdt];
@@ -1522,6 +1530,7 @@ int ora_reduce_where(int op, const ora_desc *in, const uint8_t *axmask, const or
         if (initial) {
             if (cdt == DT_F64) a64 = *(const double *)initial;
             else if (cdt == DT_F32) a32 = *(const float *)initial;
+            else if (cdt == DT_F16) memcpy(&ah, initial, 2);
             else {
                 ua = (uint64_t)load_i(cdt, (const char *)initial);
                 sa = load_i(cdt, (const char *)initial);
@@ -1552,7 +1561,34 @@ int ora_reduce_where(int op, const ora_desc *in, const uint8_t *axmask, const or
                     while (b < len && mp[moff[s + b]]) b++;
                     if (!wo) wo = &zero;
                     /* one call of the inner loop on elements [a, b) */
-                    if (cdt == DT_F64 || cdt == DT_F32) {
+                    if (cdt == DT_F16) {
+                        /* NumPy's HALF loop: a float32 accumulator for the
+                           call, stored back to float16 at its end (add:
+                           float(acc) + pw_f32(run); multiply: a float32
+                           chain); maximum / minimum element by element on
+                           the halves; W empty: every element its own call */
+                        float *fv = (float *)buf;
+                        uint16_t h;
+                        if (op == OP_ADD && p.nw) {
+                            for (e = 0; e < b - a; e++) {
+                                cast_elem(in->dtype, pb + wo[e], DT_F16, (char *)&h);
+                                fv[e] = half_f(h);
+                            }
+                            ah = f32_to_half(f32_bits(f32_arith(OP_ADD, half_f(ah), pw_f32(buf, foff, b - a))));
+                        } else if (op == OP_MULTIPLY && p.nw) {
+                            float f = half_f(ah);
+                            for (e = 0; e < b - a; e++) {
+                                cast_elem(in->dtype, pb + wo[e], DT_F16, (char *)&h);
+                                f = f32_arith(OP_MULTIPLY, f, half_f(h));
+                            }
+                            ah = f32_to_half(f32_bits(f));
+                        } else {
+                            for (e = 0; e < b - a; e++) {
+                                cast_elem(in->dtype, pb + wo[e], DT_F16, (char *)&h);
+                                ah = half_step(op, ah, h);
+                            }
+                        }
+                    } else if (cdt == DT_F64 || cdt == DT_F32) {
                         if (op == OP_ADD && p.nw) {
                             if (cdt == DT_F64) a64 = f64_arith(OP_ADD, a64, pw_f64(pb, wo, b - a));
                             else a32 = f32_arith(OP_ADD, a32, pw_f32(pb, wo, b - a));
@@ -1603,6 +1639,7 @@ int ora_reduce_where(int op, const ora_desc *in, const uint8_t *axmask, const or
         }
         if (cdt == DT_F64) *(double *)po = a64;
         else if (cdt == DT_F32) *(float *)po = a32;
+        else if (cdt == DT_F16) memcpy(po, &ah, 2);
         else {
             uint64_t acc = uns ? ua : (uint64_t)sa;
             if (cdt == DT_BOOL) acc = acc != 0; /* add: or; multiply: and */
@@ -1620,6 +1657,7 @@ int ora_reduce_where(int op, const ora_desc *in, const uint8_t *axmask, const or
     }
     free(woff);
     free(moff);
+    free(foff);
     free(buf);
     return ORA_OK;
 }
@@ -1644,7 +1682,6 @@ int ora_reduceat(int op, const ora_desc *in, int axis, const int64_t *idx, int64
     char *buf;
     if (op != OP_ADD && op != OP_MULTIPLY && op != OP_MAXIMUM && op != OP_MINIMUM) return ORA_ENOLOOP;
     if (axis < 0 || axis >= n || out->ndim != n) return ORA_EAXIS;
-    if (is_float_any(in->dtype) && !is_float_any(cdt)) return ORA_ETYPE;
     if (is_float(cdt) && in->dtype != cdt) return ORA_EVALUE;
     len = in->shape[axis];
     st = in->strides[axis];
@@ -1779,7 +1816,6 @@ int ora_accumulate(int op, const ora_desc *in, int axis, const ora_desc *out)
     int64_t idx[ORA_MAXDIM], len, nfib = 1, f, i;
     if (op != OP_ADD && op != OP_MULTIPLY && op != OP_MAXIMUM && op != OP_MINIMUM) return ORA_ENOLOOP;
     if (axis < 0 || axis >= (n > 0 ? n : 1)) return ORA_EAXIS;
-    if (is_float_any(in->dtype) && !is_float_any(cdt)) return ORA_ETYPE;
     if (out->ndim != n) return ORA_EVALUE;
     for (k = 0; k < n; k++)
         if (out->shape[k] != in->shape[k]) return ORA_EVALUE;

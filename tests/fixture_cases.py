@@ -32,8 +32,21 @@ def big():
         return json.load(f)
 
 
+def py_scalar(spec):
+    """A Python bool / int / float operand recorded as its type and exact value."""
+    t, v = spec["py"], spec["value"]
+    if t == "bool":
+        return v == "True"
+    if t == "int":
+        return int(v)
+    return float.fromhex(v)
+
+
 def host_input(spec, npz):
-    """The operand as a host numpy view (strides preserved)."""
+    """The operand as a host numpy view (strides preserved), or the Python
+    scalar itself."""
+    if "py" in spec:
+        return py_scalar(spec)
     if "gen" in spec:  # shift + scale * uniform[-1, 1) in float64, cast, reshaped, stepped
         x = np.random.default_rng(spec["seed"]).uniform(-1, 1, spec["n"])
         if "scale" in spec:
@@ -78,6 +91,6 @@ def check_result(case, got, npz):
         return "" if sha(got) == exp["sha256"] else "value digest differs"
     want = npz[exp["buf"]].reshape(exp["shape"])
     if np.ascontiguousarray(got).tobytes() != want.tobytes():
-        diff = np.flatnonzero(np.ascontiguousarray(got).view(np.uint8).reshape(-1) != want.view(np.uint8).reshape(-1))
+        diff = np.flatnonzero(np.ascontiguousarray(got).reshape(-1).view(np.uint8) != want.reshape(-1).view(np.uint8))
         return f"value bytes differ at byte {diff[:4]} (got {got.reshape(-1)[:3]} want {want.reshape(-1)[:3]})"
     return ""

@@ -70,6 +70,22 @@ class Gen:
         return d
 
 
+class PyScalar:
+    """A Python bool / int / float operand (a NEP 50 weak scalar): recorded
+    as its type and exact value, passed to the ufunc as the scalar itself."""
+
+    def __init__(self, v):
+        self.v = v
+
+    def spec(self):
+        v = self.v
+        if isinstance(v, bool):
+            return {"py": "bool", "value": str(v)}
+        if isinstance(v, int):
+            return {"py": "int", "value": str(v)}
+        return {"py": "float", "value": float(v).hex()}
+
+
 class Store:
     def __init__(self):
         self.arrays = {}
@@ -85,7 +101,7 @@ class Store:
     def view(self, v):
         """Record an operand view as (stored root buffer, element offset,
         shape, strides): views of one array share its stored buffer."""
-        if isinstance(v, Gen):
+        if isinstance(v, (Gen, PyScalar)):
             return v.spec()
         v = np.asarray(v)
         isz = v.itemsize
@@ -204,9 +220,24 @@ def binary_cases(S, rng):
         c2 = specials(dt, 16, rng).reshape(16, 1)
         S.add(f"rows/multiply/{nm}/aligned", "binary", "multiply", [A2, c2], lambda: np.multiply(A2, c2))
         S.add(f"rows/multiply/{nm}/offset1", "binary", "multiply", [A2[:, 1:], c2], lambda: np.multiply(A2[:, 1:], c2))
-        V2 = A2.T[::2]
-        S.add(f"rows/multiply/{nm}/Tview", "binary", "multiply", [V2, c2[::2]], lambda: np.multiply(V2, c2[::2]))
+        rows_view_cases(S, A2, nm, dt)
     transpose_cases(S)
+
+
+def rows_view_cases(S, A2, nm, dt):
+    """The transposed stepped view of the bench (X.T[::2] * c[::2]) on a
+    (16, 2048) array: A2.T[::2] is (1024, 16) with element strides (2, 2048)
+    (the stride-2 gather of the row kernel), against a (1024, 1) column and
+    a (16,) row; an odd element offset (A2.T[1::2]: 16-byte pairs not
+    aligned) and stride 3 (A2.T[::3]: element by element).  Partners from
+    their own generator (seed 90 + dtype)."""
+    rv = np.random.default_rng(90 + np.dtype(dt).num)
+    for vn, V in (("Tview", A2.T[::2]), ("Tview_odd", A2.T[1::2]), ("Tview_s3", A2.T[::3])):
+        col = specials(dt, V.shape[0], rv).reshape(V.shape[0], 1)
+        row = specials(dt, V.shape[1], rv)
+        S.add(f"rows/multiply/{nm}/{vn}/col", "binary", "multiply", [V, col], lambda: np.multiply(V, col))
+        S.add(f"rows/add/{nm}/{vn}/row", "binary", "add", [V, row], lambda: np.add(V, row))
+        S.add(f"rows/less/{nm}/{vn}/col", "binary", "less", [col, V], lambda: np.less(col, V))
 
 
 def transpose_cases(S):
@@ -993,10 +1024,10 @@ def argext_cases(S, rng):
             if np.dtype(dt).kind == "f":
                 x = np.nan_to_num(x, nan=0.0) if rng.random() < 0.5 else x
                 x.reshape(-1)[rng.integers(0, x.size)] = x.max() if not np.isnan(x).any() else np.nan
-            for V in (x, x[::-1] if x.ndim == 1 else x.transpose()):
+            for vn, V in (("C", x), ("rev" if x.ndim == 1 else "T", x[::-1] if x.ndim == 1 else x.transpose())):
                 for ax in [None] + list(range(V.ndim)):
                     for fn in ("argmax", "argmin"):
-                        S.add(f"arg/{fn}/{nm}/{shp}/{ax}", "argext", fn, [V],
+                        S.add(f"arg/{fn}/{nm}/{shp}/{vn}/{ax}", "argext", fn, [V],
                               lambda: getattr(np, fn)(V, axis=ax), {"axis": ax})
     x = np.array([1.0, 3.0, 3.0, np.nan, 5.0, np.nan])
     S.add("arg/argmax/nan_first", "argext", "argmax", [x], lambda: np.argmax(x), {"axis": None})
@@ -1006,6 +1037,186 @@ def argext_cases(S, rng):
     S.add("arg/argmax/keepdims", "argext", "argmax", [x], lambda: np.argmax(x, axis=1, keepdims=True),
           {"axis": 1, "keepdims": True})
     S.add("error/argmax_empty", "argext", "argmax", [np.zeros((0,))], lambda: np.argmax(np.zeros((0,))), {"axis": None})
+
+
+ALL_DTYPES = DTYPES + [np.float16]
+
+
+def weak_scalar_cases(S):
+    """NEP 50 weak Python scalars (ionp-py lib.rs:8238-8272
+    extract_binary_pair -> scalar_against 8104): a Python bool / int /
+    float against every dtype, on either side -- the scalar takes the
+    array's dtype within its kind; an int outside it raises OverflowError
+    (arithmetic) or compares by value (comparisons, also beyond 64 bits);
+    true_divide's integer targets are float64; float scalars that overflow
+    the array's float dtype.  Own generator (seed 91)."""
+    r = np.random.default_rng(91)
+    scal = [0, 1, -1, 7, 127, 128, 255, 256, -129, 40000, 2 ** 31, -2 ** 31 - 1, 2 ** 63 - 1, 2 ** 63, 2 ** 64 - 1,
+            2 ** 64, -2 ** 63, -2 ** 63 - 1, 2 ** 70, -2 ** 70, 1.5, -0.0, 0.1, 1e300, 1e-50, float("nan"),
+            float("inf"), True, False]
+    ops = ["add", "subtract", "multiply", "true_divide", "maximum", "less", "equal", "greater_equal"]
+    for dt in ALL_DTYPES:
+        nm = np.dtype(dt).name
+        x = specials(dt, 7, r) if dt != np.float16 else (r.standard_normal(7) * 100).astype(np.float16)
+        for v in scal:
+            for op in ops:
+                if op == "subtract" and dt == np.bool_ and isinstance(v, bool):
+                    continue
+                f = getattr(np, op)
+                S.add(f"weak/{op}/{nm}/{v!r}", "binary", op, [x, PyScalar(v)], lambda: f(x, v))
+                if op in ("subtract", "less", "true_divide"):
+                    S.add(f"weak/{op}/{v!r}/{nm}", "binary", op, [PyScalar(v), x], lambda: f(v, x))
+    # both operands Python scalars
+    for a, b in ((3, 4), (3, 4.5), (True, 2), (2 ** 63, -1), (1.5, True), (2 ** 64 - 1, 1)):
+        for op in ("add", "less", "multiply"):
+            f = getattr(np, op)
+            S.add(f"weak/{op}/{a!r}/{b!r}", "binary", op, [PyScalar(a), PyScalar(b)], lambda: f(a, b))
+    # where with a scalar branch
+    c = r.random(9) < 0.5
+    for dt in (np.int8, np.uint8, np.float32, np.float64, np.int64):
+        y = specials(dt, 9, r)
+        for v in (1, 300, -1, 2.5, True):
+            S.add(f"weak/where/{np.dtype(dt).name}/{v!r}", "where", "where", [c, PyScalar(v), y],
+                  lambda: np.where(c, v, y))
+
+
+def dtype_arg_cases(S):
+    """dtype= (and casting=) on Ufunc.__call__ (ionp-py lib.rs:9568-9804,
+    resolve_binary_output_dtype ufunc.rs:1031): the inputs are cast into the
+    loop of the requested output dtype first; rejected casts raise
+    UFuncTypeError, a dtype no loop outputs raises TypeError; weak scalars
+    are converted straight into the loop dtype.  Own generator (seed 92)."""
+    r = np.random.default_rng(92)
+    ops = ["add", "subtract", "multiply", "true_divide", "maximum", "less", "equal"]
+    pairs = [(np.float64, np.float64), (np.float32, np.float64), (np.int32, np.int32), (np.int64, np.uint64),
+             (np.int8, np.uint8), (np.bool_, np.bool_), (np.float16, np.int16), (np.int16, np.float32),
+             (np.uint32, np.int8), (np.float64, np.int64)]
+    for da, db in pairs:
+        x = specials(da, 30, r).reshape(5, 6) if da != np.float16 else (r.standard_normal((5, 6)) * 50).astype(da)
+        y = specials(db, 6, r) if db != np.float16 else (r.standard_normal(6) * 50).astype(db)
+        for op in ops:
+            f = getattr(np, op)
+            for req in ALL_DTYPES:
+                for casting in ("same_kind", "unsafe"):
+                    if casting == "unsafe" and op not in ("add", "multiply"):
+                        continue
+                    if casting == "unsafe" and req == np.uint32 and "f" in (np.dtype(da).kind, np.dtype(db).kind):
+                        continue  # float -> uint32 out of range: platform-defined in NumPy (unpinned)
+                    rn = np.dtype(req).name
+                    S.add(f"dtype/{op}/{np.dtype(da).name}-{np.dtype(db).name}/{rn}/{casting}", "binary", op, [x, y],
+                          lambda: f(x, y, dtype=req, casting=casting), {"dtype": rn, "casting": casting})
+    # weak scalars with dtype=
+    x8 = np.array([1, -3, 100], np.int8)
+    u8 = np.array([1, 3, 200], np.uint8)
+    f32 = np.array([1.5, -2.25, 3e38], np.float32)
+    for arr, v, req in ((x8, 300, np.int16), (u8, -1, np.int16), (u8, -1, np.uint16), (f32, 0.1, np.float64),
+                        (x8, 1.5, np.int64), (x8, 2 ** 40, np.int16), (f32, 2, np.int64), (x8, 3, np.float32),
+                        (f32, True, np.float32), (u8, 2 ** 70, np.int64)):
+        rn = np.dtype(req).name
+        for op in ("add", "multiply"):
+            f = getattr(np, op)
+            S.add(f"dtype/{op}/weak/{arr.dtype.name}/{v!r}/{rn}", "binary", op, [arr, PyScalar(v)],
+                  lambda: f(arr, v, dtype=req), {"dtype": rn})
+
+
+def where3_cases(S):
+    """where with three-way broadcasting at rank 3 and 4: size-1 axes spread
+    over condition, x and y, F-ordered and stepped / reversed operands, an
+    integer condition (truthiness), mixed dtypes.  Own generator (seed 93)."""
+    r = np.random.default_rng(93)
+    for k in range(24):
+        rank = 3 + k % 2
+        shape = [int(v) for v in r.integers(2, 9, rank)]
+        shapes = [list(shape) for _ in range(3)]
+        for ax in range(rank):
+            shapes[int(r.integers(0, 3))][ax] = 1
+        dx, dy = [(np.float64, np.float32), (np.int64, np.uint8), (np.float32, np.int32), (np.int8, np.float64),
+                  (np.bool_, np.int16), (np.float16, np.float32)][k % 6]
+        ci = r.integers(0, 3, shapes[0]).astype(np.int32 if k % 5 == 0 else np.int8)
+        c = ci if k % 5 == 0 else ci > 0
+        x = specials(dx, int(np.prod(shapes[1])), r).reshape(shapes[1]) if dx != np.float16 else \
+            (r.standard_normal(shapes[1]) * 10).astype(dx)
+        y = specials(dy, int(np.prod(shapes[2])) * 2, r).reshape(shapes[2][:-1] + [shapes[2][-1] * 2])
+        if k % 3 == 0:
+            c = np.asfortranarray(c)
+        if k % 4 == 1:
+            x = np.asfortranarray(x)[::-1]
+        ys = y[..., ::2] if k % 2 else y[..., 1::2][..., ::-1]
+        S.add(f"where3/{k}/rank{rank}", "where", "where", [c, x, ys], lambda: np.where(c, x, ys))
+
+
+def reduce_gap_cases(S):
+    """float16 where= reductions (the HALF loop per maximal True run of a
+    window: float32 accumulator stored back at the run's end;
+    reduce_axis_masked_f16_widen, ufunc.rs:7414), maximum / minimum.reduce
+    with a dtype= other than a float input's (NumPy's buffered cast: the
+    loop reads the cast values; dtype rules ufunc.rs:2080-2163), float16
+    into a float32 / float64 loop with a mask.  Own generator (seed 94)."""
+    r = np.random.default_rng(94)
+    h = np.float16
+    for shp, ax in (((20000,), 0), ((300, 50), 1), ((50, 300), 0), ((3, 9000), 1), ((9000, 3), 0),
+                    ((40, 30, 20), (0, 2)), ((17000,), None)):
+        for v in range(2):
+            x = (r.standard_normal(shp) * [1, 1e3][v]).astype(h)
+            if v:
+                x.reshape(-1)[[3, 50]] = [np.nan, -0.0]
+            xm = (1 + 0.002 * r.standard_normal(shp)).astype(h)
+            for p in (0.6, 0.05):
+                m = r.random(shp) < p
+                for op in ("add", "multiply", "maximum", "minimum"):
+                    xi = xm if op == "multiply" else x
+                    init = 1.5 if op in ("maximum", "minimum") else (-3.25 if v else None)
+                    kw = {} if init is None else {"initial": init}
+                    args = {"axis": list(ax) if isinstance(ax, tuple) else ax, "where": True, **kw}
+                    S.add(f"gap/half_where/{op}/{shp}/{v}/{p}", "reduce", op, [xi, m],
+                          lambda: getattr(np, op).reduce(xi, axis=ax, where=m, **kw), args)
+                    if xi.ndim > 1:
+                        S.add(f"gap/half_where/{op}/{shp}/{v}/{p}/T", "reduce", op, [xi.T, m.T],
+                              lambda: getattr(np, op).reduce(xi.T, axis=ax, where=m.T, **kw), args)
+    for shp, ax in (((20000,), 0), ((300, 50), 1), ((50, 300), 0), ((9000, 3), 0), ((40, 30, 20), (0, 2))):
+        for src in (np.float64, np.float32, np.float16, np.int32, np.uint8):
+            x = (r.standard_normal(shp) * 100).astype(src)
+            if np.dtype(src).kind == "f":
+                x.reshape(-1)[r.integers(0, x.size, 3)] = [np.nan, -0.0, 0.0]
+            axv = list(ax) if isinstance(ax, tuple) else ax
+            for dt in (np.float32, np.float64):
+                if dt == src:
+                    continue
+                dn = np.dtype(dt).name
+                for op in ("maximum", "minimum") + (("add",) if src == np.float16 else ()):
+                    for vn, xv in (("C", x), ("T", x.T if x.ndim > 1 else x[::-2])):
+                        S.add(f"gap/dtype/{op}/{np.dtype(src).name}->{dn}/{shp}/{vn}", "reduce", op, [xv],
+                              lambda: getattr(np, op).reduce(xv, axis=ax, dtype=dt), {"axis": axv, "dtype": dn})
+                        m = r.random(xv.shape) < 0.7
+                        init = {"maximum": -1e9, "minimum": 1e9, "add": None}[op]
+                        kw = {} if init is None else {"initial": init}
+                        S.add(f"gap/dtype_where/{op}/{np.dtype(src).name}->{dn}/{shp}/{vn}", "reduce", op, [xv, m],
+                              lambda: getattr(np, op).reduce(xv, axis=ax, dtype=dt, where=m, **kw),
+                              {"axis": axv, "dtype": dn, "where": True, **kw})
+    # float input into an integer / bool loop: the reduce family casts
+    # unsafely by default (NumPy's C cast per element; out-of-range values
+    # take the x86 conversion's result)
+    for src in (np.float64, np.float32, np.float16):
+        x = (r.standard_normal((40, 300)) * 100).astype(src)
+        x[3, 5] = 1e10 if src != np.float16 else 6e4
+        x[4, 7] = -1e10 if src != np.float16 else -6e4
+        x[5, 9] = -0.7
+        sn = np.dtype(src).name
+        for dt in (np.int64, np.int32, np.int8, np.uint8, np.uint64, np.bool_):
+            dn = np.dtype(dt).name
+            for ax in (0, 1, None):
+                for op in ("add", "maximum", "multiply"):
+                    S.add(f"gap/f2i/{op}/{sn}->{dn}/{ax}", "reduce", op, [x],
+                          lambda: getattr(np, op).reduce(x, axis=ax, dtype=dt), {"axis": ax, "dtype": dn})
+                m = r.random(x.shape) < 0.5
+                S.add(f"gap/f2i_where/add/{sn}->{dn}/{ax}", "reduce", "add", [x, m],
+                      lambda: np.add.reduce(x, axis=ax, dtype=dt, where=m), {"axis": ax, "dtype": dn, "where": True})
+            for ax in (0, 1):
+                S.add(f"gap/f2i_acc/add/{sn}->{dn}/{ax}", "accumulate", "add", [x],
+                      lambda: np.add.accumulate(x, axis=ax, dtype=dt), {"axis": ax, "dtype": dn})
+                S.add(f"gap/f2i_rat/add/{sn}->{dn}/{ax}", "reduceat", "add", [x],
+                      lambda: np.add.reduceat(x, [0, 5, 2, 20], axis=ax, dtype=dt),
+                      {"axis": ax, "indices": [0, 5, 2, 20], "dtype": dn})
 
 
 def sha(a):
@@ -1065,6 +1276,10 @@ def main():
     reduceat_cases(S)
     fpe_cases(S)
     half_cases(S)
+    weak_scalar_cases(S)
+    dtype_arg_cases(S)
+    where3_cases(S)
+    reduce_gap_cases(S)
     np.savez_compressed(os.path.join(HERE, "fixtures.npz"), **S.arrays)
     with open(os.path.join(HERE, "cases.json"), "w") as f:
         json.dump(S.cases, f)

@@ -74,6 +74,41 @@ def test_uneven_batch_full_size_matches_oracle(ap):
             assert not bad, f"{mode}: {len(bad)} calls differ from the oracle, first {bad[:5]}"
 
 
+def test_float16_calls_in_a_batch(ap):
+    """float16 calls mixed into a batch (single-call launches beside the
+    persistent kernel): every mode, every byte against the oracle."""
+    from anionpy_amd.batch import Batch, binary_call, reduce_call, where_call
+    rng = np.random.default_rng(21)
+    h = lambda *s: rng.standard_normal(s).astype(np.float16) * 10  # noqa: E731
+    xs = [h(300, 70), h(70), h(5000), h(40, 9000)]
+    specs = [binary_call("add", xs[0], xs[1]), binary_call("multiply", xs[0], rng.standard_normal((300, 1))),
+             binary_call("less", xs[1], xs[1][::-1]), reduce_call(xs[2], axis=0), reduce_call(xs[3], axis=1),
+             reduce_call(xs[3], axis=0, dtype=np.float32),
+             where_call(rng.random((300, 70)) < 0.5, xs[0], np.float16(2.5)),
+             binary_call("add", rng.standard_normal((64, 64)), rng.standard_normal(64))]
+    with Batch(specs) as bt:
+        ref = None
+        for mode in ("per_call", "persistent", "graph", "persistent"):
+            for s in specs:
+                s.out._buf.fill_(0xAB)
+            bt.run(mode)
+            torch.cuda.synchronize()
+            got = [s.out.get().copy() for s in specs]
+            if ref is None:
+                ref = got
+            assert all(a.tobytes() == b.tobytes() for a, b in zip(ref, got)), mode
+    for i, s in enumerate(specs):
+        ins = [x.get() for x in s.inputs]
+        if s.kind == ap._lib.CALL_BINARY:
+            want = O.binary({0: "add", 2: "multiply", 6: "less"}[s.op], ins[0], ins[1])
+        elif s.kind == ap._lib.CALL_WHERE:
+            want = O.where(*ins)
+        else:
+            x = ins[0] if s.out.dtype == np.float16 else O.cast(ins[0], s.out.dtype)
+            want = O.reduce("add", x, s.rec.axes[0], dtype=s.out.dtype)
+        assert np.asarray(ref[i]).tobytes() == np.ascontiguousarray(want).tobytes(), i
+
+
 def test_batch_rejects_bad_calls(ap):
     from anionpy_amd.batch import Batch, reduce_call
     x = ap.asarray(np.zeros((4, 5)))

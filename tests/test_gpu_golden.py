@@ -3,18 +3,21 @@ import numpy as np
 import pytest
 import torch
 
-from fixture_cases import args_axis, check_result, host_input, load
+from fixture_cases import args_axis, check_result, host_input, load, py_scalar
 
 pytestmark = pytest.mark.gpu
 
 KINDS = ["binary", "where", "cast", "reduce", "argext", "accumulate", "reduceat", "fpe"]
 
 EXC = {"ValueError": ValueError, "TypeError": TypeError, "IndexError": IndexError,
-       "AxisError": ValueError, "UFuncTypeError": TypeError}
+       "AxisError": np.exceptions.AxisError, "UFuncTypeError": TypeError, "OverflowError": OverflowError}
 
 
 def device_input(ap, spec, npz, cache):
-    """Rebuild the exact operand view on the device (same offset/strides)."""
+    """Rebuild the exact operand view on the device (same offset/strides);
+    a Python scalar operand stays the scalar."""
+    if "py" in spec:
+        return py_scalar(spec)
     if "gen" in spec:
         return ap.asarray(host_input(spec, npz))
     buf = npz[spec["buf"]]
@@ -38,7 +41,13 @@ def run_device(ap, case, npz, cache):
     ins = [device_input(ap, s, npz, cache) for s in case["inputs"]]
     k, op, args = case["kind"], case["op"], case["args"]
     if k == "binary":
-        return getattr(ap, op)(ins[0], ins[1])
+        kw = {}
+        if "dtype" in args:
+            kw["dtype"] = np.dtype(args["dtype"])
+        if "casting" in args:
+            kw["casting"] = args["casting"]
+        r = getattr(ap, op)(ins[0], ins[1], **kw)
+        return r
     if k == "where":
         return ap.where(*ins)
     if k == "cast":
@@ -59,7 +68,7 @@ def run_device(ap, case, npz, cache):
             r |= f
         return _Host(np.array([r], dtype=np.int64))
     if k == "reduceat":
-        return getattr(ap, op).reduceat(ins[0], args["indices"], axis=args["axis"])
+        return getattr(ap, op).reduceat(ins[0], args["indices"], axis=args["axis"], dtype=args.get("dtype"))
     if k == "accumulate":
         if op in ("cumsum", "cumprod"):
             return getattr(ap, op)(ins[0], axis=args.get("axis"))
@@ -89,9 +98,8 @@ def test_golden_on_gpu(ap, kind, tracked):
                 got = run_device(ap, c, npz, cache)
         except Exception as e:  # noqa: BLE001
             if "error" in exp and isinstance(e, EXC.get(exp["error"], Exception)):
-                if exp["error"] != "AxisError" and not exp["msg"].startswith(str(e)[:20]) and \
-                        "boolean subtract" not in exp["msg"]:
-                    pass  # same exception type; message checked where NumPy's is reproduced
+                if str(e) != exp["msg"]:
+                    bad.append((c["name"], f"message {str(e)!r} != NumPy's {exp['msg']!r}"))
                 continue
             bad.append((c["name"], f"raised {type(e).__name__}: {e}"))
             continue

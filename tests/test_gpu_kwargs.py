@@ -82,12 +82,11 @@ def test_reduce_out_overlapping_input(ap):
     _same_bytes(X.get(), want)
 
 
-def test_float16_masked_reduce_not_provided(ap):
-    """float16 where= reductions are reported as not provided (the caller's
-    CPU path), never computed wrongly; the unmasked float16 reduce works."""
+def test_float16_masked_reduce_matches_oracle(ap):
+    """float16 where= reductions (the HALF loop per True run of a window),
+    against the oracle, masked and unmasked."""
     x = np.random.default_rng(7).standard_normal((30, 40)).astype(np.float16)
     m = np.random.default_rng(8).random((30, 40)) < 0.5
     X = ap.asarray(x)
-    with pytest.raises(NotImplementedError, match="float16 where"):
-        ap.add.reduce(X, axis=1, where=ap.asarray(m))
+    _same_bytes(np.asarray(ap.add.reduce(X, axis=1, where=ap.asarray(m)).get()), O.reduce("add", x, 1, where=m))
     _same_bytes(np.asarray(ap.add.reduce(X, axis=1).get()), O.reduce("add", x, 1))
