@@ -585,7 +585,143 @@ template <typename T, int OPK> __global__ __launch_bounds__(256) void ew_tiled(c
     }
 }
 
+// The vector form: X contiguous (x_st0 == 1) or broadcast (x_st0 == 0)
+// along d0, every operand row 16-byte aligned.  A TJ (d0) x TI (d*) tile:
+// Y is read in 16-byte vectors along d* (rows of TI elements) into LDS, X
+// and the output in 16-byte vectors along d0 (rows of TJ elements), all
+// nontemporal (each byte is touched once).  Wide rows give HBM long bursts
+// on all three streams: measured on MI355X (8192 x 8192, out = x + y.T),
+// 4-byte elements with 128 x 128 tiles 130-136 us (5.9-6.1 TB/s), 8-byte
+// elements with 32 (d*) x 128 (d0) tiles 270-278 us (5.8 TB/s), where the
+// scalar 64 x 64 form takes 246 / 389 us.  Partial tiles at the edges take
+// an element-by-element path.
+// XB: X is broadcast along d0 (a template parameter: a runtime branch
+// between the loads splits their batch, measured 10 % slower).
+template <typename T, int OPK, int TI, int TJ, bool XB>
+__global__ __launch_bounds__(256) void ew_tiled_v(const TileArgs P) {
+    constexpr int V = 16 / sizeof(T);
+    using VT = T __attribute__((ext_vector_type(V)));
+    using OT = typename std::conditional<(OPK >= IONP_LESS), uint8_t, T>::type;
+    using OV = OT __attribute__((ext_vector_type(V)));
+    constexpr int YV = TJ * TI / V / 256, XV = TI * TJ / V / 256;  // vectors per thread
+    static_assert(YV >= 1 && XV >= 1 && TJ * TI % (256 * V) == 0, "tile shape");
+    __shared__ T tile[TJ][TI + 1];  // [d0][d*], padded
+    const T *__restrict__ x = (const T *)P.x;
+    const T *__restrict__ y = (const T *)P.y;
+    OT *__restrict__ o = (OT *)P.out;
+    for (int64_t t = blockIdx.x; t < P.ntiles; t += gridDim.x) {
+        // t = (batch, tile along d*, tile along d0), d0 fastest
+        uint32_t rem = (uint32_t)t;
+        const uint32_t q0 = P.t0div.div(rem);
+        const int64_t j0 = (int64_t)(rem - q0 * (uint32_t)P.t0) * TJ;  // along d0
+        rem = q0;
+        const uint32_t q1 = P.t1div.div(rem);
+        const int64_t i0 = (int64_t)(rem - q1 * (uint32_t)P.t1) * TI;  // along d*
+        rem = q1;
+        int64_t bx = 0, by = 0, bo = 0;
+        for (int d = 0; d < P.nb; d++) {
+            uint32_t c;
+            if (d + 1 < P.nb) {
+                const uint32_t q = P.bdiv[d].div(rem);
+                c = rem - q * (uint32_t)P.bshape[d];
+                rem = q;
+            } else {
+                c = rem;
+            }
+            bx += (int64_t)c * P.bx[d];
+            by += (int64_t)c * P.by[d];
+            bo += (int64_t)c * P.bo[d];
+        }
+        const int64_t m0 = min((int64_t)TJ, P.n0 - j0), m1 = min((int64_t)TI, P.n1 - i0);
+        if (m0 < TJ || m1 < TI) {  // an edge tile: element by element, straight from memory
+            for (int e = threadIdx.x; e < TI * TJ; e += blockDim.x) {
+                const int i = e / TJ, j = e % TJ;
+                if (i >= m1 || j >= m0) continue;
+                const T xv = x[bx + (j0 + j) * P.x_st0 + (i0 + i) * P.x_st1];
+                const T yv = y[by + (j0 + j) * P.y_st0 + (i0 + i)];
+                o[bo + (j0 + j) + (i0 + i) * P.o_st1] = P.y_first ? row_apply<T, OPK>(yv, xv) : row_apply<T, OPK>(xv, yv);
+            }
+            continue;  // no LDS use: no barrier needed
+        }
+        VT yv[YV], xv[XV];
+#pragma unroll
+        for (int k = 0; k < YV; k++) {
+            const int idx = threadIdx.x + 256 * k, j = idx / (TI / V), iv = idx % (TI / V);
+            yv[k] = __builtin_nontemporal_load((const VT *)(y + by + (j0 + j) * P.y_st0 + i0 + iv * V));
+        }
+#pragma unroll
+        for (int k = 0; k < XV; k++) {
+            const int idx = threadIdx.x + 256 * k, i = idx / (TJ / V), jv = idx % (TJ / V);
+            const T *px = x + bx + (i0 + i) * P.x_st1;
+            if constexpr (!XB) {
+                xv[k] = __builtin_nontemporal_load((const VT *)(px + j0 + jv * V));
+            } else {
+                const T v = *px;  // broadcast along d0
+#pragma unroll
+                for (int e = 0; e < V; e++) xv[k][e] = v;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < YV; k++) {
+            const int idx = threadIdx.x + 256 * k, j = idx / (TI / V), iv = idx % (TI / V);
+#pragma unroll
+            for (int e = 0; e < V; e++) tile[j][iv * V + e] = yv[k][e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < XV; k++) {
+            const int idx = threadIdx.x + 256 * k, i = idx / (TJ / V), jv = idx % (TJ / V);
+            OV r;
+#pragma unroll
+            for (int e = 0; e < V; e++) {
+                const T a = xv[k][e], b = tile[jv * V + e][i];
+                r[e] = P.y_first ? row_apply<T, OPK>(b, a) : row_apply<T, OPK>(a, b);
+            }
+            __builtin_nontemporal_store(r, (OV *)(o + bo + (i0 + i) * P.o_st1 + j0 + jv * V));
+        }
+        __syncthreads();
+    }
+}
+
 typedef void (*TileFn)(const TileArgs);
+
+// 4-byte elements: 128 x 128 tiles; 8-byte: 32 (d*) x 128 (d0)
+template <typename T> constexpr int tile_ti() { return sizeof(T) == 8 ? 32 : 128; }
+constexpr int kTileTJ = 128;
+
+template <typename T, bool XB> static TileFn tiled_v_t(int op) {
+    constexpr int TI = tile_ti<T>();
+#define IONP_TILEV_CASE(OPK) \
+    case OPK: return ew_tiled_v<T, OPK, TI, kTileTJ, XB>;
+    switch (op) {
+        IONP_TILEV_CASE(IONP_ADD)
+        IONP_TILEV_CASE(IONP_SUBTRACT)
+        IONP_TILEV_CASE(IONP_MULTIPLY)
+        IONP_TILEV_CASE(IONP_MAXIMUM)
+        IONP_TILEV_CASE(IONP_MINIMUM)
+        IONP_TILEV_CASE(IONP_LESS)
+        IONP_TILEV_CASE(IONP_LESS_EQUAL)
+        IONP_TILEV_CASE(IONP_GREATER)
+        IONP_TILEV_CASE(IONP_GREATER_EQUAL)
+        IONP_TILEV_CASE(IONP_EQUAL)
+        IONP_TILEV_CASE(IONP_NOT_EQUAL)
+        case IONP_TRUE_DIVIDE:
+            if constexpr (std::is_floating_point<T>::value) return ew_tiled_v<T, IONP_TRUE_DIVIDE, TI, kTileTJ, XB>;
+            else return nullptr;
+        default: return nullptr;
+    }
+#undef IONP_TILEV_CASE
+}
+
+static TileFn tiled_v_kernel(int dt, int op, bool xb) {
+    switch (dt) {
+        case IONP_INT32: return xb ? tiled_v_t<int32_t, true>(op) : tiled_v_t<int32_t, false>(op);
+        case IONP_INT64: return xb ? tiled_v_t<int64_t, true>(op) : tiled_v_t<int64_t, false>(op);
+        case IONP_FLOAT32: return xb ? tiled_v_t<float, true>(op) : tiled_v_t<float, false>(op);
+        case IONP_FLOAT64: return xb ? tiled_v_t<double, true>(op) : tiled_v_t<double, false>(op);
+        default: return nullptr;
+    }
+}
 
 template <typename T> static TileFn tiled_t(int op) {
 #define IONP_TILE_CASE(OPK) \
@@ -672,6 +808,32 @@ bool try_tiled(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_stat
     P.y = A.in[yk];
     P.out = A.out;
     P.y_first = yk == 0;
+    // the vector form: X contiguous or broadcast along d0 and every row (of
+    // every batch) 16-byte aligned, output rows aligned to one vector
+    const int64_t isz = itemsize(A.loop_dt), V = 16 / isz, osz = itemsize(A.out_dt);
+    auto al = [](const void *p, int64_t a) { return ((uintptr_t)p % (uintptr_t)a) == 0; };
+    bool vec = (P.x_st0 == 1 || P.x_st0 == 0) && al(P.y, 16) && (P.y_st0 * isz) % 16 == 0 && al(P.out, V * osz) &&
+               P.o_st1 % V == 0 && (P.x_st0 == 0 || (al(P.x, 16) && (P.x_st1 * isz) % 16 == 0));
+    for (int d = 0; vec && d < P.nb; d++)
+        vec = (P.by[d] * isz) % 16 == 0 && P.bo[d] % V == 0 && (P.x_st0 == 0 || (P.bx[d] * isz) % 16 == 0);
+    const TileFn vfn = vec ? tiled_v_kernel(A.loop_dt, A.op, P.x_st0 == 0) : nullptr;
+    if (vfn) {
+        const int ti = isz == 8 ? 32 : 128;
+        P.t0 = (P.n0 + kTileTJ - 1) / kTileTJ;
+        P.t1 = (P.n1 + ti - 1) / ti;
+        P.ntiles = P.t0 * P.t1 * nbatch;
+        if (P.ntiles >= (1ll << 31) || P.t0 >= (1ll << 31) || P.t1 >= (1ll << 31)) return false;
+        P.t0div = FastDiv((uint32_t)P.t0);
+        P.t1div = FastDiv((uint32_t)P.t1);
+        // measured (8192 x 8192): 2 workgroups per CU (4-byte: 66 KB of LDS
+        // each, 2 fit), 1-2 for 8-byte
+        const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 2;
+        int64_t grid = (int64_t)bpc * num_cus();
+        if (grid > P.ntiles) grid = P.ntiles;
+        hipLaunchKernelGGL(vfn, dim3((unsigned)grid), dim3(256), 0, s, P);
+        *st = hipGetLastError() == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, "kernel launch failed");
+        return true;
+    }
     // measured on MI355X (8192 x 8192, A + B.T): 4 workgroups per CU for
     // 8-byte elements, 16 for 4-byte ones (their LDS tile is half the size)
     const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : (itemsize(A.loop_dt) == 8 ? 4 : 16);

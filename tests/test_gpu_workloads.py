@@ -182,3 +182,28 @@ def test_streaming_sum_misaligned_views(ap):
             exp = np.asarray(O.reduce("add", R[sl], 1))
             got = ap.add.reduce(Rd[sl], axis=1).get()
             assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), (dt, sl)
+
+
+def test_transposed_operand_tiles(ap):
+    """One operand contiguous along another axis than the output's (A + B.T):
+    full 128 x 128 / 32 x 128 vector tiles plus partial edge tiles, both
+    operand orders, comparisons (bool output), a broadcast row, a 3-D batch
+    and a misaligned view (the scalar tile kernel), against the oracle."""
+    rng = np.random.default_rng(17)
+    for dt in (np.float32, np.float64, np.int32, np.int64):
+        kind = np.dtype(dt).kind
+        mk = (lambda s: rng.standard_normal(s).astype(dt)) if kind == "f" else \
+            (lambda s: rng.integers(-10 ** 6, 10 ** 6, s).astype(dt))
+        A, B = mk((300, 1000)), mk((1000, 300))
+        row = mk((1, 1000))
+        C, D = mk((3, 260, 129)), mk((3, 129, 260))
+        Ad, Bd, rowd, Cd, Dd = (ap.asarray(v) for v in (A, B, row, C, D))
+        cases = [("add", A, B.T, Ad, Bd.T), ("subtract", B.T, A, Bd.T, Ad), ("less", A, B.T, Ad, Bd.T),
+                 ("maximum", row, B.T, rowd, Bd.T), ("multiply", C, D.transpose(0, 2, 1), Cd, Dd.transpose(0, 2, 1)),
+                 ("add", A[:, 1:], B.T[:, 1:], Ad[:, 1:], Bd.T[:, 1:])]
+        if kind == "f":
+            cases.append(("true_divide", A, B.T, Ad, Bd.T))
+        for op, a, b, ad, bd in cases:
+            want = O.binary(op, a, b)
+            got = getattr(ap, op)(ad, bd).get()
+            assert got.strides == want.strides and got.tobytes() == want.tobytes(), (dt, op, a.shape)
