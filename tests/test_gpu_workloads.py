@@ -207,3 +207,24 @@ def test_transposed_operand_tiles(ap):
             want = O.binary(op, a, b)
             got = getattr(ap, op)(ad, bd).get()
             assert got.strides == want.strides and got.tobytes() == want.tobytes(), (dt, op, a.shape)
+
+
+def test_blocked_accumulate(ap):
+    """Scans over many fibers (the blocked kernel): along the contiguous axis
+    and along the outer axis of C-ordered arrays, extents that are not
+    multiples of the block shape, a NaN (the fiber redone with x86 NaN
+    rules), integer wrap, against the oracle."""
+    rng = np.random.default_rng(18)
+    for shape in ((3000, 2100), (2100, 3000), (4, 700, 530)):
+        for dt in (np.float64, np.float32, np.int64, np.int32):
+            if np.dtype(dt).kind == "f":
+                h = rng.standard_normal(shape).astype(dt)
+                h.reshape(-1)[12345] = np.nan
+            else:
+                h = rng.integers(-2 ** 30, 2 ** 30, shape).astype(dt)
+            d = ap.asarray(h)
+            for ax in range(h.ndim):
+                for op in ("add", "maximum") + (("multiply",) if np.dtype(dt).kind == "i" else ()):
+                    want = O.accumulate(op, h, ax)
+                    got = getattr(ap, op).accumulate(d, axis=ax).get()
+                    assert got.tobytes() == want.tobytes(), (shape, dt, ax, op)
