@@ -38,6 +38,11 @@ class Config(ctypes.Structure):
                 ("blocks_per_cu", ctypes.c_int32), ("split", ctypes.c_int32)]
 
 
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("ew_tile", ctypes.c_int64), ("int_seg", ctypes.c_int64), ("claim", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
 class Call(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("op", ctypes.c_int32), ("naxes", ctypes.c_int32),
                 ("keepdims", ctypes.c_int32), ("axes", ctypes.c_int32 * MAX_DIMS),
@@ -93,6 +98,7 @@ SIGNATURES = {
     "ionp_reduceat": [_I, _D, _P(ctypes.c_int64), ctypes.c_int64, _I, _D, _C, _V],
     "ionp_accumulate": [_I, _D, _I, _D, _C, _V],
     "ionp_batch_create": [_P(Call), _I, _V, _P(_V)],
+    "ionp_batch_create_ex": [_P(Call), _I, _P(BatchConfig), _V, _P(_V)],
     "ionp_batch_run": [_V, _I, _V],
     "ionp_batch_info": [_V, _P(_L), _P(_L)],
     "ionp_batch_destroy": [_V],

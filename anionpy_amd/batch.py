@@ -115,11 +115,16 @@ def call_array(specs):
 class Batch:
     """A planned batch; `run(mode)` executes it on torch's current stream."""
 
-    def __init__(self, specs):
+    def __init__(self, specs, ew_tile=0, int_seg=0, claim=0):
+        """ew_tile / int_seg / claim: tile sizes and tiles per claim of the
+        persistent kernel (0: the library's defaults; see ionp_batch_config)."""
         self.specs = list(specs)
         self._arr = call_array(self.specs)
         self._h = ctypes.c_void_p()
-        _call(_lib.load().ionp_batch_create, self._arr, len(self.specs), _lib.stream_handle(), ctypes.byref(self._h))
+        cfg = _lib.BatchConfig()
+        cfg.ew_tile, cfg.int_seg, cfg.claim = int(ew_tile), int(int_seg), int(claim)
+        _call(_lib.load().ionp_batch_create_ex, self._arr, len(self.specs), ctypes.byref(cfg), _lib.stream_handle(),
+              ctypes.byref(self._h))
 
     def run(self, mode="persistent"):
         _call(_lib.load().ionp_batch_run, self._h, MODES[mode], _lib.stream_handle())

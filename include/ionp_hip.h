@@ -338,12 +338,25 @@ typedef enum {
     IONP_BATCH_PER_CALL = 2    /* one stream launch sequence per call */
 } ionp_batch_mode;
 
+/* Tile sizes of a batch's persistent kernel (zero fields: the defaults).
+ * Like ionp_config they move work between workgroups only: results are
+ * bit-identical whatever is chosen. */
+typedef struct {
+    int64_t ew_tile; /* elements per elementwise tile: a multiple of 1024 (default 16384) */
+    int64_t int_seg; /* elements per integer-reduce segment (default 8192) */
+    int32_t claim;   /* consecutive tiles a workgroup claims at once (1..64, default 4) */
+    int32_t reserved;
+} ionp_batch_config;
+
 /* Validate and plan `n` calls, pack their descriptors and tile table into
  * device memory.  The calls' device buffers must outlive the batch.  Calls
  * are independent: no call's output may overlap another call's operands.
  * In PERSISTENT mode float16 calls run as their own single-call launches,
  * in stream order ahead of the persistent kernel. */
 ionp_status ionp_batch_create(const ionp_call *calls, int32_t n, void *stream, ionp_batch **out);
+/* ionp_batch_create with tile sizes (null: the defaults). */
+ionp_status ionp_batch_create_ex(const ionp_call *calls, int32_t n, const ionp_batch_config *config, void *stream,
+                                 ionp_batch **out);
 ionp_status ionp_batch_run(ionp_batch *batch, int32_t mode, void *stream);
 /* number of tiles / bytes moved (algorithmic) by one run */
 ionp_status ionp_batch_info(const ionp_batch *batch, int64_t *ntiles, int64_t *bytes);
