@@ -34,6 +34,7 @@ enum : int32_t {
     BK_IADD_WAVE = 8,    // integer / bool add.reduce, groups of 65 .. kIntWaveMax: one wavefront per output
     BK_SIDE = 9,         // float16 call: its own single-call launches, ahead of the persistent kernel
     BK_EW_FLAT = 10,     // binary / where / cast over one contiguous, aligned dim: nontemporal vectors
+    BK_EW_SIZED = 11,    // binary / where, inputs of 4- / 8-byte items, 31-bit offsets (sized_ok)
 };
 
 constexpr int kBatchThreads = 256;
@@ -97,8 +98,8 @@ __device__ inline void load_k(int dt, const char *p, const int64_t (&off)[K], ty
     });
 }
 
-template <typename T, int K>
-__device__ inline void store_k(int dt, char *p, const int64_t (&off)[K], const T (&v)[K], const bool (&ok)[K]) {
+template <typename T, int K, typename XT>
+__device__ inline void store_k(int dt, char *p, const XT (&off)[K], const T (&v)[K], const bool (&ok)[K]) {
     auto st = [&](auto ot, auto cv) {
         using OT = decltype(ot);
 #pragma unroll
@@ -241,6 +242,146 @@ __device__ void ew_tile(const EwArgs &A, int64_t t, int64_t tile) {
         case IONP_INT32: ew_range<IONP_INT32, 0>(A, e0, e1); break;
         case IONP_UINT32: ew_range<IONP_UINT32, 0>(A, e0, e1); break;
         default: ew_range<IONP_UINT64, 0>(A, e0, e1); break;
+    }
+}
+
+// ---- the sized tile path ----------------------------------------------------
+// For calls whose inputs have 4- or 8-byte items (where's condition: bool)
+// and whose offsets fit in 31 bits (sized_ok).  The item sizes are template
+// arguments, so each operand's K loads are plain fixed-width loads issued
+// back to back for every operand, and the dtype conversion runs after all of
+// them.  (In ew_range each operand's loads sit in a dtype switch with their
+// conversion, which makes the next operand's loads wait for that data.)
+template <int S> struct uint_sz;
+template <> struct uint_sz<1> { using T = uint8_t; };
+template <> struct uint_sz<4> { using T = uint32_t; };
+template <> struct uint_sz<8> { using T = uint64_t; };
+
+// K raw items (unsigned U of the dtype's size) of dtype dt converted to the
+// loop dtype LT as ld_t converts them.  dt's item size must be sizeof(U)
+// (bool: U is uint8_t).
+template <int LT, int K, typename U>
+__device__ inline void cvt_raw_k(int dt, const U (&r)[K], typename dt_type<LT>::T (&v)[K]) {
+    in_type_dispatch(dt, [&](auto it) {
+        using IT = decltype(it);
+        if constexpr (std::is_same<IT, bool>::value) {
+#pragma unroll
+            for (int k = 0; k < K; k++) v[k] = cvt<LT>((uint8_t)(r[k] != 0));
+        } else if constexpr (sizeof(IT) == sizeof(U)) {
+#pragma unroll
+            for (int k = 0; k < K; k++) v[k] = cvt<LT>(__builtin_bit_cast(IT, r[k]));
+        } else {
+            __builtin_unreachable();  // sized_ok admits only dtypes of U's size
+        }
+        return 0;
+    });
+}
+
+// Elements [e0, e1) as ew_range, for NIN inputs of item sizes SA, SB, SC
+// over exactly ND dims (the host pads the call's dims to 2 or 4).
+template <int LT, int NIN, int SA, int SB, int SC, int ND>
+__device__ void ew_range_sz(const EwArgs &A, uint32_t e0, uint32_t e1) {
+    using T = typename dt_type<LT>::T;
+    using UA = typename uint_sz<SA>::T;
+    using UB = typename uint_sz<(NIN > 1 ? SB : 1)>::T;
+    using UC = typename uint_sz<(NIN > 2 ? SC : 1)>::T;
+    constexpr int K = kRangeK;
+    for (uint32_t base = e0 + threadIdx.x; base < e1; base += K * blockDim.x) {
+        int32_t off[3][K], oo[K];
+        bool ok[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            uint32_t e = base + k * blockDim.x;
+            ok[k] = e < e1;
+            uint32_t rem = ok[k] ? e : e0;
+            int32_t o0 = 0, o1 = 0, o2 = 0, ou = 0;
+#pragma unroll
+            for (int d = 0; d < ND; d++) {
+                uint32_t c;
+                if (d + 1 < ND) {
+                    const uint32_t q = A.div[d].div(rem);
+                    c = rem - q * (uint32_t)A.shape[d];
+                    rem = q;
+                } else {
+                    c = rem;
+                }
+                o0 += (int32_t)c * (int32_t)A.in_st[0][d];
+                if constexpr (NIN > 1) o1 += (int32_t)c * (int32_t)A.in_st[1][d];
+                if constexpr (NIN > 2) o2 += (int32_t)c * (int32_t)A.in_st[2][d];
+                ou += (int32_t)c * (int32_t)A.out_st[d];
+            }
+            off[0][k] = o0;
+            off[1][k] = o1;
+            off[2][k] = o2;
+            oo[k] = ou;
+        }
+        UA ra[K];
+        UB rb[K];
+        UC rc[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) ra[k] = ((const UA *)A.in[0])[off[0][k]];
+        if constexpr (NIN > 1) {
+#pragma unroll
+            for (int k = 0; k < K; k++) rb[k] = ((const UB *)A.in[1])[off[1][k]];
+        }
+        if constexpr (NIN > 2) {
+#pragma unroll
+            for (int k = 0; k < K; k++) rc[k] = ((const UC *)A.in[2])[off[2][k]];
+        }
+        T r[K];
+        if constexpr (NIN == 1) {
+            cvt_raw_k<LT, K>(A.in_dt[0], ra, r);
+        } else if constexpr (NIN == 3) {
+            T x[K], y[K];
+            cvt_raw_k<LT, K>(A.in_dt[1], rb, x);
+            cvt_raw_k<LT, K>(A.in_dt[2], rc, y);
+#pragma unroll
+            for (int k = 0; k < K; k++) r[k] = ra[k] ? x[k] : y[k];
+        } else {
+            T a[K], b[K];
+            cvt_raw_k<LT, K>(A.in_dt[0], ra, a);
+            cvt_raw_k<LT, K>(A.in_dt[1], rb, b);
+            if (A.op >= IONP_LESS) {
+                uint8_t cr[K];
+#pragma unroll
+                for (int k = 0; k < K; k++) cr[k] = (uint8_t)cmp_op<T>(A.op, a[k], b[k]);
+                store_k<uint8_t, K>(A.out_dt, A.out, oo, cr, ok);
+                continue;
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++) r[k] = arith_op<LT>(A.op, a[k], b[k]);
+        }
+        store_k<T, K>(A.out_dt, A.out, oo, r, ok);
+    }
+}
+
+template <int LT, int ND> __device__ inline void ew_sized(const EwArgs &A, uint32_t e0, uint32_t e1) {
+    const bool a8 = itemsize(A.in_dt[0]) == 8, b8 = itemsize(A.in_dt[1]) == 8,
+               c8 = A.nin > 2 && itemsize(A.in_dt[2]) == 8;
+    if (A.nin == 2) {
+        if (a8 && b8) ew_range_sz<LT, 2, 8, 8, 0, ND>(A, e0, e1);
+        else if (a8) ew_range_sz<LT, 2, 8, 4, 0, ND>(A, e0, e1);
+        else if (b8) ew_range_sz<LT, 2, 4, 8, 0, ND>(A, e0, e1);
+        else ew_range_sz<LT, 2, 4, 4, 0, ND>(A, e0, e1);
+    } else {
+        if (b8 && c8) ew_range_sz<LT, 3, 1, 8, 8, ND>(A, e0, e1);
+        else if (b8) ew_range_sz<LT, 3, 1, 8, 4, ND>(A, e0, e1);
+        else if (c8) ew_range_sz<LT, 3, 1, 4, 8, ND>(A, e0, e1);
+        else ew_range_sz<LT, 3, 1, 4, 4, ND>(A, e0, e1);
+    }
+}
+
+template <int LT> __device__ inline void ew_sized_nd(const EwArgs &A, uint32_t e0, uint32_t e1) {
+    if (A.ndim == 2) ew_sized<LT, 2>(A, e0, e1);
+    else ew_sized<LT, 4>(A, e0, e1);
+}
+
+__device__ void ew_sized_tile(const EwArgs &A, int64_t t, int64_t tile) {
+    const uint32_t e0 = (uint32_t)(t * tile), e1 = (uint32_t)min(A.total, t * tile + tile);
+    switch (A.loop_dt) {
+        case IONP_FLOAT64: ew_sized_nd<IONP_FLOAT64>(A, e0, e1); break;
+        case IONP_FLOAT32: ew_sized_nd<IONP_FLOAT32>(A, e0, e1); break;
+        default: ew_sized_nd<IONP_INT64>(A, e0, e1); break;
     }
 }
 
@@ -627,6 +768,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_persistent(
         switch (C.kind) {
             case BK_ELEMENTWISE: ew_tile(C.ew, lt, C.tile); break;
             case BK_EW_FLAT: ew_flat_tile(C.ew, lt, C.tile); break;
+            case BK_EW_SIZED: ew_sized_tile(C.ew, lt, C.tile); break;
             case BK_FADD_WIDE:
                 if (C.red.out_dt == IONP_FLOAT64) fadd_window<double>(C, partials, lt, S);
                 else fadd_window<float>(C, partials, lt, S);
@@ -695,6 +837,37 @@ static bool vec_ok(const EwArgs &E) {
     return true;
 }
 
+// The sized tile path applies: a binary call or where, loop dtype float64 /
+// float32 / int64, inputs of 4- or 8-byte items (where's condition bool), at
+// most 4 dims, fewer than 2^31 elements and every operand's element offsets
+// in (-2^31, 2^31).
+static bool sized_ok(const EwArgs &E) {
+    if (E.loop_dt != IONP_FLOAT64 && E.loop_dt != IONP_FLOAT32 && E.loop_dt != IONP_INT64) return false;
+    if (E.nin < 2 || E.ndim < 1 || E.ndim > 4 || E.mixed_su || E.total >= (int64_t(1) << 31)) return false;
+    for (int k = 0; k < E.nin; k++) {
+        const int sz = itemsize(E.in_dt[k]);
+        if (E.op == OP_WHERE && k == 0 ? E.in_dt[0] != IONP_BOOL : (sz != 4 && sz != 8)) return false;
+    }
+    for (int k = 0; k <= E.nin; k++) {
+        const int64_t *st = k < E.nin ? E.in_st[k] : E.out_st;
+        int64_t span = 0;
+        for (int d = 0; d < E.ndim; d++) span += (E.shape[d] - 1) * (st[d] < 0 ? -st[d] : st[d]);
+        if (span >= (int64_t(1) << 31)) return false;
+    }
+    return true;
+}
+
+// Trailing size-1 dims (stride 0) up to nd: the same iteration space.
+static void pad_dims(EwArgs &E, int nd) {
+    for (int d = E.ndim; d < nd; d++) {
+        E.shape[d] = 1;
+        E.div[d] = FastDiv(1);
+        for (int k = 0; k < 3; k++) E.in_st[k][d] = 0;
+        E.out_st[d] = 0;
+    }
+    E.ndim = nd;
+}
+
 ionp_status plan_call(const ionp_call &c, const ionp_batch_config &cfg, BatchCall *B, int64_t *bytes) {
     std::memset(B, 0, sizeof *B);
     const int nops = c.kind == IONP_CALL_WHERE ? 3 : (c.kind == IONP_CALL_BINARY ? 2 : 1);
@@ -747,7 +920,8 @@ ionp_status plan_call_tiles(const ionp_call &c, const ionp_batch_config &cfg, Ba
         // other calls (broadcast operands) keep the generic tiles: 4-element
         // vectors along their rows measured slower on the uneven batch
         // (1.9-2.2 against 1.8 ms for that subset)
-        B->kind = vec_ok(B->ew) && flat_ok(B->ew) ? BK_EW_FLAT : BK_ELEMENTWISE;
+        B->kind = vec_ok(B->ew) && flat_ok(B->ew) ? BK_EW_FLAT : (sized_ok(B->ew) ? BK_EW_SIZED : BK_ELEMENTWISE);
+        if (B->kind == BK_EW_SIZED) pad_dims(B->ew, B->ew.ndim <= 2 ? 2 : 4);
         B->tile = cfg.ew_tile;
         B->tiles = (B->ew.total + B->tile - 1) / B->tile;
         *bytes = B->ew.total * itemsize(c.out.dtype);
