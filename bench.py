@@ -12,9 +12,11 @@ once, the output written once) divided by the step time, inputs resident in
 HBM.  One JSON line is printed; `roofline` covers the dominant kernel (the
 float64 multiply), timed with HIP events on the stream it runs on, and
 `cpu_baseline` times the scalar C oracle on the same four operations, on one
-core and (OpenMP) on every host core.  `extra` carries the reduction workloads (float64
-add.reduce of 2^28 elements, float32 (4096,65536) sums, maximum/argmax, also
-with NaN at rate 1e-6).
+core and (OpenMP) on every host core.  `extra` carries the other elementwise
+and scan workloads (float64 broadcast add, the float64 view, A + B.T,
+cumsum), the reduction workloads (float64 add.reduce of 2^28 elements,
+float32 (4096,65536) sums, maximum/argmax, also with NaN at rate 1e-6), the
+uneven batch and the learned configuration.
 """
 import argparse
 import json
@@ -38,6 +40,9 @@ def parse():
     p.add_argument("--reductions", action="store_true",
                    help="only run the reduction workloads and print their timings (the workload of the "
                         "reduction kernel profiles)")
+    p.add_argument("--kernels", action="store_true",
+                   help="only run the elementwise and scan workloads beside the step (broadcast add, "
+                        "X.T[::2] view, A + B.T, cumsum) and print their timings")
     p.add_argument("--roofline-pmc", action="store_true",
                    help="only run the dominant kernel and a plain copy of known size, 5 times each "
                         "(the workload of the rocprofv3 FETCH_SIZE / WRITE_SIZE passes)")
@@ -77,6 +82,21 @@ def main():
     args = parse()
     if args.roofline_pmc:
         roofline_pmc_workload()
+        return
+    if args.kernels:
+        import torch
+        import anionpy_amd as ap
+        torch.cuda.set_device(0)
+        ap.seterr(all="ignore")
+        res = elementwise_extras(ap, torch)
+        # calibration for counter passes: torch's device copy of 2^26
+        # float64 (536,870,912 B read and written), 5 times
+        src = torch.ones(2 ** 26, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps(res), flush=True)
         return
     if args.reductions:
         import torch
@@ -170,6 +190,7 @@ def main():
     del X64, c64, X32, c32, I, j, V32, v32, outs
     torch.cuda.empty_cache()
     if not args.no_extra:
+        extra.update(elementwise_extras(ap, torch))
         extra.update(reduction_extras(ap, torch))
         extra["batch"] = batch_extras(ap, torch)
         extra["learned_config"] = learned_extras(torch)
@@ -198,6 +219,46 @@ def timed(torch, fn, reps):
     e1.record(s)
     e1.synchronize()
     return e0.elapsed_time(e1) / reps * 1e-3
+
+
+def _rate(nbytes, t):
+    return {"us": round(t * 1e6, 1), "GB/s": round(nbytes / t / 1e9, 1), "frac": round(nbytes / t / PEAK_HBM, 4)}
+
+
+def elementwise_extras(ap, torch):
+    """Elementwise and scan workloads beside the step: float64 broadcast add
+    (row and column vector), the float64 X.T[::2] view, A + B.T (the
+    LDS-transposed operand) in float32 and float64, and float64 cumsum along
+    either axis, all 8192 x 8192."""
+    log("elementwise and scan workloads")
+    out = {}
+    A = ap.asarray(np.random.default_rng(0).uniform(-1, 1, (8192, 8192)))
+    row = ap.asarray(np.random.default_rng(9).uniform(-1, 1, 8192))
+    col = ap.asarray(np.random.default_rng(10).uniform(-1, 1, (8192, 1)))
+    for key, b in (("add_f64_row", row), ("add_f64_col", col)):
+        o = ap.add(A, b)
+        out[key] = _rate(A.nbytes + b.nbytes + o.nbytes, timed(torch, lambda: ap.add(A, b, out=o), 10))
+        del o
+    V, v = A.T[::2], col[::2]
+    o = ap.multiply(V, v)
+    out["view_f64"] = _rate(V.nbytes + v.nbytes + o.nbytes, timed(torch, lambda: ap.multiply(V, v, out=o), 10))
+    del o, V, v
+    for dt in (np.float32, np.float64):
+        Ad = A if dt == np.float64 else ap.asarray(np.random.default_rng(0).uniform(-1, 1, (8192, 8192)).astype(dt))
+        Bd = ap.asarray(np.random.default_rng(11).uniform(-1, 1, (8192, 8192)).astype(dt))
+        BT = Bd.T
+        o = ap.add(Ad, BT)
+        out[f"add_T_{np.dtype(dt).name}"] = _rate(Ad.nbytes + BT.nbytes + o.nbytes,
+                                                  timed(torch, lambda: ap.add(Ad, BT, out=o), 10))
+        del o, Bd, BT, Ad
+    for ax in (0, 1):
+        o = ap.add.accumulate(A, axis=ax)
+        out[f"cumsum_f64_axis{ax}"] = _rate(A.nbytes + o.nbytes,
+                                            timed(torch, lambda: ap.add.accumulate(A, axis=ax, out=o), 10))
+        del o
+    del A, row, col
+    torch.cuda.empty_cache()
+    return out
 
 
 def reduction_extras(ap, torch):
