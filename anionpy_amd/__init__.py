@@ -656,40 +656,46 @@ def _finish(out):
 
 
 # ---- launch-configuration policy ---------------------------------------------
-_advisor = None
+_policy = None  # installed at the first call: "learned" when the model loads
 
 
 def set_config_policy(policy):
-    """"heuristic" (default): every call uses the built-in launch heuristic.
-    "learned": the cost model (anionpy_amd/costmodel.json) proposes a launch
-    configuration per call layout; results are bit-identical either way."""
-    global _advisor
+    """"learned" (the default): every call runs the launch configuration the
+    cost model (anionpy_amd/costmodel.json) proposes for its layout, served
+    by the library's advisor (ionp_advisor_set); a layout's first call runs
+    the built-in heuristic while its proposal is computed on the GPU.
+    "heuristic": every call runs the built-in heuristic.  Results are
+    bit-identical either way."""
+    global _policy
+    from . import costmodel
     if policy == "heuristic":
-        _advisor = None
+        costmodel.uninstall()
     elif policy == "learned":
-        from . import costmodel
-        model, _ = costmodel.load()
-        _advisor = costmodel.Advisor(model)
+        costmodel.install_file()
     else:
         raise ValueError(f"unknown configuration policy {policy!r}")
+    _policy = policy
 
 
 def get_config_policy():
-    return "heuristic" if _advisor is None else "learned"
+    if _policy is None:
+        _default_policy()
+    return _policy
+
+
+def _default_policy():
+    try:
+        set_config_policy("learned")
+    except (OSError, ValueError, KeyError, IonpError):  # no usable model: the heuristic
+        set_config_policy("heuristic")
 
 
 def _cfg(kind, op, ins, out, axes=None, naxes=0, keepdims=False):
-    """ctypes pointer to the configuration of this call (None: heuristic)."""
-    if _advisor is None:
-        return None
-    rec = _lib.Call()
-    rec.kind, rec.op, rec.naxes, rec.keepdims = kind, op, naxes, int(bool(keepdims))
-    for i in range(builtins.max(naxes, 0)):
-        rec.axes[i] = axes[i]
-    for i, d in enumerate(ins):
-        rec.inp[i] = d
-    rec.out = out
-    return ctypes.byref(_advisor.config(rec))
+    """Configuration argument of a call: None, so that the library applies
+    the installed policy (the advisor's proposal, else the heuristic)."""
+    if _policy is None:
+        _default_policy()
+    return None
 
 
 class Ufunc:

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libionp_hip.so")
 
 MAX_DIMS = 16
 AXIS_NONE = -1000000
-NFEATURES = 24
+NFEATURES = 32
 
 BOOL, INT8, UINT8, INT16, UINT16, INT32, UINT32, INT64, UINT64, FLOAT32, FLOAT64 = range(11)
 ADD, SUBTRACT, MULTIPLY, TRUE_DIVIDE, MAXIMUM, MINIMUM, LESS, LESS_EQUAL, GREATER, GREATER_EQUAL, EQUAL, NOT_EQUAL = range(12)
@@ -102,11 +102,17 @@ SIGNATURES = {
     "ionp_batch_run": [_V, _I, _V],
     "ionp_batch_info": [_V, _P(_L), _P(_L)],
     "ionp_batch_destroy": [_V],
+    "ionp_batch_autotune": [_P(Call), _I, _P(BatchConfig), _I, _I, _F, _V],
     "ionp_autotune": [_P(Call), _I, _C, _I, _I, _F, _V],
     "ionp_call_features": [_P(Call), _F],
+    "ionp_advisor_set": [_V, _I, _V, _V, _I, _V],
+    "ionp_advisor_prime": [_P(Call), _I, _V],
+    "ionp_advisor_stats": [_P(_L)],
+    "ionp_advisor_lookup": [_P(Call), _P(Config), _P(_I)],
     "ionp_mlp_linear_fwd": [_V, _V, _V, _V, _I, _I, _I, _I, _V],
     "ionp_mlp_linear_bwd": [_V, _V, _V, _V, _V, _V, _V, _I, _I, _I, _I, _V],
     "ionp_mlp_softmax_xent": [_V, _V, _V, _I, _I, _I, _V, _V, _V],
+    "ionp_mlp_softmax_xent_soft": [_V, _V, _V, _I, _I, _I, _V, _V, _V],
     "ionp_mlp_adam": [_V, _V, _V, _V, _L, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                       ctypes.c_float, _I, _V],
 }

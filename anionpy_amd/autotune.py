@@ -44,8 +44,15 @@ def config_index(labels, grid=GRID):
     return grid.index((b, v, p, s))
 
 
-def sweep(specs, reps=3, grid=GRID, chunk=256):
-    """times[i, j]: microseconds of call i under grid[j] (negative: unusable)."""
+def sweep(specs, reps=3, grid=GRID, chunk=256, refine=0):
+    """times[i, j]: microseconds of call i under grid[j] (negative: unusable).
+
+    refine=0: every configuration timed `reps` times (fastest kept).
+    refine=k: successive halving; every configuration is timed once, then
+    the k fastest of each call and the heuristic (entry 0) are timed `reps`
+    more times and keep their fastest sample; the others keep their single
+    timing."""
+    first_reps = 1 if refine else reps
     cfgs = config_array(grid)
     out = np.empty((len(specs), len(grid)), dtype=np.float32)
     L = _lib.load()
@@ -53,8 +60,19 @@ def sweep(specs, reps=3, grid=GRID, chunk=256):
         part = specs[s0:s0 + chunk]
         arr = call_array(part)
         t = (ctypes.c_float * (len(part) * len(grid)))()
-        _call(L.ionp_autotune, arr, len(part), cfgs, len(grid), reps, t, _lib.stream_handle())
+        _call(L.ionp_autotune, arr, len(part), cfgs, len(grid), first_reps, t, _lib.stream_handle())
         out[s0:s0 + len(part)] = np.frombuffer(t, dtype=np.float32).reshape(len(part), len(grid))
+    if refine:
+        for i, spec in enumerate(specs):
+            row = np.where(out[i] > 0, out[i], np.inf)
+            cand = sorted(set(np.argsort(row)[:refine].tolist()) | {0})
+            cand = [j for j in cand if np.isfinite(row[j]) or j == 0]
+            sub = config_array([grid[j] for j in cand])
+            t = (ctypes.c_float * len(cand))()
+            _call(L.ionp_autotune, call_array([spec]), 1, sub, len(cand), reps, t, _lib.stream_handle())
+            for k, j in enumerate(cand):
+                if t[k] > 0 and (out[i, j] <= 0 or t[k] < out[i, j]):
+                    out[i, j] = t[k]
     return out
 
 

@@ -112,6 +112,24 @@ def call_array(specs):
     return arr
 
 
+# tile schedules the batch autotuner tries (entry 0: the library's defaults)
+BATCH_GRID = [(0, 0, 0)] + [(e, i, c) for e in (8192, 16384, 32768, 65536) for i in (4096, 8192, 16384)
+                            for c in (1, 2, 4, 8)]
+
+
+def autotune(specs, grid=BATCH_GRID, reps=5):
+    """Persistent-run time (us, fastest of `reps`) of a batch of `specs`
+    under each (ew_tile, int_seg, claim) schedule of `grid`; negative:
+    unusable.  Outputs are bit-identical whatever the schedule."""
+    cands = (_lib.BatchConfig * len(grid))()
+    for j, (e, i, c) in enumerate(grid):
+        cands[j].ew_tile, cands[j].int_seg, cands[j].claim = e, i, c
+    t = (ctypes.c_float * len(grid))()
+    arr = call_array(specs)
+    _call(_lib.load().ionp_batch_autotune, arr, len(specs), cands, len(grid), reps, t, _lib.stream_handle())
+    return np.frombuffer(t, dtype=np.float32).copy()
+
+
 class Batch:
     """A planned batch; `run(mode)` executes it on torch's current stream."""
 
@@ -125,6 +143,17 @@ class Batch:
         cfg.ew_tile, cfg.int_seg, cfg.claim = int(ew_tile), int(int_seg), int(claim)
         _call(_lib.load().ionp_batch_create_ex, self._arr, len(self.specs), ctypes.byref(cfg), _lib.stream_handle(),
               ctypes.byref(self._h))
+
+    @classmethod
+    def tuned(cls, specs, grid=BATCH_GRID, reps=5):
+        """A batch built with the fastest tile schedule of `grid` for these
+        calls (the autotuner's timings in .tune_times, the schedule in
+        .schedule)."""
+        t = autotune(specs, grid, reps)
+        j = int(np.argmin(np.where(t > 0, t, np.inf)))
+        bt = cls(specs, *grid[j])
+        bt.tune_times, bt.schedule = t, grid[j]
+        return bt
 
     def run(self, mode="persistent"):
         _call(_lib.load().ionp_batch_run, self._h, MODES[mode], _lib.stream_handle())

@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libionp_hip.so")
 ARCH = os.environ.get("IONP_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["planner.cpp", "api.cpp", "elementwise.hip", "reduce.hip", "accumulate.hip", "batch.hip", "mlp.hip",
-           "features.cpp"]
+           "features.cpp", "advisor.cpp"]
 
 FLAGS = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contract=off",

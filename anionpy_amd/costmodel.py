@@ -1,14 +1,21 @@
-"""Learned launch-configuration model: an fp32 MLP (24 layout features ->
+"""Learned launch-configuration model: an fp32 MLP (32 layout features ->
 3 x 64 hidden ReLU -> one softmax head per ionp_config knob) that proposes
-a configuration per call.  Its forward, backward and Adam kernels are the
-library's own HIP kernels (ionp_mlp_* in include/ionp_hip.h); torch tensors
-only hold the parameters in device memory.
+a configuration per call layout.  Its forward, backward and Adam kernels
+are the library's own HIP kernels (ionp_mlp_* in include/ionp_hip.h); torch
+tensors only hold the parameters in device memory.
 
-Training data: autotune.sweep timings of seeded random layouts; the label
-of a layout is the knob setting of its fastest configuration.  The built-in
-heuristic (config 0) remains the fallback, and the exhaustive sweep is the
-judge of the model's regret.  Whatever configuration it proposes, results
-are bit-identical: configurations only move work between threads.
+Training data: autotune.sweep timings of seeded random layouts.  The target
+of a layout is, per knob, a soft distribution over the knob's values: value
+v gets weight exp(-(r_v - 1) / TAU), where r_v is the best time among the
+grid configurations with that value divided by the layout's best time, so
+values within a few percent of the best share the mass; a fallback head
+prefers the heuristic unless the knobs' best beats it by FALLBACK.  Each
+layout's targets are scaled by layout_weights (longer layouts count more).  `install` loads
+the model into the library's advisor (ionp_advisor_set), which then serves
+every call made without a configuration; the built-in heuristic (config 0)
+remains the fallback, and the exhaustive sweep is the judge of the model's
+regret.  Whatever configuration it proposes, results are bit-identical:
+configurations only move work between threads.
 """
 import ctypes
 import json
@@ -23,9 +30,18 @@ from .autotune import GRID, HEADS, config_index, knob_labels
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "costmodel.json")
 SWEEP_DATA = os.path.join(HERE, "costmodel_sweep.npz")
-HEAD_SIZES = [len(v) for v in HEADS.values()]
+# one head per knob, then the fallback head: value 1 = run the built-in
+# heuristic (grid entry 0, whose per-kernel defaults are not one point of
+# the knob product), value 0 = run the knobs the other heads pick
+KNOB_HEADS = list(HEADS.values())
+HEAD_SIZES = [len(v) for v in KNOB_HEADS] + [2]
+HEAD_VALUES = [v for knob in KNOB_HEADS for v in knob] + [0, 1]
 HEAD_OFF = np.concatenate([[0], np.cumsum(HEAD_SIZES)]).astype(np.int32)
 NOUT = int(HEAD_OFF[-1])
+# knob labels of grid entries 1.. (the full knob product)
+GRID_LABELS = np.array([knob_labels(i) for i in range(1, len(GRID))], dtype=np.int32)
+TAU = 0.01       # soft-target temperature, in units of relative slowdown
+FALLBACK = 0.01  # the heuristic is preferred unless the knobs' best is this much faster
 
 
 def _ptr(t):
@@ -89,8 +105,8 @@ class MLP:
         dz = torch.empty_like(acts[-1])
         loss = torch.empty(n + 1, device=x.device)
         off = torch.tensor(HEAD_OFF, dtype=torch.int32, device=x.device)
-        _call(L.ionp_mlp_softmax_xent, _ptr(acts[-1]), _ptr(labels), _ptr(off), len(HEAD_SIZES), n, NOUT, _ptr(dz),
-              _ptr(loss), s)
+        xent = L.ionp_mlp_softmax_xent_soft if labels.dtype == torch.float32 else L.ionp_mlp_softmax_xent
+        _call(xent, _ptr(acts[-1]), _ptr(labels), _ptr(off), len(HEAD_SIZES), n, NOUT, _ptr(dz), _ptr(loss), s)
         grads_W, grads_b = [None] * len(self.W), [None] * len(self.W)
         dy = dz
         for i in range(len(self.W) - 1, -1, -1):
@@ -116,9 +132,12 @@ class MLP:
             _call(L.ionp_mlp_adam, _ptr(p), _ptr(g), _ptr(m), _ptr(v), p.numel(), lr, b1, b2, eps, self.step, s)
 
     def train(self, feats, labels, epochs=40, batch=512, lr=3e-3, seed=0, log=None):
+        """labels: int32 (n, heads) knob indices, or float32 (n, NOUT) soft
+        targets (a distribution over each head's values)."""
         dev = _device()
         X = torch.as_tensor(np.asarray(feats, dtype=np.float32), device=dev)
-        Y = torch.as_tensor(np.asarray(labels, dtype=np.int32), device=dev)
+        labels = np.asarray(labels)
+        Y = torch.as_tensor(labels.astype(np.float32 if labels.dtype.kind == "f" else np.int32), device=dev)
         rng = np.random.default_rng(seed)
         n = X.shape[0]
         hist = []
@@ -139,14 +158,56 @@ class MLP:
         """Grid index of the proposed configuration for each feature row."""
         X = torch.as_tensor(np.asarray(feats, dtype=np.float32), device=_device())
         z = self.forward(X)[-1].cpu().numpy()
-        labels = np.stack([z[:, HEAD_OFF[h]:HEAD_OFF[h + 1]].argmax(axis=1) for h in range(len(HEAD_SIZES))], 1)
-        return np.array([config_index(list(l)) for l in labels])
+        return decode(z)
 
 
-def labels_of(times):
-    """Knob labels of each layout's fastest configuration."""
+def decode(logits):
+    """Grid index proposed by each row of logits: 0 (the heuristic) when
+    the fallback head says so, else the knob heads' argmaxes."""
+    labels = np.stack([logits[:, HEAD_OFF[h]:HEAD_OFF[h + 1]].argmax(axis=1) for h in range(len(HEAD_SIZES))], 1)
+    return np.array([0 if l[-1] == 1 else config_index(list(l[:-1])) for l in labels])
+
+
+def best_index(times):
+    """Grid index of each layout's fastest usable configuration."""
+    return np.argmin(np.where(times > 0, times, np.inf), axis=1)
+
+
+def time_ratios(times):
+    """Each layout's times divided by its best (unusable: 100)."""
     t = np.where(times > 0, times, np.inf)
-    return np.array([knob_labels(int(i)) for i in np.argmin(t, axis=1)], dtype=np.int32)
+    r = t / t.min(axis=1, keepdims=True)
+    return np.where(np.isfinite(r), r, 100.0).astype(np.float32)
+
+
+def soft_targets(ratios, tau=TAU):
+    """(n, NOUT) per-knob target distributions from time ratios (n, grid)."""
+    r = np.asarray(ratios, dtype=np.float64)
+    rg = r[:, 1:]
+    q = np.zeros((len(rg), NOUT), dtype=np.float32)
+
+    def softmin(rv):
+        z = -(rv - 1.0) / tau
+        z -= z.max(axis=1, keepdims=True)
+        e = np.exp(z)
+        return e / e.sum(axis=1, keepdims=True)
+
+    for h, size in enumerate(HEAD_SIZES[:-1]):
+        rv = np.stack([rg[:, GRID_LABELS[:, h] == v].min(axis=1) for v in range(size)], axis=1)
+        q[:, HEAD_OFF[h]:HEAD_OFF[h + 1]] = softmin(rv)
+    # fallback head: the knobs' best against the heuristic, which wins ties
+    # within FALLBACK
+    fb = np.stack([rg.min(axis=1), r[:, 0] - FALLBACK], axis=1)
+    q[:, HEAD_OFF[-2]:HEAD_OFF[-1]] = softmin(fb)
+    return q
+
+
+def layout_weights(t_best):
+    """Training weight of each layout: 1 + log2(best time / 10 us), at least
+    1, normalised to mean 1 -- a layout that takes longer counts more (the
+    aggregate throughput is theirs), without drowning the many short ones."""
+    w = np.clip(1.0 + np.log2(np.asarray(t_best, dtype=np.float64) / 10.0), 1.0, None)
+    return (w / w.mean()).astype(np.float32)
 
 
 def regret(times, chosen):
@@ -173,23 +234,57 @@ def load(path=WEIGHTS):
     return MLP.from_state(st), st.get("meta", {})
 
 
-class Advisor:
-    """Per-call launch configuration proposed by the model, cached by the
-    call's layout features (ionp_call_features): the MLP's HIP forward runs
-    once per distinct layout.  Grid entry 0 is the built-in heuristic."""
+def install(model):
+    """Load `model` (an MLP) into the library's advisor: from now on every
+    call made without a configuration runs the model's proposal for its
+    layout (a layout's first call runs the heuristic while the proposal is
+    computed on the GPU, without waiting)."""
+    _install(model.sizes, [w.detach().cpu().numpy() for w in model.W], [b.detach().cpu().numpy() for b in model.b])
 
-    def __init__(self, model):
-        self.model = model
-        self.cache = {}
 
-    def config(self, rec):
-        f = (ctypes.c_float * _lib.NFEATURES)()
-        _call(_lib.load().ionp_call_features, ctypes.byref(rec), f)
-        key = bytes(f)
-        cfg = self.cache.get(key)
-        if cfg is None:
-            idx = int(self.model.predict(np.frombuffer(f, dtype=np.float32)[None])[0])
-            cfg = _lib.Config(*GRID[idx])
-            self.cache[key] = cfg
-        return cfg
+def install_file(path=WEIGHTS):
+    """install() from saved weights, without building device tensors."""
+    with open(path) as f:
+        st = json.load(f)
+    _install(st["sizes"], [np.asarray(w, np.float32) for w in st["W"]], [np.asarray(b, np.float32) for b in st["b"]])
 
+
+def _install(sizes_, Ws, bs):
+    sizes = (ctypes.c_int32 * len(sizes_))(*sizes_)
+    flat = np.concatenate([np.concatenate([np.ravel(w), np.ravel(b)]) for w, b in zip(Ws, bs)]).astype(np.float32)
+    heads = (ctypes.c_int32 * len(HEAD_SIZES))(*HEAD_SIZES)
+    values = (ctypes.c_int32 * len(HEAD_VALUES))(*HEAD_VALUES)
+    _call(_lib.load().ionp_advisor_set, sizes, len(sizes_) - 1, flat.ctypes.data_as(ctypes.c_void_p), heads,
+          len(HEAD_SIZES), values)
+
+
+def uninstall():
+    """Every call made without a configuration runs the heuristic."""
+    _call(_lib.load().ionp_advisor_set, None, 0, None, None, 0, None)
+
+
+def prime(specs):
+    """Evaluate the installed model for these calls' layouts now (batched
+    passes, one wait), so their first calls already run the learned
+    configuration."""
+    from .batch import call_array
+    _call(_lib.load().ionp_advisor_prime, call_array(specs), len(specs), _lib.stream_handle())
+
+
+def stats():
+    """Counters of the advisor: calls that ran its configuration, calls that
+    ran the heuristic while their proposal was pending or under trial,
+    layouts evaluated, layouts pending, and the trials' outcomes (each
+    proposal is timed against the heuristic on live calls and the faster
+    one kept)."""
+    v = (ctypes.c_int64 * 6)()
+    _call(_lib.load().ionp_advisor_stats, v)
+    return {"learned_calls": v[0], "heuristic_calls": v[1], "layouts_evaluated": v[2], "pending": v[3],
+            "trials_kept_proposal": v[4], "trials_kept_heuristic": v[5]}
+
+
+def lookup(spec):
+    """The advisor's cached configuration for a call's layout, or None."""
+    cfg, found = _lib.Config(), ctypes.c_int32()
+    _call(_lib.load().ionp_advisor_lookup, ctypes.byref(spec.rec), ctypes.byref(cfg), ctypes.byref(found))
+    return (cfg.block, cfg.vec, cfg.blocks_per_cu, cfg.split) if found.value else None

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "elementwise.h"
+#include "advisor.h"
 #include "planner.h"
 #include "reduce.h"
 #include "accumulate.h"
@@ -467,6 +468,12 @@ ionp_status ionp_plan_argext(int32_t ismax, const ionp_array_desc *in, int32_t a
 ionp_status ionp_binary(int32_t op, const ionp_array_desc *a, const ionp_array_desc *b, const ionp_array_desc *out,
                         const ionp_config *cfg, void *stream) {
     if (ionp_status cs = check_cfg(cfg)) return cs;
+    ionp_config learned;
+    AdviseTiming timing;
+    if (!cfg) {
+        const ionp_array_desc *ops[2] = {a, b};
+        cfg = advise(IONP_CALL_BINARY, op, 2, ops, out, nullptr, 0, 0, as_stream(stream), &learned, &timing);
+    }
     int loop, odt, nd;
     int64_t shape[IONP_MAX_DIMS];
     ionp_status s = binary_common(op, a, b, &loop, &odt, &nd, shape);
@@ -479,12 +486,19 @@ ionp_status ionp_binary(int32_t op, const ionp_array_desc *a, const ionp_array_d
     if ((s = plan_elementwise(op, 2, ins, out, loop, &A))) return s;
     if ((s = attach_fpe(A, op, loop, a->dtype, out->dtype))) return s;
     A.mixed_su = is_mixed_su(op, a->dtype, b->dtype);
+    timing.begin();
     return run_elementwise(A, cfg, as_stream(stream));
 }
 
 ionp_status ionp_where(const ionp_array_desc *c, const ionp_array_desc *x, const ionp_array_desc *y,
                        const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
     if (ionp_status cs = check_cfg(cfg)) return cs;
+    ionp_config learned;
+    AdviseTiming timing;
+    if (!cfg) {
+        const ionp_array_desc *ops[3] = {c, x, y};
+        cfg = advise(IONP_CALL_WHERE, 0, 3, ops, out, nullptr, 0, 0, as_stream(stream), &learned, &timing);
+    }
     ionp_array_desc t;
     ionp_status s = ionp_plan_where(c, x, y, &t);
     if (s) return s;
@@ -492,11 +506,18 @@ ionp_status ionp_where(const ionp_array_desc *c, const ionp_array_desc *x, const
     const ionp_array_desc *ins[3] = {c, x, y};
     EwArgs A;
     if ((s = plan_elementwise(OP_WHERE, 3, ins, out, out->dtype, &A))) return s;
+    timing.begin();
     return run_elementwise(A, cfg, as_stream(stream));
 }
 
 ionp_status ionp_cast(const ionp_array_desc *in, const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
     if (ionp_status cs = check_cfg(cfg)) return cs;
+    ionp_config learned;
+    AdviseTiming timing;
+    if (!cfg) {
+        const ionp_array_desc *ops[1] = {in};
+        cfg = advise(IONP_CALL_CAST, 0, 1, ops, out, nullptr, 0, 0, as_stream(stream), &learned, &timing);
+    }
     ionp_status s;
     if ((s = check_desc(in, "array")) || (s = check_desc(out, "out"))) return s;
     if ((s = same_shape(out, in->ndim, in->shape, "astype"))) return s;
@@ -504,6 +525,7 @@ ionp_status ionp_cast(const ionp_array_desc *in, const ionp_array_desc *out, con
     EwArgs A;
     if ((s = plan_elementwise(OP_CAST, 1, ins, out, out->dtype, &A))) return s;
     if ((s = attach_fpe(A, OP_CAST, out->dtype, in->dtype, out->dtype))) return s;
+    timing.begin();
     return run_elementwise(A, cfg, as_stream(stream));
 }
 
@@ -511,6 +533,12 @@ ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *ax
                         const void *initial, const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
     if (ionp_status cs = check_cfg(cfg)) return cs;
     if (fold_error_take()) return fold_failed();
+    ionp_config learned;
+    AdviseTiming timing;
+    if (!cfg) {
+        const ionp_array_desc *ops[1] = {in};
+        cfg = advise(IONP_CALL_REDUCE, op, 1, ops, out, axes, naxes, keepdims, as_stream(stream), &learned, &timing);
+    }
     ionp_array_desc t;
     ionp_status s;
     if ((s = check_desc(out, "out"))) return s;
@@ -525,6 +553,7 @@ ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *ax
     }
     ReducePlan p;
     if ((s = plan_reduce(op, in, mask, out, &p))) return s;
+    timing.begin();
     return run_reduce(p, g_bufsize, initial, cfg, as_stream(stream));
 }
 
@@ -542,8 +571,10 @@ ionp_status ionp_reduce_noalloc(int32_t op, const ionp_array_desc *in, const int
     A.size.stage = kNoAllocStage;
     A.size.slots = kNoAllocSlots;
     {
+        // never the learned configuration: its first use on a device allocates
+        static const ionp_config heuristic = {0, 0, 0, 0};
         ScratchScope scope(&A);
-        s = ionp_reduce(op, in, axes, naxes, keepdims, initial, out, cfg, stream);
+        s = ionp_reduce(op, in, axes, naxes, keepdims, initial, out, cfg ? cfg : &heuristic, stream);
     }
     if (A.short_) {
         auto mib = [](size_t b) { return std::to_string((b + (1u << 20) - 1) >> 20) + " MiB"; };

@@ -1007,7 +1007,12 @@ ionp_status plan_call_tiles(const ionp_call &c, const ionp_batch_config &cfg, Ba
     return IONP_OK;
 }
 
+// one call through its entry point; no configuration means the built-in
+// heuristic (a zero ionp_config), never the learned one: per-call and
+// graph batches and the autotuner's timings run fixed kernels
 static ionp_status run_one(const ionp_call &c, hipStream_t s, const ionp_config *cfg = nullptr) {
+    static const ionp_config heuristic = {0, 0, 0, 0};
+    if (!cfg) cfg = &heuristic;
     switch (c.kind) {
         case IONP_CALL_BINARY: return ionp_binary(c.op, &c.in[0], &c.in[1], &c.out, cfg, s);
         case IONP_CALL_WHERE: return ionp_where(&c.in[0], &c.in[1], &c.in[2], &c.out, cfg, s);
@@ -1272,6 +1277,39 @@ ionp_status ionp_autotune(const ionp_call *calls, int32_t n, const ionp_config *
         }
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
+    return st;
+}
+
+ionp_status ionp_batch_autotune(const ionp_call *calls, int32_t n, const ionp_batch_config *cands, int32_t ncand,
+                                int32_t reps, float *times_us, void *stream) {
+    if (n < 0 || ncand <= 0 || reps <= 0 || !cands || !times_us || (n > 0 && !calls))
+        return set_error(IONP_ERR_VALUE, "bad batch autotune arguments");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return set_error(IONP_ERR_DEVICE, "hipEventCreate failed");
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return set_error(IONP_ERR_DEVICE, "hipEventCreate failed");
+    }
+    ionp_status st = IONP_OK;
+    for (int32_t j = 0; j < ncand && !st; j++) {
+        times_us[j] = -1.0f;
+        ionp_batch *bt = nullptr;
+        if (ionp_batch_create_ex(calls, n, &cands[j], stream, &bt) != IONP_OK) continue;  // unusable schedule
+        st = ionp_batch_run(bt, IONP_BATCH_PERSISTENT, stream);  // warm
+        for (int32_t r = 0; r < reps && !st; r++) {
+            (void)hipEventRecord(e0, s);
+            st = ionp_batch_run(bt, IONP_BATCH_PERSISTENT, stream);
+            (void)hipEventRecord(e1, s);
+            if (!st && hipEventSynchronize(e1) != hipSuccess) st = set_error(IONP_ERR_DEVICE, "batch autotune run failed");
+            float ms = 0.0f;
+            if (!st && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && (times_us[j] < 0.0f || ms * 1000.0f < times_us[j]))
+                times_us[j] = ms * 1000.0f;
+        }
+        ionp_batch_destroy(bt);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return st;
 }
 

@@ -86,5 +86,25 @@ extern "C" ionp_status ionp_call_features(const ionp_call *call, float *f) {
     }
     f[17 + cls] = 1.0f;
     f[23] = (float)itemsize(dt) / 8.0f;
+    f[24] = (float)itemsize(call->out.dtype) / 8.0f;
+    int64_t maxst = 0;
+    for (int o = 0; o < nop && o < 3 && n > 0; o++) {
+        if (cst[o][0] < 0) f[25 + o] = 1.0f;
+        const int64_t a = cst[o][0] < 0 ? -cst[o][0] : cst[o][0];
+        if (a > maxst) maxst = a;
+    }
+    f[28] = (float)log2((double)maxst + 1.0) / 24.0f;
+    double bytes = (double)total * itemsize(call->out.dtype);
+    for (int o = 0; o < nin; o++) {
+        int64_t e = 1;  // elements of the operand itself (a broadcast operand counts its own size)
+        for (int k = 0; k < call->in[o].ndim; k++) e *= call->in[o].shape[k];
+        bytes += (double)e * itemsize(call->in[o].dtype);
+    }
+    if (call->kind == IONP_CALL_REDUCE) {
+        int64_t e = 1;
+        for (int k = 0; k < call->out.ndim; k++) e *= call->out.shape[k];
+        bytes = (double)total * itemsize(call->in[0].dtype) + (double)e * itemsize(call->out.dtype);
+    }
+    f[29] = (float)log2(bytes + 1.0) / 36.0f;
     return IONP_OK;
 }

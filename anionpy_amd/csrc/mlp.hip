@@ -73,6 +73,35 @@ __global__ void mlp_softmax_xent(const float *logits, const int *labels, const i
     loss_per_sample[r] = l;
 }
 
+// the same loss against soft targets q (n x total, non-negative; a head's
+// targets may sum to any mass m_h, which weights that sample's head):
+// loss = -sum_h sum_i q_i log p_i, dz = (m_h p - q) / n.  One-hot q gives
+// mlp_softmax_xent.
+__global__ void mlp_softmax_xent_soft(const float *logits, const float *q, const int *head_off, int heads, int n,
+                                      int total, float *dlogits, float *loss_per_sample) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const float *z = logits + (int64_t)r * total;
+    const float *qr = q + (int64_t)r * total;
+    float *dz = dlogits + (int64_t)r * total;
+    float l = 0.0f;
+    for (int h = 0; h < heads; h++) {
+        const int a = head_off[h], b = head_off[h + 1];
+        float m = z[a];
+        for (int i = a + 1; i < b; i++) m = z[i] > m ? z[i] : m;
+        float s = 0.0f;
+        for (int i = a; i < b; i++) s += expf(z[i] - m);
+        const float ls = logf(s);
+        float mass = 0.0f;
+        for (int i = a; i < b; i++) mass += qr[i];
+        for (int i = a; i < b; i++) {
+            dz[i] = (mass * (expf(z[i] - m) / s) - qr[i]) / (float)n;
+            l -= qr[i] * (z[i] - m - ls);
+        }
+    }
+    loss_per_sample[r] = l;
+}
+
 __global__ void sum_reduce_f32(const float *v, int n, float *out) {
     __shared__ float red[256];
     float acc = 0.0f;
@@ -136,6 +165,17 @@ ionp_status ionp_mlp_softmax_xent(const float *logits, const int32_t *labels, co
     // per-sample losses live after the scalar slot: loss[1 .. n]
     hipLaunchKernelGGL(mlp_softmax_xent, dim3(blocks(n)), dim3(256), 0, s, logits, labels, head_off, heads, n, total,
                        dlogits, loss + 1);
+    hipLaunchKernelGGL(sum_reduce_f32, dim3(1), dim3(256), 0, s, loss + 1, n, loss);
+    return done();
+}
+
+ionp_status ionp_mlp_softmax_xent_soft(const float *logits, const float *targets, const int32_t *head_off,
+                                       int32_t heads, int32_t n, int32_t total, float *dlogits, float *loss,
+                                       void *stream) {
+    if (n <= 0 || heads <= 0) return set_error(IONP_ERR_VALUE, "bad head layout");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(mlp_softmax_xent_soft, dim3(blocks(n)), dim3(256), 0, s, logits, targets, head_off, heads, n,
+                       total, dlogits, loss + 1);
     hipLaunchKernelGGL(sum_reduce_f32, dim3(1), dim3(256), 0, s, loss + 1, n, loss);
     return done();
 }

@@ -137,6 +137,7 @@ def _c_strides(shape):
 
 _TUNE_DTYPES = (np.float32, np.float64, np.int64, np.int32, np.uint8)
 _KINDS = ("contiguous", "contiguous", "transposed", "stepped", "reversed")
+CLASSES = ("arith", "compare", "where", "cast", "reduce")
 
 
 def tuning_layouts(seed, n, pool, min_log2=10, max_log2=22):
@@ -156,22 +157,24 @@ def tuning_layouts(seed, n, pool, min_log2=10, max_log2=22):
                 a = pool.view(rng, shape, dt, kind)
                 b = pool.view(rng, tuple(b_shape), dt, "contiguous")
                 op = ("add", "multiply", "maximum")[int(rng.integers(0, 3))] if cls == 0 else "less"
-                specs.append(binary_call(op, a, b, out=_out_for(pool, rng, shape, np.bool_ if cls else dt)))
+                spec = binary_call(op, a, b, out=_out_for(pool, rng, shape, np.bool_ if cls else dt))
             elif cls == 2:
                 c = pool.view(rng, shape, np.bool_, "contiguous")
                 x = pool.view(rng, shape, dt, kind)
                 y = pool.view(rng, tuple(1 for _ in shape), dt, "contiguous")
-                specs.append(where_call(c, x, y, out=_out_for(pool, rng, shape, dt)))
+                spec = where_call(c, x, y, out=_out_for(pool, rng, shape, dt))
             elif cls == 3:
                 x = pool.view(rng, shape, dt, kind)
                 to = _TUNE_DTYPES[int(rng.integers(0, len(_TUNE_DTYPES)))]
-                specs.append(cast_call(x, to, out=_out_for(pool, rng, shape, to)))
+                spec = cast_call(x, to, out=_out_for(pool, rng, shape, to))
             else:
                 x = pool.view(rng, shape, dt if dt != np.uint8 else np.float64, kind)
                 axes = [None] + list(range(rank))
-                specs.append(reduce_call(x, axis=axes[int(rng.integers(0, len(axes)))]))
+                spec = reduce_call(x, axis=axes[int(rng.integers(0, len(axes)))])
         except (ValueError, TypeError, NotImplementedError):
             continue
+        spec.tag = (CLASSES[cls], kind, total)
+        specs.append(spec)
     return specs
 
 

@@ -165,7 +165,8 @@ def main():
         e1.synchronize()
         t = e0.elapsed_time(e1) / reps * 1e-3
         per_op[name] = {"us": round(t * 1e6, 1), "GB/s": round(alg_bytes(a, b, o) / t / 1e9, 1),
-                        "frac": round(alg_bytes(a, b, o) / t / PEAK_HBM, 4)}
+                        "frac": round(alg_bytes(a, b, o) / t / PEAK_HBM, 4),
+                        "config": launch_config(ap, f, a, b, o)}
     # the dominant op with IEEE flags tracked (NumPy's default modes): flag
     # variant of the kernel plus a stream synchronisation and a 4-byte read
     with ap.errstate(divide="warn", over="warn", invalid="warn"):
@@ -186,7 +187,7 @@ def main():
         with open(traffic_file) as f:
             roofline["traffic"] = json.load(f).get("mul_f64")
 
-    extra = {"per_op": per_op, "fpe_tracking": fpe_tracking}
+    extra = {"config_policy": ap.get_config_policy(), "per_op": per_op, "fpe_tracking": fpe_tracking}
     del X64, c64, X32, c32, I, j, V32, v32, outs
     torch.cuda.empty_cache()
     if not args.no_extra:
@@ -205,6 +206,16 @@ def main():
                        "bytes_per_step": step_bytes, "frac_of_peak": round(value * 1e9 / PEAK_HBM, 4)},
             "roofline": roofline, "cpu_baseline": cpu, "extra": extra}
     print(json.dumps(line), flush=True)
+
+
+def launch_config(ap, f, a, b, o):
+    """The launch configuration the call ran with: the advisor's proposal
+    (block, vec, blocks_per_cu, split; zeros = the built-in heuristic) or
+    "heuristic" when no proposal is cached."""
+    from anionpy_amd import costmodel as CM
+    from anionpy_amd.batch import binary_call
+    got = CM.lookup(binary_call(f.__name__, a, b, out=o))
+    return "heuristic" if got is None or not any(got) else list(got)
 
 
 def timed(torch, fn, reps):
@@ -331,39 +342,70 @@ def batch_extras(ap, torch):
     res["speedup_persistent_vs_per_call"] = round(res["per_call"]["ms"] / res["persistent"]["ms"], 2)
     res["speedup_graph_vs_per_call"] = round(res["per_call"]["ms"] / res["graph"]["ms"], 2)
     bt.close()
+    # the batch autotuner: the persistent run under each tile schedule of
+    # BATCH_GRID, then the batch rebuilt with the fastest and timed as above
+    log("uneven batch: tile schedule autotuning")
+    t0 = time.perf_counter()
+    tb = Batch.tuned(specs)
+    tune_s = time.perf_counter() - t0
+    tb.run("persistent")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        tb.run("persistent")
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 10
+    tuned_same = all(torch.equal(a, s.out._buf) for a, s in zip(outs["persistent"], specs))
+    res["persistent_tuned"] = {"ms": round(dt * 1e3, 3), "calls_per_s": round(len(specs) / dt, 1),
+                               "GB/s": round(info["bytes"] / dt / 1e9, 1),
+                               "schedule": {"ew_tile": tb.schedule[0], "int_seg": tb.schedule[1],
+                                            "claim": tb.schedule[2]},
+                               "default_schedule_us": round(float(tb.tune_times[0]), 1),
+                               "tuned_schedule_us": round(float(np.min(tb.tune_times[tb.tune_times > 0])), 1),
+                               "schedules_tried": int(len(tb.tune_times)), "tuning_s": round(tune_s, 2),
+                               "identical_outputs": bool(tuned_same)}
+    res["speedup_persistent_tuned_vs_per_call"] = round(res["per_call"]["ms"] / res["persistent_tuned"]["ms"], 2)
+    tb.close()
     del specs, outs
     torch.cuda.empty_cache()
     return res
 
 
 def learned_extras(torch):
-    """The committed cost model on 300 layouts generated now (seed 909, not
-    seen in training) and timed under every grid configuration: throughput
-    of the learned choices and of the heuristic as a fraction of the
-    exhaustive best, beside the stored held-out figures."""
+    """The committed cost model on 200 layouts generated now (seed 909, not
+    seen in training; sizes 2^10-2^27) and timed under every grid
+    configuration, 5 reps, twice: throughput of the learned choices and of
+    the heuristic as a fraction of the exhaustive best, the noise floor (the
+    best of one sweep judged by the other), and the stored held-out figures."""
     path = os.path.join(HERE, "anionpy_amd", "costmodel.json")
     if not os.path.exists(path):
         return None
     from anionpy_amd import autotune, costmodel as CM
     from anionpy_amd.layouts import Pool, tuning_layouts
+    from anionpy_amd.train_costmodel import MAX_LOG2, MIN_LOG2, POOL_BYTES
     log("learned configuration: live sweep")
     model, meta = CM.load()
-    pool = Pool(1 << 28)
-    specs = tuning_layouts(909, 300, pool)
-    t = autotune.sweep(specs, reps=2)
+    pool = Pool(POOL_BYTES)
+    specs = tuning_layouts(909, 200, pool, min_log2=MIN_LOG2, max_log2=MAX_LOG2)
+    t = autotune.sweep(specs, reps=5)
+    t2 = autotune.sweep(specs, reps=5)
     chosen = model.predict(np.stack([s.features() for s in specs]))
     tt = np.where(t > 0, t, np.inf)
     tb = tt.min(axis=1)
-    tl = tt[np.arange(len(tt)), chosen]
-    tl = np.where(np.isfinite(tl), tl, tt[:, 0])
-    th = tt[:, 0]
-    res = {"live_layouts": len(specs), "exhaustive_us": round(float(tb.sum()), 1), "learned_us": round(float(tl.sum()), 1),
-           "heuristic_us": round(float(th.sum()), 1),
-           "learned_throughput_vs_exhaustive": round(float(tb.sum() / tl.sum()), 4),
-           "heuristic_throughput_vs_exhaustive": round(float(tb.sum() / th.sum()), 4),
-           "stored_held_out": {k: meta.get(k) for k in ("test_layouts", "regret_learned_aggregate",
-                                                        "regret_heuristic_aggregate", "regret_learned_mean",
-                                                        "regret_heuristic_mean")}}
+
+    def total(pick):
+        tc = tt[np.arange(len(tt)), pick]
+        return float(np.where(np.isfinite(tc), tc, tt[:, 0]).sum())
+
+    tl, th, tn = total(chosen), total(np.zeros(len(specs), dtype=int)), total(CM.best_index(t2))
+    ho = meta.get("held_out", {}).get("all", {})
+    res = {"live_layouts": len(specs), "exhaustive_us": round(float(tb.sum()), 1), "learned_us": round(tl, 1),
+           "heuristic_us": round(th, 1), "noise_floor_us": round(tn, 1),
+           "learned_throughput_vs_exhaustive": round(float(tb.sum() / tl), 4),
+           "heuristic_throughput_vs_exhaustive": round(float(tb.sum() / th), 4),
+           "noise_floor_throughput_vs_exhaustive": round(float(tb.sum() / tn), 4),
+           "learned_fallback_share": round(float(np.mean(chosen == 0)), 3),
+           "stored_held_out": ho}
     del specs, pool
     torch.cuda.empty_cache()
     return res

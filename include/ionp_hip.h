@@ -362,6 +362,13 @@ ionp_status ionp_batch_run(ionp_batch *batch, int32_t mode, void *stream);
 ionp_status ionp_batch_info(const ionp_batch *batch, int64_t *ntiles, int64_t *bytes);
 void ionp_batch_destroy(ionp_batch *batch);
 
+/* Times the batch's PERSISTENT run under each candidate tile schedule:
+ * times_us[j] is the fastest of `reps` runs with cands[j] (a schedule the
+ * batch cannot use: a negative time).  The calls' outputs are overwritten;
+ * each candidate builds and frees its own batch. */
+ionp_status ionp_batch_autotune(const ionp_call *calls, int32_t n, const ionp_batch_config *cands, int32_t ncand,
+                                int32_t reps, float *times_us, void *stream);
+
 /* ---- autotuning (device) ----------------------------------------------- */
 /* No counterpart in the reference (its CPU loops have no launch knobs). */
 
@@ -372,13 +379,59 @@ void ionp_batch_destroy(ionp_batch *batch);
 ionp_status ionp_autotune(const ionp_call *calls, int32_t n, const ionp_config *cfgs, int32_t ncfg, int32_t reps,
                           float *times_us, void *stream);
 
+/* ---- learned launch configuration (the advisor) ----------------------- */
+/* No counterpart in the reference.
+ *
+ * With a model installed, ionp_binary / ionp_where / ionp_cast /
+ * ionp_reduce called with cfg == NULL run the configuration the cost model
+ * proposes for the call's layout (a zero ionp_config still means the
+ * built-in heuristic).  Proposals are cached per layout (its feature vector).
+ * A layout's first call runs the heuristic and queues the layout; the MLP's
+ * forward kernels evaluate the queue on that call's stream (at most 256
+ * layouts per pass, one pass in flight per device, never under graph
+ * capture) and a later call installs the result once the pass has finished,
+ * so no call waits for the model.  Each proposal is then verified on live
+ * calls: two calls of the heuristic (the layout's first calls) and two of
+ * the proposal are timed with HIP events, read back without waiting by
+ * later calls, and the layout keeps whichever ran faster.  Results are
+ * bit-identical whatever runs.  ionp_reduce_noalloc, batches and
+ * ionp_autotune always run the heuristic.  Thread-safe, except that
+ * ionp_advisor_set must not run while other threads make calls; the cache
+ * is shared by all devices. */
+
+/* Installs a model (nlayers > 0) or removes it (nlayers == 0: every call
+ * runs the heuristic).  sizes[0..nlayers]: layer widths, sizes[0] ==
+ * IONP_NFEATURES; params: per layer W (sizes[l] x sizes[l+1], row-major)
+ * then b; heads[nheads]: values per knob (block, vec, blocks_per_cu,
+ * split), and with nheads == 5 a fallback head of two values whose second
+ * (value 1) means "run the built-in heuristic"; values: the knob values,
+ * head after head (sum(heads) == sizes[nlayers]).
+ * Clears the cache; device copies are made at each device's first pass. */
+ionp_status ionp_advisor_set(const int32_t *sizes, int32_t nlayers, const float *params, const int32_t *heads,
+                             int32_t nheads, const int32_t *values);
+/* Evaluates the model for these calls' layouts now (batched passes on
+ * `stream`, waiting for them), so their first calls already run the
+ * learned configuration. */
+ionp_status ionp_advisor_prime(const ionp_call *calls, int32_t n, void *stream);
+/* stats[0]: calls that ran the advisor's configuration for their layout
+ * (the proposal, or the heuristic a trial kept), [1]: calls that ran the
+ * heuristic while their layout's proposal was pending or under trial, [2]:
+ * layouts evaluated, [3]: layouts queued or in flight, [4]: trials that
+ * kept the proposal, [5]: trials that kept the heuristic. */
+ionp_status ionp_advisor_stats(int64_t *stats);
+/* The cached proposal for a call's layout (*found = 0 if none yet). */
+ionp_status ionp_advisor_lookup(const ionp_call *call, ionp_config *cfg, int32_t *found);
+
 /* ---- layout features for the cost model (host only) ------------------ */
 /* No counterpart in the reference. */
 
-#define IONP_NFEATURES 24
+#define IONP_NFEATURES 32
 
-/* Feature vector of a call's coalesced layout (rank, log2 dims,
- * stride/contiguity/broadcast bits, dtype and op class one-hots). */
+/* Feature vector of a call's coalesced layout: rank, log2 dims and total
+ * size, per-operand contiguity / broadcast / negative-stride bits along the
+ * innermost coalesced axis, the output's contiguity there, input dtype and
+ * op class one-hots, input and output item sizes, log2 of the largest inner
+ * stride, log2 of the call's bytes.  Entries 30-31 are zero (reserved). */
 ionp_status ionp_call_features(const ionp_call *call, float *features);
 
 /* ---- MLP cost model kernels (device) ---------------------------------- */
@@ -399,6 +452,14 @@ ionp_status ionp_mlp_linear_bwd(const float *x, const float *w, const float *y, 
 ionp_status ionp_mlp_softmax_xent(const float *logits, const int32_t *labels, const int32_t *head_off,
                                   int32_t heads, int32_t n, int32_t total, float *dlogits, float *loss,
                                   void *stream);
+/* The same loss against soft targets: targets[n, total] holds non-negative
+ * weights over each head's values (a distribution scaled by the sample's
+ * weight m: a head's targets sum to m); loss = -sum_h sum_i q_i
+ * log softmax(z)_i (over n samples, averaged), dlogits = (m softmax(z) - q)
+ * / n. */
+ionp_status ionp_mlp_softmax_xent_soft(const float *logits, const float *targets, const int32_t *head_off,
+                                       int32_t heads, int32_t n, int32_t total, float *dlogits, float *loss,
+                                       void *stream);
 /* Adam step over `count` parameters. */
 ionp_status ionp_mlp_adam(float *param, const float *grad, float *m, float *v, int64_t count, float lr,
                           float beta1, float beta2, float eps, int32_t step, void *stream);

@@ -1,6 +1,6 @@
 """float64 NumPy restatement of the cost-model MLP (test infrastructure
-only): forward, summed-head softmax cross-entropy and backward, used to
-check the HIP kernels' gradients."""
+only): forward, summed-head softmax cross-entropy (hard labels or soft targets)
+and backward, used to check the HIP kernels' gradients."""
 import numpy as np
 
 
@@ -13,6 +13,10 @@ def forward(x, Ws, bs):
 
 
 def loss_and_grads(x, labels, Ws, bs, head_off):
+    """labels: int class per head (n x heads), or float soft targets
+    (n x total, non-negative weights over each head's values; a row whose
+    head sums to m counts m times)."""
+    soft = np.asarray(labels).dtype.kind == "f"
     acts = forward(x, Ws, bs)
     z = acts[-1]
     n = z.shape[0]
@@ -22,6 +26,11 @@ def loss_and_grads(x, labels, Ws, bs, head_off):
         a, b = head_off[h], head_off[h + 1]
         zz = z[:, a:b] - z[:, a:b].max(axis=1, keepdims=True)
         p = np.exp(zz) / np.exp(zz).sum(axis=1, keepdims=True)
+        if soft:  # labels is q[n, total]: non-negative weights per head (mass m per row)
+            q = np.asarray(labels[:, a:b], np.float64)
+            loss += -(q * np.log(p)).sum()
+            dz[:, a:b] = (p * q.sum(axis=1, keepdims=True) - q) / n
+            continue
         lab = labels[:, h]
         loss += -np.log(p[np.arange(n), lab]).sum()
         g = p.copy()

@@ -114,3 +114,34 @@ def test_batch_rejects_bad_calls(ap):
     x = ap.asarray(np.zeros((4, 5)))
     with pytest.raises(NotImplementedError):
         Batch([reduce_call(x, axis=1, op="maximum")])
+
+
+def test_batch_autotuner_schedules_are_bit_identical(ap):
+    """ionp_batch_autotune times the persistent run under each tile schedule;
+    the tuned batch (and every schedule it tried) writes the same bytes as
+    the default schedule."""
+    from anionpy_amd.batch import BATCH_GRID, Batch, autotune
+    from anionpy_amd.layouts import uneven_batch
+    specs = uneven_batch(seed=12, n=400, max_log2=16)
+    with Batch(specs) as bt:
+        bt.run("persistent")
+        torch.cuda.synchronize()
+        want = [s.out.get().copy() for s in specs]
+    grid = [BATCH_GRID[0], (8192, 4096, 1), (65536, 16384, 8), (32768, 8192, 2)]
+    t = autotune(specs, grid=grid, reps=2)
+    assert t.shape == (4,) and (t > 0).all()
+    # the last schedule timed left its outputs in place
+    assert all(a.tobytes() == s.out.get().tobytes() for a, s in zip(want, specs))
+    bt = Batch.tuned(specs, grid=grid, reps=2)
+    try:
+        assert bt.schedule in grid and bt.tune_times.shape == (4,)
+        for s in specs:
+            s.out._buf.fill_(0xAB)
+        bt.run("persistent")
+        torch.cuda.synchronize()
+        assert all(a.tobytes() == s.out.get().tobytes() for a, s in zip(want, specs))
+    finally:
+        bt.close()
+    # an ew_tile that is not a multiple of 1024 is a schedule the batch
+    # cannot use: reported as a negative time
+    assert autotune(specs, grid=[(1000, 8192, 4)], reps=1)[0] < 0

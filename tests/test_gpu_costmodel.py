@@ -9,19 +9,24 @@ import oracle.mlp_ref as R
 pytestmark = pytest.mark.gpu
 
 
-def test_mlp_gradients_match_float64_oracle(ap):
-    from anionpy_amd import costmodel as CM
+@pytest.mark.parametrize("soft", [False, True], ids=["hard labels", "soft targets"])
+def test_mlp_gradients_match_float64_oracle(ap, soft):
+    from anionpy_amd import _lib, costmodel as CM
     m = CM.MLP(seed=3)
     rng = np.random.default_rng(0)
     n = 37
-    x = rng.uniform(0, 1, (n, 24)).astype(np.float32)
-    labels = np.stack([rng.integers(0, k, n) for k in CM.HEAD_SIZES], 1).astype(np.int32)
+    x = rng.uniform(0, 1, (n, _lib.NFEATURES)).astype(np.float32)
+    if soft:
+        labels = CM.soft_targets(rng.uniform(1.0, 1.2, (n, len(CM.GRID))).astype(np.float32))
+        labels *= rng.uniform(0.2, 3.0, (n, 1)).astype(np.float32)  # per-sample weights
+    else:
+        labels = np.stack([rng.integers(0, k, n) for k in CM.HEAD_SIZES], 1).astype(np.int32)
     xt = torch.as_tensor(x, device="cuda")
     yt = torch.as_tensor(labels, device="cuda")
     loss, gW, gb = m.loss_and_grads(xt, yt)
     Ws = [w.cpu().numpy() for w in m.W]
     bs = [b.cpu().numpy() for b in m.b]
-    ref_loss, rW, rb = R.loss_and_grads(x, labels, Ws, bs, CM.HEAD_OFF)
+    ref_loss, rW, rb = R.loss_and_grads(x, labels.astype(np.float64) if soft else labels, Ws, bs, CM.HEAD_OFF)
     assert abs(float(loss[0]) - ref_loss) <= 1e-5 * max(1.0, abs(ref_loss))
     for g, r in zip(gW + gb, rW + rb):
         g = g.cpu().numpy().astype(np.float64)
@@ -30,10 +35,8 @@ def test_mlp_gradients_match_float64_oracle(ap):
     # forward of the trained-model path (predict) agrees with the oracle's argmax
     acts = R.forward(x, Ws, bs)
     z = acts[-1]
-    want = np.stack([z[:, CM.HEAD_OFF[h]:CM.HEAD_OFF[h + 1]].argmax(1) for h in range(len(CM.HEAD_SIZES))], 1)
     got_idx = m.predict(x)
-    from anionpy_amd.autotune import config_index
-    assert list(got_idx) == [config_index(list(w)) for w in want]
+    assert list(got_idx) == list(CM.decode(z))
 
 
 def test_adam_step_matches_reference(ap):
@@ -54,40 +57,50 @@ def test_adam_step_matches_reference(ap):
 
 
 def test_learned_config_beats_heuristic_on_held_out(ap):
-    """Train on the stored MI355X sweep (50k layouts, fastest configuration
-    of each) with the HIP kernels, score on the 5k held-out layouts' full
-    timings: the learned choices must beat the built-in heuristic in
-    aggregate and per layout, and stay within 10% of the exhaustive best."""
+    """Train on the stored MI355X sweep (50k layouts) with the HIP kernels and
+    score on the 5k held-out layouts' full timings (every configuration, 5
+    reps, swept twice): the learned choices beat the built-in heuristic in
+    aggregate by more than twice the noise floor (the best of one sweep
+    judged by the other) and per layout, and stay within 10% of the
+    exhaustive best."""
     from anionpy_amd import costmodel as CM
-    from anionpy_amd.train_costmodel import fit, load_data
-    f_train, best_train, f_test, t_test = load_data(CM.SWEEP_DATA)
-    m, meta = fit(f_train, best_train, f_test, t_test, epochs=40)
-    print(meta)
-    assert meta["loss_last"] < meta["loss_first"]
-    assert meta["regret_learned_aggregate"] < meta["regret_heuristic_aggregate"]
-    assert meta["regret_learned_mean"] < meta["regret_heuristic_mean"]
-    assert meta["regret_learned_aggregate"] < 0.10
+    from anionpy_amd.train_costmodel import fit, load_data, score
+    f_train, r_train, tb_train, f_test, t_test, t_test2, tags_test = load_data(CM.SWEEP_DATA)
+    m, hist = fit(f_train, r_train, tb_train, epochs=60)
+    res = score(m, f_test, t_test, t_test2, tags_test)
+    print(res)
+    a = res["all"]
+    assert hist[-1] < hist[0]
+    noise = max(a["regret_noise_aggregate"], 0.002)
+    assert a["regret_learned_aggregate"] < a["regret_heuristic_aggregate"] - 2 * noise
+    assert a["regret_learned_mean"] < a["regret_heuristic_mean"]
+    assert a["regret_learned_aggregate"] < 0.10
 
 
 def test_committed_weights_on_live_sweep(ap):
-    """The committed weights, on layouts generated and timed now: the
-    learned choices are within 10% of exhaustive autotuning in aggregate."""
+    """The committed weights, on layouts generated and timed now (every
+    configuration, 5 reps): the learned choices beat the heuristic in
+    aggregate and are within 10% of exhaustive autotuning."""
     from anionpy_amd import autotune, costmodel as CM
     from anionpy_amd.layouts import Pool, tuning_layouts
+    from anionpy_amd.train_costmodel import MAX_LOG2, MIN_LOG2, POOL_BYTES
     m, _ = CM.load()
-    pool = Pool(1 << 28)
-    specs = tuning_layouts(909, 200, pool)
-    t = autotune.sweep(specs, reps=2)
+    pool = Pool(POOL_BYTES)
+    specs = tuning_layouts(909, 200, pool, min_log2=MIN_LOG2, max_log2=MAX_LOG2)
+    t = autotune.sweep(specs, reps=5)
     chosen = m.predict(np.stack([s.features() for s in specs]))
     learned, _ = CM.regret(t, chosen)
     heuristic, _ = CM.regret(t, np.zeros(len(specs), dtype=int))
     print(f"live aggregate regret: learned {learned:.4f}, heuristic {heuristic:.4f}")
+    assert learned < heuristic
     assert learned < 0.10
 
 
 def test_learned_policy_is_bit_identical(ap):
-    """Calls under the learned policy (a configuration per call layout)
-    write the same bytes as under the heuristic."""
+    """Calls under the learned policy (a configuration per call layout,
+    served by the library's advisor) write the same bytes as under the
+    heuristic, and the learned configurations really run."""
+    from anionpy_amd import costmodel as CM
     rng = np.random.default_rng(31)
     X = ap.asarray(rng.standard_normal((300, 700)))
     c = ap.asarray(rng.uniform(0.5, 2, (300, 1)))
@@ -100,14 +113,78 @@ def test_learned_policy_is_bit_identical(ap):
                 ap.where(cond, X, c).get(), X.astype(np.float32).get()]
 
     ap.set_config_policy("heuristic")
-    want = run()
-    ap.set_config_policy("learned")
     try:
+        want = run()
+        ap.set_config_policy("learned")
         assert ap.get_config_policy() == "learned"
-        got = run()
-        got2 = run()  # second pass: proposals from the cache
+        got = run()  # first calls: the heuristic while proposals are computed
+        torch.cuda.synchronize()
+        got2 = run()  # proposals installed
+        st = CM.stats()
     finally:
-        ap.set_config_policy("heuristic")
+        ap.set_config_policy("learned")
+    assert st["learned_calls"] >= 8 and st["layouts_evaluated"] >= 8, st
     for w, g, g2 in zip(want, got, got2):
         assert w.dtype == g.dtype and w.shape == g.shape
         assert w.tobytes() == g.tobytes() == g2.tobytes()
+
+
+def test_advisor_serves_without_waiting(ap):
+    """A new layout's first call runs the heuristic and queues the layout;
+    once the pass has run, the next call runs the proposal, which equals the
+    MLP's own prediction for the layout's features; prime() evaluates
+    layouts up front."""
+    from anionpy_amd import costmodel as CM
+    from anionpy_amd.autotune import GRID
+    from anionpy_amd.batch import binary_call, reduce_call
+    model, _ = CM.load()
+    ap.set_config_policy("learned")  # a fresh advisor with the committed weights
+    try:
+        a = ap.asarray(np.arange(12 * 345, dtype=np.float64).reshape(12, 345))
+        b = ap.asarray(np.ones((12, 1)))
+        spec = binary_call("add", a, b)
+        assert CM.lookup(spec) is None
+        s0 = CM.stats()
+        ap.add(a, b)
+        s1 = CM.stats()
+        assert s1["heuristic_calls"] == s0["heuristic_calls"] + 1
+        torch.cuda.synchronize()
+        ap.add(a, b)
+        s2 = CM.stats()
+        assert s2["learned_calls"] == s1["learned_calls"] + 1
+        want = GRID[int(model.predict(spec.features()[None])[0])]
+        assert CM.lookup(spec) == want
+        # prime: a batch of new layouts evaluated in one wait
+        xs = [ap.asarray(np.ones((7, 3 + k))) for k in range(40)]
+        specs = [reduce_call(x, axis=1) for x in xs]
+        CM.prime(specs)
+        got = [CM.lookup(sp) for sp in specs]
+        pred = model.predict(np.stack([sp.features() for sp in specs]))
+        assert got == [GRID[int(i)] for i in pred]
+        assert CM.stats()["pending"] == 0
+    finally:
+        ap.set_config_policy("learned")
+
+
+def test_advisor_trial_keeps_the_faster_arm(ap):
+    """Each proposal is timed against the heuristic on live calls (two of
+    each); after the trial the layout runs whichever was faster, and the
+    bytes never change."""
+    from anionpy_amd import costmodel as CM
+    from anionpy_amd.batch import binary_call
+    ap.set_config_policy("learned")
+    a = ap.asarray(np.random.default_rng(3).standard_normal((2048, 3000)))
+    b = ap.asarray(np.random.default_rng(4).standard_normal((2048, 1)))
+    want = ap.multiply(a, b).get()
+    spec = binary_call("multiply", a, b)
+    for _ in range(12):  # heuristic calls, the pass, proposal calls, then the decision
+        got = ap.multiply(a, b)
+        torch.cuda.synchronize()
+        assert got.get().tobytes() == want.tobytes()
+    st = CM.stats()
+    assert st["trials_kept_proposal"] + st["trials_kept_heuristic"] >= 1, st
+    cfg = CM.lookup(spec)
+    assert cfg is not None
+    before = CM.stats()["learned_calls"]
+    ap.multiply(a, b)
+    assert CM.stats()["learned_calls"] == before + 1  # decided: the advisor's choice, untimed
