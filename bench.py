@@ -37,6 +37,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extra", action="store_true")
+    p.add_argument("--no-sweep", action="store_true",
+                   help="skip the learned configuration's live sweep (its ~150k timed launches swamp a kernel trace)")
     p.add_argument("--reductions", action="store_true",
                    help="only run the reduction workloads and print their timings (the workload of the "
                         "reduction kernel profiles)")
@@ -194,7 +196,7 @@ def main():
         extra.update(elementwise_extras(ap, torch))
         extra.update(reduction_extras(ap, torch))
         extra["batch"] = batch_extras(ap, torch)
-        extra["learned_config"] = learned_extras(torch)
+        extra["learned_config"] = None if args.no_sweep else learned_extras(torch)
 
     cpu = None if args.no_cpu else cpu_baseline()
     line = {"metric": "HBM GB/s (% of peak) ufunc+reduce, bit-exact; batched vs per-call calls/s",
