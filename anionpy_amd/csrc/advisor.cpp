@@ -11,10 +11,11 @@
 // installs them once hipEventQuery reports the copy done.  No call waits for
 // the model, and nothing is launched on a stream under graph capture.
 //
-// A proposal is then verified on the calls themselves: the layout's first
-// calls (run with the heuristic while the proposal is pending) and the
-// proposal's first calls are timed with event pairs, two of each, and the
-// layout keeps whichever ran faster (a trial).  The events are read with
+// A proposal is then verified on the calls themselves: the layout's early
+// calls (run with the heuristic while the proposal is pending; not the very
+// first, which pays one-time costs) and the proposal's first calls are timed
+// with event pairs, three of each, and the layout keeps whichever ran faster
+// (a trial).  The events are read with
 // hipEventQuery by later calls, so verifying waits for nothing either.
 //
 // Whatever the model proposes, results are bit-identical: a configuration
@@ -34,7 +35,7 @@ namespace {
 
 constexpr int kMaxBatch = 256;      // layouts per forward pass
 constexpr size_t kMaxQueue = 4096;  // layouts waiting for a pass (later ones: heuristic until queued)
-constexpr int kTrialSamples = 2;    // timed calls per arm (heuristic, proposal) of a trial
+constexpr int kTrialSamples = 3;    // timed calls per arm (heuristic, proposal) of a trial
 constexpr int kTrialAttempts = 8;   // timings issued per arm before deciding on what was measured
 constexpr size_t kMaxTrials = 1024; // layouts under trial at once (later ones keep the proposal untried)
 
@@ -81,6 +82,7 @@ enum : uint8_t { SLOT_FREE = 0, SLOT_RESERVED = 1, SLOT_RECORDED = 2 };
 enum { ARM_HEURISTIC = 0, ARM_PROPOSAL = 1 };
 struct Trial {
     int dev = 0;
+    int calls = 0;  // calls seen while pending: the first (cold) one is not timed
     hipEvent_t ev[2][kTrialSamples][2] = {};
     uint8_t state[2][kTrialSamples] = {};
     int valid[2] = {0, 0}, attempts[2] = {0, 0};
@@ -399,7 +401,9 @@ const ionp_config *advise(int kind, int op, int nin, const ionp_array_desc *cons
         launch(D, s);
         if (Trial *T = trial_get(k, D, dev, !capturing(s))) {
             trial_read(*T);
-            trial_reserve(*T, ARM_HEURISTIC, s, timing);
+            // a layout's first call pays one-time costs (first touch of its
+            // buffers, code loading): not a sample
+            if (T->calls++ > 0) trial_reserve(*T, ARM_HEURISTIC, s, timing);
         }
         return nullptr;
     }

@@ -47,8 +47,8 @@ ionp_status check_cfg(const ionp_config *cfg) {
         return set_error(IONP_ERR_VALUE, "ionp_config.block must be 64, 128, 192 or 256");
     if (cfg->vec != 0 && cfg->vec != 1 && cfg->vec != 2 && cfg->vec != 4 && cfg->vec != 8)
         return set_error(IONP_ERR_VALUE, "ionp_config.vec must be 1, 2, 4 or 8");
-    if (cfg->blocks_per_cu < 0 || cfg->blocks_per_cu > 64 || cfg->split < 0 || cfg->split > 64)
-        return set_error(IONP_ERR_VALUE, "ionp_config.blocks_per_cu and split must be in [0, 64]");
+    if (cfg->blocks_per_cu < 0 || cfg->blocks_per_cu > 4096 || cfg->split < 0 || cfg->split > 64)
+        return set_error(IONP_ERR_VALUE, "ionp_config.blocks_per_cu must be in [0, 4096] and split in [0, 64]");
     return IONP_OK;
 }
 

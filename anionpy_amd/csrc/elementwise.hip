@@ -16,6 +16,7 @@
 //                input, all operands of one dtype: 16-byte vector loads and
 //                stores, several vectors in flight per thread.
 //   ew_generic - any strides (rank <= 8 after coalescing), any dtype mix.
+// (where and casts over rows layouts: rows_select_cast.hip)
 #include "elementwise.h"
 
 #include "device_ops.h"
@@ -431,10 +432,12 @@ bool try_rows(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_statu
         ost1[d] = A.in_st[1][d];
     }
     const int oes = itemsize(A.out_dt);
-    // defaults measured on MI355X over the 8192 x 8192 broadcast workloads
-    // (f64/f32/int64, rows and stride-2 views): 32 workgroups per CU with 2
-    // vectors in flight per thread beat 8 per CU with 4 by 10-15%
-    int unroll = (cfg && cfg->split > 0) ? cfg->split : 2;
+    // defaults measured on MI355X over the 8192 x 8192 broadcast workloads:
+    // one 16-byte vector per thread and one tile per workgroup (no
+    // grid-stride loop; the dispatcher balances the short workgroups over
+    // the CUs) streams f64 at 6.5 TB/s, against 6.2 with 32 workgroups per
+    // CU looping over tiles and 5.6-6.0 with 2-4 vectors in flight
+    int unroll = (cfg && cfg->split > 0) ? cfg->split : 1;
     // IEEE flags tracked: the float arithmetic ops have flag variants (one
     // vector width, 4 in flight); other ops raise no flags
     const bool fpe = A.fpe && is_float_dt(A.loop_dt) && A.op <= IONP_TRUE_DIVIDE;
@@ -494,8 +497,9 @@ bool try_rows(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_statu
         R.nt_loads = in_bytes > kInfinityCacheBytes;
     }
     const int64_t ntiles = R.rows * R.tiles_per_row;
-    const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 32;
-    int64_t grid = (int64_t)bpc * num_cus();
+    // heuristic: a workgroup per tile; a configured blocks_per_cu caps the
+    // grid (the kernel loops over the remaining tiles)
+    int64_t grid = (cfg && cfg->blocks_per_cu > 0) ? (int64_t)cfg->blocks_per_cu * num_cus() : ntiles;
     if (grid > ntiles) grid = ntiles;
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(block), 0, s, R);
     *st = hipGetLastError() == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, "kernel launch failed");
@@ -852,6 +856,7 @@ ionp_status run_elementwise(EwArgs &A, const ionp_config *cfg, hipStream_t s) {
     ionp_status st;
     if (try_tiled(A, cfg, s, &st)) return st;
     if (try_rows(A, cfg, s, &st)) return st;
+    if (try_rows_select_cast(A, cfg, s, &st)) return st;
     return launch_generic(A, cfg, s);
 }
 

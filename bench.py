@@ -241,8 +241,9 @@ def _rate(nbytes, t):
 def elementwise_extras(ap, torch):
     """Elementwise and scan workloads beside the step: float64 broadcast add
     (row and column vector), the float64 X.T[::2] view, A + B.T (the
-    LDS-transposed operand) in float32 and float64, and float64 cumsum along
-    either axis, all 8192 x 8192."""
+    LDS-transposed operand) in float32 and float64, where(cond, A, column)
+    and the float64 -> float32 cast, and float64 cumsum along either axis,
+    all 8192 x 8192."""
     log("elementwise and scan workloads")
     out = {}
     A = ap.asarray(np.random.default_rng(0).uniform(-1, 1, (8192, 8192)))
@@ -264,6 +265,14 @@ def elementwise_extras(ap, torch):
         out[f"add_T_{np.dtype(dt).name}"] = _rate(Ad.nbytes + BT.nbytes + o.nbytes,
                                                   timed(torch, lambda: ap.add(Ad, BT, out=o), 10))
         del o, Bd, BT, Ad
+    cond = ap.asarray(np.random.default_rng(12).random((8192, 8192)) < 0.5)
+    o = ap.where(cond, A, col)
+    out["where_f64"] = _rate(cond.nbytes + A.nbytes + col.nbytes + o.nbytes,
+                             timed(torch, lambda: ap.where(cond, A, col), 10))
+    del o, cond
+    o = A.astype(np.float32)
+    out["cast_f64_f32"] = _rate(A.nbytes + o.nbytes, timed(torch, lambda: A.astype(np.float32), 10))
+    del o
     for ax in (0, 1):
         o = ap.add.accumulate(A, axis=ax)
         out[f"cumsum_f64_axis{ax}"] = _rate(A.nbytes + o.nbytes,

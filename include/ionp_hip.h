@@ -98,7 +98,7 @@ typedef struct {
 typedef struct {
     int32_t block;         /* threads per workgroup: 64, 128, 192 or 256 */
     int32_t vec;           /* elements per vector access: 1, 2, 4, 8 */
-    int32_t blocks_per_cu; /* grid size = blocks_per_cu * CUs (grid-stride) */
+    int32_t blocks_per_cu; /* grid size = blocks_per_cu * CUs, at most the work items (grid-stride) */
     int32_t split;         /* reduce: outputs per thread (narrow) / unroll */
 } ionp_config;
 
@@ -391,9 +391,10 @@ ionp_status ionp_autotune(const ionp_call *calls, int32_t n, const ionp_config *
  * layouts per pass, one pass in flight per device, never under graph
  * capture) and a later call installs the result once the pass has finished,
  * so no call waits for the model.  Each proposal is then verified on live
- * calls: two calls of the heuristic (the layout's first calls) and two of
- * the proposal are timed with HIP events, read back without waiting by
- * later calls, and the layout keeps whichever ran faster.  Results are
+ * calls: three calls of the heuristic (the layout's early calls, not its
+ * first) and three of the proposal are timed with HIP events, read back
+ * without waiting by later calls, and the layout keeps whichever ran
+ * faster.  Results are
  * bit-identical whatever runs.  ionp_reduce_noalloc, batches and
  * ionp_autotune always run the heuristic.  Thread-safe, except that
  * ionp_advisor_set must not run while other threads make calls; the cache

@@ -228,3 +228,69 @@ def test_blocked_accumulate(ap):
                     want = O.accumulate(op, h, ax)
                     got = getattr(ap, op).accumulate(d, axis=ax).get()
                     assert got.tobytes() == want.tobytes(), (shape, dt, ax, op)
+
+
+def _values(rng, shape, dt):
+    """Random values of dtype dt with the edge cases of casts mixed in."""
+    dt = np.dtype(dt)
+    n = int(np.prod(shape))
+    if dt.kind == "f":
+        v = rng.standard_normal(n) * 1e3
+        sp = np.array([np.nan, -np.nan, np.inf, -np.inf, 1e20, -1e20, -0.0, 0.5, -0.5, 2.0 ** 63, 2.0 ** 64, -1.0,
+                       2.0 ** 31, -2.0 ** 31 - 1, 255.5, 65535.9, 3e38, -3e38, 1e-310])
+        idx = rng.choice(n, size=min(n, 4 * len(sp)), replace=False)
+        v[idx] = np.resize(sp, len(idx))
+        with np.errstate(all="ignore"):
+            return v.astype(dt).reshape(shape)
+    if dt.kind == "b":
+        return (rng.random(n) < 0.5).reshape(shape)
+    info = np.iinfo(dt)
+    return rng.integers(info.min, info.max, n, dtype=dt, endpoint=True).reshape(shape)
+
+
+def test_rows_where_and_cast(ap):
+    """where and casts over rows layouts (the vectorised where / cast
+    kernels): every value dtype, every cast pair, NaN payloads and
+    out-of-range values, broadcast rows / columns, ragged tails, a 3-D
+    layout and a misaligned view (the generic kernel), byte-for-byte
+    against the oracle; one large case of each streams through the
+    nontemporal path."""
+    rng = np.random.default_rng(23)
+    dts = (np.bool_, np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64, np.uint64,
+           np.float32, np.float64)
+    for dt in dts:
+        # contiguous (one coalesced dim with a ragged tail), aligned rows of
+        # 1024 with broadcast columns / rows and a ragged 1000-wide view,
+        # a 3-D layout; then layouts the vector path declines (a 1003
+        # pitch, an odd element offset)
+        cond, x, y = rng.random((300, 1003)) < 0.5, _values(rng, (300, 1003), dt), _values(rng, (300, 1003), dt)
+        cond2, x2, y2 = rng.random((64, 1024)) < 0.5, _values(rng, (64, 1024), dt), _values(rng, (64, 1024), dt)
+        col, row = _values(rng, (64, 1), dt), _values(rng, (1, 1024), dt)
+        c3, x3, y3 = rng.random((3, 50, 776)) < 0.3, _values(rng, (3, 50, 776), dt), _values(rng, (1, 50, 1), dt)
+        cases = [(cond, x, y), (cond2, col, row), (cond2[:1], x2, col), (cond2, x2, y2[:1]),
+                 (cond2[:, :1000], x2[:, :1000], row[:, :1000]), (c3, x3, y3),
+                 (cond[:, :1000], x[:, :1000], y[:, :1000]), (cond[:, 1:], x[:, 1:], y[:, 1:])]
+        for c, a, b in cases:
+            want = O.where(c, a, b)
+            got = ap.where(ap.asarray(c), ap.asarray(a), ap.asarray(b)).get()
+            assert got.dtype == want.dtype and got.strides == want.strides, (dt, a.shape)
+            assert got.tobytes() == want.tobytes(), ("where", dt, c.shape, a.shape, b.shape)
+    for src in dts:
+        a, b = _values(rng, (200, 1001), src), _values(rng, (64, 1024), src)
+        ad, bd = ap.asarray(a), ap.asarray(b)
+        for to in dts:
+            # contiguous with a tail, aligned 2-D rows with a tail, then
+            # layouts the vector path declines (odd offset, 1001 pitch)
+            for view, vd in ((a, ad), (b[:, :1000], bd[:, :1000]), (a[:, 1:], ad[:, 1:]), (a[:7, :992], ad[:7, :992])):
+                want = O.cast(view, to)
+                got = vd.astype(to).get()
+                assert got.tobytes() == want.tobytes(), ("cast", np.dtype(src).name, np.dtype(to).name, view.shape)
+    # large: inputs beyond the Infinity Cache take nontemporal loads
+    big = rng.random((8192, 8192)) < 0.5
+    xb = rng.standard_normal((8192, 8192)).astype(np.float32)
+    yb = rng.standard_normal((8192, 1)).astype(np.float32)
+    got = ap.where(ap.asarray(big), ap.asarray(xb), ap.asarray(yb)).get()
+    assert got.tobytes() == O.where(big, xb, yb).tobytes()
+    del big, yb, got
+    xd = xb.astype(np.float64) * 1e35
+    assert ap.asarray(xd).astype(np.float32).get().tobytes() == O.cast(xd, np.float32).tobytes()
