@@ -277,10 +277,10 @@ def stats():
     layouts evaluated, layouts pending, and the trials' outcomes (each
     proposal is timed against the heuristic on live calls and the faster
     one kept)."""
-    v = (ctypes.c_int64 * 6)()
+    v = (ctypes.c_int64 * 7)()
     _call(_lib.load().ionp_advisor_stats, v)
     return {"learned_calls": v[0], "heuristic_calls": v[1], "layouts_evaluated": v[2], "pending": v[3],
-            "trials_kept_proposal": v[4], "trials_kept_heuristic": v[5]}
+            "trials_kept_proposal": v[4], "trials_kept_heuristic": v[5], "trials_running": v[6]}
 
 
 def lookup(spec):

@@ -9,7 +9,8 @@
 // batch of up to kMaxBatch layouts in flight per device), the logits come
 // back by an asynchronous copy into pinned host memory, and a later call
 // installs them once hipEventQuery reports the copy done.  No call waits for
-// the model, and nothing is launched on a stream under graph capture.
+// the model; a call on a stream under graph capture only reads the cache
+// (no event query, launch or timing enters the capture).
 //
 // A proposal is then verified on the calls themselves: the layout's early
 // calls (run with the heuristic while the proposal is pending; not the very
@@ -390,6 +391,20 @@ const ionp_config *advise(int kind, int op, int nin, const ionp_array_desc *cons
     if (ionp_call_features(&c, k.f) != IONP_OK) return nullptr;
     std::lock_guard<std::mutex> lock(g_mu);
     if (!g_model) return nullptr;
+    if (capturing(s)) {
+        // under graph capture only the cache is read: no event queries, no
+        // passes, no timing (a capture in global mode forbids them); a new
+        // layout is queued for the first call outside the capture
+        auto it = g_cache.find(k);
+        if (it == g_cache.end()) {
+            g_heuristic++;
+            enqueue(k);
+            return nullptr;
+        }
+        g_hits++;
+        *slot = it->second.cfg;
+        return slot;
+    }
     int dev = 0;
     (void)hipGetDevice(&dev);
     Device &D = device();
@@ -399,7 +414,7 @@ const ionp_config *advise(int kind, int op, int nin, const ionp_array_desc *cons
         g_heuristic++;
         enqueue(k);
         launch(D, s);
-        if (Trial *T = trial_get(k, D, dev, !capturing(s))) {
+        if (Trial *T = trial_get(k, D, dev, true)) {
             trial_read(*T);
             // a layout's first call pays one-time costs (first touch of its
             // buffers, code loading): not a sample
@@ -413,7 +428,7 @@ const ionp_config *advise(int kind, int op, int nin, const ionp_array_desc *cons
         trial_read(*tr->second);
         if (!trial_busy(*tr->second)) trial_drop(k, tr->second, D);
     } else if (!E.decided) {
-        Trial *T = capturing(s) ? nullptr : trial_get(k, D, dev, true);
+        Trial *T = trial_get(k, D, dev, true);
         if (!T) {
             if (g_trials.size() >= kMaxTrials) E.decided = true;  // no room: the proposal untried
         } else {
@@ -526,6 +541,7 @@ ionp_status ionp_advisor_stats(int64_t *stats) {
     stats[3] = (int64_t)g_pending.size();
     stats[4] = g_kept;
     stats[5] = g_reverted;
+    stats[6] = (int64_t)g_trials.size();
     return IONP_OK;
 }
 

@@ -142,6 +142,10 @@ def main():
         for (_, f, a, b), o in zip(ops, outs):
             f(a, b, out=o)
 
+    # the learned launch configuration settles first: each step layout's
+    # proposal is evaluated and verified against the heuristic on live calls
+    # (a trial); untimed steps run until no layout is pending or under trial
+    settle = settle_policy(torch, step)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -189,7 +193,8 @@ def main():
         with open(traffic_file) as f:
             roofline["traffic"] = json.load(f).get("mul_f64")
 
-    extra = {"config_policy": ap.get_config_policy(), "per_op": per_op, "fpe_tracking": fpe_tracking}
+    extra = {"config_policy": ap.get_config_policy(), "policy_settle_steps": settle, "per_op": per_op,
+             "fpe_tracking": fpe_tracking}
     del X64, c64, X32, c32, I, j, V32, v32, outs
     torch.cuda.empty_cache()
     if not args.no_extra:
@@ -208,6 +213,19 @@ def main():
                        "bytes_per_step": step_bytes, "frac_of_peak": round(value * 1e9 / PEAK_HBM, 4)},
             "roofline": roofline, "cpu_baseline": cpu, "extra": extra}
     print(json.dumps(line), flush=True)
+
+
+def settle_policy(torch, step, limit=40):
+    """Untimed steps until the advisor has no layout pending or under trial
+    (at most `limit`); returns how many ran."""
+    from anionpy_amd import costmodel as CM
+    for k in range(limit):
+        step()
+        torch.cuda.synchronize()
+        st = CM.stats()
+        if st["pending"] == 0 and st["trials_running"] == 0:
+            return k + 1
+    return limit
 
 
 def launch_config(ap, f, a, b, o):

@@ -388,8 +388,10 @@ ionp_status ionp_autotune(const ionp_call *calls, int32_t n, const ionp_config *
  * built-in heuristic).  Proposals are cached per layout (its feature vector).
  * A layout's first call runs the heuristic and queues the layout; the MLP's
  * forward kernels evaluate the queue on that call's stream (at most 256
- * layouts per pass, one pass in flight per device, never under graph
- * capture) and a later call installs the result once the pass has finished,
+ * layouts per pass, one pass in flight per device) and a later call
+ * installs the result once the pass has finished; a call on a stream under
+ * graph capture only reads the cache (no event query, pass or timing enters
+ * the capture),
  * so no call waits for the model.  Each proposal is then verified on live
  * calls: three calls of the heuristic (the layout's early calls, not its
  * first) and three of the proposal are timed with HIP events, read back
@@ -418,7 +420,8 @@ ionp_status ionp_advisor_prime(const ionp_call *calls, int32_t n, void *stream);
  * (the proposal, or the heuristic a trial kept), [1]: calls that ran the
  * heuristic while their layout's proposal was pending or under trial, [2]:
  * layouts evaluated, [3]: layouts queued or in flight, [4]: trials that
- * kept the proposal, [5]: trials that kept the heuristic. */
+ * kept the proposal, [5]: trials that kept the heuristic, [6]: trials in
+ * progress (stats holds 7 entries). */
 ionp_status ionp_advisor_stats(int64_t *stats);
 /* The cached proposal for a call's layout (*found = 0 if none yet). */
 ionp_status ionp_advisor_lookup(const ionp_call *call, ionp_config *cfg, int32_t *found);

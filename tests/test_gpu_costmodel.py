@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 import torch
 
+import oracle as O
 import oracle.mlp_ref as R
 
 pytestmark = pytest.mark.gpu
@@ -188,3 +189,41 @@ def test_advisor_trial_keeps_the_faster_arm(ap):
     before = CM.stats()["learned_calls"]
     ap.multiply(a, b)
     assert CM.stats()["learned_calls"] == before + 1  # decided: the advisor's choice, untimed
+
+
+def test_advisor_under_graph_capture(ap):
+    """Calls captured into a graph (torch's global capture mode) with the
+    learned policy: the advisor only reads its cache under capture (no event
+    query, pass or timing enters the graph), the capture succeeds, and every
+    replay writes the eager result."""
+    from anionpy_amd import costmodel as CM
+    ap.set_config_policy("learned")
+    rng = np.random.default_rng(41)
+    a = ap.asarray(rng.standard_normal((1000, 3000)))
+    b = ap.asarray(rng.standard_normal((1000, 1)))
+    a2 = ap.asarray(rng.standard_normal((700, 2048)))  # a layout first seen under capture
+    o, o2 = ap.multiply(a, b), ap.add(a2, 2.5)
+    want = o.get()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for _ in range(3):
+            ap.multiply(a, b, out=o)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        ap.multiply(a, b, out=o)
+        ap.add(a2, a2, out=o2)
+    want2 = O.binary("add", a2.get(), a2.get())
+    for _ in range(3):
+        o._buf.zero_()
+        o2._buf.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert o.get().tobytes() == want.tobytes()
+        assert o2.get().tobytes() == want2.tobytes()
+    ap.add(a2, a2, out=o2)  # outside the capture: the queued layout is evaluated now
+    torch.cuda.synchronize()
+    assert CM.stats()["pending"] <= 1
