@@ -397,38 +397,43 @@ typedef void (*AccFn)(const AccArgs);
 enum { ACC_THREAD = 0, ACC_WAVE = 1, ACC_TILED = 2, ACC_BLOCK_AX = 3, ACC_BLOCK_ADJ = 4 };
 constexpr int kAccBlockF = 16;  // fibers per acc_block workgroup
 
-template <int LT, int OP> static AccFn acc_pick(int kind) {
+// fibers per acc_block workgroup: 8, 16 (default) or 32 (ionp_config.split)
+template <int LT, int OP> static AccFn acc_pick(int kind, int fpb) {
     switch (kind) {
         case ACC_WAVE: return acc_wave<LT, OP>;
         case ACC_TILED: return acc_tiled<LT, OP>;
-        case ACC_BLOCK_AX: return acc_block<LT, OP, kAccBlockF, 256, true>;
-        case ACC_BLOCK_ADJ: return acc_block<LT, OP, kAccBlockF, 128, false>;
+        case ACC_BLOCK_AX:
+            return fpb == 8 ? acc_block<LT, OP, 8, 256, true>
+                 : fpb == 32 ? acc_block<LT, OP, 32, 128, true> : acc_block<LT, OP, 16, 256, true>;
+        case ACC_BLOCK_ADJ:
+            return fpb == 8 ? acc_block<LT, OP, 8, 256, false>
+                 : fpb == 32 ? acc_block<LT, OP, 32, 64, false> : acc_block<LT, OP, 16, 128, false>;
         default: return acc_thread<LT, OP>;
     }
 }
 
-template <int LT> static AccFn acc_fn(int op, int kind) {
+template <int LT> static AccFn acc_fn(int op, int kind, int fpb) {
     switch (op) {
-        case IONP_ADD: return acc_pick<LT, IONP_ADD>(kind);
-        case IONP_MULTIPLY: return acc_pick<LT, IONP_MULTIPLY>(kind);
-        case IONP_MAXIMUM: return acc_pick<LT, IONP_MAXIMUM>(kind);
-        default: return acc_pick<LT, IONP_MINIMUM>(kind);
+        case IONP_ADD: return acc_pick<LT, IONP_ADD>(kind, fpb);
+        case IONP_MULTIPLY: return acc_pick<LT, IONP_MULTIPLY>(kind, fpb);
+        case IONP_MAXIMUM: return acc_pick<LT, IONP_MAXIMUM>(kind, fpb);
+        default: return acc_pick<LT, IONP_MINIMUM>(kind, fpb);
     }
 }
 
-static AccFn acc_kernel(int dt, int op, int wave) {
+static AccFn acc_kernel(int dt, int op, int wave, int fpb) {
     switch (dt) {
-        case IONP_BOOL: return acc_fn<IONP_BOOL>(op, wave);
-        case IONP_INT8: return acc_fn<IONP_INT8>(op, wave);
-        case IONP_UINT8: return acc_fn<IONP_UINT8>(op, wave);
-        case IONP_INT16: return acc_fn<IONP_INT16>(op, wave);
-        case IONP_UINT16: return acc_fn<IONP_UINT16>(op, wave);
-        case IONP_INT32: return acc_fn<IONP_INT32>(op, wave);
-        case IONP_UINT32: return acc_fn<IONP_UINT32>(op, wave);
-        case IONP_INT64: return acc_fn<IONP_INT64>(op, wave);
-        case IONP_UINT64: return acc_fn<IONP_UINT64>(op, wave);
-        case IONP_FLOAT32: return acc_fn<IONP_FLOAT32>(op, wave);
-        default: return acc_fn<IONP_FLOAT64>(op, wave);
+        case IONP_BOOL: return acc_fn<IONP_BOOL>(op, wave, fpb);
+        case IONP_INT8: return acc_fn<IONP_INT8>(op, wave, fpb);
+        case IONP_UINT8: return acc_fn<IONP_UINT8>(op, wave, fpb);
+        case IONP_INT16: return acc_fn<IONP_INT16>(op, wave, fpb);
+        case IONP_UINT16: return acc_fn<IONP_UINT16>(op, wave, fpb);
+        case IONP_INT32: return acc_fn<IONP_INT32>(op, wave, fpb);
+        case IONP_UINT32: return acc_fn<IONP_UINT32>(op, wave, fpb);
+        case IONP_INT64: return acc_fn<IONP_INT64>(op, wave, fpb);
+        case IONP_UINT64: return acc_fn<IONP_UINT64>(op, wave, fpb);
+        case IONP_FLOAT32: return acc_fn<IONP_FLOAT32>(op, wave, fpb);
+        default: return acc_fn<IONP_FLOAT64>(op, wave, fpb);
     }
 }
 
@@ -486,13 +491,14 @@ ionp_status run_accumulate(int op, const ionp_array_desc *in, int axis, const io
     else if (A.in_st == 1 && A.nfib >= 256) kind = ACC_TILED;
     else if (A.nfib < 4096 && A.len >= 256) kind = ACC_WAVE;
     else kind = ACC_THREAD;
-    const AccFn fn = acc_kernel(out->dtype, op, kind);
     const bool block = kind == ACC_BLOCK_AX || kind == ACC_BLOCK_ADJ;
+    const int fpb = (cfg && (cfg->split == 8 || cfg->split == 32)) ? cfg->split : kAccBlockF;
+    const AccFn fn = acc_kernel(out->dtype, op, kind, fpb);
     const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : 8;
     const int threads = kind == ACC_THREAD || block ? 256 : 64;
     const int64_t units = kind == ACC_THREAD ? (A.nfib + 255) / 256
                           : kind == ACC_TILED ? (A.nfib + kAccFpw - 1) / kAccFpw
-                          : block             ? (A.nfib + kAccBlockF - 1) / kAccBlockF
+                          : block             ? (A.nfib + fpb - 1) / fpb
                                               : A.nfib;
     int64_t grid = (int64_t)bpc * num_cus() * (threads == 64 ? 4 : 1);
     if (grid > units) grid = units;

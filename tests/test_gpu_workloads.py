@@ -228,6 +228,30 @@ def test_blocked_accumulate(ap):
                     want = O.accumulate(op, h, ax)
                     got = getattr(ap, op).accumulate(d, axis=ax).get()
                     assert got.tobytes() == want.tobytes(), (shape, dt, ax, op)
+                    if op == "add":  # 8 and 32 fibers per workgroup (ionp_config.split)
+                        for fpb in (8, 32):
+                            o = getattr(ap, op).accumulate(d, axis=ax)
+                            o._buf.fill_(0xAB)
+                            rc = _lib_accumulate(ap, _lib_code(op), d, ax, o, split=fpb)
+                            assert rc == 0 and o.get().tobytes() == want.tobytes(), (shape, dt, ax, op, fpb)
+
+
+def _lib_code(op):
+    from anionpy_amd import _lib
+    return {"add": _lib.ADD, "multiply": _lib.MULTIPLY, "maximum": _lib.MAXIMUM}[op]
+
+
+def _lib_accumulate(ap, code, x, axis, out, split=0):
+    """ionp_accumulate through the C ABI with an explicit configuration."""
+    import ctypes
+    from anionpy_amd import _lib
+    dx, do = x._desc(), out._desc()
+    cfg = _lib.Config(0, 0, 0, split)
+    rc = _lib.load().ionp_accumulate(code, ctypes.byref(dx), axis, ctypes.byref(do), ctypes.byref(cfg),
+                                     _lib.stream_handle())
+    import torch
+    torch.cuda.synchronize()
+    return rc
 
 
 def _values(rng, shape, dt):
