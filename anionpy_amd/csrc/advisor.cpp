@@ -39,6 +39,7 @@ constexpr size_t kMaxQueue = 4096;  // layouts waiting for a pass (later ones: h
 constexpr int kTrialSamples = 3;    // timed calls per arm (heuristic, proposal) of a trial
 constexpr int kTrialAttempts = 8;   // timings issued per arm before deciding on what was measured
 constexpr size_t kMaxTrials = 1024; // layouts under trial at once (later ones keep the proposal untried)
+constexpr size_t kMaxCache = 1u << 20;  // layouts with a proposal (~150 B each); later new layouts: heuristic
 
 struct Key {
     float f[IONP_NFEATURES];
@@ -242,7 +243,9 @@ Device &device() {
 }
 
 void enqueue(const Key &k) {
-    if (g_cache.count(k) || g_pending.count(k) || g_queue.size() >= kMaxQueue) return;
+    if (g_cache.count(k) || g_pending.count(k) || g_queue.size() >= kMaxQueue ||
+        g_cache.size() + g_pending.size() >= kMaxCache)
+        return;
     g_queue.push_back(k);
     g_pending[k] = 1;
 }
