@@ -54,8 +54,10 @@ def parse():
 def roofline_pmc_workload():
     """ew_rows<double,multiply> on X(8192,8192) * c(8192,1), then torch's
     device copy of 2^26 float64 (536,870,912 B read and written), 5 times
-    each: the counter passes scale the multiply's FETCH_SIZE / WRITE_SIZE by
-    what the same counters read for the copy."""
+    each after the launch configuration has settled (the last 5 multiply
+    launches are the ones measured): the counter passes scale the
+    multiply's FETCH_SIZE / WRITE_SIZE by what the same counters read for
+    the copy."""
     import torch
     import anionpy_amd as ap
     torch.cuda.set_device(0)
@@ -65,7 +67,8 @@ def roofline_pmc_workload():
     out = ap.multiply(X, c)
     src = torch.ones(2 ** 26, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
-    torch.cuda.synchronize()
+    # the launch configuration settles as in the timed bench run
+    settled = settle_policy(torch, lambda: ap.multiply(X, c, out=out))
     for _ in range(5):
         ap.multiply(X, c, out=out)
     torch.cuda.synchronize()
@@ -73,7 +76,8 @@ def roofline_pmc_workload():
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"multiply_bytes": X.nbytes + c.nbytes + out.nbytes, "copy_bytes_each_way": src.numel() * 8,
-                      "launches": 5}), flush=True)
+                      "launches": 5, "settle_calls": settled, "config": launch_config(ap, ap.multiply, X, c, out)}),
+          flush=True)
 
 
 def log(*a):
