@@ -4,7 +4,8 @@
 // Replaces where_select's select loop (ionp-core/src/sort.rs:1885-1913) and
 // Buffer::cast_to's per-element conversion (ionp-core/src/buffer.rs:81-210)
 // for "rows x inner" layouts: the innermost coalesced axis is contiguous in
-// the output and contiguous or broadcast (stride 0) in every input.
+// the output and contiguous, reversed (stride -1) or broadcast (stride 0)
+// in every input.
 //   ew_rows_where<T> - a bool condition and two value operands of the
 //                      output's dtype: a select of bit patterns, so one
 //                      kernel per item size serves every dtype.
@@ -32,7 +33,7 @@ struct RowArgs3 {
     int64_t oshape[kMaxKDims];
     FastDiv odiv[kMaxKDims];
     int64_t ost[3][kMaxKDims], ost_out[kMaxKDims];
-    int64_t inner_st[3];  // 0 (broadcast) or 1
+    int64_t inner_st[3];  // 0 (broadcast), 1 or -1 (a reversed axis)
     const char *in[3];
     char *out;
     int32_t nt_loads;     // stream the inputs with nontemporal loads
@@ -69,6 +70,17 @@ template <typename T, int V> __device__ inline VecT<T, V> vload(const T *p, bool
     return nt ? __builtin_nontemporal_load((const VecT<T, V> *)p) : *(const VecT<T, V> *)p;
 }
 
+// elements i .. i + V - 1 of a row read at stride st (1, or -1: a reversed
+// axis, whose V elements are the aligned vector ending at p - i, reversed)
+template <typename T, int V> __device__ inline VecT<T, V> vload_dir(const T *p, int64_t i, int64_t st, bool nt) {
+    if (st > 0) return vload<T, V>(p + i, nt);
+    const VecT<T, V> v = vload<T, V>(p - i - (V - 1), nt);
+    VecT<T, V> r;
+#pragma unroll
+    for (int k = 0; k < V; k++) r[k] = v[V - 1 - k];
+    return r;
+}
+
 // chunks per thread of the where kernel: fewer for narrow items, whose
 // select unpacks every item into a register
 template <typename T> constexpr int where_u() { return sizeof(T) >= 4 ? kRowsU : (sizeof(T) == 2 ? 2 : 1); }
@@ -95,9 +107,9 @@ template <typename T> __global__ __launch_bounds__(256) void ew_rows_where(const
         for (int j = 0; j < U; j++) {
             const int64_t i = i0 + ((int64_t)j * blockDim.x + threadIdx.x) * V;
             if (i + V <= R.inner) {
-                if (R.inner_st[0]) c[j] = vload<uint8_t, V>(pc + i, nt);
-                if (R.inner_st[1]) x[j] = vload<T, V>(px + i, nt);
-                if (R.inner_st[2]) y[j] = vload<T, V>(py + i, nt);
+                if (R.inner_st[0]) c[j] = vload_dir<uint8_t, V>(pc, i, R.inner_st[0], nt);
+                if (R.inner_st[1]) x[j] = vload_dir<T, V>(px, i, R.inner_st[1], nt);
+                if (R.inner_st[2]) y[j] = vload_dir<T, V>(py, i, R.inner_st[2], nt);
             }
         }
 #pragma unroll
@@ -142,7 +154,7 @@ template <int LI, int LO> __global__ __launch_bounds__(256) void ew_rows_cast(co
 #pragma unroll
             for (int j = 0; j < U; j++) {
                 const int64_t i = i0 + ((int64_t)j * blockDim.x + threadIdx.x) * V;
-                if (i + V <= R.inner) a[j] = vload<TI, V>(pa + i, nt);
+                if (i + V <= R.inner) a[j] = vload_dir<TI, V>(pa, i, R.inner_st[0], nt);
             }
         }
         const TO b = R.inner_st[0] ? TO(0) : conv(pa[0]);
@@ -213,8 +225,8 @@ bool try_rows_select_cast(const EwArgs &A, const ionp_config *cfg, hipStream_t s
     if (A.ndim > 4 || A.shape[0] < 64 || A.out_st[0] != 1) return false;
     const int nin = A.op == OP_WHERE ? 3 : 1;
     if (A.nin != nin) return false;
-    for (int k = 0; k < nin; k++)
-        if (A.in_st[k][0] != 0 && A.in_st[k][0] != 1) return false;
+    for (int k = 0; k < nin; k++)  // contiguous, broadcast or reversed along the inner axis
+        if (A.in_st[k][0] != 0 && A.in_st[k][0] != 1 && A.in_st[k][0] != -1) return false;
     int isz[3] = {0, 0, 0}, V, U = kRowsU;
     const int osz = itemsize(A.out_dt);
     Row3Fn fn;
@@ -233,11 +245,12 @@ bool try_rows_select_cast(const EwArgs &A, const ionp_config *cfg, hipStream_t s
         fn = cast_kernel(A.in_dt[0], A.out_dt);
     }
     if (!fn) return false;
-    // every contiguous row must start on its vector access size (V items)
+    // every contiguous row must start on its vector access size (V items);
+    // a reversed row must end there (its first element is the last of a vector)
     auto aligned = [&](const char *p, int sz, const int64_t *st, int64_t inner_st) {
         if (inner_st == 0) return true;
         const int64_t vb = (int64_t)V * sz;
-        if ((uintptr_t)p % (uintptr_t)vb) return false;
+        if (((uintptr_t)p + (inner_st < 0 ? (uintptr_t)sz : 0)) % (uintptr_t)vb) return false;
         for (int d = 1; d < A.ndim; d++)
             if ((st[d] * sz) % vb) return false;
         return true;

@@ -293,6 +293,7 @@ def test_rows_where_and_cast(ap):
         c3, x3, y3 = rng.random((3, 50, 776)) < 0.3, _values(rng, (3, 50, 776), dt), _values(rng, (1, 50, 1), dt)
         cases = [(cond, x, y), (cond2, col, row), (cond2[:1], x2, col), (cond2, x2, y2[:1]),
                  (cond2[:, :1000], x2[:, :1000], row[:, :1000]), (c3, x3, y3),
+                 (cond2, x2[:, ::-1], y2), (cond2[:, ::-1], col, y2[:, ::-1]), (cond2[:, :100], x2[:, 99::-1], y2[:, :100]),
                  (cond[:, :1000], x[:, :1000], y[:, :1000]), (cond[:, 1:], x[:, 1:], y[:, 1:])]
         for c, a, b in cases:
             want = O.where(c, a, b)
@@ -305,7 +306,8 @@ def test_rows_where_and_cast(ap):
         for to in dts:
             # contiguous with a tail, aligned 2-D rows with a tail, then
             # layouts the vector path declines (odd offset, 1001 pitch)
-            for view, vd in ((a, ad), (b[:, :1000], bd[:, :1000]), (a[:, 1:], ad[:, 1:]), (a[:7, :992], ad[:7, :992])):
+            for view, vd in ((a, ad), (b[:, :1000], bd[:, :1000]), (b[:, ::-1], bd[:, ::-1]),
+                             (b[:, 999::-1], bd[:, 999::-1]), (a[:, 1:], ad[:, 1:]), (a[:7, :992], ad[:7, :992])):
                 want = O.cast(view, to)
                 got = vd.astype(to).get()
                 assert got.tobytes() == want.tobytes(), ("cast", np.dtype(src).name, np.dtype(to).name, view.shape)
