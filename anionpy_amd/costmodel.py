@@ -41,7 +41,7 @@ NOUT = int(HEAD_OFF[-1])
 # knob labels of grid entries 1.. (the full knob product)
 GRID_LABELS = np.array([knob_labels(i) for i in range(1, len(GRID))], dtype=np.int32)
 TAU = 0.01       # soft-target temperature, in units of relative slowdown
-FALLBACK = 0.01  # the heuristic is preferred unless the knobs' best is this much faster
+FALLBACK = 0.03  # the heuristic is preferred unless the knobs' best is this much faster
 
 
 def _ptr(t):

@@ -227,3 +227,44 @@ def test_advisor_under_graph_capture(ap):
     ap.add(a2, a2, out=o2)  # outside the capture: the queued layout is evaluated now
     torch.cuda.synchronize()
     assert CM.stats()["pending"] <= 1
+
+
+def test_advisor_two_threads(ap):
+    """Two host threads make learned-policy calls on many new layouts at
+    once (the library runs them concurrently: ctypes releases the GIL), so
+    proposals, passes and trials of both threads interleave; every result
+    matches the oracle."""
+    import threading
+    ap.set_config_policy("learned")
+    rng = np.random.default_rng(53)
+    work = []
+    for t in range(2):
+        items = []
+        for k in range(24):
+            shape = (37 + 11 * k + t, 129 + 7 * k)
+            a = rng.standard_normal(shape)
+            b = rng.standard_normal((shape[0], 1))
+            items.append((a, b, ap.asarray(a), ap.asarray(b)))
+        work.append(items)
+    errors = []
+
+    def run(items):
+        try:
+            for _ in range(6):
+                for a, b, ad, bd in items:
+                    got = ap.multiply(ad, bd)
+                    cast = ad.astype(np.float32)
+                    torch.cuda.synchronize()
+                    if got.get().tobytes() != O.binary("multiply", a, b).tobytes():
+                        errors.append(("multiply", a.shape))
+                    if cast.get().tobytes() != O.cast(a, np.float32).tobytes():
+                        errors.append(("cast", a.shape))
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=run, args=(w,)) for w in work]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors[:5]
