@@ -19,6 +19,8 @@ float32 (4096,65536) sums, maximum/argmax, also with NaN at rate 1e-6), the
 uneven batch and the learned configuration.
 """
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -82,6 +84,19 @@ def roofline_pmc_workload():
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+@contextlib.contextmanager
+def quiet_gc():
+    """A timed region without Python garbage-collection pauses: collected
+    first, the collector off inside (a pause of tens of ms inside a loop of
+    launches leaves the GPU idle and inflates the time)."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
 
 
 def main():
@@ -154,11 +169,12 @@ def main():
         step()
     torch.cuda.synchronize()
     log(f"timing {args.steps} steps")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    with quiet_gc():
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
     ms = dt / args.steps * 1e3
     value = step_bytes * args.steps / dt / 1e9
 
@@ -168,11 +184,12 @@ def main():
         s = torch.cuda.current_stream()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         reps = 10
-        e0.record(s)
-        for _ in range(reps):
-            f(a, b, out=o)
-        e1.record(s)
-        e1.synchronize()
+        with quiet_gc():
+            e0.record(s)
+            for _ in range(reps):
+                f(a, b, out=o)
+            e1.record(s)
+            e1.synchronize()
         t = e0.elapsed_time(e1) / reps * 1e-3
         per_op[name] = {"us": round(t * 1e6, 1), "GB/s": round(alg_bytes(a, b, o) / t / 1e9, 1),
                         "frac": round(alg_bytes(a, b, o) / t / PEAK_HBM, 4),
@@ -182,10 +199,11 @@ def main():
     with ap.errstate(divide="warn", over="warn", invalid="warn"):
         f64o = outs[0]
         ap.multiply(X64, c64, out=f64o)
-        t0 = time.perf_counter()
-        for _ in range(10):
-            ap.multiply(X64, c64, out=f64o)
-        tt = (time.perf_counter() - t0) / 10
+        with quiet_gc():
+            t0 = time.perf_counter()
+            for _ in range(10):
+                ap.multiply(X64, c64, out=f64o)
+            tt = (time.perf_counter() - t0) / 10
     fpe_tracking = {"op": "mul_f64", "us": round(tt * 1e6, 1), "GB/s": round(alg_bytes(X64, c64, f64o) / tt / 1e9, 1),
                     "note": "per call incl. the flag fetch (host wall clock)"}
     dom = per_op["mul_f64"]
@@ -248,11 +266,12 @@ def timed(torch, fn, reps):
         fn()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(s)
-    for _ in range(reps):
-        fn()
-    e1.record(s)
-    e1.synchronize()
+    with quiet_gc():
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        e1.synchronize()
     return e0.elapsed_time(e1) / reps * 1e-3
 
 
@@ -361,11 +380,12 @@ def batch_extras(ap, torch):
         bt.run(mode)  # warm (the graph is captured here)
         torch.cuda.synchronize()
         reps = 3 if mode == "per_call" else 10
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            bt.run(mode)
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / reps
+        with quiet_gc():
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                bt.run(mode)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / reps
         res[mode] = {"ms": round(dt * 1e3, 3), "calls_per_s": round(len(specs) / dt, 1),
                      "GB/s": round(info["bytes"] / dt / 1e9, 1)}
         outs[mode] = [s.out._buf.clone() for s in specs]
@@ -383,11 +403,12 @@ def batch_extras(ap, torch):
     tune_s = time.perf_counter() - t0
     tb.run("persistent")
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(10):
-        tb.run("persistent")
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / 10
+    with quiet_gc():
+        t0 = time.perf_counter()
+        for _ in range(10):
+            tb.run("persistent")
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 10
     tuned_same = all(torch.equal(a, s.out._buf) for a, s in zip(outs["persistent"], specs))
     res["persistent_tuned"] = {"ms": round(dt * 1e3, 3), "calls_per_s": round(len(specs) / dt, 1),
                                "GB/s": round(info["bytes"] / dt / 1e9, 1),
