@@ -365,14 +365,20 @@ void AdviseTiming::begin() {
 
 AdviseTiming::~AdviseTiming() {
     if (!trial) return;
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (generation != g_generation) {
+        // the trial went with the model it was for; ionp_advisor_set left
+        // this reserved slot's events to the call that holds them
+        (void)hipEventDestroy(start);
+        (void)hipEventDestroy(end);
+        return;
+    }
     bool recorded = false;
     if (started) {
         recorded = hipEventRecord(end, stream) == hipSuccess;
         if (!recorded) (void)hipGetLastError();
     }
-    std::lock_guard<std::mutex> lock(g_mu);
-    if (generation == g_generation)  // else the trial went with the model it was for
-        static_cast<Trial *>(trial)->state[arm][slot] = recorded ? SLOT_RECORDED : SLOT_FREE;
+    static_cast<Trial *>(trial)->state[arm][slot] = recorded ? SLOT_RECORDED : SLOT_FREE;
 }
 
 const ionp_config *advise(int kind, int op, int nin, const ionp_array_desc *const *ins, const ionp_array_desc *out,
@@ -495,8 +501,13 @@ ionp_status ionp_advisor_set(const int32_t *sizes, int32_t nlayers, const float 
     g_generation++;
     for (auto &kv : g_trials) {
         for (int a = 0; a < 2; a++)
-            for (int j = 0; j < kTrialSamples; j++)
+            for (int j = 0; j < kTrialSamples; j++) {
+                // a reserved slot's call (on another thread) still records
+                // into its events: it destroys them when it sees the new
+                // generation
+                if (kv.second->state[a][j] == SLOT_RESERVED) continue;
                 for (int e = 0; e < 2; e++) (void)hipEventDestroy(kv.second->ev[a][j][e]);
+            }
         delete kv.second;
     }
     g_trials.clear();

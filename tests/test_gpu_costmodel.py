@@ -268,3 +268,53 @@ def test_advisor_two_threads(ap):
     for th in threads:
         th.join()
     assert not errors, errors[:5]
+
+
+def test_advisor_model_swapped_under_running_calls(ap):
+    """One thread makes learned-policy calls on new layouts (so trial slots
+    are reserved while its calls run) while another swaps the model in and
+    out thirty times; a swap never destroys the events a running call holds,
+    and every result matches the oracle."""
+    import threading
+    ap.set_config_policy("learned")
+    rng = np.random.default_rng(59)
+    items = []
+    for k in range(32):
+        shape = (41 + 13 * k, 97 + 5 * k)
+        a = rng.standard_normal(shape)
+        b = rng.standard_normal((1, shape[1]))
+        items.append((a, b, ap.asarray(a), ap.asarray(b)))
+    errors = []
+    stop = threading.Event()
+
+    def calls():
+        try:
+            for _ in range(4):
+                for a, b, ad, bd in items:
+                    got = ap.add(ad, bd)
+                    torch.cuda.synchronize()
+                    if got.get().tobytes() != O.binary("add", a, b).tobytes():
+                        errors.append(("add", a.shape))
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(repr(e))
+        finally:
+            stop.set()
+
+    def swaps():
+        try:
+            for i in range(30):
+                if stop.is_set():
+                    break
+                ap.set_config_policy("heuristic" if i % 2 else "learned")
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=calls), threading.Thread(target=swaps)]
+    try:
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+    finally:
+        ap.set_config_policy("learned")
+    assert not errors, errors[:5]
