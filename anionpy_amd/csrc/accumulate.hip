@@ -280,10 +280,14 @@ template <int LT, int OP> __global__ __launch_bounds__(64) void acc_tiled(const 
 // -- F lanes each run their fiber's sequential chain over the block in LDS
 // (NumPy's order, one dependent op per step), and all threads store the
 // block.  The next block's loads are in flight during the chain.  Measured
-// on MI355X, float64 8192 x 8192 cumsum: axis 0 (F 16, R 128) 189 us, axis 1
-// (F 16, R 256) 211 us (5.7 / 5.1 TB/s), where a thread per fiber, or 16
-// fibers per wavefront in 64-element chunks, kept too few bytes in flight
-// (685 / 529 us).
+// on MI355X, 8192 x 8192 cumsum: float64 axis 0 (F 16, R 128) 204-210 us,
+// axis 1 (F 16, R 256) 210-215 us (5.0-5.3 TB/s); float32 axis 0 163 us,
+// axis 1 135 us (with a guard on each load and store: 170 / 148 us, the
+// compiler then waiting for the previous block's stores on every block);
+// a thread per fiber, or 16 fibers per wavefront in 64-element chunks, kept
+// too few bytes in flight (float64 685 / 529 us).  Neither two blocks of
+// loads in flight nor two LDS buffers (stores drained during the next
+// chain) ran faster.
 template <int LT, int OP, int F, int R, bool AX1> __global__ __launch_bounds__(256) void acc_block(const AccArgs A) {
     using T = typename dt_type<LT>::T;
     constexpr int PER = F * R / 256;  // elements per thread per block
@@ -311,31 +315,43 @@ template <int LT, int OP, int F, int R, bool AX1> __global__ __launch_bounds__(2
         __syncthreads();
         const T *x = (const T *)A.in;
         T *o = (T *)A.out;
-        // this thread's elements: offsets at step 0 of a block, fiber in range
-        int64_t pin[PER], pout[PER];
-        bool ok[PER];
+        // Every load and store is issued unguarded, so the compiler's wait
+        // for a block's loads can count exactly and leave the previous
+        // block's stores in flight (behind a guard that may skip them it
+        // waits for everything, the stores' write latency included, on every
+        // block).  An element outside the group's fibers or past the axis
+        // end uses the nearest one inside instead: its load reads a valid
+        // address and is never used, its store rewrites that element's own
+        // value at its own address.
+        int fc[PER];  // this thread's fibers, clamped to the group
+        int64_t ib[PER], ob[PER];
 #pragma unroll
         for (int e = 0; e < PER; e++) {
-            const int f = fl(e), k = st(e);
-            ok[e] = f < nf;
-            pin[e] = ok[e] ? sib[f] + k * A.in_st : 0;
-            pout[e] = ok[e] ? sob[f] + k * A.out_st : 0;
+            fc[e] = min(fl(e), nf - 1);
+            ib[e] = sib[fc[e]];
+            ob[e] = sob[fc[e]];
         }
-        auto load = [&](int64_t i0, T (&d)[PER]) {
-            const bool whole = i0 + R <= A.len;
+        T regs[PER];
+        auto load = [&](int64_t i0) {  // block i0 (steps past the end: the last step)
+            const int64_t lim = A.len - 1 - i0;
 #pragma unroll
-            for (int e = 0; e < PER; e++)
-                d[e] = (ok[e] && (whole || i0 + st(e) < A.len)) ? __builtin_nontemporal_load(x + pin[e] + i0 * A.in_st)
-                                                                : T(0);
+            for (int e = 0; e < PER; e++) {
+                const int64_t k = min((int64_t)st(e), lim);
+                regs[e] = __builtin_nontemporal_load(x + ib[e] + (i0 + k) * A.in_st);
+            }
+        };
+        auto stage = [&]() {
+#pragma unroll
+            for (int e = 0; e < PER; e++) lds(fl(e), st(e)) = regs[e];
         };
         T acc = T(0);
-        T cur[PER], nxt[PER];
-        load(0, cur);
+        load(0);
+        stage();
+        // per block of R steps, staged in LDS: the next block's loads are
+        // issued, the chains run, the block is stored, the next is staged
         for (int64_t i0 = 0; i0 < A.len; i0 += R) {
             const int m = (int)min((int64_t)R, A.len - i0);
-#pragma unroll
-            for (int e = 0; e < PER; e++) lds(fl(e), st(e)) = cur[e];
-            if (i0 + R < A.len) load(i0 + R, nxt);
+            load(i0 + R);  // past the end: rereads the last step, unused
             __syncthreads();
             if (tid < nf) {  // the chain of fiber `tid` over this block
                 int k = 0;
@@ -360,11 +376,12 @@ template <int LT, int OP, int F, int R, bool AX1> __global__ __launch_bounds__(2
             }
             __syncthreads();
 #pragma unroll
-            for (int e = 0; e < PER; e++)
-                if (ok[e] && st(e) < m) __builtin_nontemporal_store(lds(fl(e), st(e)), o + pout[e] + i0 * A.out_st);
-            __syncthreads();
-#pragma unroll
-            for (int e = 0; e < PER; e++) cur[e] = nxt[e];
+            for (int e = 0; e < PER; e++) {
+                const int k = min(st(e), m - 1);
+                __builtin_nontemporal_store(lds(fc[e], k), o + ob[e] + (i0 + k) * A.out_st);
+            }
+            __syncthreads();  // also orders every wave's stores before a fiber's redo below
+            stage();
         }
         if constexpr (acc_plain<T, OP>()) {
             if (tid < nf && acc != acc) {
