@@ -87,13 +87,20 @@ def log(*a):
 
 
 @contextlib.contextmanager
-def quiet_gc():
+def quiet_gc(torch, warm):
     """A timed region without Python garbage-collection pauses: collected
     first, the collector off inside (a pause of tens of ms inside a loop of
-    launches leaves the GPU idle and inflates the time)."""
+    launches leaves the GPU idle and inflates the time).  `warm` runs after
+    the collection and before the region, and the device synchronises: the
+    first launches after a collection are slow on the host (measured on
+    MI355X: 10 float64 8192 x 8192 adds queued in 300-350 us instead of
+    200-220 us), which idles the GPU at the start of the region and added
+    about 10 us per call to a 10-call loop."""
     gc.collect()
     gc.disable()
     try:
+        warm()
+        torch.cuda.synchronize()
         yield
     finally:
         gc.enable()
@@ -169,7 +176,7 @@ def main():
         step()
     torch.cuda.synchronize()
     log(f"timing {args.steps} steps")
-    with quiet_gc():
+    with quiet_gc(torch, step):
         t0 = time.perf_counter()
         for _ in range(args.steps):
             step()
@@ -184,7 +191,7 @@ def main():
         s = torch.cuda.current_stream()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         reps = 10
-        with quiet_gc():
+        with quiet_gc(torch, lambda: f(a, b, out=o)):
             e0.record(s)
             for _ in range(reps):
                 f(a, b, out=o)
@@ -198,8 +205,7 @@ def main():
     # variant of the kernel plus a stream synchronisation and a 4-byte read
     with ap.errstate(divide="warn", over="warn", invalid="warn"):
         f64o = outs[0]
-        ap.multiply(X64, c64, out=f64o)
-        with quiet_gc():
+        with quiet_gc(torch, lambda: ap.multiply(X64, c64, out=f64o)):
             t0 = time.perf_counter()
             for _ in range(10):
                 ap.multiply(X64, c64, out=f64o)
@@ -266,7 +272,7 @@ def timed(torch, fn, reps):
         fn()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    with quiet_gc():
+    with quiet_gc(torch, fn):
         e0.record(s)
         for _ in range(reps):
             fn()
@@ -380,7 +386,7 @@ def batch_extras(ap, torch):
         bt.run(mode)  # warm (the graph is captured here)
         torch.cuda.synchronize()
         reps = 3 if mode == "per_call" else 10
-        with quiet_gc():
+        with quiet_gc(torch, lambda: bt.run(mode)):
             t0 = time.perf_counter()
             for _ in range(reps):
                 bt.run(mode)
@@ -403,7 +409,7 @@ def batch_extras(ap, torch):
     tune_s = time.perf_counter() - t0
     tb.run("persistent")
     torch.cuda.synchronize()
-    with quiet_gc():
+    with quiet_gc(torch, lambda: tb.run("persistent")):
         t0 = time.perf_counter()
         for _ in range(10):
             tb.run("persistent")
