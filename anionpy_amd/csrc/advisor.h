@@ -19,6 +19,11 @@ struct AdviseTiming {
     hipStream_t stream = nullptr;
     bool started = false;
     void begin();
+    // the call failed after begin(): its slot is freed without a sample
+    ionp_status done(ionp_status s) {
+        if (s != IONP_OK) started = false;
+        return s;
+    }
     ~AdviseTiming();
 };
 

@@ -70,6 +70,7 @@ constexpr int kFpeSlots = 1024;
 struct FpePool {
     std::mutex mu;
     uint32_t *words[64] = {};  // per device
+    uint32_t *host = nullptr;  // pinned, one word per slot: where a fetch copies its word
     std::vector<int> free_slots;
     int next = 0;
 };
@@ -114,6 +115,12 @@ ionp_status fpe_word(uint32_t **w) {
             return set_error(IONP_ERR_DEVICE, "clearing the flag words failed");
         }
         P.words[dev] = (uint32_t *)p;
+    }
+    if (!P.host) {
+        void *h = nullptr;
+        if (hipHostMalloc(&h, kFpeSlots * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+            return set_error(IONP_ERR_DEVICE, "out of pinned host memory for the flag words");
+        P.host = (uint32_t *)h;
     }
     *w = P.words[dev] + t_fpe.slot;
     return IONP_OK;
@@ -261,10 +268,13 @@ ionp_status ionp_fpe_fetch(void *stream, uint32_t *flags) {
     uint32_t *w;
     ionp_status s;
     if ((s = fpe_word(&w))) return s;
-    uint32_t v = 0;
-    if (hipStreamSynchronize(as_stream(stream)) != hipSuccess ||
-        hipMemcpy(&v, w, sizeof v, hipMemcpyDeviceToHost) != hipSuccess)
+    // the word is copied behind the stream's launches into this thread's
+    // pinned slot, and one synchronisation waits for both
+    volatile uint32_t *h = fpe_pool().host + t_fpe.slot;
+    if (hipMemcpyAsync((void *)h, w, sizeof *w, hipMemcpyDeviceToHost, as_stream(stream)) != hipSuccess ||
+        hipStreamSynchronize(as_stream(stream)) != hipSuccess)
         return set_error(IONP_ERR_DEVICE, "reading the flag word failed");
+    const uint32_t v = *h;
     if (v && hipMemset(w, 0, sizeof v) != hipSuccess) return set_error(IONP_ERR_DEVICE, "clearing the flag word failed");
     *flags = v;
     return IONP_OK;
@@ -487,7 +497,7 @@ ionp_status ionp_binary(int32_t op, const ionp_array_desc *a, const ionp_array_d
     if ((s = attach_fpe(A, op, loop, a->dtype, out->dtype))) return s;
     A.mixed_su = is_mixed_su(op, a->dtype, b->dtype);
     timing.begin();
-    return run_elementwise(A, cfg, as_stream(stream));
+    return timing.done(run_elementwise(A, cfg, as_stream(stream)));
 }
 
 ionp_status ionp_where(const ionp_array_desc *c, const ionp_array_desc *x, const ionp_array_desc *y,
@@ -507,7 +517,7 @@ ionp_status ionp_where(const ionp_array_desc *c, const ionp_array_desc *x, const
     EwArgs A;
     if ((s = plan_elementwise(OP_WHERE, 3, ins, out, out->dtype, &A))) return s;
     timing.begin();
-    return run_elementwise(A, cfg, as_stream(stream));
+    return timing.done(run_elementwise(A, cfg, as_stream(stream)));
 }
 
 ionp_status ionp_cast(const ionp_array_desc *in, const ionp_array_desc *out, const ionp_config *cfg, void *stream) {
@@ -526,7 +536,7 @@ ionp_status ionp_cast(const ionp_array_desc *in, const ionp_array_desc *out, con
     if ((s = plan_elementwise(OP_CAST, 1, ins, out, out->dtype, &A))) return s;
     if ((s = attach_fpe(A, OP_CAST, out->dtype, in->dtype, out->dtype))) return s;
     timing.begin();
-    return run_elementwise(A, cfg, as_stream(stream));
+    return timing.done(run_elementwise(A, cfg, as_stream(stream)));
 }
 
 ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes, int32_t keepdims,
@@ -554,7 +564,7 @@ ionp_status ionp_reduce(int32_t op, const ionp_array_desc *in, const int32_t *ax
     ReducePlan p;
     if ((s = plan_reduce(op, in, mask, out, &p))) return s;
     timing.begin();
-    return run_reduce(p, g_bufsize, initial, cfg, as_stream(stream));
+    return timing.done(run_reduce(p, g_bufsize, initial, cfg, as_stream(stream)));
 }
 
 ionp_status ionp_reduce_noalloc(int32_t op, const ionp_array_desc *in, const int32_t *axes, int32_t naxes,
