@@ -334,19 +334,21 @@ static RowFn row_gather_t(int op) {
     }
 }
 
-// IEEE-flag variants: float add / subtract / multiply / true_divide
+// IEEE-flag variants: float add / subtract / multiply / true_divide; the
+// contiguous ones one vector per thread (as the plain kernels run by
+// default), the gathers 4 in flight
 template <typename T> static RowFn row_fpe_t(int op, bool gather) {
     constexpr int V = 16 / sizeof(T) > 4 ? 4 : 16 / sizeof(T);
     constexpr int VR = 16 / sizeof(T);
     switch (op) {
-        case IONP_ADD: return gather ? ew_rows<T, IONP_ADD, V, 4, true, true> : ew_rows<T, IONP_ADD, VR, 4, false, true>;
+        case IONP_ADD: return gather ? ew_rows<T, IONP_ADD, V, 4, true, true> : ew_rows<T, IONP_ADD, VR, 1, false, true>;
         case IONP_SUBTRACT:
-            return gather ? ew_rows<T, IONP_SUBTRACT, V, 4, true, true> : ew_rows<T, IONP_SUBTRACT, VR, 4, false, true>;
+            return gather ? ew_rows<T, IONP_SUBTRACT, V, 4, true, true> : ew_rows<T, IONP_SUBTRACT, VR, 1, false, true>;
         case IONP_MULTIPLY:
-            return gather ? ew_rows<T, IONP_MULTIPLY, V, 4, true, true> : ew_rows<T, IONP_MULTIPLY, VR, 4, false, true>;
+            return gather ? ew_rows<T, IONP_MULTIPLY, V, 4, true, true> : ew_rows<T, IONP_MULTIPLY, VR, 1, false, true>;
         case IONP_TRUE_DIVIDE:
             return gather ? ew_rows<T, IONP_TRUE_DIVIDE, V, 4, true, true>
-                          : ew_rows<T, IONP_TRUE_DIVIDE, 4, 4, false, true>;
+                          : ew_rows<T, IONP_TRUE_DIVIDE, 4, 1, false, true>;
         default: return nullptr;
     }
 }
@@ -439,7 +441,8 @@ bool try_rows(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_statu
     // CU looping over tiles and 5.6-6.0 with 2-4 vectors in flight
     int unroll = (cfg && cfg->split > 0) ? cfg->split : 1;
     // IEEE flags tracked: the float arithmetic ops have flag variants (one
-    // vector width, 4 in flight); other ops raise no flags
+    // vector width; one vector per thread, the gathers 4); other ops raise
+    // no flags
     const bool fpe = A.fpe && is_float_dt(A.loop_dt) && A.op <= IONP_TRUE_DIVIDE;
     RowFn fn;
     if (gather) {
@@ -452,7 +455,7 @@ bool try_rows(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_statu
         if (A.shape[0] % vec || !aligned(A.in[0], esz, ost0, A.in_st[0][0]) ||
             !aligned(A.in[1], esz, ost1, A.in_st[1][0]) || !aligned(A.out, oes, A.out_st, 1))
             return false;
-        unroll = 4;
+        unroll = 1;
         fn = row_fpe(A.loop_dt, A.op, false);
     } else {
         while (vec > 1 && (A.shape[0] % vec || !aligned(A.in[0], esz, ost0, A.in_st[0][0]) ||
