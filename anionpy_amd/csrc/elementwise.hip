@@ -597,9 +597,10 @@ template <typename T, int OPK> __global__ __launch_bounds__(256) void ew_tiled(c
 // and the output in 16-byte vectors along d0 (rows of TJ elements), all
 // nontemporal (each byte is touched once).  Wide rows give HBM long bursts
 // on all three streams: measured on MI355X (8192 x 8192, out = x + y.T),
-// 4-byte elements with 128 x 128 tiles 130-136 us (5.9-6.1 TB/s), 8-byte
-// elements with 32 (d*) x 128 (d0) tiles 270-278 us (5.8 TB/s), where the
-// scalar 64 x 64 form takes 246 / 389 us.  Partial tiles at the edges take
+// 4-byte elements with 128 x 128 tiles 138-143 us (5.6-5.8 TB/s; 64 (d*)
+// x 256 (d0) tiles 144-152 us), 8-byte elements with 32 (d*) x 128 (d0)
+// tiles 270-278 us (5.8 TB/s), where the scalar 64 x 64 form takes
+// 246 / 389 us.  Partial tiles at the edges take
 // an element-by-element path.
 // XB: X is broadcast along d0 (a template parameter: a runtime branch
 // between the loads splits their batch, measured 10 % slower).
