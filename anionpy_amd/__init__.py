@@ -382,6 +382,13 @@ def _needs_temp(out, inputs):
     """True when `out` shares memory with an input other than as the very
     same view: writing results straight into it could overwrite elements
     before they are read (NumPy then computes into a temporary)."""
+    if out._buf is None or out.size == 0:
+        return False
+    ob = out._buf.data_ptr()
+    oe = ob + out._buf.numel()
+    # buffers that do not intersect cannot share an element
+    if all(x._buf is None or x._buf.data_ptr() >= oe or x._buf.data_ptr() + x._buf.numel() <= ob for x in inputs):
+        return False
     ro = _byte_range(out)
     if ro is None:
         return False

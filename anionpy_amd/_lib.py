@@ -145,6 +145,14 @@ def check(code):
         raise IonpError(code, msg)
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_get_device = getattr(torch._C, "_cuda_getDevice", None)
+
+
 def stream_handle():
-    """hipStream_t of torch's current stream on the current device."""
+    """hipStream_t of torch's current stream on the current device (torch's
+    raw accessor where it exists: 0.3 us against 2.7 us per call through
+    torch.cuda.current_stream())."""
+    if _raw_stream is not None and _get_device is not None:
+        return ctypes.c_void_p(_raw_stream(_get_device()))
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)

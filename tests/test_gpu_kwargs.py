@@ -92,3 +92,20 @@ def test_float16_masked_reduce_matches_oracle(ap):
     X = ap.asarray(x)
     _same_bytes(np.asarray(ap.add.reduce(X, axis=1, where=ap.asarray(m)).get()), O.reduce("add", x, 1, where=m))
     _same_bytes(np.asarray(ap.add.reduce(X, axis=1).get()), O.reduce("add", x, 1))
+
+
+def test_stream_handle_is_the_current_stream(ap):
+    """Calls launch on torch's current stream: the handle the facade passes
+    is that stream's, on the default stream and inside a side stream."""
+    import torch
+    from anionpy_amd import _lib
+    handle = lambda: _lib.stream_handle().value or 0  # noqa: E731  (the default stream is the null handle)
+    assert handle() == torch.cuda.current_stream().cuda_stream
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        assert handle() == s.cuda_stream == torch.cuda.current_stream().cuda_stream != 0
+        x = ap.asarray(np.arange(4096, dtype=np.float64).reshape(64, 64))
+        y = ap.add(x, 1.0)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    assert y.get().tobytes() == (np.arange(4096, dtype=np.float64).reshape(64, 64) + 1.0).tobytes()

@@ -162,3 +162,24 @@ def test_mlp_reference_finite_differences():
                 lm, _, _ = R.loss_and_grads(x, lab, Ws, bs, off)
                 p[idx] = old
                 assert abs((lp - lm) / (2 * eps) - grads[k][idx]) < 1e-6
+
+
+def test_out_overlap_check():
+    """The out= overlap check (NumPy computes into a temporary when out
+    shares memory with an input other than as the very same view), on
+    host byte buffers: separate buffers, the same view, a shifted view, a
+    disjoint part of one buffer and a transposed view of the same bytes."""
+    import torch
+    import anionpy_amd as ap
+    f8 = np.dtype(np.float64)
+    a_buf = torch.zeros(8 * 100, dtype=torch.uint8)
+    o_buf = torch.zeros(8 * 100, dtype=torch.uint8)
+    a = ap.ndarray(a_buf, f8, (10, 10), (10, 1))
+    assert not ap._needs_temp(ap.ndarray(o_buf, f8, (10, 10), (10, 1)), [a])
+    assert not ap._needs_temp(ap.ndarray(a_buf, f8, (10, 10), (10, 1)), [a])
+    assert ap._needs_temp(ap.ndarray(a_buf, f8, (9, 10), (10, 1), 1), [ap.ndarray(a_buf, f8, (9, 10), (10, 1))])
+    half = ap.ndarray(a_buf, f8, (5, 10), (10, 1))
+    assert not ap._needs_temp(ap.ndarray(a_buf, f8, (5, 10), (10, 1), 50), [half])
+    assert ap._needs_temp(ap.ndarray(a_buf, f8, (10, 10), (1, 10)), [a])
+    # a view of the same bytes held by another tensor object
+    assert ap._needs_temp(ap.ndarray(a_buf[8:], f8, (9, 10), (10, 1)), [ap.ndarray(a_buf, f8, (9, 10), (10, 1))])
