@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libionp_hip.so")
 ARCH = os.environ.get("IONP_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["planner.cpp", "api.cpp", "elementwise.hip", "rows_select_cast.hip", "reduce.hip", "accumulate.hip", "batch.hip", "mlp.hip",
+SOURCES = ["planner.cpp", "api.cpp", "elementwise.hip", "rows_select_cast.hip", "tiled_any.hip", "reduce.hip", "accumulate.hip", "batch.hip", "mlp.hip",
            "features.cpp", "advisor.cpp"]
 
 FLAGS = [

@@ -126,27 +126,70 @@ template <typename T> __device__ inline void store_as(int dt, char *p, T v) {
     }
 }
 
-// One output element of the generic kernel, operand addresses resolved.
-template <int LT>
-__device__ inline void ew_element(const EwArgs &A, const char *p0, const char *p1, const char *p2, char *po) {
+// the value of an item whose raw bytes are the low bytes of b (little
+// endian), converted to the loop dtype as load_as converts
+template <int LT> __device__ inline typename dt_type<LT>::T from_bits(int dt, uint64_t b) {
+    switch (dt) {
+        case IONP_BOOL: return cvt<LT>((uint8_t)((uint8_t)b != 0));
+        case IONP_INT8: return cvt<LT>((int8_t)(uint8_t)b);
+        case IONP_UINT8: return cvt<LT>((uint8_t)b);
+        case IONP_INT16: return cvt<LT>((int16_t)(uint16_t)b);
+        case IONP_UINT16: return cvt<LT>((uint16_t)b);
+        case IONP_INT32: return cvt<LT>((int32_t)(uint32_t)b);
+        case IONP_UINT32: return cvt<LT>((uint32_t)b);
+        case IONP_INT64: return cvt<LT>((int64_t)b);
+        case IONP_UINT64: return cvt<LT>((uint64_t)b);
+        case IONP_FLOAT32: return cvt<LT>(__builtin_bit_cast(float, (uint32_t)b));
+        default: return cvt<LT>(__builtin_bit_cast(double, b));
+    }
+}
+
+// Operand sources of one element: operands read from their addresses, or
+// held as raw bit patterns in registers (loaded ahead by the caller).
+struct PtrSrc {
+    const char *p[3];
+    template <int L> __device__ typename dt_type<L>::T val(int k, int dt) const { return load_as<L>(dt, p[k]); }
+    __device__ int64_t i64(int k) const { return *(const int64_t *)p[k]; }
+    __device__ uint64_t u64(int k) const { return *(const uint64_t *)p[k]; }
+    // a float32 / float64 operand as double
+    __device__ double fx(int k, int dt) const { return dt == IONP_FLOAT32 ? (double)*(const float *)p[k] : *(const double *)p[k]; }
+};
+struct BitSrc {
+    uint64_t b[3];
+    template <int L> __device__ typename dt_type<L>::T val(int k, int dt) const { return from_bits<L>(dt, b[k]); }
+    __device__ int64_t i64(int k) const { return (int64_t)b[k]; }
+    __device__ uint64_t u64(int k) const { return b[k]; }
+    __device__ double fx(int k, int dt) const {
+        return dt == IONP_FLOAT32 ? (double)__builtin_bit_cast(float, (uint32_t)b[k]) : __builtin_bit_cast(double, b[k]);
+    }
+};
+
+// One output element of the generic kernel from its operand source.
+template <int LT, class S> __device__ inline void ew_element_src(const EwArgs &A, const S &s, char *po) {
     using T = typename dt_type<LT>::T;
     if (A.op == OP_CAST) {
-        store_as<T>(A.out_dt, po, load_as<LT>(A.in_dt[0], p0));
+        store_as<T>(A.out_dt, po, s.template val<LT>(0, A.in_dt[0]));
     } else if (A.op == OP_WHERE) {
-        const bool c = load_as<IONP_BOOL>(A.in_dt[0], p0) != 0;
-        store_as<T>(A.out_dt, po, c ? load_as<LT>(A.in_dt[1], p1) : load_as<LT>(A.in_dt[2], p2));
+        const bool c = s.template val<IONP_BOOL>(0, A.in_dt[0]) != 0;
+        store_as<T>(A.out_dt, po, c ? s.template val<LT>(1, A.in_dt[1]) : s.template val<LT>(2, A.in_dt[2]));
     } else if (A.op >= IONP_LESS) {
         bool r;
         if (A.mixed_su) {
-            r = A.in_dt[0] == IONP_INT64 ? cmp_su(A.op, *(const int64_t *)p0, *(const uint64_t *)p1)
-                                         : cmp_su(swap_cmp(A.op), *(const int64_t *)p1, *(const uint64_t *)p0);
+            r = A.in_dt[0] == IONP_INT64 ? cmp_su(A.op, s.i64(0), s.u64(1)) : cmp_su(swap_cmp(A.op), s.i64(1), s.u64(0));
         } else {
-            r = cmp_op<T>(A.op, load_as<LT>(A.in_dt[0], p0), load_as<LT>(A.in_dt[1], p1));
+            r = cmp_op<T>(A.op, s.template val<LT>(0, A.in_dt[0]), s.template val<LT>(1, A.in_dt[1]));
         }
         *(uint8_t *)po = (uint8_t)r;
     } else {
-        store_as<T>(A.out_dt, po, arith_op<LT>(A.op, load_as<LT>(A.in_dt[0], p0), load_as<LT>(A.in_dt[1], p1)));
+        store_as<T>(A.out_dt, po,
+                    arith_op<LT>(A.op, s.template val<LT>(0, A.in_dt[0]), s.template val<LT>(1, A.in_dt[1])));
     }
+}
+
+// One output element of the generic kernel, operand addresses resolved.
+template <int LT>
+__device__ inline void ew_element(const EwArgs &A, const char *p0, const char *p1, const char *p2, char *po) {
+    ew_element_src<LT>(A, PtrSrc{{p0, p1, p2}}, po);
 }
 
 // ---------------------------------------------------------------------------
@@ -222,9 +265,8 @@ template <typename T> __device__ inline uint32_t fpe_arith_dyn(int op, T a, T b,
 // compiler uses (32-bit for int8..int32, 64-bit for uint32 / int64, the
 // two-step 64-bit sequence for uint64); float64 -> float32 overflows or
 // underflows like an arithmetic result
-__device__ inline uint32_t fpe_cast(int in_dt, int out_dt, const char *p) {
-    if (!is_float_dt(in_dt)) return 0;
-    const double x = in_dt == IONP_FLOAT32 ? (double)*(const float *)p : *(const double *)p;
+// the flags of casting the float32 / float64 value x (as double) to out_dt
+__device__ inline uint32_t fpe_cast_x(int in_dt, int out_dt, double x) {
     switch (out_dt) {
         case IONP_INT8:
         case IONP_UINT8:
@@ -247,6 +289,10 @@ __device__ inline uint32_t fpe_cast(int in_dt, int out_dt, const char *p) {
         default: return 0;
     }
 }
+__device__ inline uint32_t fpe_cast(int in_dt, int out_dt, const char *p) {
+    if (!is_float_dt(in_dt)) return 0;
+    return fpe_cast_x(in_dt, out_dt, in_dt == IONP_FLOAT32 ? (double)*(const float *)p : *(const double *)p);
+}
 // OR of the wavefront's flags into *word by one lane (all lanes call it)
 __device__ inline void fpe_commit(uint32_t *word, uint32_t fl) {
     uint32_t w = 0;
@@ -258,23 +304,26 @@ __device__ inline void fpe_commit(uint32_t *word, uint32_t fl) {
 }
 
 // One output element of the generic kernel with its IEEE flags.
-template <int LT>
-__device__ inline uint32_t ew_element_fpe(const EwArgs &A, const char *p0, const char *p1, const char *p2, char *po) {
+template <int LT, class S> __device__ inline uint32_t ew_element_fpe_src(const EwArgs &A, const S &s, char *po) {
     using T = typename dt_type<LT>::T;
     if (A.op == OP_CAST) {
-        store_as<T>(A.out_dt, po, load_as<LT>(A.in_dt[0], p0));
-        return fpe_cast(A.in_dt[0], A.out_dt, p0);
+        store_as<T>(A.out_dt, po, s.template val<LT>(0, A.in_dt[0]));
+        return is_float_dt(A.in_dt[0]) ? fpe_cast_x(A.in_dt[0], A.out_dt, s.fx(0, A.in_dt[0])) : 0;
     }
     if constexpr (std::is_floating_point<T>::value) {
         if (A.op <= IONP_TRUE_DIVIDE) {
-            const T a = load_as<LT>(A.in_dt[0], p0), b = load_as<LT>(A.in_dt[1], p1);
+            const T a = s.template val<LT>(0, A.in_dt[0]), b = s.template val<LT>(1, A.in_dt[1]);
             const T r = arith_op<LT>(A.op, a, b);
             store_as<T>(A.out_dt, po, r);
             return fpe_arith_dyn<T>(A.op, a, b, r);
         }
     }
-    ew_element<LT>(A, p0, p1, p2, po);
+    ew_element_src<LT>(A, s, po);
     return 0;
+}
+template <int LT>
+__device__ inline uint32_t ew_element_fpe(const EwArgs &A, const char *p0, const char *p1, const char *p2, char *po) {
+    return ew_element_fpe_src<LT>(A, PtrSrc{{p0, p1, p2}}, po);
 }
 
 // ---------------------------------------------------------------------------

@@ -28,6 +28,9 @@ constexpr int64_t kInfinityCacheBytes = 256ll << 20;
 ionp_status run_elementwise(EwArgs &A, const ionp_config *cfg, hipStream_t s);
 // where / cast over a rows layout (rows_select_cast.hip); false: not eligible
 bool try_rows_select_cast(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_status *st);
+// any elementwise call with an input contiguous along a dim other than the
+// output's contiguous one: that input staged through LDS (tiled_any.hip)
+bool try_tiled_any(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_status *st);
 ionp_status launch_generic(EwArgs A, const ionp_config *cfg, hipStream_t s);
 // IEEE flag tracking of the calling thread: sets it on/off, returns the old state
 int fpe_thread_swap(int on);

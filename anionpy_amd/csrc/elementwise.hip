@@ -860,6 +860,7 @@ ionp_status run_elementwise(EwArgs &A, const ionp_config *cfg, hipStream_t s) {
     if (try_tiled(A, cfg, s, &st)) return st;
     if (try_rows(A, cfg, s, &st)) return st;
     if (try_rows_select_cast(A, cfg, s, &st)) return st;
+    if (try_tiled_any(A, cfg, s, &st)) return st;
     return launch_generic(A, cfg, s);
 }
 

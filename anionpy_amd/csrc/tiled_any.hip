@@ -1,0 +1,247 @@
+// tiled_any.hip -- the LDS-staged transpose for any elementwise call.
+//
+// Replaces the per-element gathers of binary_elementwise (ionp-core/src/
+// ufunc.rs:3539), where_select (sort.rs:1885-1913) and Buffer::cast_to
+// (buffer.rs:81-210) when an input is contiguous along an axis other than
+// the output's contiguous one (B.T against a C-ordered output), for the
+// calls the typed tile kernels do not take: where, casts, mixed dtypes,
+// and the IEEE-flag variants.  Output dim 0 is contiguous (d0); d* is a
+// dim along which some input is contiguous.  Per 64 (d0) x 32 (d*) tile,
+// every operand is copied into LDS as raw items, each read along its own
+// contiguous dim (lanes along d* for the transposed operands, along d0
+// for the others; each thread's 8 loads in flight before its LDS stores),
+// then each lane computes the elements of one d0 position from LDS through
+// ew_element_src / ew_element_fpe_src -- the generic kernel's own element
+// function, so every dtype, conversion, select and comparison rule is
+// that kernel's -- and stores them along d0.  (Exact int64 / uint64
+// comparisons stay on the generic kernel.)
+#include "elementwise.h"
+
+#include "device_ops.h"
+
+namespace ionp {
+
+constexpr int kTn0 = 64;             // tile extent along d0 (the output's contiguous dim)
+constexpr int kTn1 = 32;             // tile extent along d*
+constexpr int kTnPad = kTn1 + 1;     // LDS row pitch in items (d* fastest)
+constexpr int kTnMaxLds = 64 << 10;  // LDS bytes per workgroup at most
+
+struct TileNArgs {
+    EwArgs E;                 // op, dtypes, operand and output addresses, flag word
+    int64_t n0, n1;           // extents along d0 and d*
+    int64_t st0[3], st1[3];   // input strides along d0 and d*
+    int64_t o_st1;            // output stride along d* (along d0 it is 1)
+    int32_t staged;           // bit k: input k read along d* (contiguous there)
+    int32_t lds_off[3];       // byte offset of input k's LDS tile
+    int64_t t0, t1;           // tiles along d0, d*
+    FastDiv t0div, t1div;
+    int32_t nb;               // remaining ("batch") dims
+    int64_t bshape[kMaxKDims];
+    FastDiv bdiv[kMaxKDims];
+    int64_t bin[3][kMaxKDims], bo[kMaxKDims];
+    int64_t ntiles;
+};
+
+// One operand's tile into LDS ([d0][d*] items, pitch kTnPad): the 8
+// elements of each thread loaded into registers first, then stored.
+// along_dstar: lanes walk d* (the operand is contiguous there); else d0.
+template <typename U>
+__device__ inline void stage_tile(char *tile, const char *src, int64_t st0, int64_t st1, int m0, int m1,
+                                  bool along_dstar) {
+    constexpr int Q = kTn0 * kTn1 / 256;
+    U v[Q];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int r = along_dstar ? (t >> 5) + 8 * q : (t & 63);
+        const int c = along_dstar ? (t & 31) : (t >> 6) + 4 * q;
+        v[q] = (r < m0 && c < m1) ? *(const U *)(src + ((int64_t)r * st0 + (int64_t)c * st1) * (int64_t)sizeof(U)) : U(0);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int r = along_dstar ? (t >> 5) + 8 * q : (t & 63);
+        const int c = along_dstar ? (t & 31) : (t >> 6) + 4 * q;
+        ((U *)tile)[r * kTnPad + c] = v[q];
+    }
+}
+
+template <int N> struct UintOf;
+template <> struct UintOf<1> { using T = uint8_t; };
+template <> struct UintOf<2> { using T = uint16_t; };
+template <> struct UintOf<4> { using T = uint32_t; };
+template <> struct UintOf<8> { using T = uint64_t; };
+
+// WISZ 0: any call, each element through ew_element_src / _fpe_src in
+// loop dtype LT.  WISZ 1 / 2 / 4 / 8: where with a bool condition and
+// values of the output's dtype (not bool), a select of WISZ-byte bit
+// patterns.
+template <int LT, bool FPE, int WISZ> __global__ __launch_bounds__(256) void ew_tiled_n(const TileNArgs P) {
+    extern __shared__ char smem[];
+    const EwArgs &A = P.E;
+    const int nin = A.nin;
+    const int io = itemsize(A.out_dt);
+    const int lo = threadIdx.x & 63, hi = threadIdx.x >> 6;  // 4 wavefronts
+    uint32_t fl = 0;
+    for (int64_t t = blockIdx.x; t < P.ntiles; t += gridDim.x) {
+        // t = (batch, tile along d*, tile along d0), d0 fastest
+        uint32_t rem = (uint32_t)t;
+        const uint32_t q0 = P.t0div.div(rem);
+        const int64_t j0 = (int64_t)(rem - q0 * (uint32_t)P.t0) * kTn0;  // along d0
+        rem = q0;
+        const uint32_t q1 = P.t1div.div(rem);
+        const int64_t i0 = (int64_t)(rem - q1 * (uint32_t)P.t1) * kTn1;  // along d*
+        rem = q1;
+        int64_t b[3] = {0, 0, 0}, bo = 0;
+        for (int d = 0; d < P.nb; d++) {
+            uint32_t c;
+            if (d + 1 < P.nb) {
+                const uint32_t q = P.bdiv[d].div(rem);
+                c = rem - q * (uint32_t)P.bshape[d];
+                rem = q;
+            } else {
+                c = rem;
+            }
+            for (int k = 0; k < 3; k++) b[k] += (int64_t)c * P.bin[k][d];
+            bo += (int64_t)c * P.bo[d];
+        }
+        const int m0 = (int)min((int64_t)kTn0, P.n0 - j0), m1 = (int)min((int64_t)kTn1, P.n1 - i0);
+        // every operand's tile into LDS, each read along its contiguous dim
+        for (int k = 0; k < nin; k++) {
+            const int isz = itemsize(A.in_dt[k]);
+            char *tile = smem + P.lds_off[k];
+            const char *src = A.in[k] + (b[k] + j0 * P.st0[k] + i0 * P.st1[k]) * isz;
+            const bool ds = (P.staged >> k) & 1;
+            switch (isz) {
+                case 1: stage_tile<uint8_t>(tile, src, P.st0[k], P.st1[k], m0, m1, ds); break;
+                case 2: stage_tile<uint16_t>(tile, src, P.st0[k], P.st1[k], m0, m1, ds); break;
+                case 4: stage_tile<uint32_t>(tile, src, P.st0[k], P.st1[k], m0, m1, ds); break;
+                default: stage_tile<uint64_t>(tile, src, P.st0[k], P.st1[k], m0, m1, ds); break;
+            }
+        }
+        __syncthreads();
+        // lanes along d0 (the output's contiguous dim), 8 positions of d*
+        // each, operands read back from LDS
+        if (lo < m0) {
+            char *orow = A.out + (bo + j0 + lo + i0 * P.o_st1) * io;
+            for (int q = 0; q < kTn1 / 4; q++) {
+                const int c = hi + 4 * q;
+                if (c >= m1) break;
+                char *po = orow + (int64_t)c * P.o_st1 * io;
+                if constexpr (WISZ > 0) {
+                    using U = typename UintOf<WISZ>::T;
+                    const int e = lo * kTnPad + c;
+                    const bool cond = ((const uint8_t *)(smem + P.lds_off[0]))[e] != 0;
+                    *(U *)po = cond ? ((const U *)(smem + P.lds_off[1]))[e] : ((const U *)(smem + P.lds_off[2]))[e];
+                } else {
+                    PtrSrc src;
+                    for (int k = 0; k < 3; k++)
+                        src.p[k] = k < nin ? smem + P.lds_off[k] + (lo * kTnPad + c) * itemsize(A.in_dt[k]) : nullptr;
+                    if constexpr (FPE) fl |= ew_element_fpe_src<LT>(A, src, po);
+                    else ew_element_src<LT>(A, src, po);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if constexpr (FPE) fpe_commit(A.fpe, fl);
+}
+
+typedef void (*TileNFn)(const TileNArgs);
+
+template <bool FPE> static TileNFn tiled_n_kernel(int lt) {
+    switch (lt) {
+        case IONP_BOOL: return ew_tiled_n<IONP_BOOL, FPE, 0>;
+        case IONP_INT8: return ew_tiled_n<IONP_INT8, FPE, 0>;
+        case IONP_UINT8: return ew_tiled_n<IONP_UINT8, FPE, 0>;
+        case IONP_INT16: return ew_tiled_n<IONP_INT16, FPE, 0>;
+        case IONP_UINT16: return ew_tiled_n<IONP_UINT16, FPE, 0>;
+        case IONP_INT32: return ew_tiled_n<IONP_INT32, FPE, 0>;
+        case IONP_UINT32: return ew_tiled_n<IONP_UINT32, FPE, 0>;
+        case IONP_INT64: return ew_tiled_n<IONP_INT64, FPE, 0>;
+        case IONP_UINT64: return ew_tiled_n<IONP_UINT64, FPE, 0>;
+        case IONP_FLOAT32: return ew_tiled_n<IONP_FLOAT32, FPE, 0>;
+        case IONP_FLOAT64: return ew_tiled_n<IONP_FLOAT64, FPE, 0>;
+        default: return nullptr;
+    }
+}
+
+static TileNFn tiled_where_kernel(int isz) {
+    switch (isz) {
+        case 1: return ew_tiled_n<IONP_UINT8, false, 1>;
+        case 2: return ew_tiled_n<IONP_UINT8, false, 2>;
+        case 4: return ew_tiled_n<IONP_UINT8, false, 4>;
+        case 8: return ew_tiled_n<IONP_UINT8, false, 8>;
+        default: return nullptr;
+    }
+}
+
+bool try_tiled_any(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_status *st) {
+    if (A.ndim < 2 || A.ndim > kMaxKDims || A.out_st[0] != 1 || A.nin < 1 || A.nin > 3) return false;
+    if (A.mixed_su) return false;  // exact int64 / uint64 comparisons: the generic kernel
+    if (A.out_dt == IONP_FLOAT16) return false;
+    for (int k = 0; k < A.nin; k++)
+        if (A.in_dt[k] == IONP_FLOAT16) return false;
+    // d*: the first dim (other than 0) along which some input is
+    // contiguous while not contiguous or broadcast along d0
+    int dstar = -1;
+    for (int k = 0; k < A.nin && dstar < 0; k++) {
+        const int64_t s0 = A.in_st[k][0];
+        if (s0 == 0 || s0 == 1 || s0 == -1) continue;
+        for (int d = 1; d < A.ndim; d++)
+            if (A.in_st[k][d] == 1) {
+                dstar = d;
+                break;
+            }
+    }
+    if (dstar < 0) return false;
+    const bool fpe = A.fpe != nullptr;
+    TileNFn fn = fpe ? tiled_n_kernel<true>(A.loop_dt) : tiled_n_kernel<false>(A.loop_dt);
+    // where over values of the output's dtype: a bit-pattern select (bool
+    // values are read as 0 / 1 by the element function: not a bit copy)
+    if (A.op == OP_WHERE && A.in_dt[0] == IONP_BOOL && A.in_dt[1] == A.out_dt && A.in_dt[2] == A.out_dt &&
+        A.out_dt != IONP_BOOL)
+        fn = tiled_where_kernel(itemsize(A.out_dt));
+    if (!fn) return false;
+    TileNArgs P;
+    std::memset(&P, 0, sizeof P);
+    P.E = A;
+    P.n0 = A.shape[0];
+    P.n1 = A.shape[dstar];
+    int lds = 0;
+    for (int k = 0; k < A.nin; k++) {
+        P.st0[k] = A.in_st[k][0];
+        P.st1[k] = A.in_st[k][dstar];
+        const int64_t s0 = P.st0[k];
+        if (P.st1[k] == 1 && s0 != 0 && s0 != 1 && s0 != -1) P.staged |= 1 << k;
+        P.lds_off[k] = lds;
+        lds += kTn0 * kTnPad * itemsize(A.in_dt[k]);
+        lds = (lds + 15) & ~15;
+    }
+    if (lds > kTnMaxLds) return false;
+    P.o_st1 = A.out_st[dstar];
+    P.t0 = (P.n0 + kTn0 - 1) / kTn0;
+    P.t1 = (P.n1 + kTn1 - 1) / kTn1;
+    int64_t nbatch = 1;
+    for (int d = 1; d < A.ndim; d++) {
+        if (d == dstar) continue;
+        P.bshape[P.nb] = A.shape[d];
+        P.bdiv[P.nb] = FastDiv((uint32_t)A.shape[d]);
+        for (int k = 0; k < A.nin; k++) P.bin[k][P.nb] = A.in_st[k][d];
+        P.bo[P.nb] = A.out_st[d];
+        P.nb++;
+        nbatch *= A.shape[d];
+    }
+    P.ntiles = P.t0 * P.t1 * nbatch;
+    if (P.ntiles >= (1ll << 31) || P.t0 >= (1ll << 31) || P.t1 >= (1ll << 31)) return false;
+    P.t0div = FastDiv((uint32_t)P.t0);
+    P.t1div = FastDiv((uint32_t)P.t1);
+    // a workgroup per tile; a configured blocks_per_cu caps the grid (the
+    // kernel loops over the remaining tiles)
+    int64_t grid = (cfg && cfg->blocks_per_cu > 0) ? (int64_t)cfg->blocks_per_cu * num_cus() : P.ntiles;
+    if (grid > P.ntiles) grid = P.ntiles;
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), lds, s, P);
+    *st = hipGetLastError() == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, "kernel launch failed");
+    return true;
+}
+
+}  // namespace ionp

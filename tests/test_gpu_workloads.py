@@ -209,6 +209,64 @@ def test_transposed_operand_tiles(ap):
             assert got.strides == want.strides and got.tobytes() == want.tobytes(), (dt, op, a.shape)
 
 
+def test_transposed_operand_any_call(ap):
+    """Calls with an input contiguous along another axis than the output's
+    that the typed tile kernels do not take -- where, casts into a C-ordered
+    output, mixed-dtype arithmetic and comparisons (int64 against uint64
+    exactly) -- staged through LDS in 64 x 64 tiles: partial edge tiles, a
+    3-D batch, a reversed view and broadcast operands, against the oracle;
+    and the IEEE-flag variant (overflow raised, a clean call silent)."""
+    rng = np.random.default_rng(23)
+    f8 = rng.standard_normal((300, 1000))
+    f4 = rng.standard_normal((1000, 300)).astype(np.float32)
+    i8 = rng.integers(-10 ** 9, 10 ** 9, (300, 1000))
+    u8 = rng.integers(0, 2 ** 63, (1000, 300), dtype=np.uint64)
+    i4 = rng.integers(-10 ** 6, 10 ** 6, (1000, 300)).astype(np.int32)
+    cond = rng.random((300, 1000)) < 0.5
+    condT = rng.random((1000, 300)) < 0.5
+    f8b = rng.standard_normal((1000, 300))
+    c3 = rng.random((3, 130, 70)) < 0.5
+    x3 = rng.standard_normal((3, 70, 130))
+    y3 = rng.standard_normal((3, 130, 70)).astype(np.float32)
+    D = {k: ap.asarray(v) for k, v in dict(f8=f8, f4=f4, i8=i8, u8=u8, i4=i4, cond=cond, condT=condT, f8b=f8b,
+                                            c3=c3, x3=x3, y3=y3).items()}
+    where_cases = [
+        (cond, f8, f8b.T, D["cond"], D["f8"], D["f8b"].T),
+        (condT.T, f8, f8b.T, D["condT"].T, D["f8"], D["f8b"].T),
+        (cond, f4.T, np.float32(2.5), D["cond"], D["f4"].T, 2.5),  # a weak Python float: float32
+        (cond, i8, i4.T, D["cond"], D["i8"], D["i4"].T),
+        (c3, x3.transpose(0, 2, 1), y3, D["c3"], D["x3"].transpose(0, 2, 1), D["y3"]),
+        (cond[::-1], f8[::-1], f8b.T[::-1], D["cond"][::-1], D["f8"][::-1], D["f8b"].T[::-1]),
+    ]
+    for c, x, y, cd, xd, yd in where_cases:
+        want = O.where(c, x, y)
+        got = ap.where(cd, xd, yd).get()
+        assert got.dtype == want.dtype and got.strides == want.strides and got.tobytes() == want.tobytes(), \
+            (np.shape(c), np.asarray(x).dtype)
+    for src, srcd in ((f4, D["f4"]), (f8b, D["f8b"]), (u8, D["u8"]), (condT, D["condT"])):
+        for dt in (np.float64, np.float32, np.int64, np.int8, np.uint16, np.bool_):
+            want = np.ascontiguousarray(O.cast(src.T, dt))
+            got = srcd.T.astype(dt, order="C").get()
+            assert got.dtype == want.dtype and got.tobytes() == want.tobytes(), (src.dtype, dt)
+    bin_cases = [("add", f8, f4.T, D["f8"], D["f4"].T), ("multiply", i8, i4.T, D["i8"], D["i4"].T),
+                 ("less", i8, u8.T, D["i8"], D["u8"].T), ("greater_equal", f4.T, i8, D["f4"].T, D["i8"]),
+                 ("subtract", x3.transpose(0, 2, 1)[:, :, 1:], y3[:, :, 1:], D["x3"].transpose(0, 2, 1)[:, :, 1:],
+                  D["y3"][:, :, 1:])]
+    for op, a, b, ad, bd in bin_cases:
+        want = O.binary(op, a, b)
+        got = getattr(ap, op)(ad, bd).get()
+        assert got.strides == want.strides and got.tobytes() == want.tobytes(), (op, a.dtype, b.dtype)
+    big = np.full((200, 130), 3e38, dtype=np.float32)
+    huge = np.full((130, 200), 1e300)
+    assert O.binary_fpe("multiply", huge, big.T) & 2 and O.binary_fpe("add", f8, f4.T) == 0
+    bigd, huged = ap.asarray(big), ap.asarray(huge)
+    with ap.errstate(over="raise", invalid="ignore", under="ignore", divide="ignore"):
+        with pytest.raises(FloatingPointError, match="overflow"):
+            ap.multiply(huged, bigd.T)
+        got = ap.add(D["f8"], D["f4"].T).get()
+    assert got.tobytes() == O.binary("add", f8, f4.T).tobytes()
+
+
 def test_blocked_accumulate(ap):
     """Scans over many fibers (the blocked kernel): along the contiguous axis
     and along the outer axis of C-ordered arrays, extents that are not
