@@ -16,7 +16,8 @@
 //                input, all operands of one dtype: 16-byte vector loads and
 //                stores, several vectors in flight per thread.
 //   ew_generic - any strides (rank <= 8 after coalescing), any dtype mix.
-// (where and casts over rows layouts: rows_select_cast.hip)
+// (where and casts over rows layouts: rows_select_cast.hip; any call with
+// a transposed input that ew_tiled does not take: tiled_any.hip)
 #include "elementwise.h"
 
 #include "device_ops.h"

@@ -289,8 +289,9 @@ def elementwise_extras(ap, torch):
     """Elementwise and scan workloads beside the step: float64 broadcast add
     (row and column vector), the float64 X.T[::2] view, A + B.T (the
     LDS-transposed operand) in float32 and float64, where(cond, A, column)
-    and the float64 -> float32 cast, and float64 cumsum along either axis,
-    all 8192 x 8192."""
+    and the float64 -> float32 cast, where(cond, A, B.T), B.T cast into a
+    C-ordered float32, float64 A + float32 B.T, and float64 cumsum along
+    either axis, all 8192 x 8192."""
     log("elementwise and scan workloads")
     out = {}
     A = ap.asarray(np.random.default_rng(0).uniform(-1, 1, (8192, 8192)))
@@ -316,10 +317,23 @@ def elementwise_extras(ap, torch):
     o = ap.where(cond, A, col)
     out["where_f64"] = _rate(cond.nbytes + A.nbytes + col.nbytes + o.nbytes,
                              timed(torch, lambda: ap.where(cond, A, col), 10))
+    del o
+    # a transposed operand through the any-call LDS tile kernel
+    B = ap.asarray(np.random.default_rng(13).uniform(-1, 1, (8192, 8192)))
+    o = ap.where(cond, A, B.T)
+    out["where_f64_T"] = _rate(cond.nbytes + A.nbytes + B.nbytes + o.nbytes,
+                               timed(torch, lambda: ap.where(cond, A, B.T), 10))
     del o, cond
     o = A.astype(np.float32)
     out["cast_f64_f32"] = _rate(A.nbytes + o.nbytes, timed(torch, lambda: A.astype(np.float32), 10))
     del o
+    o = B.T.astype(np.float32, order="C")
+    out["cast_f64_f32_T_to_C"] = _rate(B.nbytes + o.nbytes, timed(torch, lambda: B.T.astype(np.float32, order="C"), 10))
+    del o
+    Bf = ap.asarray(np.random.default_rng(14).uniform(-1, 1, (8192, 8192)).astype(np.float32))
+    o = ap.add(A, Bf.T)
+    out["add_f64_f32T"] = _rate(A.nbytes + Bf.nbytes + o.nbytes, timed(torch, lambda: ap.add(A, Bf.T, out=o), 10))
+    del o, B, Bf
     for ax in (0, 1):
         o = ap.add.accumulate(A, axis=ax)
         out[f"cumsum_f64_axis{ax}"] = _rate(A.nbytes + o.nbytes,
