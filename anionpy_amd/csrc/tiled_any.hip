@@ -194,6 +194,20 @@ bool try_tiled_any(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_
             }
     }
     if (dstar < 0) return false;
+    // tiles at least half full along d0 and a few elements deep along d*:
+    // measured on MI355X, a short output-contiguous dim (2-5 elements, the
+    // tile 3-8 % full) ran 7-30 x slower here than in the generic kernel,
+    // whose lanes then alternate between a few contiguous rows; a long d0
+    // with d* of 6 ran 3 x faster here
+    if (A.shape[0] < kTn0 / 2 || A.shape[dstar] < 4) return false;
+    // where with a value broadcast along both tile dims (a scalar): the
+    // tuning sweep measured the generic kernel 15 % faster over such calls
+    // (it streams the two full operands); with both values full or the
+    // condition transposed this kernel wins (8192^2 f64: 459 -> 391 us,
+    // 620 -> 340 us)
+    if (A.op == OP_WHERE)
+        for (int k = 1; k < 3; k++)
+            if (A.in_st[k][0] == 0 && A.in_st[k][dstar] == 0) return false;
     const bool fpe = A.fpe != nullptr;
     TileNFn fn = fpe ? tiled_n_kernel<true>(A.loop_dt) : tiled_n_kernel<false>(A.loop_dt);
     // where over values of the output's dtype: a bit-pattern select (bool
