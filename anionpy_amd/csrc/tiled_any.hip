@@ -74,8 +74,10 @@ template <> struct UintOf<8> { using T = uint64_t; };
 // WISZ 0: any call, each element through ew_element_src / _fpe_src in
 // loop dtype LT.  WISZ 1 / 2 / 4 / 8: where with a bool condition and
 // values of the output's dtype (not bool), a select of WISZ-byte bit
-// patterns.
-template <int LT, bool FPE, int WISZ> __global__ __launch_bounds__(256) void ew_tiled_n(const TileNArgs P) {
+// patterns.  CI >= 0: a cast from dtype CI to CO (flags not tracked), the
+// conversion of the generic kernel (cvt; a bool input read as 0 / 1).
+template <int LT, bool FPE, int WISZ, int CI = -1, int CO = -1>
+__global__ __launch_bounds__(256) void ew_tiled_n(const TileNArgs P) {
     extern __shared__ char smem[];
     const EwArgs &A = P.E;
     const int nin = A.nin;
@@ -127,7 +129,13 @@ template <int LT, bool FPE, int WISZ> __global__ __launch_bounds__(256) void ew_
                 const int c = hi + 4 * q;
                 if (c >= m1) break;
                 char *po = orow + (int64_t)c * P.o_st1 * io;
-                if constexpr (WISZ > 0) {
+                if constexpr (CI >= 0) {
+                    using TI = typename dt_type<CI>::T;
+                    using TO = typename dt_type<CO>::T;
+                    const TI v = ((const TI *)(smem + P.lds_off[0]))[lo * kTnPad + c];
+                    if constexpr (CI == IONP_BOOL) *(TO *)po = cvt<CO>((uint8_t)(v != 0));
+                    else *(TO *)po = cvt<CO>(v);
+                } else if constexpr (WISZ > 0) {
                     using U = typename UintOf<WISZ>::T;
                     const int e = lo * kTnPad + c;
                     const bool cond = ((const uint8_t *)(smem + P.lds_off[0]))[e] != 0;
@@ -161,6 +169,40 @@ template <bool FPE> static TileNFn tiled_n_kernel(int lt) {
         case IONP_UINT64: return ew_tiled_n<IONP_UINT64, FPE, 0>;
         case IONP_FLOAT32: return ew_tiled_n<IONP_FLOAT32, FPE, 0>;
         case IONP_FLOAT64: return ew_tiled_n<IONP_FLOAT64, FPE, 0>;
+        default: return nullptr;
+    }
+}
+
+template <int CI> static TileNFn tiled_cast_from(int co) {
+    switch (co) {
+        case IONP_BOOL: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_BOOL>;
+        case IONP_INT8: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_INT8>;
+        case IONP_UINT8: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_UINT8>;
+        case IONP_INT16: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_INT16>;
+        case IONP_UINT16: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_UINT16>;
+        case IONP_INT32: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_INT32>;
+        case IONP_UINT32: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_UINT32>;
+        case IONP_INT64: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_INT64>;
+        case IONP_UINT64: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_UINT64>;
+        case IONP_FLOAT32: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_FLOAT32>;
+        case IONP_FLOAT64: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_FLOAT64>;
+        default: return nullptr;
+    }
+}
+
+static TileNFn tiled_cast_kernel(int ci, int co) {
+    switch (ci) {
+        case IONP_BOOL: return tiled_cast_from<IONP_BOOL>(co);
+        case IONP_INT8: return tiled_cast_from<IONP_INT8>(co);
+        case IONP_UINT8: return tiled_cast_from<IONP_UINT8>(co);
+        case IONP_INT16: return tiled_cast_from<IONP_INT16>(co);
+        case IONP_UINT16: return tiled_cast_from<IONP_UINT16>(co);
+        case IONP_INT32: return tiled_cast_from<IONP_INT32>(co);
+        case IONP_UINT32: return tiled_cast_from<IONP_UINT32>(co);
+        case IONP_INT64: return tiled_cast_from<IONP_INT64>(co);
+        case IONP_UINT64: return tiled_cast_from<IONP_UINT64>(co);
+        case IONP_FLOAT32: return tiled_cast_from<IONP_FLOAT32>(co);
+        case IONP_FLOAT64: return tiled_cast_from<IONP_FLOAT64>(co);
         default: return nullptr;
     }
 }
@@ -215,6 +257,8 @@ bool try_tiled_any(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_
     if (A.op == OP_WHERE && A.in_dt[0] == IONP_BOOL && A.in_dt[1] == A.out_dt && A.in_dt[2] == A.out_dt &&
         A.out_dt != IONP_BOOL)
         fn = tiled_where_kernel(itemsize(A.out_dt));
+    // casts with no flags tracked: typed conversion (as ew_rows_cast)
+    if (A.op == OP_CAST && !fpe && tiled_cast_kernel(A.in_dt[0], A.out_dt)) fn = tiled_cast_kernel(A.in_dt[0], A.out_dt);
     if (!fn) return false;
     TileNArgs P;
     std::memset(&P, 0, sizeof P);
