@@ -74,8 +74,8 @@ template <> struct UintOf<8> { using T = uint64_t; };
 // WISZ 0: any call, each element through ew_element_src / _fpe_src in
 // loop dtype LT.  WISZ 1 / 2 / 4 / 8: where with a bool condition and
 // values of the output's dtype (not bool), a select of WISZ-byte bit
-// patterns.  CI >= 0: a cast from dtype CI to CO (flags not tracked), the
-// conversion of the generic kernel (cvt; a bool input read as 0 / 1).
+// patterns.  CI >= 0: a cast from dtype CI to CO, the conversion of the
+// generic kernel (cvt; a bool input read as 0 / 1) and, FPE, its flags.
 template <int LT, bool FPE, int WISZ, int CI = -1, int CO = -1>
 __global__ __launch_bounds__(256) void ew_tiled_n(const TileNArgs P) {
     extern __shared__ char smem[];
@@ -135,6 +135,7 @@ __global__ __launch_bounds__(256) void ew_tiled_n(const TileNArgs P) {
                     const TI v = ((const TI *)(smem + P.lds_off[0]))[lo * kTnPad + c];
                     if constexpr (CI == IONP_BOOL) *(TO *)po = cvt<CO>((uint8_t)(v != 0));
                     else *(TO *)po = cvt<CO>(v);
+                    if constexpr (FPE && (CI == IONP_FLOAT32 || CI == IONP_FLOAT64)) fl |= fpe_cast_x(CI, CO, (double)v);
                 } else if constexpr (WISZ > 0) {
                     using U = typename UintOf<WISZ>::T;
                     const int e = lo * kTnPad + c;
@@ -173,36 +174,37 @@ template <bool FPE> static TileNFn tiled_n_kernel(int lt) {
     }
 }
 
-template <int CI> static TileNFn tiled_cast_from(int co) {
+// casts: the flag variant only for float inputs (the others raise none)
+template <int CI, bool FPE> static TileNFn tiled_cast_from(int co) {
     switch (co) {
-        case IONP_BOOL: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_BOOL>;
-        case IONP_INT8: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_INT8>;
-        case IONP_UINT8: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_UINT8>;
-        case IONP_INT16: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_INT16>;
-        case IONP_UINT16: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_UINT16>;
-        case IONP_INT32: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_INT32>;
-        case IONP_UINT32: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_UINT32>;
-        case IONP_INT64: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_INT64>;
-        case IONP_UINT64: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_UINT64>;
-        case IONP_FLOAT32: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_FLOAT32>;
-        case IONP_FLOAT64: return ew_tiled_n<IONP_UINT8, false, 0, CI, IONP_FLOAT64>;
+        case IONP_BOOL: return ew_tiled_n<IONP_UINT8, FPE, 0, CI, IONP_BOOL>;
+        case IONP_INT8: return ew_tiled_n<IONP_UINT8, FPE, 0, CI, IONP_INT8>;
+        case IONP_UINT8: return ew_tiled_n<IONP_UINT8, FPE, 0, CI, IONP_UINT8>;
+        case IONP_INT16: return ew_tiled_n<IONP_UINT8, FPE, 0, CI, IONP_INT16>;
+        case IONP_UINT16: return ew_tiled_n<IONP_UINT8, FPE, 0, CI, IONP_UINT16>;
+        case IONP_INT32: return ew_tiled_n<IONP_UINT8, FPE, 0, CI, IONP_INT32>;
+        case IONP_UINT32: return ew_tiled_n<IONP_UINT8, FPE, 0, CI, IONP_UINT32>;
+        case IONP_INT64: return ew_tiled_n<IONP_UINT8, FPE, 0, CI, IONP_INT64>;
+        case IONP_UINT64: return ew_tiled_n<IONP_UINT8, FPE, 0, CI, IONP_UINT64>;
+        case IONP_FLOAT32: return ew_tiled_n<IONP_UINT8, FPE, 0, CI, IONP_FLOAT32>;
+        case IONP_FLOAT64: return ew_tiled_n<IONP_UINT8, FPE, 0, CI, IONP_FLOAT64>;
         default: return nullptr;
     }
 }
 
-static TileNFn tiled_cast_kernel(int ci, int co) {
+static TileNFn tiled_cast_kernel(int ci, int co, bool fpe) {
     switch (ci) {
-        case IONP_BOOL: return tiled_cast_from<IONP_BOOL>(co);
-        case IONP_INT8: return tiled_cast_from<IONP_INT8>(co);
-        case IONP_UINT8: return tiled_cast_from<IONP_UINT8>(co);
-        case IONP_INT16: return tiled_cast_from<IONP_INT16>(co);
-        case IONP_UINT16: return tiled_cast_from<IONP_UINT16>(co);
-        case IONP_INT32: return tiled_cast_from<IONP_INT32>(co);
-        case IONP_UINT32: return tiled_cast_from<IONP_UINT32>(co);
-        case IONP_INT64: return tiled_cast_from<IONP_INT64>(co);
-        case IONP_UINT64: return tiled_cast_from<IONP_UINT64>(co);
-        case IONP_FLOAT32: return tiled_cast_from<IONP_FLOAT32>(co);
-        case IONP_FLOAT64: return tiled_cast_from<IONP_FLOAT64>(co);
+        case IONP_BOOL: return tiled_cast_from<IONP_BOOL, false>(co);
+        case IONP_INT8: return tiled_cast_from<IONP_INT8, false>(co);
+        case IONP_UINT8: return tiled_cast_from<IONP_UINT8, false>(co);
+        case IONP_INT16: return tiled_cast_from<IONP_INT16, false>(co);
+        case IONP_UINT16: return tiled_cast_from<IONP_UINT16, false>(co);
+        case IONP_INT32: return tiled_cast_from<IONP_INT32, false>(co);
+        case IONP_UINT32: return tiled_cast_from<IONP_UINT32, false>(co);
+        case IONP_INT64: return tiled_cast_from<IONP_INT64, false>(co);
+        case IONP_UINT64: return tiled_cast_from<IONP_UINT64, false>(co);
+        case IONP_FLOAT32: return fpe ? tiled_cast_from<IONP_FLOAT32, true>(co) : tiled_cast_from<IONP_FLOAT32, false>(co);
+        case IONP_FLOAT64: return fpe ? tiled_cast_from<IONP_FLOAT64, true>(co) : tiled_cast_from<IONP_FLOAT64, false>(co);
         default: return nullptr;
     }
 }
@@ -257,8 +259,10 @@ bool try_tiled_any(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_
     if (A.op == OP_WHERE && A.in_dt[0] == IONP_BOOL && A.in_dt[1] == A.out_dt && A.in_dt[2] == A.out_dt &&
         A.out_dt != IONP_BOOL)
         fn = tiled_where_kernel(itemsize(A.out_dt));
-    // casts with no flags tracked: typed conversion (as ew_rows_cast)
-    if (A.op == OP_CAST && !fpe && tiled_cast_kernel(A.in_dt[0], A.out_dt)) fn = tiled_cast_kernel(A.in_dt[0], A.out_dt);
+    // casts: typed conversion (as ew_rows_cast), flags as the generic kernel
+    if (A.op == OP_CAST) {
+        if (const TileNFn cf = tiled_cast_kernel(A.in_dt[0], A.out_dt, fpe)) fn = cf;
+    }
     if (!fn) return false;
     TileNArgs P;
     std::memset(&P, 0, sizeof P);

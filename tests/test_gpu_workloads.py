@@ -263,6 +263,13 @@ def test_transposed_operand_any_call(ap):
     with ap.errstate(over="raise", invalid="ignore", under="ignore", divide="ignore"):
         with pytest.raises(FloatingPointError, match="overflow"):
             ap.multiply(huged, bigd.T)
+        # the typed cast into another order with flags tracked
+        with pytest.raises(FloatingPointError, match="overflow"):
+            huged.T.astype(np.float32, order="C")
+        got_cast = D["f8b"].T.astype(np.float32, order="C").get()
+    assert O.cast_fpe(huge.T, np.float32) & 2 and O.cast_fpe(f8b.T, np.float32) == 0
+    assert got_cast.tobytes() == np.ascontiguousarray(O.cast(f8b.T, np.float32)).tobytes()
+    with ap.errstate(over="raise", invalid="ignore", under="ignore", divide="ignore"):
         got = ap.add(D["f8"], D["f4"].T).get()
     assert got.tobytes() == O.binary("add", f8, f4.T).tobytes()
 
