@@ -76,7 +76,10 @@ template <> struct UintOf<8> { using T = uint64_t; };
 // values of the output's dtype (not bool), a select of WISZ-byte bit
 // patterns.  CI >= 0: a cast from dtype CI to CO, the conversion of the
 // generic kernel (cvt; a bool input read as 0 / 1) and, FPE, its flags.
-template <int LT, bool FPE, int WISZ, int CI = -1, int CO = -1>
+// BA >= 0: a binary op on inputs of dtypes BA and BB, both converted to
+// the loop dtype LT by cvt, the op applied as the generic kernel applies
+// it (output LT, or bool for comparisons) and, FPE, its flags.
+template <int LT, bool FPE, int WISZ, int CI = -1, int CO = -1, int BA = -1, int BB = -1>
 __global__ __launch_bounds__(256) void ew_tiled_n(const TileNArgs P) {
     extern __shared__ char smem[];
     const EwArgs &A = P.E;
@@ -129,7 +132,20 @@ __global__ __launch_bounds__(256) void ew_tiled_n(const TileNArgs P) {
                 const int c = hi + 4 * q;
                 if (c >= m1) break;
                 char *po = orow + (int64_t)c * P.o_st1 * io;
-                if constexpr (CI >= 0) {
+                if constexpr (BA >= 0) {
+                    using T = typename dt_type<LT>::T;
+                    const int e = lo * kTnPad + c;
+                    const T a = cvt<LT>(((const typename dt_type<BA>::T *)(smem + P.lds_off[0]))[e]);
+                    const T bv = cvt<LT>(((const typename dt_type<BB>::T *)(smem + P.lds_off[1]))[e]);
+                    if (A.op >= IONP_LESS) {
+                        *(uint8_t *)po = (uint8_t)cmp_op<T>(A.op, a, bv);
+                    } else {
+                        const T r = arith_op<LT>(A.op, a, bv);
+                        *(T *)po = r;
+                        if constexpr (FPE && std::is_floating_point<T>::value)
+                            if (A.op <= IONP_TRUE_DIVIDE) fl |= fpe_arith_dyn<T>(A.op, a, bv, r);
+                    }
+                } else if constexpr (CI >= 0) {
                     using TI = typename dt_type<CI>::T;
                     using TO = typename dt_type<CO>::T;
                     const TI v = ((const TI *)(smem + P.lds_off[0]))[lo * kTnPad + c];
@@ -209,6 +225,31 @@ static TileNFn tiled_cast_kernel(int ci, int co, bool fpe) {
     }
 }
 
+// mixed-dtype binaries over int32 / int64 / float32 / float64 (loop dtype:
+// int64 for two integers, float64 otherwise, as NumPy promotes these pairs)
+template <int BA, int BB, bool FPE> static TileNFn tiled_bin_pair() {
+    constexpr bool ints = (BA == IONP_INT32 || BA == IONP_INT64) && (BB == IONP_INT32 || BB == IONP_INT64);
+    return ew_tiled_n<ints ? IONP_INT64 : IONP_FLOAT64, FPE, 0, -1, -1, BA, BB>;
+}
+template <int BA, bool FPE> static TileNFn tiled_bin_from(int bb) {
+    switch (bb) {
+        case IONP_INT32: return BA == IONP_INT32 ? nullptr : tiled_bin_pair<BA, IONP_INT32, FPE>();
+        case IONP_INT64: return BA == IONP_INT64 ? nullptr : tiled_bin_pair<BA, IONP_INT64, FPE>();
+        case IONP_FLOAT32: return BA == IONP_FLOAT32 ? nullptr : tiled_bin_pair<BA, IONP_FLOAT32, FPE>();
+        case IONP_FLOAT64: return BA == IONP_FLOAT64 ? nullptr : tiled_bin_pair<BA, IONP_FLOAT64, FPE>();
+        default: return nullptr;
+    }
+}
+template <bool FPE> static TileNFn tiled_bin_kernel(int ba, int bb) {
+    switch (ba) {
+        case IONP_INT32: return tiled_bin_from<IONP_INT32, FPE>(bb);
+        case IONP_INT64: return tiled_bin_from<IONP_INT64, FPE>(bb);
+        case IONP_FLOAT32: return tiled_bin_from<IONP_FLOAT32, FPE>(bb);
+        case IONP_FLOAT64: return tiled_bin_from<IONP_FLOAT64, FPE>(bb);
+        default: return nullptr;
+    }
+}
+
 static TileNFn tiled_where_kernel(int isz) {
     switch (isz) {
         case 1: return ew_tiled_n<IONP_UINT8, false, 1>;
@@ -262,6 +303,17 @@ bool try_tiled_any(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_
     // casts: typed conversion (as ew_rows_cast), flags as the generic kernel
     if (A.op == OP_CAST) {
         if (const TileNFn cf = tiled_cast_kernel(A.in_dt[0], A.out_dt, fpe)) fn = cf;
+    }
+    // mixed-dtype binaries: typed loads, the loop dtype NumPy promotes to
+    if (A.op >= 0 && A.op < IONP_NOPS && A.nin == 2 && A.in_dt[0] != A.in_dt[1]) {
+        const bool ints = (A.in_dt[0] == IONP_INT32 || A.in_dt[0] == IONP_INT64) &&
+                          (A.in_dt[1] == IONP_INT32 || A.in_dt[1] == IONP_INT64);
+        const int lt = ints ? IONP_INT64 : IONP_FLOAT64;
+        const int odt = A.op >= IONP_LESS ? (int)IONP_BOOL : lt;
+        if (A.loop_dt == lt && A.out_dt == odt)
+            if (const TileNFn bf = fpe ? tiled_bin_kernel<true>(A.in_dt[0], A.in_dt[1])
+                                       : tiled_bin_kernel<false>(A.in_dt[0], A.in_dt[1]))
+                fn = bf;
     }
     if (!fn) return false;
     TileNArgs P;
