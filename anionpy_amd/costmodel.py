@@ -13,8 +13,8 @@ prefers the heuristic unless the knobs' best beats it by FALLBACK.  Each
 layout's targets are scaled by layout_weights (longer layouts count more).  `install` loads
 the model into the library's advisor (ionp_advisor_set), which then serves
 every call made without a configuration; the built-in heuristic (config 0)
-remains the fallback, and the exhaustive sweep is the judge of the model's
-regret.  Whatever configuration it proposes, results are bit-identical:
+remains the fallback, and the model's regret is measured against the
+exhaustive sweep.  Whatever configuration it proposes, results are bit-identical:
 configurations only move work between threads.
 """
 import ctypes

@@ -134,8 +134,8 @@ def main():
         print(json.dumps(reduction_extras(ap, torch)), flush=True)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this backend runs on one GPU; multi-GPU sharding is described "
-                                   "in DESIGN.md and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this backend runs on one GPU; multi-GPU sharding is a design "
+                                   "only and not launched", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
     import anionpy_amd as ap
