@@ -274,6 +274,56 @@ def test_transposed_operand_any_call(ap):
     assert got.tobytes() == O.binary("add", f8, f4.T).tobytes()
 
 
+def test_transposed_operand_any_call_edges(ap):
+    """The LDS tile kernel's other variants and limits, against the oracle:
+    where bit selects of 2- and 4-byte values, a transposed condition in a
+    3-D batch, typed casts from bool and from float64 into uint8 (cvt edge
+    values), int32 / int64 maximum and minimum, a staged operand reversed
+    along the output's contiguous dim, and the output-contiguous extent at
+    the kernel's 32-element limit and just below it (the generic kernel)."""
+    rng = np.random.default_rng(29)
+    for dt in (np.int16, np.float32, np.int32):
+        c = rng.random((130, 257)) < 0.5
+        x = rng.integers(-1000, 1000, (130, 257)).astype(dt)
+        y = rng.integers(-1000, 1000, (257, 130)).astype(dt)
+        got = ap.where(ap.asarray(c), ap.asarray(x), ap.asarray(y).T).get()
+        want = O.where(c, x, y.T)
+        assert got.dtype == want.dtype and got.strides == want.strides and got.tobytes() == want.tobytes(), dt
+    cT = rng.random((3, 90, 70)) < 0.5
+    x3 = rng.standard_normal((3, 70, 90))
+    y3 = rng.standard_normal((3, 70, 90))
+    got = ap.where(ap.asarray(cT).transpose(0, 2, 1), ap.asarray(x3), ap.asarray(y3)).get()
+    want = O.where(cT.transpose(0, 2, 1), x3, y3)
+    assert got.strides == want.strides and got.tobytes() == want.tobytes()
+    b = rng.random((300, 200)) < 0.5
+    v = np.concatenate([rng.uniform(-300, 300, 59990), [np.nan, np.inf, -np.inf, 255.5, -0.5, 256.0, 1e20, -1e20,
+                                                         0.0, -0.0]]).reshape(300, 200)
+    for src, dt in ((b, np.float64), (b, np.int32), (v, np.uint8), (v, np.int16)):
+        want = np.ascontiguousarray(O.cast(src.T, dt))
+        with ap.errstate(all="ignore"):
+            got = ap.asarray(src).T.astype(dt, order="C").get()
+        assert got.dtype == want.dtype and got.tobytes() == want.tobytes(), (src.dtype, dt)
+    i4 = rng.integers(-2 ** 31, 2 ** 31, (200, 300)).astype(np.int32)
+    i8 = rng.integers(-2 ** 40, 2 ** 40, (300, 200))
+    for op in ("maximum", "minimum", "subtract"):
+        want = O.binary(op, i4, i8.T)
+        got = getattr(ap, op)(ap.asarray(i4), ap.asarray(i8).T).get()
+        assert got.strides == want.strides and got.tobytes() == want.tobytes(), op
+    f8 = rng.standard_normal((300, 200))
+    f4 = rng.standard_normal((200, 300)).astype(np.float32)
+    for a, bt, A, BT in ((f8, f4.T[:, ::-1], ap.asarray(f8), ap.asarray(f4).T[:, ::-1]),
+                         (f8[::-1], f4.T[::-1], ap.asarray(f8)[::-1], ap.asarray(f4).T[::-1])):
+        want = O.binary("add", a, bt)
+        got = ap.add(A, BT).get()
+        assert got.strides == want.strides and got.tobytes() == want.tobytes()
+    for n0 in (32, 31):
+        a = rng.standard_normal((500, n0))
+        bb = rng.standard_normal((n0, 500)).astype(np.float32)
+        want = O.binary("multiply", a, bb.T)
+        got = ap.multiply(ap.asarray(a), ap.asarray(bb).T).get()
+        assert got.strides == want.strides and got.tobytes() == want.tobytes(), n0
+
+
 def test_blocked_accumulate(ap):
     """Scans over many fibers (the blocked kernel): along the contiguous axis
     and along the outer axis of C-ordered arrays, extents that are not
