@@ -14,7 +14,13 @@
 // ew_element_src / ew_element_fpe_src -- the generic kernel's own element
 // function, so every dtype, conversion, select and comparison rule is
 // that kernel's -- and stores them along d0.  (Exact int64 / uint64
-// comparisons stay on the generic kernel.)
+// comparisons stay on the generic kernel.)  Measured and not kept: full
+// tiles of 16-byte-aligned operands staged as 16-byte vectors (every
+// operand's loads in flight before the LDS stores): 8192^2 f64 where and
+// f64 + f32 within 5 % either way, f32 where 9 % and unaligned (8190 x
+// 8191) calls 18 % slower, the vector path's registers costing occupancy
+// on every call; vectors for the transposed or the d0-contiguous operands
+// alone were 20-55 % slower.
 #include "elementwise.h"
 
 #include "device_ops.h"
