@@ -38,6 +38,7 @@ struct TileNArgs {
     int64_t st0[3], st1[3];   // input strides along d0 and d*
     int64_t o_st1;            // output stride along d* (along d0 it is 1)
     int32_t staged;           // bit k: input k read along d* (contiguous there)
+    int32_t ovec;             // the output's full tiles stored as aligned 16-byte vectors
     int32_t lds_off[3];       // byte offset of input k's LDS tile
     int64_t t0, t1;           // tiles along d0, d*
     FastDiv t0div, t1div;
@@ -130,6 +131,52 @@ __global__ __launch_bounds__(256) void ew_tiled_n(const TileNArgs P) {
             }
         }
         __syncthreads();
+        if constexpr (WISZ > 0 || CI >= 0 || BA >= 0) {
+            // where selects, casts and mixed-dtype arithmetic, a full tile
+            // of a 16-byte-aligned output: each lane W consecutive d0
+            // positions of one d* position, stored as one nontemporal
+            // 16-byte vector (comparisons keep the item stores below)
+            using OT = typename std::conditional<
+                (WISZ > 0), typename UintOf<(WISZ > 0 ? WISZ : 1)>::T,
+                typename std::conditional<(CI >= 0), typename dt_type<(CI >= 0 ? CO : IONP_UINT8)>::T,
+                                          typename dt_type<LT>::T>::type>::type;
+            if (P.ovec && m0 == kTn0 && m1 == kTn1 && (BA < 0 || A.op < IONP_LESS)) {
+                constexpr int W = 16 / (int)sizeof(OT);
+                constexpr int NV = kTn0 * kTn1 / W;  // vectors per tile, a multiple of 64
+                using OV = OT __attribute__((ext_vector_type(W)));
+#pragma unroll
+                for (int q = 0; q < (NV + 255) / 256; q++) {
+                    const int vi = (int)threadIdx.x + 256 * q;
+                    if (NV % 256 != 0 && vi >= NV) break;  // whole wavefronts
+                    const int c = vi / (kTn0 / W), r = vi % (kTn0 / W) * W;
+                    OV res;
+#pragma unroll
+                    for (int w = 0; w < W; w++) {
+                        const int e = (r + w) * kTnPad + c;
+                        if constexpr (WISZ > 0) {
+                            const bool cond = ((const uint8_t *)(smem + P.lds_off[0]))[e] != 0;
+                            res[w] = cond ? ((const OT *)(smem + P.lds_off[1]))[e] : ((const OT *)(smem + P.lds_off[2]))[e];
+                        } else if constexpr (CI >= 0) {
+                            using TI = typename dt_type<CI>::T;
+                            const TI v = ((const TI *)(smem + P.lds_off[0]))[e];
+                            if constexpr (CI == IONP_BOOL) res[w] = cvt<CO>((uint8_t)(v != 0));
+                            else res[w] = cvt<CO>(v);
+                            if constexpr (FPE && (CI == IONP_FLOAT32 || CI == IONP_FLOAT64)) fl |= fpe_cast_x(CI, CO, (double)v);
+                        } else {
+                            const OT a = cvt<LT>(((const typename dt_type<(BA >= 0 ? BA : LT)>::T *)(smem + P.lds_off[0]))[e]);
+                            const OT bv = cvt<LT>(((const typename dt_type<(BB >= 0 ? BB : LT)>::T *)(smem + P.lds_off[1]))[e]);
+                            const OT r = arith_op<LT>(A.op, a, bv);
+                            res[w] = r;
+                            if constexpr (FPE && std::is_floating_point<OT>::value)
+                                if (A.op <= IONP_TRUE_DIVIDE) fl |= fpe_arith_dyn<OT>(A.op, a, bv, r);
+                        }
+                    }
+                    __builtin_nontemporal_store(res, (OV *)(A.out + (bo + j0 + r + (i0 + c) * P.o_st1) * (int64_t)sizeof(OT)));
+                }
+                __syncthreads();
+                continue;
+            }
+        }
         // lanes along d0 (the output's contiguous dim), 8 positions of d*
         // each, operands read back from LDS
         if (lo < m0) {
@@ -350,6 +397,14 @@ bool try_tiled_any(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_
         P.bo[P.nb] = A.out_st[d];
         P.nb++;
         nbatch *= A.shape[d];
+    }
+    // 16-byte output vectors: base, stride along d* and batch strides
+    // multiples of 16 bytes (tile origins advance by 64 items along d0)
+    {
+        const int64_t io = itemsize(A.out_dt);
+        bool v = reinterpret_cast<uintptr_t>(A.out) % 16 == 0 && (P.o_st1 * io) % 16 == 0;
+        for (int d = 0; d < P.nb; d++) v = v && (P.bo[d] * io) % 16 == 0;
+        P.ovec = v;
     }
     P.ntiles = P.t0 * P.t1 * nbatch;
     if (P.ntiles >= (1ll << 31) || P.t0 >= (1ll << 31) || P.t1 >= (1ll << 31)) return false;

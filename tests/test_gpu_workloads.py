@@ -322,6 +322,41 @@ def test_transposed_operand_any_call_edges(ap):
         want = O.binary("multiply", a, bb.T)
         got = ap.multiply(ap.asarray(a), ap.asarray(bb).T).get()
         assert got.strides == want.strides and got.tobytes() == want.tobytes(), n0
+    # outputs whose rows are 16-byte multiples: full tiles stored as 16-byte
+    # vectors (where of every value size, casts with the cvt edges and the
+    # flag variant, mixed-dtype arithmetic; comparisons keep item stores)
+    for dt in (np.int8, np.int16, np.float32, np.float64):
+        c = rng.random((130, 256)) < 0.5
+        x = rng.integers(-100, 100, (130, 256)).astype(dt)
+        y = rng.integers(-100, 100, (256, 130)).astype(dt)
+        got = ap.where(ap.asarray(c), ap.asarray(x), ap.asarray(y).T).get()
+        want = O.where(c, x, y.T)
+        assert got.strides == want.strides and got.tobytes() == want.tobytes(), dt
+    va = np.concatenate([rng.uniform(-300, 300, 256 * 320 - 10), [np.nan, np.inf, -np.inf, 255.5, -0.5, 256.0, 1e20,
+                                                                  -1e20, 0.0, -0.0]]).reshape(256, 320)
+    ba = rng.random((256, 320)) < 0.5
+    for src, dt in ((va, np.uint8), (va, np.bool_), (va, np.int16), (va, np.float32), (va, np.int64),
+                    (ba, np.float64), (ba, np.uint8)):
+        want = np.ascontiguousarray(O.cast(src.T, dt))
+        with ap.errstate(all="ignore"):
+            got = ap.asarray(src).T.astype(dt, order="C").get()
+        assert got.dtype == want.dtype and got.tobytes() == want.tobytes(), (src.dtype, dt)
+    hugea = np.full((256, 192), 1e300)
+    assert O.cast_fpe(hugea.T, np.float32) & 2
+    with ap.errstate(over="raise", invalid="ignore", under="ignore", divide="ignore"):
+        with pytest.raises(FloatingPointError, match="overflow"):
+            ap.asarray(hugea).T.astype(np.float32, order="C")
+        got = ap.asarray(va[:, :192] / 1e3).T.astype(np.float32, order="C").get()
+    assert got.tobytes() == np.ascontiguousarray(O.cast((va[:, :192] / 1e3).T, np.float32)).tobytes()
+    ia = rng.integers(-2 ** 31, 2 ** 31, (256, 320)).astype(np.int32)
+    la = rng.integers(-2 ** 40, 2 ** 40, (320, 256))
+    fa = rng.standard_normal((320, 256)).astype(np.float32)
+    for op, a, b, ad, bd in (("maximum", ia, la.T, ap.asarray(ia), ap.asarray(la).T),
+                             ("add", ia, fa.T, ap.asarray(ia), ap.asarray(fa).T),
+                             ("less", ia, la.T, ap.asarray(ia), ap.asarray(la).T)):
+        want = O.binary(op, a, b)
+        got = getattr(ap, op)(ad, bd).get()
+        assert got.strides == want.strides and got.tobytes() == want.tobytes(), op
 
 
 def test_blocked_accumulate(ap):
