@@ -859,6 +859,11 @@ ionp_status run_elementwise(EwArgs &A, const ionp_config *cfg, hipStream_t s) {
     if (half) return launch_generic(A, cfg, s);  // float16: the generic walk only
     ionp_status st;
     if (try_tiled(A, cfg, s, &st)) return st;
+    // with IEEE flags tracked try_tiled leaves float arithmetic over a
+    // transposed operand alone; the LDS tile kernel (typed, with flags)
+    // takes it before the rows gather, which would read that operand at a
+    // stride (8192^2 f64 A + B.T: 550 us in the gather's flag variant)
+    if (A.fpe && try_tiled_any(A, cfg, s, &st)) return st;
     if (try_rows(A, cfg, s, &st)) return st;
     if (try_rows_select_cast(A, cfg, s, &st)) return st;
     if (try_tiled_any(A, cfg, s, &st)) return st;

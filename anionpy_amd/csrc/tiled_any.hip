@@ -283,18 +283,20 @@ static TileNFn tiled_cast_kernel(int ci, int co, bool fpe) {
     }
 }
 
-// mixed-dtype binaries over int32 / int64 / float32 / float64 (loop dtype:
-// int64 for two integers, float64 otherwise, as NumPy promotes these pairs)
+// binaries over int32 / int64 / float32 / float64 (loop dtype: the dtype
+// of a same-dtype pair; int64 for two integers, float64 otherwise, as
+// NumPy promotes these pairs).  Same-dtype pairs come here when ew_tiled_v
+// does not take them: with IEEE flags tracked, unaligned rows.
 template <int BA, int BB, bool FPE> static TileNFn tiled_bin_pair() {
     constexpr bool ints = (BA == IONP_INT32 || BA == IONP_INT64) && (BB == IONP_INT32 || BB == IONP_INT64);
-    return ew_tiled_n<ints ? IONP_INT64 : IONP_FLOAT64, FPE, 0, -1, -1, BA, BB>;
+    return ew_tiled_n<BA == BB ? BA : ints ? IONP_INT64 : IONP_FLOAT64, FPE, 0, -1, -1, BA, BB>;
 }
 template <int BA, bool FPE> static TileNFn tiled_bin_from(int bb) {
     switch (bb) {
-        case IONP_INT32: return BA == IONP_INT32 ? nullptr : tiled_bin_pair<BA, IONP_INT32, FPE>();
-        case IONP_INT64: return BA == IONP_INT64 ? nullptr : tiled_bin_pair<BA, IONP_INT64, FPE>();
-        case IONP_FLOAT32: return BA == IONP_FLOAT32 ? nullptr : tiled_bin_pair<BA, IONP_FLOAT32, FPE>();
-        case IONP_FLOAT64: return BA == IONP_FLOAT64 ? nullptr : tiled_bin_pair<BA, IONP_FLOAT64, FPE>();
+        case IONP_INT32: return tiled_bin_pair<BA, IONP_INT32, FPE>();
+        case IONP_INT64: return tiled_bin_pair<BA, IONP_INT64, FPE>();
+        case IONP_FLOAT32: return tiled_bin_pair<BA, IONP_FLOAT32, FPE>();
+        case IONP_FLOAT64: return tiled_bin_pair<BA, IONP_FLOAT64, FPE>();
         default: return nullptr;
     }
 }
@@ -362,11 +364,11 @@ bool try_tiled_any(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_
     if (A.op == OP_CAST) {
         if (const TileNFn cf = tiled_cast_kernel(A.in_dt[0], A.out_dt, fpe)) fn = cf;
     }
-    // mixed-dtype binaries: typed loads, the loop dtype NumPy promotes to
-    if (A.op >= 0 && A.op < IONP_NOPS && A.nin == 2 && A.in_dt[0] != A.in_dt[1]) {
+    // binaries: typed loads, the loop dtype NumPy promotes to
+    if (A.op >= 0 && A.op < IONP_NOPS && A.nin == 2) {
         const bool ints = (A.in_dt[0] == IONP_INT32 || A.in_dt[0] == IONP_INT64) &&
                           (A.in_dt[1] == IONP_INT32 || A.in_dt[1] == IONP_INT64);
-        const int lt = ints ? IONP_INT64 : IONP_FLOAT64;
+        const int lt = A.in_dt[0] == A.in_dt[1] ? A.in_dt[0] : ints ? IONP_INT64 : IONP_FLOAT64;
         const int odt = A.op >= IONP_LESS ? (int)IONP_BOOL : lt;
         if (A.loop_dt == lt && A.out_dt == odt)
             if (const TileNFn bf = fpe ? tiled_bin_kernel<true>(A.in_dt[0], A.in_dt[1])

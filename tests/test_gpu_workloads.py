@@ -348,6 +348,22 @@ def test_transposed_operand_any_call_edges(ap):
             ap.asarray(hugea).T.astype(np.float32, order="C")
         got = ap.asarray(va[:, :192] / 1e3).T.astype(np.float32, order="C").get()
     assert got.tobytes() == np.ascontiguousarray(O.cast((va[:, :192] / 1e3).T, np.float32)).tobytes()
+    # same-dtype float arithmetic with IEEE flags tracked (the typed tile
+    # kernel with flags; untracked, ew_tiled_v takes these): partial tiles,
+    # a clean call under raise, an overflow raised
+    for dt in (np.float64, np.float32):
+        p = rng.standard_normal((300, 1000)).astype(dt)
+        qT = rng.standard_normal((1000, 300)).astype(dt)
+        with ap.errstate(all="raise"):
+            got = ap.subtract(ap.asarray(p), ap.asarray(qT).T).get()
+            got_m = ap.multiply(ap.asarray(p), ap.asarray(qT).T).get()
+        assert got.tobytes() == O.binary("subtract", p, qT.T).tobytes(), dt
+        assert got_m.tobytes() == O.binary("multiply", p, qT.T).tobytes(), dt
+        hp = np.full((256, 192), 1e30 if dt == np.float32 else 1e300, dtype=dt)
+        assert O.binary_fpe("multiply", hp, hp.T.copy().T) & 2
+        with ap.errstate(over="raise", invalid="ignore", under="ignore", divide="ignore"):
+            with pytest.raises(FloatingPointError, match="overflow"):
+                ap.multiply(ap.asarray(hp), ap.asarray(np.ascontiguousarray(hp.T)).T)
     ia = rng.integers(-2 ** 31, 2 ** 31, (256, 320)).astype(np.int32)
     la = rng.integers(-2 ** 40, 2 ** 40, (320, 256))
     fa = rng.standard_normal((320, 256)).astype(np.float32)
