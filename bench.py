@@ -312,6 +312,11 @@ def elementwise_extras(ap, torch):
         o = ap.add(Ad, BT)
         out[f"add_T_{np.dtype(dt).name}"] = _rate(Ad.nbytes + BT.nbytes + o.nbytes,
                                                   timed(torch, lambda: ap.add(Ad, BT, out=o), 10))
+        # the same call with IEEE flags tracked (NumPy's default modes): the
+        # typed LDS tile kernel with flags, plus each call's flag fetch
+        with ap.errstate(divide="warn", over="warn", invalid="warn"):
+            out[f"add_T_{np.dtype(dt).name}_tracked"] = _rate(Ad.nbytes + BT.nbytes + o.nbytes,
+                                                              timed(torch, lambda: ap.add(Ad, BT, out=o), 10))
         del o, Bd, BT, Ad
     cond = ap.asarray(np.random.default_rng(12).random((8192, 8192)) < 0.5)
     o = ap.where(cond, A, col)
