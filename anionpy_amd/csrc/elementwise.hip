@@ -531,6 +531,7 @@ struct TileArgs {
     const char *x, *y;
     char *out;
     int32_t y_first;       // Y is the op's left operand
+    uint32_t *fpe;         // IEEE flag word (ew_tiled_v's flag variant only)
 };
 
 constexpr int kTile = 64;
@@ -604,8 +605,10 @@ template <typename T, int OPK> __global__ __launch_bounds__(256) void ew_tiled(c
 // 246 / 389 us.  Partial tiles at the edges take
 // an element-by-element path.
 // XB: X is broadcast along d0 (a template parameter: a runtime branch
-// between the loads splits their batch, measured 10 % slower).
-template <typename T, int OPK, int TI, int TJ, bool XB>
+// between the loads splits their batch, measured 10 % slower).  FPE: float
+// arithmetic with IEEE flags tracked, each element's flags ORed into the
+// flag word once per wavefront (NumPy's default modes track them).
+template <typename T, int OPK, int TI, int TJ, bool XB, bool FPE = false>
 __global__ __launch_bounds__(256) void ew_tiled_v(const TileArgs P) {
     constexpr int V = 16 / sizeof(T);
     using VT = T __attribute__((ext_vector_type(V)));
@@ -617,6 +620,7 @@ __global__ __launch_bounds__(256) void ew_tiled_v(const TileArgs P) {
     const T *__restrict__ x = (const T *)P.x;
     const T *__restrict__ y = (const T *)P.y;
     OT *__restrict__ o = (OT *)P.out;
+    uint32_t fl = 0;
     for (int64_t t = blockIdx.x; t < P.ntiles; t += gridDim.x) {
         // t = (batch, tile along d*, tile along d0), d0 fastest
         uint32_t rem = (uint32_t)t;
@@ -647,7 +651,9 @@ __global__ __launch_bounds__(256) void ew_tiled_v(const TileArgs P) {
                 if (i >= m1 || j >= m0) continue;
                 const T xv = x[bx + (j0 + j) * P.x_st0 + (i0 + i) * P.x_st1];
                 const T yv = y[by + (j0 + j) * P.y_st0 + (i0 + i)];
-                o[bo + (j0 + j) + (i0 + i) * P.o_st1] = P.y_first ? row_apply<T, OPK>(yv, xv) : row_apply<T, OPK>(xv, yv);
+                const OT rv = P.y_first ? row_apply<T, OPK>(yv, xv) : row_apply<T, OPK>(xv, yv);
+                o[bo + (j0 + j) + (i0 + i) * P.o_st1] = rv;
+                if constexpr (FPE) fl |= P.y_first ? fpe_arith<T, OPK>(yv, xv, rv) : fpe_arith<T, OPK>(xv, yv, rv);
             }
             continue;  // no LDS use: no barrier needed
         }
@@ -684,11 +690,13 @@ __global__ __launch_bounds__(256) void ew_tiled_v(const TileArgs P) {
             for (int e = 0; e < V; e++) {
                 const T a = xv[k][e], b = tile[jv * V + e][i];
                 r[e] = P.y_first ? row_apply<T, OPK>(b, a) : row_apply<T, OPK>(a, b);
+                if constexpr (FPE) fl |= P.y_first ? fpe_arith<T, OPK>(b, a, r[e]) : fpe_arith<T, OPK>(a, b, r[e]);
             }
             __builtin_nontemporal_store(r, (OV *)(o + bo + (i0 + i) * P.o_st1 + j0 + jv * V));
         }
         __syncthreads();
     }
+    if constexpr (FPE) fpe_commit(P.fpe, fl);
 }
 
 typedef void (*TileFn)(const TileArgs);
@@ -721,7 +729,26 @@ template <typename T, bool XB> static TileFn tiled_v_t(int op) {
 #undef IONP_TILEV_CASE
 }
 
-static TileFn tiled_v_kernel(int dt, int op, bool xb) {
+// the flag variants: float arithmetic only (the ops that raise flags)
+template <typename T, bool XB> static TileFn tiled_v_fpe_t(int op) {
+    constexpr int TI = tile_ti<T>();
+    switch (op) {
+        case IONP_ADD: return ew_tiled_v<T, IONP_ADD, TI, kTileTJ, XB, true>;
+        case IONP_SUBTRACT: return ew_tiled_v<T, IONP_SUBTRACT, TI, kTileTJ, XB, true>;
+        case IONP_MULTIPLY: return ew_tiled_v<T, IONP_MULTIPLY, TI, kTileTJ, XB, true>;
+        case IONP_TRUE_DIVIDE: return ew_tiled_v<T, IONP_TRUE_DIVIDE, TI, kTileTJ, XB, true>;
+        default: return nullptr;
+    }
+}
+
+static TileFn tiled_v_kernel(int dt, int op, bool xb, bool fpe) {
+    if (fpe) {
+        switch (dt) {
+            case IONP_FLOAT32: return xb ? tiled_v_fpe_t<float, true>(op) : tiled_v_fpe_t<float, false>(op);
+            case IONP_FLOAT64: return xb ? tiled_v_fpe_t<double, true>(op) : tiled_v_fpe_t<double, false>(op);
+            default: return nullptr;
+        }
+    }
     switch (dt) {
         case IONP_INT32: return xb ? tiled_v_t<int32_t, true>(op) : tiled_v_t<int32_t, false>(op);
         case IONP_INT64: return xb ? tiled_v_t<int64_t, true>(op) : tiled_v_t<int64_t, false>(op);
@@ -769,7 +796,9 @@ static TileFn tiled_kernel(int dt, int op) {
 // whatever stride it has).
 bool try_tiled(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_status *st) {
     if (A.nin != 2 || A.op < 0 || A.op >= IONP_NOPS || A.mixed_su || A.ndim < 2) return false;
-    if (A.fpe && is_float_dt(A.loop_dt) && A.op <= IONP_TRUE_DIVIDE) return false;  // flags: rows / generic
+    // float arithmetic with IEEE flags tracked: the vector form's flag
+    // variant, else nothing here (the LDS tile kernel has flags)
+    const bool fpe = A.fpe && is_float_dt(A.loop_dt) && A.op <= IONP_TRUE_DIVIDE;
     if (A.in_dt[0] != A.loop_dt || A.in_dt[1] != A.loop_dt) return false;
     if (A.out_dt != (A.op >= IONP_LESS ? (int)IONP_BOOL : A.loop_dt) || A.out_st[0] != 1) return false;
     int yk = -1, dstar = -1;
@@ -824,7 +853,8 @@ bool try_tiled(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_stat
                P.o_st1 % V == 0 && (P.x_st0 == 0 || (al(P.x, 16) && (P.x_st1 * isz) % 16 == 0));
     for (int d = 0; vec && d < P.nb; d++)
         vec = (P.by[d] * isz) % 16 == 0 && P.bo[d] % V == 0 && (P.x_st0 == 0 || (P.bx[d] * isz) % 16 == 0);
-    const TileFn vfn = vec ? tiled_v_kernel(A.loop_dt, A.op, P.x_st0 == 0) : nullptr;
+    P.fpe = fpe ? A.fpe : nullptr;
+    const TileFn vfn = vec ? tiled_v_kernel(A.loop_dt, A.op, P.x_st0 == 0, fpe) : nullptr;
     if (vfn) {
         const int ti = isz == 8 ? 32 : 128;
         P.t0 = (P.n0 + kTileTJ - 1) / kTileTJ;
@@ -842,6 +872,7 @@ bool try_tiled(const EwArgs &A, const ionp_config *cfg, hipStream_t s, ionp_stat
         *st = hipGetLastError() == hipSuccess ? IONP_OK : set_error(IONP_ERR_DEVICE, "kernel launch failed");
         return true;
     }
+    if (fpe) return false;  // the scalar form has no flag variant
     // measured on MI355X (8192 x 8192, A + B.T): 4 workgroups per CU for
     // 8-byte elements, 16 for 4-byte ones (their LDS tile is half the size)
     const int bpc = (cfg && cfg->blocks_per_cu > 0) ? cfg->blocks_per_cu : (itemsize(A.loop_dt) == 8 ? 4 : 16);
@@ -859,10 +890,11 @@ ionp_status run_elementwise(EwArgs &A, const ionp_config *cfg, hipStream_t s) {
     if (half) return launch_generic(A, cfg, s);  // float16: the generic walk only
     ionp_status st;
     if (try_tiled(A, cfg, s, &st)) return st;
-    // with IEEE flags tracked try_tiled leaves float arithmetic over a
-    // transposed operand alone; the LDS tile kernel (typed, with flags)
-    // takes it before the rows gather, which would read that operand at a
-    // stride (8192^2 f64 A + B.T: 550 us in the gather's flag variant)
+    // with IEEE flags tracked try_tiled takes float arithmetic over a
+    // transposed operand only in its vector form; the LDS tile kernel
+    // (typed, with flags) takes the rest before the rows gather, which
+    // would read that operand at a stride (8192^2 f64 A + B.T: 550 us in
+    // the gather's flag variant, 313.5 us in the tile kernel's)
     if (A.fpe && try_tiled_any(A, cfg, s, &st)) return st;
     if (try_rows(A, cfg, s, &st)) return st;
     if (try_rows_select_cast(A, cfg, s, &st)) return st;

@@ -286,7 +286,7 @@ static TileNFn tiled_cast_kernel(int ci, int co, bool fpe) {
 // binaries over int32 / int64 / float32 / float64 (loop dtype: the dtype
 // of a same-dtype pair; int64 for two integers, float64 otherwise, as
 // NumPy promotes these pairs).  Same-dtype pairs come here when ew_tiled_v
-// does not take them: with IEEE flags tracked, unaligned rows.
+// does not take them (with IEEE flags tracked: rows not 16-byte aligned).
 template <int BA, int BB, bool FPE> static TileNFn tiled_bin_pair() {
     constexpr bool ints = (BA == IONP_INT32 || BA == IONP_INT64) && (BB == IONP_INT32 || BB == IONP_INT64);
     return ew_tiled_n<BA == BB ? BA : ints ? IONP_INT64 : IONP_FLOAT64, FPE, 0, -1, -1, BA, BB>;
